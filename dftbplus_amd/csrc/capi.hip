@@ -1,0 +1,1499 @@
+// C ABI of the dftbhip library and the resident SCC context.
+//
+// Stateless drop-ins (sygvd, density matrices, sparse kernels) take host arrays with the
+// reference's conventions and stage them through device memory. The context keeps everything
+// resident: geometry, neighbour blocks, H0/S, the gamma matrix, the dense matrices and the
+// eigensolver workspaces, and runs the SCC cycle of src/dftbp/dftbplus/main.F90:1429-1563 with
+// the charge mixing (Broyden, src/dftbp/mixer/broydenmixer.F90:180-342) and the Fermi filling
+// (src/dftbp/dftb/etemp.F90:59-455) on the host, everything of O(N^2) or more on the device.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/dftbhip.h"
+#include "common.h"
+#include "eigen.h"
+#include "sparse.h"
+
+namespace dftbhip {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const std::string& last_error() { return g_err; }
+
+struct Runtime {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  EigenSolver eig;
+};
+static Runtime g_rt;
+
+static int ensure_runtime() {
+  if (g_rt.stream) return 0;
+  int dev = 0;
+  int n = 0;
+  DH_HIP(hipGetDeviceCount(&n));
+  if (n <= 0) {
+    set_error("dftbhip: no HIP device available");
+    return kErrDevice;
+  }
+  if (g_rt.device >= 0) dev = g_rt.device;
+  DH_HIP(hipSetDevice(dev));
+  g_rt.device = dev;
+  DH_HIP(hipStreamCreate(&g_rt.stream));
+  return 0;
+}
+
+// y = G^T x for a dense (n x n) column-major G: one wave per output element.
+__global__ void gemv_t_kernel(int n, const double* G, const double* x, double* y) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const double* col = G + (long)j * n;
+  double acc = 0.0;
+  for (int i = lane; i < n; i += 64) acc += col[i] * x[i];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if (lane == 0) y[j] = acc;
+}
+
+__global__ void expand_kernel(int nat, const long* iAtomStart, const double* va, double* vo) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nat) return;
+  for (long o = iAtomStart[a]; o < iAtomStart[a + 1]; ++o) vo[o] = va[a];
+}
+
+__global__ void scale_kernel(int n, const double* a, const double* b, double* c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = a[i] * b[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// Slater-Koster parameters on the device
+// ------------------------------------------------------------------------------------------
+struct SkTables {
+  SkDev dev{};
+  DevBuf<double> htab, stab, repdata;
+  DevBuf<RepDev> dreps;
+  int nsp = 0;
+  std::vector<int> norb;
+  std::vector<double> hubbu, q0;
+  std::vector<double> repcut;
+  double skcut = 0.0;
+  std::vector<double> gcut;   // expGamma cutoffs (sp2 * nsp + sp1)
+  DevBuf<double> dhubbu, dgcut;
+};
+
+static double expgamma_h(double r, double ua, double ub) {
+  const double ta = 3.2 * ua, tb = 3.2 * ub;
+  auto gsub = [](double rr, double t1, double t2) {
+    const double d2 = t1 * t1 - t2 * t2;
+    return std::exp(-t1 * rr) * ((0.5 * std::pow(t2, 4) * t1 / (d2 * d2)) -
+                                 (std::pow(t2, 6) - 3.0 * std::pow(t2, 4) * t1 * t1) /
+                                     (rr * d2 * d2 * d2));
+  };
+  if (r < 1.0e-5) {
+    if (std::fabs(ua - ub) < 1.0e-10) return -0.5 * (ua + ub);
+    return -0.5 * ((ta * tb) / (ta + tb) + (ta * tb) * (ta * tb) / std::pow(ta + tb, 3));
+  }
+  if (std::fabs(ua - ub) < 1.0e-10) {
+    const double tm = 0.5 * (ta + tb);
+    return std::exp(-tm * r) * (1.0 / r + 0.6875 * tm + 0.1875 * r * tm * tm +
+                                0.02083333333333333333 * r * r * tm * tm * tm);
+  }
+  return gsub(r, ta, tb) + gsub(r, tb, ta);
+}
+
+// expGammaCutoff (src/dftbp/dftb/shortgammafuncs.F90:27-92)
+static double expgamma_cutoff_h(double u2, double u1) {
+  const double minv = 1.0e-10, tol = 1.0e-10;
+  double rab = 1.0;
+  while (expgamma_h(rab, u2, u1) > minv) rab *= 2.0;
+  double mincut = rab + 0.1, maxcut = 0.5 * rab - 0.1;
+  const double maxg = expgamma_h(maxcut, u2, u1), ming = expgamma_h(mincut, u2, u1);
+  double lowerg = expgamma_h(mincut, u2, u1);
+  double cut = maxcut + 0.1;
+  double g = expgamma_h(cut, u2, u1);
+  while ((g - lowerg) > tol) {
+    maxcut = 0.5 * (cut + mincut);
+    if ((maxg >= ming) == (minv >= expgamma_h(maxcut, u2, u1))) {
+      mincut = maxcut;
+      lowerg = expgamma_h(mincut, u2, u1);
+    } else {
+      cut = maxcut;
+      g = expgamma_h(cut, u2, u1);
+    }
+  }
+  return mincut;
+}
+
+static int sk_setup(SkTables& t, const dftbhip_params* p, hipStream_t s) {
+  DH_CHECK(p->nspecies > 0 && p->nspecies <= kMaxSpecies, "params: bad number of species");
+  DH_CHECK(p->nrows >= 9 && p->dist > 0.0, "params: SK grid needs at least 9 rows");
+  const int ns = p->nspecies;
+  t.nsp = ns;
+  t.dev.nsp = ns;
+  t.dev.nrows = p->nrows;
+  t.dev.dist = p->dist;
+  t.norb.assign(ns, 0);
+  t.hubbu.assign(ns, 0.0);
+  t.q0.assign(ns, 0.0);
+  for (int a = 0; a < ns; ++a) {
+    SpeciesDev& sp = t.dev.sp[a];
+    DH_CHECK(p->nshell[a] >= 1 && p->nshell[a] <= 2, "params: 1 or 2 shells per species");
+    sp.nshell = p->nshell[a];
+    int no = 0;
+    for (int sh = 0; sh < sp.nshell; ++sh) {
+      const int l = p->angshell[a][sh];
+      DH_CHECK(l == 0 || l == 1, "params: only s and p shells are supported");
+      sp.ang[sh] = l;
+      for (int m = 0; m < 2 * l + 1; ++m) {
+        DH_CHECK(no < 4, "params: at most 4 orbitals per atom");
+        sp.onsite[no++] = p->onsite[a][sh];
+      }
+      t.q0[a] += p->occ[a][sh];
+    }
+    sp.norb = no;
+    t.norb[a] = no;
+    t.hubbu[a] = p->hubbu[a];
+  }
+  std::vector<double> hall, sall;
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b < ns; ++b) {
+      const int pr = a * ns + b;
+      int nint = 0;
+      for (int i = 0; i < t.dev.sp[a].nshell; ++i)
+        for (int j = 0; j < t.dev.sp[b].nshell; ++j)
+          nint += std::min(t.dev.sp[a].ang[i], t.dev.sp[b].ang[j]) + 1;
+      DH_CHECK(p->ninteg[pr] == nint, "params: ninteg does not match the shells");
+      DH_CHECK(p->htab[pr] && p->stab[pr], "params: missing SK table");
+      t.dev.nint[pr] = nint;
+      t.dev.tabOff[pr] = (long)hall.size();
+      hall.insert(hall.end(), p->htab[pr], p->htab[pr] + (size_t)p->nrows * nint);
+      sall.insert(sall.end(), p->stab[pr], p->stab[pr] + (size_t)p->nrows * nint);
+    }
+  DH_TRY(t.htab.upload(hall.data(), hall.size(), s));
+  DH_TRY(t.stab.upload(sall.data(), sall.size(), s));
+  t.dev.htab = t.htab.p;
+  t.dev.stab = t.stab.p;
+  t.skcut = p->nrows * p->dist + 1.0;
+  // repulsives
+  std::vector<double> rd;
+  std::vector<RepDev> reps(ns * ns);
+  std::vector<long> offs(ns * ns * 3);
+  t.repcut.assign(ns * ns, 0.0);
+  for (int pr = 0; pr < ns * ns; ++pr) {
+    const int ni = p->rep_nint[pr];
+    DH_CHECK(ni >= 1 && p->rep_xs[pr] && p->rep_last[pr], "params: missing repulsive");
+    reps[pr].nint = ni;
+    reps[pr].cutoff = p->rep_cutoff[pr];
+    t.repcut[pr] = p->rep_cutoff[pr];
+    for (int c = 0; c < 3; ++c) reps[pr].expc[c] = p->rep_expc[pr][c];
+    offs[3 * pr] = (long)rd.size();
+    rd.insert(rd.end(), p->rep_xs[pr], p->rep_xs[pr] + ni);
+    offs[3 * pr + 1] = (long)rd.size();
+    if (ni > 1) rd.insert(rd.end(), p->rep_coef[pr], p->rep_coef[pr] + 4 * (ni - 1));
+    offs[3 * pr + 2] = (long)rd.size();
+    rd.insert(rd.end(), p->rep_last[pr], p->rep_last[pr] + 6);
+  }
+  DH_TRY(t.repdata.upload(rd.data(), rd.size(), s));
+  for (int pr = 0; pr < ns * ns; ++pr) {
+    reps[pr].xs = t.repdata.p + offs[3 * pr];
+    reps[pr].coef = t.repdata.p + offs[3 * pr + 1];
+    reps[pr].last = t.repdata.p + offs[3 * pr + 2];
+  }
+  DH_TRY(t.dreps.upload(reps.data(), reps.size(), s));
+  t.gcut.assign(ns * ns, 0.0);
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b < ns; ++b) t.gcut[b * ns + a] = expgamma_cutoff_h(t.hubbu[b], t.hubbu[a]);
+  DH_TRY(t.dhubbu.upload(t.hubbu.data(), ns, s));
+  DH_TRY(t.dgcut.upload(t.gcut.data(), ns * ns, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace dftbhip
+
+struct dftbhip_sk {
+  dftbhip::SkTables t;
+};
+
+using namespace dftbhip;
+
+// ------------------------------------------------------------------------------------------
+// device management
+// ------------------------------------------------------------------------------------------
+extern "C" int dftbhip_get_gpus_available(int* ngpus) {
+  DH_CHECK(ngpus, "ngpus is null");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *ngpus = (e == hipSuccess) ? n : 0;
+  return 0;
+}
+
+extern "C" int dftbhip_init(int device) {
+  if (g_rt.stream) return 0;
+  g_rt.device = device;
+  return ensure_runtime();
+}
+
+extern "C" int dftbhip_finalize(void) {
+  g_rt.eig.release();
+  if (g_rt.stream) (void)hipStreamDestroy(g_rt.stream);
+  g_rt.stream = nullptr;
+  return 0;
+}
+
+extern "C" int dftbhip_last_error(char* buf, int len) {
+  if (!buf || len <= 0) return kErrArg;
+  std::strncpy(buf, g_err.c_str(), len - 1);
+  buf[len - 1] = 0;
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// generalized eigensolver and density matrices (stateless drop-ins)
+// ------------------------------------------------------------------------------------------
+extern "C" int dftbhip_sygvd(int itype, char jobz, char uplo, int n, double* a, int lda,
+                             double* b, int ldb, double* w) {
+  DH_CHECK(itype == 1, "sygvd: only itype = 1");
+  DH_CHECK(uplo == 'L' || uplo == 'l', "sygvd: only uplo = 'L'");
+  DH_CHECK(jobz == 'V' || jobz == 'v' || jobz == 'N' || jobz == 'n', "sygvd: bad jobz");
+  DH_CHECK(n >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), "sygvd: bad dimensions");
+  DH_CHECK(a && b && w, "sygvd: null pointer");
+  if (n == 0) return 0;
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  EigenSolver& E = g_rt.eig;
+  DH_TRY(E.ensure(n));
+  DevBuf<double> H, S;
+  DH_TRY(H.alloc((size_t)n * n));
+  DH_TRY(S.alloc((size_t)n * n));
+  DH_HIP(hipMemcpy2DAsync(H.p, n * sizeof(double), a, lda * sizeof(double), n * sizeof(double), n,
+                          hipMemcpyHostToDevice, s));
+  DH_HIP(hipMemcpy2DAsync(S.p, n * sizeof(double), b, ldb * sizeof(double), n * sizeof(double), n,
+                          hipMemcpyHostToDevice, s));
+  DH_TRY(symmetrize_lower(s, H.p, n, n));
+  int info = 0;
+  E.factored = false;
+  DH_TRY(E.factor_overlap(s, S.p, n, &info));
+  if (info != 0) {
+    set_error("sygvd: overlap matrix not positive definite");
+    return n + info;
+  }
+  std::vector<double> wh;
+  DH_TRY(E.solve(s, H.p, n, wh, &info));
+  if (info != 0) return info;
+  std::copy(wh.begin(), wh.end(), w);
+  if (jobz == 'V' || jobz == 'v') {
+    DH_TRY(E.vectors(s, n, H.p, n));
+    DH_HIP(hipMemcpy2DAsync(a, lda * sizeof(double), H.p, n * sizeof(double), n * sizeof(double),
+                            n, hipMemcpyDeviceToHost, s));
+  }
+  DH_HIP(hipMemcpy2DAsync(b, ldb * sizeof(double), E.L.p, n * sizeof(double), n * sizeof(double),
+                          n, hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+static int dm_common(int n, int nlev, const double* evec, int lde, const double* wts, double* dm,
+                     int lddm) {
+  DH_CHECK(n >= 0 && nlev >= 0 && nlev <= n && lde >= std::max(1, n) && lddm >= std::max(1, n),
+           "density matrix: bad dimensions");
+  if (n == 0) return 0;
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  DevBuf<double> X, P, f;
+  DH_TRY(X.alloc((size_t)n * std::max(nlev, 1)));
+  DH_TRY(P.alloc((size_t)n * n));
+  DH_TRY(f.upload(wts, std::max(nlev, 1), s));
+  if (nlev > 0)
+    DH_HIP(hipMemcpy2DAsync(X.p, n * sizeof(double), evec, lde * sizeof(double),
+                            n * sizeof(double), nlev, hipMemcpyHostToDevice, s));
+  DH_TRY(gemm<double>(s, false, true, n, n, nlev, 1.0, X.p, n, X.p, n, 0.0, P.p, n, true, kKFull,
+                      f.p));
+  DH_HIP(hipMemcpy2DAsync(dm, lddm * sizeof(double), P.p, n * sizeof(double), n * sizeof(double),
+                          n, hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_dm_real(int n, int nlevels, const double* evec, int lde,
+                               const double* filling, double* dm, int lddm) {
+  DH_CHECK(evec && filling && dm, "dm_real: null pointer");
+  return dm_common(n, nlevels, evec, lde, filling, dm, lddm);
+}
+
+extern "C" int dftbhip_edm_real(int n, int nlevels, const double* evec, int lde,
+                                const double* filling, const double* eigen, double* edm,
+                                int lddm) {
+  DH_CHECK(evec && filling && eigen && edm, "edm_real: null pointer");
+  std::vector<double> w(std::max(nlevels, 1));
+  for (int i = 0; i < nlevels; ++i) w[i] = filling[i] * eigen[i];
+  return dm_common(n, nlevels, evec, lde, w.data(), edm, lddm);
+}
+
+// ------------------------------------------------------------------------------------------
+// SK parameters and sparse drop-ins
+// ------------------------------------------------------------------------------------------
+extern "C" int dftbhip_sk_create(dftbhip_sk** sk, const dftbhip_params* p) {
+  DH_CHECK(sk && p, "sk_create: null pointer");
+  DH_TRY(ensure_runtime());
+  auto* h = new dftbhip_sk();
+  const int rc = sk_setup(h->t, p, g_rt.stream);
+  if (rc != 0) {
+    delete h;
+    return rc;
+  }
+  *sk = h;
+  return 0;
+}
+
+extern "C" int dftbhip_sk_destroy(dftbhip_sk* sk) {
+  delete sk;
+  return 0;
+}
+
+namespace {
+
+// Flattened blocks from the reference's neighbour arrays (1-based atom/image indices).
+struct HostBlocks {
+  std::vector<int> at1, img2, at2;
+  std::vector<long> start;
+};
+
+int blocks_from_reference(int nAtom, int nAll, const int* img2CentCell, int mNeighbour,
+                          const int* iNeighbour, const int* nNeighbourSK, const int* iPair,
+                          int sparseSize, const std::vector<int>& norbAtom, HostBlocks& hb) {
+  const long ld = (long)mNeighbour + 1;
+  for (int a = 0; a < nAtom; ++a) {
+    DH_CHECK(nNeighbourSK[a] >= 0 && nNeighbourSK[a] <= mNeighbour, "nNeighbourSK out of range");
+    for (int k = 0; k <= nNeighbourSK[a]; ++k) {
+      const int img = iNeighbour[k + a * ld] - 1;
+      DH_CHECK(img >= 0 && img < nAll, "iNeighbour out of range");
+      const int jc = img2CentCell[img] - 1;
+      DH_CHECK(jc >= 0 && jc < nAtom, "img2CentCell out of range");
+      const long st = iPair[k + a * ld];
+      DH_CHECK(st >= 0 && st + (long)norbAtom[a] * norbAtom[jc] <= sparseSize,
+               "iPair out of range");
+      hb.at1.push_back(a);
+      hb.img2.push_back(img);
+      hb.at2.push_back(jc);
+      hb.start.push_back(st);
+    }
+  }
+  return 0;
+}
+
+struct DevBlocks {
+  DevBuf<int> at1, img2, at2, species, norb;
+  DevBuf<long> start, atomStart;
+  PairsDev pr{};
+  int upload(const HostBlocks& hb, const std::vector<int>& species, const std::vector<int>& norbAtom,
+             const std::vector<long>& atomStart, hipStream_t s) {
+    const size_t n = hb.at1.size();
+    DH_TRY(at1.upload(hb.at1.data(), std::max<size_t>(n, 1), s));
+    DH_TRY(img2.upload(hb.img2.data(), std::max<size_t>(n, 1), s));
+    DH_TRY(at2.upload(hb.at2.data(), std::max<size_t>(n, 1), s));
+    DH_TRY(start.upload(hb.start.data(), std::max<size_t>(n, 1), s));
+    if (!species.empty()) DH_TRY(this->species.upload(species.data(), species.size(), s));
+    DH_TRY(norb.upload(norbAtom.data(), norbAtom.size(), s));
+    DH_TRY(this->atomStart.upload(atomStart.data(), atomStart.size(), s));
+    pr.nblk = (int)n;
+    pr.at1 = at1.p;
+    pr.img2 = img2.p;
+    pr.at2 = at2.p;
+    pr.start = start.p;
+    pr.species = this->species.p;
+    return 0;
+  }
+};
+
+int atom_layout(int nAtom, const int* iAtomStart, std::vector<int>& norb, std::vector<long>& off) {
+  norb.resize(nAtom);
+  off.resize(nAtom + 1);
+  for (int a = 0; a <= nAtom; ++a) off[a] = iAtomStart[a] - 1;
+  for (int a = 0; a < nAtom; ++a) {
+    norb[a] = (int)(off[a + 1] - off[a]);
+    DH_CHECK(norb[a] >= 1 && norb[a] <= 4, "iAtomStart: 1..4 orbitals per atom");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int dftbhip_build_h0_s(const dftbhip_sk* sk, int nAtom, int nAll, const double* coords,
+                                  const int* species, const int* img2CentCell, int mNeighbour,
+                                  const int* iNeighbour, const int* nNeighbourSK,
+                                  const int* iPair, int sparseSize, double* h0, double* over) {
+  DH_CHECK(sk && coords && species && img2CentCell && iNeighbour && nNeighbourSK && iPair && h0 &&
+               over,
+           "build_h0_s: null pointer");
+  DH_CHECK(nAtom > 0 && nAll >= nAtom && sparseSize > 0, "build_h0_s: bad sizes");
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  std::vector<int> sp(nAtom), norb(nAtom);
+  std::vector<long> off(nAtom + 1, 0);
+  for (int a = 0; a < nAtom; ++a) {
+    sp[a] = species[a] - 1;
+    DH_CHECK(sp[a] >= 0 && sp[a] < sk->t.nsp, "species out of range");
+    norb[a] = sk->t.norb[sp[a]];
+    off[a + 1] = off[a] + norb[a];
+  }
+  HostBlocks hb;
+  DH_TRY(blocks_from_reference(nAtom, nAll, img2CentCell, mNeighbour, iNeighbour, nNeighbourSK,
+                               iPair, sparseSize, norb, hb));
+  DevBlocks db;
+  DH_TRY(db.upload(hb, sp, norb, off, s));
+  DevBuf<double> c, dh0, ds;
+  DH_TRY(c.upload(coords, 3 * (size_t)nAll, s));
+  DH_TRY(dh0.alloc(sparseSize));
+  DH_TRY(ds.alloc(sparseSize));
+  DH_TRY(fill<double>(s, dh0.p, sparseSize, 0.0));
+  DH_TRY(fill<double>(s, ds.p, sparseSize, 0.0));
+  DH_TRY(launch_build_h0s(s, sk->t.dev, db.pr, c.p, dh0.p, ds.p));
+  DH_HIP(hipMemcpyAsync(h0, dh0.p, sparseSize * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(over, ds.p, sparseSize * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_unpack_hs_real(int nAtom, const int* iAtomStart, const int* img2CentCell,
+                                      int mNeighbour, const int* iNeighbour,
+                                      const int* nNeighbourSK, const int* iPair, int sparseSize,
+                                      const double* orig, double* square, int ldsq) {
+  DH_CHECK(iAtomStart && img2CentCell && iNeighbour && nNeighbourSK && iPair && orig && square,
+           "unpack: null pointer");
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  std::vector<int> norb;
+  std::vector<long> off;
+  DH_TRY(atom_layout(nAtom, iAtomStart, norb, off));
+  const int n = (int)off[nAtom];
+  DH_CHECK(ldsq >= n, "unpack: ldsq too small");
+  int nAll = 0;
+  for (int a = 0; a < nAtom; ++a)
+    for (int k = 0; k <= nNeighbourSK[a]; ++k)
+      nAll = std::max(nAll, iNeighbour[k + a * ((long)mNeighbour + 1)]);
+  HostBlocks hb;
+  DH_TRY(blocks_from_reference(nAtom, nAll, img2CentCell, mNeighbour, iNeighbour, nNeighbourSK,
+                               iPair, sparseSize, norb, hb));
+  DevBlocks db;
+  DH_TRY(db.upload(hb, {}, norb, off, s));
+  DevBuf<double> x, D;
+  DH_TRY(x.upload(orig, sparseSize, s));
+  DH_TRY(D.alloc((size_t)n * n));
+  DH_TRY(launch_unpack<double>(s, db.pr, db.norb.p, db.atomStart.p, x.p, D.p, n, n));
+  DH_HIP(hipMemcpy2DAsync(square, ldsq * sizeof(double), D.p, n * sizeof(double),
+                          n * sizeof(double), n, hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_pack_hs_real(int nAtom, const int* iAtomStart, const int* img2CentCell,
+                                    int mNeighbour, const int* iNeighbour,
+                                    const int* nNeighbourSK, const int* iPair, int sparseSize,
+                                    const double* square, int ldsq, double* primitive) {
+  DH_CHECK(iAtomStart && img2CentCell && iNeighbour && nNeighbourSK && iPair && square &&
+               primitive,
+           "pack: null pointer");
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  std::vector<int> norb;
+  std::vector<long> off;
+  DH_TRY(atom_layout(nAtom, iAtomStart, norb, off));
+  const int n = (int)off[nAtom];
+  DH_CHECK(ldsq >= n, "pack: ldsq too small");
+  int nAll = 0;
+  for (int a = 0; a < nAtom; ++a)
+    for (int k = 0; k <= nNeighbourSK[a]; ++k)
+      nAll = std::max(nAll, iNeighbour[k + a * ((long)mNeighbour + 1)]);
+  HostBlocks hb;
+  DH_TRY(blocks_from_reference(nAtom, nAll, img2CentCell, mNeighbour, iNeighbour, nNeighbourSK,
+                               iPair, sparseSize, norb, hb));
+  DevBlocks db;
+  DH_TRY(db.upload(hb, {}, norb, off, s));
+  DevBuf<double> x, D;
+  DH_TRY(D.alloc((size_t)n * n));
+  DH_HIP(hipMemcpy2DAsync(D.p, n * sizeof(double), square, ldsq * sizeof(double),
+                          n * sizeof(double), n, hipMemcpyHostToDevice, s));
+  DH_TRY(x.alloc(sparseSize));
+  DH_TRY(fill<double>(s, x.p, sparseSize, 0.0));
+  DH_TRY(launch_pack<double>(s, db.pr, db.norb.p, db.atomStart.p, D.p, n, x.p));
+  DH_HIP(hipMemcpyAsync(primitive, x.p, sparseSize * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_mulliken(int nAtom, const int* iAtomStart, const int* img2CentCell,
+                                int mNeighbour, const int* iNeighbour, const int* nNeighbourSK,
+                                const int* iPair, int sparseSize, const double* over,
+                                const double* rho, double* qorb) {
+  DH_CHECK(iAtomStart && img2CentCell && iNeighbour && nNeighbourSK && iPair && over && rho &&
+               qorb,
+           "mulliken: null pointer");
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  std::vector<int> norb;
+  std::vector<long> off;
+  DH_TRY(atom_layout(nAtom, iAtomStart, norb, off));
+  const int n = (int)off[nAtom];
+  int nAll = 0;
+  for (int a = 0; a < nAtom; ++a)
+    for (int k = 0; k <= nNeighbourSK[a]; ++k)
+      nAll = std::max(nAll, iNeighbour[k + a * ((long)mNeighbour + 1)]);
+  HostBlocks hb;
+  DH_TRY(blocks_from_reference(nAtom, nAll, img2CentCell, mNeighbour, iNeighbour, nNeighbourSK,
+                               iPair, sparseSize, norb, hb));
+  DevBlocks db;
+  DH_TRY(db.upload(hb, {}, norb, off, s));
+  DevBuf<double> ds, dr, q;
+  DH_TRY(ds.upload(over, sparseSize, s));
+  DH_TRY(dr.upload(rho, sparseSize, s));
+  DH_TRY(q.alloc(n));
+  DH_TRY(launch_mulliken(s, db.pr, db.norb.p, db.atomStart.p, ds.p, nullptr, dr.p, q.p, nullptr,
+                         n));
+  DH_HIP(hipMemcpyAsync(qorb, q.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Resident SCC context
+// ------------------------------------------------------------------------------------------
+namespace dftbhip {
+
+// Half neighbour list (src/dftbp/dftb/periodic.F90:446-696): atom a lists the images of atoms
+// b >= a within the cutoff, sorted by distance; images are numbered after the central atoms in
+// (translation, atom) order. Candidate images are binned on a grid of cutoff-sized cells.
+struct NeighbourList {
+  std::vector<double> coordsAll;               // 3 * nAll
+  std::vector<int> img2cent;                   // nAll
+  std::vector<std::vector<int>> nb;            // per atom: images (self first)
+  std::vector<std::vector<double>> d2;
+};
+
+static void cell_translations(const double* lat, double cutoff, std::vector<double>& tr) {
+  tr.assign(3, 0.0);
+  if (!lat) return;
+  // reciprocal vectors / 2 pi: rows of inv(lat)^T
+  const double* a = lat;
+  const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+  double inv[9];
+  inv[0] = (a[4] * a[8] - a[5] * a[7]) / det;
+  inv[1] = -(a[1] * a[8] - a[2] * a[7]) / det;
+  inv[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+  inv[3] = -(a[3] * a[8] - a[5] * a[6]) / det;
+  inv[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+  inv[5] = -(a[0] * a[5] - a[2] * a[3]) / det;
+  inv[6] = (a[3] * a[7] - a[4] * a[6]) / det;
+  inv[7] = -(a[0] * a[7] - a[1] * a[6]) / det;
+  inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+  int nmax[3];
+  for (int i = 0; i < 3; ++i) {
+    // row i of inv^T = column i of inv
+    const double nr = std::sqrt(inv[i] * inv[i] + inv[3 + i] * inv[3 + i] + inv[6 + i] * inv[6 + i]);
+    nmax[i] = (int)std::ceil(cutoff * nr) + 1;
+  }
+  for (int x = -nmax[0]; x <= nmax[0]; ++x)
+    for (int y = -nmax[1]; y <= nmax[1]; ++y)
+      for (int z = -nmax[2]; z <= nmax[2]; ++z) {
+        if (x == 0 && y == 0 && z == 0) continue;
+        for (int c = 0; c < 3; ++c) tr.push_back(x * lat[c] + y * lat[3 + c] + z * lat[6 + c]);
+      }
+}
+
+static int build_neighbours(int nat, const double* coords, const double* lat, double cutoff,
+                            NeighbourList& nl) {
+  std::vector<double> tr;
+  cell_translations(lat, cutoff, tr);
+  const int ntr = (int)tr.size() / 3;
+  double lo[3], hi[3];
+  for (int c = 0; c < 3; ++c) {
+    lo[c] = 1e300;
+    hi[c] = -1e300;
+  }
+  for (int a = 0; a < nat; ++a)
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = std::min(lo[c], coords[3 * a + c]);
+      hi[c] = std::max(hi[c], coords[3 * a + c]);
+    }
+  // candidate points (translation, atom) inside the box grown by the cutoff
+  struct Pt {
+    double x[3];
+    int t, a;
+  };
+  std::vector<Pt> pts;
+  for (int t = 0; t < ntr; ++t)
+    for (int a = 0; a < nat; ++a) {
+      Pt p;
+      bool in = true;
+      for (int c = 0; c < 3; ++c) {
+        p.x[c] = coords[3 * a + c] + tr[3 * t + c];
+        if (p.x[c] < lo[c] - cutoff || p.x[c] > hi[c] + cutoff) in = false;
+      }
+      if (!in) continue;
+      p.t = t;
+      p.a = a;
+      pts.push_back(p);
+    }
+  int nbin[3];
+  for (int c = 0; c < 3; ++c)
+    nbin[c] = std::max(1, (int)std::floor((hi[c] - lo[c] + 2 * cutoff) / cutoff));
+  auto binOf = [&](const double* x, int* b) {
+    for (int c = 0; c < 3; ++c) {
+      int v = (int)std::floor((x[c] - (lo[c] - cutoff)) / cutoff);
+      b[c] = std::min(std::max(v, 0), nbin[c] - 1);
+    }
+  };
+  std::vector<std::vector<int>> bins((size_t)nbin[0] * nbin[1] * nbin[2]);
+  for (size_t i = 0; i < pts.size(); ++i) {
+    int b[3];
+    binOf(pts[i].x, b);
+    bins[((size_t)b[0] * nbin[1] + b[1]) * nbin[2] + b[2]].push_back((int)i);
+  }
+  const double cut2 = cutoff * cutoff;
+  std::vector<std::vector<std::pair<double, int>>> found(nat);   // (d2, point index)
+  for (int a2 = 0; a2 < nat; ++a2) {
+    int b[3];
+    binOf(coords + 3 * a2, b);
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int bx = b[0] + dx, by = b[1] + dy, bz = b[2] + dz;
+          if (bx < 0 || by < 0 || bz < 0 || bx >= nbin[0] || by >= nbin[1] || bz >= nbin[2])
+            continue;
+          for (int pi : bins[((size_t)bx * nbin[1] + by) * nbin[2] + bz]) {
+            const Pt& p = pts[pi];
+            if (p.a < a2) continue;
+            double d2 = 0.0;
+            for (int c = 0; c < 3; ++c) d2 += (p.x[c] - coords[3 * a2 + c]) * (p.x[c] - coords[3 * a2 + c]);
+            if (d2 > cut2) continue;
+            if (d2 < 1.0e-4) {
+              if (p.t == 0 && p.a == a2) continue;
+              set_error("Atoms " + std::to_string(p.a + 1) + " and " + std::to_string(a2 + 1) +
+                        " too close together");
+              return kErrArg;
+            }
+            found[a2].push_back({d2, pi});
+          }
+        }
+  }
+  // image numbering: central atoms first, then used images in (translation, atom) order
+  std::vector<int> used(pts.size(), -1);
+  std::vector<int> order;
+  for (int a2 = 0; a2 < nat; ++a2)
+    for (auto& f : found[a2]) {
+      const Pt& p = pts[f.second];
+      if (p.t != 0 && used[f.second] < 0) {
+        used[f.second] = 0;
+        order.push_back(f.second);
+      }
+    }
+  std::sort(order.begin(), order.end(), [&](int x, int y) {
+    return pts[x].t != pts[y].t ? pts[x].t < pts[y].t : pts[x].a < pts[y].a;
+  });
+  nl.coordsAll.assign(coords, coords + 3 * nat);
+  nl.img2cent.resize(nat);
+  std::iota(nl.img2cent.begin(), nl.img2cent.end(), 0);
+  for (int pi : order) {
+    used[pi] = (int)nl.img2cent.size();
+    nl.img2cent.push_back(pts[pi].a);
+    for (int c = 0; c < 3; ++c) nl.coordsAll.push_back(pts[pi].x[c]);
+  }
+  nl.nb.assign(nat, {});
+  nl.d2.assign(nat, {});
+  for (int a2 = 0; a2 < nat; ++a2) {
+    auto& f = found[a2];
+    std::sort(f.begin(), f.end(), [&](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+      if (x.first != y.first) return x.first < y.first;
+      const Pt &px = pts[x.second], &py = pts[y.second];
+      return px.t != py.t ? px.t < py.t : px.a < py.a;
+    });
+    nl.nb[a2].push_back(a2);
+    nl.d2[a2].push_back(0.0);
+    for (auto& x : f) {
+      const Pt& p = pts[x.second];
+      nl.nb[a2].push_back(p.t == 0 ? p.a : used[x.second]);
+      nl.d2[a2].push_back(x.first);
+    }
+  }
+  return 0;
+}
+
+// Ewald parameters (src/dftbp/dftb/coulomb.F90:1824-2037) and reciprocal vectors.
+struct EwaldSetup {
+  double alpha = 0, maxr = 0, maxg = 0, vol = 0;
+  std::vector<double> G, w, tr;
+};
+
+static double gterm(double g, double al, double vol) {
+  return 4.0 * M_PI * (std::exp(-0.25 * g * g / (al * al)) / (vol * g * g));
+}
+static double rterm(double r, double al) { return std::erfc(al * r) / r; }
+
+template <typename F>
+static double bisect_max(F fn, double minv) {
+  double xx = 1.0e-8, yy = fn(xx);
+  while (yy > minv) {
+    xx *= 2.0;
+    yy = fn(xx);
+  }
+  double xl = 0.5 * xx, yl = fn(xl), xr = xx, yr = yy;
+  int it = 1;
+  while (yl - yr > minv && it <= 30) {
+    xx = 0.5 * (xl + xr);
+    yy = fn(xx);
+    if (yy >= minv) {
+      xl = xx;
+      yl = yy;
+    } else {
+      xr = xx;
+      yr = yy;
+    }
+    ++it;
+  }
+  return xx;
+}
+
+static void ewald_setup(const double* lat, double tol, EwaldSetup& ew) {
+  const double* a = lat;
+  const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+  ew.vol = std::fabs(det);
+  // reciprocal vectors rec_i (rows) = 2 pi * (inv(lat)^T)_i
+  double inv[9];
+  inv[0] = (a[4] * a[8] - a[5] * a[7]) / det;
+  inv[1] = -(a[1] * a[8] - a[2] * a[7]) / det;
+  inv[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+  inv[3] = -(a[3] * a[8] - a[5] * a[6]) / det;
+  inv[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+  inv[5] = -(a[0] * a[5] - a[2] * a[3]) / det;
+  inv[6] = (a[3] * a[7] - a[4] * a[6]) / det;
+  inv[7] = -(a[0] * a[7] - a[1] * a[6]) / det;
+  inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+  double rec[9];
+  for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < 3; ++c) rec[3 * i + c] = 2.0 * M_PI * inv[3 * c + i];
+  double ming = 1e300, minr = 1e300;
+  for (int i = 0; i < 3; ++i) {
+    ming = std::min(ming, std::sqrt(rec[3 * i] * rec[3 * i] + rec[3 * i + 1] * rec[3 * i + 1] +
+                                    rec[3 * i + 2] * rec[3 * i + 2]));
+    minr = std::min(minr, std::sqrt(a[3 * i] * a[3 * i] + a[3 * i + 1] * a[3 * i + 1] +
+                                    a[3 * i + 2] * a[3 * i + 2]));
+  }
+  const double vol = ew.vol;
+  auto diff = [&](double al) {
+    return (gterm(4.0 * ming, al, vol) - gterm(5.0 * ming, al, vol)) -
+           (rterm(2.0 * minr, al) - rterm(3.0 * minr, al));
+  };
+  double alpha = 1.0e-8, d = diff(alpha);
+  while (d < -tol) {
+    alpha *= 2.0;
+    d = diff(alpha);
+  }
+  double aleft = 0.5 * alpha;
+  while (d < tol) {
+    alpha *= 2.0;
+    d = diff(alpha);
+  }
+  double aright = alpha;
+  alpha = 0.5 * (aleft + aright);
+  d = diff(alpha);
+  int it = 0;
+  while (std::fabs(d) > tol && it <= 30) {
+    if (d < 0) aleft = alpha;
+    else aright = alpha;
+    alpha = 0.5 * (aleft + aright);
+    d = diff(alpha);
+    ++it;
+  }
+  ew.alpha = alpha;
+  ew.maxr = bisect_max([&](double x) { return rterm(x, alpha); }, tol);
+  ew.maxg = bisect_max([&](double x) { return gterm(x, alpha, vol); }, tol);
+  int nmax[3];
+  for (int i = 0; i < 3; ++i) {
+    const double nr = std::sqrt(rec[3 * i] * rec[3 * i] + rec[3 * i + 1] * rec[3 * i + 1] +
+                                rec[3 * i + 2] * rec[3 * i + 2]);
+    nmax[i] = (int)std::ceil(ew.maxg / nr) + 1;
+  }
+  ew.G.clear();
+  ew.w.clear();
+  for (int x = -nmax[0]; x <= nmax[0]; ++x)
+    for (int y = -nmax[1]; y <= nmax[1]; ++y)
+      for (int z = -nmax[2]; z <= nmax[2]; ++z) {
+        // one of each +-G pair: lexicographically positive (x, y, z)
+        if (x < 0 || (x == 0 && (y < 0 || (y == 0 && z <= 0)))) continue;
+        double g[3];
+        for (int c = 0; c < 3; ++c) g[c] = x * rec[c] + y * rec[3 + c] + z * rec[6 + c];
+        const double g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+        if (g2 > ew.maxg * ew.maxg) continue;
+        ew.G.insert(ew.G.end(), g, g + 3);
+        ew.w.push_back(2.0 * 4.0 * M_PI / vol * std::exp(-g2 / (4.0 * alpha * alpha)) / g2);
+      }
+  cell_translations(lat, ew.maxr, ew.tr);
+}
+
+// Fermi filling of one spin-unpolarised k-point (etemp.F90 Efilling/electronFill/middleGap,
+// main.F90:4544-4651): fillings include the factor 2.
+struct FillResult {
+  double ef, eband, ts, e0;
+};
+
+static double electron_count(double ef, const std::vector<double>& e, double kT) {
+  double s = 0.0, comp = 0.0;
+  for (double x : e) {
+    const double arg = (x - ef) / kT;
+    const double v = 1.0 / (1.0 + std::exp(arg));
+    const double y = v - comp;
+    const double t = s + y;
+    comp = (t - s) - y;
+    s = t;
+  }
+  return s;
+}
+
+static void electron_fill(double ef, const std::vector<double>& e, double kT,
+                          std::vector<double>& f, double& eband, double& ts) {
+  const double eps = 2.220446049250313e-16;
+  const int n = (int)e.size();
+  f.assign(n, 0.0);
+  double eb = 0.0, cb = 0.0, tsum = 0.0, ct = 0.0;
+  auto kahan = [](double& s, double v, double& c) {
+    const double y = v - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  };
+  for (int j = 0; j < n; ++j) {
+    const double x = (e[j] - ef) / kT;
+    f[j] = x > 700.0 ? 0.0 : 1.0 / (1.0 + std::exp(x));
+    if (f[j] <= 1.0e-15) break;
+    if (f[j] > eps && f[j] < 1.0 - eps)
+      kahan(tsum, -(f[j] * std::log(f[j]) + (1.0 - f[j]) * std::log(1.0 - f[j])), ct);
+    kahan(eb, f[j] * e[j], cb);
+  }
+  eband = eb;
+  ts = tsum * kT;
+}
+
+static int fermi_filling(const std::vector<double>& e, double nel, double kT,
+                         std::vector<double>& f, FillResult& r) {
+  kT = std::max(kT, 1.0e-8);
+  const double ne = 0.5 * nel;
+  const int n = (int)e.size();
+  const double eps2 = 2.0 * 2.220446049250313e-16;
+  double ef, eb, ts;
+  auto finish = [&](double scale) {
+    electron_fill(ef, e, kT, f, eb, ts);
+    if (scale != 1.0)
+      for (double& x : f) x *= scale;
+    for (double& x : f) x *= 2.0;
+    r.ef = ef;
+    r.eband = 2.0 * eb;
+    r.ts = 2.0 * ts;
+    r.e0 = 2.0 * (eb - 0.5 * ts);
+  };
+  if (n <= ne) {
+    ef = e.back() + 1000.0 * (kT + 1.1920928955078125e-07);
+    finish(1.0);
+    return 0;
+  }
+  if (std::fabs(ne - std::round(ne)) <= 1.0e-15) {
+    int k = (int)std::ceil(ne - 1e-12);
+    k = std::min(k, n - 1);
+    ef = 0.5 * (e[k] + e[k - 1]);
+    if (std::fabs(ne - electron_count(ef, e, kT)) <= 1.0e-7) {
+      finish(1.0);
+      return 0;
+    }
+  }
+  double upper = e.back() + 0.01, lower = e.front() - 0.01;
+  double nmax = electron_count(upper, e, kT), nmin = electron_count(lower, e, kT);
+  while (nmin > ne) {
+    lower = 2.0 * (lower - upper) + lower;
+    nmin = electron_count(lower, e, kT);
+  }
+  while (nmax < ne) {
+    upper = 2.0 * (upper - lower) + lower;
+    nmax = electron_count(upper, e, kT);
+  }
+  ef = 0.5 * (upper + lower);
+  double nc = electron_count(ef, e, kT);
+  while (std::fabs(ne - nc) > 1.0e-15 &&
+         std::fabs(upper - lower) >= std::max(std::fabs(ef) * eps2, eps2)) {
+    if ((nmax >= nmin) == (ne >= nc)) {
+      lower = ef;
+      nmin = nc;
+    } else {
+      upper = ef;
+      nmax = nc;
+    }
+    ef = 0.5 * (upper + lower);
+    nc = electron_count(ef, e, kT);
+  }
+  if (std::fabs(ne - nc) > 1.0e-7) {
+    set_error("Fermi level search did not converge.");
+    return kErrState;
+  }
+  finish((ne > eps2 && std::fabs(ne - nc) > eps2) ? ne / nc : 1.0);
+  return 0;
+}
+
+// Modified Broyden mixing (src/dftbp/mixer/broydenmixer.F90:204-342).
+struct Broyden {
+  int n = 0, it = 0;
+  double alpha, omega0, minw, maxw, wfac;
+  std::vector<double> ww, qlast, dlast;
+  std::vector<std::vector<double>> aa, dF, uu;
+  void reset(int n_, const dftbhip_options& o, int maxIter) {
+    n = n_;
+    it = 0;
+    alpha = o.mix_param;
+    omega0 = o.omega0;
+    minw = o.min_weight;
+    maxw = o.max_weight;
+    wfac = o.weight_fac;
+    ww.assign(maxIter + 1, 0.0);
+    aa.assign(maxIter + 1, std::vector<double>(maxIter + 1, 0.0));
+    dF.clear();
+    uu.clear();
+    qlast.assign(n, 0.0);
+    dlast.assign(n, 0.0);
+  }
+  static double dot(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+  }
+  void mix(std::vector<double>& q, const std::vector<double>& d) {
+    ++it;
+    const int nn = it;
+    if (nn == 1) {
+      qlast = q;
+      dlast = d;
+      for (int i = 0; i < n; ++i) q[i] += alpha * d[i];
+      return;
+    }
+    const int n1 = nn - 1;
+    double w = std::sqrt(dot(d, d));
+    w = (w > wfac / maxw) ? wfac / w : maxw;
+    w = std::max(w, minw);
+    ww[n1 - 1] = w;
+    std::vector<double> dfu(n);
+    for (int i = 0; i < n; ++i) dfu[i] = d[i] - dlast[i];
+    double inv = std::max(std::sqrt(dot(dfu, dfu)), 2.220446049250313e-16);
+    inv = 1.0 / inv;
+    for (double& x : dfu) x *= inv;
+    std::vector<double> cc(n1);
+    for (int ii = 0; ii < nn - 2; ++ii) {
+      aa[ii][n1 - 1] = dot(dF[ii], dfu);
+      aa[n1 - 1][ii] = aa[ii][n1 - 1];
+      cc[ii] = ww[ii] * dot(dF[ii], d);
+    }
+    aa[n1 - 1][n1 - 1] = 1.0;
+    cc[n1 - 1] = ww[n1 - 1] * dot(dfu, d);
+    // beta(i,j) = w_i w_j a(i,j) + omega0^2 delta; solve beta^T x = cc (LU with pivoting)
+    std::vector<std::vector<double>> B(n1, std::vector<double>(n1));
+    for (int i = 0; i < n1; ++i)
+      for (int j = 0; j < n1; ++j)
+        B[i][j] = ww[i] * ww[j] * aa[i][j] + (i == j ? omega0 * omega0 : 0.0);
+    // B is symmetric: beta^T = beta
+    std::vector<int> piv(n1);
+    for (int k = 0; k < n1; ++k) {
+      int p = k;
+      for (int i = k + 1; i < n1; ++i)
+        if (std::fabs(B[i][k]) > std::fabs(B[p][k])) p = i;
+      std::swap(B[k], B[p]);
+      std::swap(cc[k], cc[p]);
+      for (int i = k + 1; i < n1; ++i) {
+        const double f = B[i][k] / B[k][k];
+        for (int j = k; j < n1; ++j) B[i][j] -= f * B[k][j];
+        cc[i] -= f * cc[k];
+      }
+    }
+    for (int k = n1 - 1; k >= 0; --k) {
+      double s = cc[k];
+      for (int j = k + 1; j < n1; ++j) s -= B[k][j] * cc[j];
+      cc[k] = s / B[k][k];
+    }
+    dF.push_back(dfu);
+    std::vector<double> u(n);
+    for (int i = 0; i < n; ++i) u[i] = alpha * dfu[i] + inv * (q[i] - qlast[i]);
+    qlast = q;
+    dlast = d;
+    for (int i = 0; i < n; ++i) q[i] += alpha * d[i];
+    for (int ii = 0; ii < nn - 2; ++ii)
+      for (int i = 0; i < n; ++i) q[i] -= ww[ii] * cc[ii] * uu[ii][i];
+    for (int i = 0; i < n; ++i) q[i] -= ww[n1 - 1] * cc[n1 - 1] * u[i];
+    uu.push_back(u);
+  }
+};
+
+enum Timer {
+  kTSetup = 0, kTHam, kTReduce, kTTridiag, kTDC, kTBack, kTDM, kTPackMull, kTHost, kTForces,
+  kTIter, kTCount
+};
+
+struct Context {
+  dftbhip_options opt{};
+  SkTables sk;
+  hipStream_t s = nullptr;
+  int nat = 0, norbTot = 0;
+  bool periodic = false;
+  double lat[9]{};
+  std::vector<double> coords;
+  std::vector<int> species;
+  NeighbourList nl;
+  std::vector<int> norbAtom;
+  std::vector<long> atomStart;
+  HostBlocks blk, nbp;
+  DevBlocks dblk, dnb;
+  NeighDev nbdev{};
+  long nnz = 0;
+  EwaldSetup ew;
+  DevBuf<double> coordsAll, coordsC, h0, ov, hs, rho, erho, gamma, vorb, vatom, dq, qorb, eorb;
+  DevBuf<double> Hd, Sd, X, P, f, feps, grad, eatom, ewG, ewW, ewTr, ewCS, ewCSq, ww2;
+  EigenSolver eig;
+  std::vector<double> q0atom, eigv, fill, qout, dqout;
+  double energies[8]{};
+  int nocc = 0;
+  bool sccDone = false;
+  double times[kTCount]{};
+  int ncalls = 0;
+};
+
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int ctx_geometry(Context& c, int nat, const double* coords, const int* species,
+                        const double* latvecs) {
+  hipStream_t s = c.s;
+  const double t0 = now();
+  c.nat = nat;
+  c.coords.assign(coords, coords + 3 * nat);
+  c.species.assign(species, species + nat);
+  c.periodic = latvecs != nullptr;
+  if (latvecs) std::copy(latvecs, latvecs + 9, c.lat);
+  const int ns = c.sk.nsp;
+  c.norbAtom.resize(nat);
+  c.atomStart.assign(nat + 1, 0);
+  c.q0atom.resize(nat);
+  for (int a = 0; a < nat; ++a) {
+    DH_CHECK(species[a] >= 0 && species[a] < ns, "set_geometry: species out of range");
+    c.norbAtom[a] = c.sk.norb[species[a]];
+    c.atomStart[a + 1] = c.atomStart[a] + c.norbAtom[a];
+    c.q0atom[a] = c.sk.q0[species[a]];
+  }
+  c.norbTot = (int)c.atomStart[nat];
+  double cutoff = c.sk.skcut;
+  for (double r : c.sk.repcut) cutoff = std::max(cutoff, r);
+  for (double g : c.sk.gcut) cutoff = std::max(cutoff, g);
+  DH_TRY(build_neighbours(nat, coords, c.periodic ? c.lat : nullptr, cutoff, c.nl));
+  // SK blocks and all neighbour pairs
+  c.blk = HostBlocks();
+  c.nbp = HostBlocks();
+  long st = 0;
+  const double sk2 = c.sk.skcut * c.sk.skcut;
+  for (int a = 0; a < nat; ++a) {
+    const auto& nb = c.nl.nb[a];
+    const auto& d2 = c.nl.d2[a];
+    for (size_t k = 0; k < nb.size(); ++k) {
+      const int img = nb[k], jc = c.nl.img2cent[img];
+      c.nbp.at1.push_back(a);
+      c.nbp.img2.push_back(img);
+      c.nbp.at2.push_back(jc);
+      c.nbp.start.push_back(0);
+      if (d2[k] <= sk2) {
+        c.blk.at1.push_back(a);
+        c.blk.img2.push_back(img);
+        c.blk.at2.push_back(jc);
+        c.blk.start.push_back(st);
+        st += (long)c.norbAtom[a] * c.norbAtom[jc];
+      }
+    }
+  }
+  c.nnz = st;
+  DH_TRY(c.dblk.upload(c.blk, c.species, c.norbAtom, c.atomStart, s));
+  DH_TRY(c.dnb.upload(c.nbp, c.species, c.norbAtom, c.atomStart, s));
+  c.nbdev.n = c.dnb.pr.nblk;
+  c.nbdev.at1 = c.dnb.pr.at1;
+  c.nbdev.img2 = c.dnb.pr.img2;
+  c.nbdev.at2 = c.dnb.pr.at2;
+  c.nbdev.species = c.dnb.pr.species;
+  DH_TRY(c.coordsAll.upload(c.nl.coordsAll.data(), c.nl.coordsAll.size(), s));
+  DH_TRY(c.coordsC.upload(coords, 3 * (size_t)nat, s));
+  const int n = c.norbTot;
+  DH_TRY(c.h0.alloc(c.nnz));
+  DH_TRY(c.ov.alloc(c.nnz));
+  DH_TRY(c.hs.alloc(c.nnz));
+  DH_TRY(c.rho.alloc(c.nnz));
+  DH_TRY(c.erho.alloc(c.nnz));
+  DH_TRY(c.vorb.alloc(n));
+  DH_TRY(c.vatom.alloc(nat));
+  DH_TRY(c.dq.alloc(nat));
+  DH_TRY(c.qorb.alloc(n));
+  DH_TRY(c.eorb.alloc(n));
+  DH_TRY(c.f.alloc(n));
+  DH_TRY(c.feps.alloc(n));
+  DH_TRY(c.grad.alloc(3 * (size_t)nat));
+  DH_TRY(c.eatom.alloc(nat));
+  DH_TRY(c.Hd.alloc((size_t)n * n));
+  DH_TRY(c.Sd.alloc((size_t)n * n));
+  DH_TRY(c.X.alloc((size_t)n * n));
+  DH_TRY(c.P.alloc((size_t)n * n));
+  DH_TRY(c.gamma.alloc((size_t)nat * nat));
+  // H0, S
+  DH_TRY(launch_build_h0s(s, c.sk.dev, c.dblk.pr, c.coordsAll.p, c.h0.p, c.ov.p));
+  // gamma matrix
+  DH_TRY(fill<double>(s, c.gamma.p, (size_t)nat * nat, 0.0));
+  if (!c.periodic) {
+    DH_TRY(launch_invr_cluster(s, nat, c.coordsC.p, c.gamma.p));
+  } else {
+    ewald_setup(c.lat, c.opt.ewald_tol, c.ew);
+    const int ng = (int)c.ew.w.size();
+    const int ntr = (int)c.ew.tr.size() / 3;
+    DH_TRY(c.ewTr.upload(c.ew.tr.data(), c.ew.tr.size(), s));
+    DH_TRY(launch_ewald_real(s, nat, c.coordsC.p, ntr, c.ewTr.p, c.ew.alpha, c.ew.maxr,
+                             -M_PI / (c.ew.vol * c.ew.alpha * c.ew.alpha),
+                             -2.0 * c.ew.alpha / std::sqrt(M_PI), c.gamma.p));
+    if (ng > 0) {
+      DH_TRY(c.ewG.upload(c.ew.G.data(), c.ew.G.size(), s));
+      DH_TRY(c.ewW.upload(c.ew.w.data(), ng, s));
+      std::vector<double> w2(2 * ng);
+      for (int q = 0; q < ng; ++q) w2[q] = w2[q + ng] = c.ew.w[q];
+      DH_TRY(c.ww2.upload(w2.data(), 2 * ng, s));
+      DH_TRY(c.ewCS.alloc((size_t)nat * 2 * ng));
+      DH_TRY(c.ewCSq.alloc(2 * (size_t)ng));
+      DH_TRY(launch_strucfac(s, nat, c.coordsC.p, ng, c.ewG.p, c.ewCS.p));
+      DH_TRY(gemm<double>(s, false, true, nat, nat, 2 * ng, 1.0, c.ewCS.p, nat, c.ewCS.p, nat, 1.0,
+                          c.gamma.p, nat, false, kKFull, c.ww2.p));
+      DH_HIP(hipStreamSynchronize(s));
+    }
+  }
+  DH_TRY(launch_short_gamma(s, c.nbdev, c.coordsAll.p, c.sk.dhubbu.p, c.sk.dgcut.p, ns, nat,
+                            c.gamma.p));
+  // overlap factorisation (S fixed during the SCC cycle)
+  DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.Sd.p, n,
+                               n));
+  DH_TRY(c.eig.ensure(n));
+  c.eig.factored = false;
+  int info = 0;
+  DH_TRY(c.eig.factor_overlap(s, c.Sd.p, n, &info));
+  if (info != 0) {
+    set_error("set_geometry: overlap matrix not positive definite (minor " +
+              std::to_string(info) + ")");
+    return n + info;
+  }
+  DH_HIP(hipStreamSynchronize(s));
+  c.sccDone = false;
+  c.times[kTSetup] = now() - t0;
+  return 0;
+}
+
+static int gamma_times(Context& c, const std::vector<double>& dqa, std::vector<double>* vout) {
+  hipStream_t s = c.s;
+  DH_HIP(hipMemcpyAsync(c.dq.p, dqa.data(), c.nat * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(gemv_t_kernel, dim3(cdiv(c.nat, 4)), dim3(256), 0, s, c.nat, c.gamma.p, c.dq.p,
+                     c.vatom.p);
+  DH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(expand_kernel, dim3(cdiv(c.nat, 256)), dim3(256), 0, s, c.nat,
+                     c.dblk.atomStart.p, c.vatom.p, c.vorb.p);
+  DH_LAUNCH_CHECK();
+  if (vout) {
+    vout->resize(c.nat);
+    DH_HIP(hipMemcpyAsync(vout->data(), c.vatom.p, c.nat * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipStreamSynchronize(s));
+  }
+  return 0;
+}
+
+static int ctx_scc(Context& c, double* energies, double* charges, int* niter, int* converged) {
+  hipStream_t s = c.s;
+  const int nat = c.nat, n = c.norbTot;
+  const dftbhip_options& o = c.opt;
+  for (int t = 1; t < kTCount; ++t) c.times[t] = 0.0;
+  double nel = 0.0;
+  for (double q : c.q0atom) nel += q;
+  std::vector<double> qin = c.q0atom, dqa(nat), qorbH(n), eorbH(n), vout;
+  Broyden mixer;
+  mixer.reset(nat, o, o.max_scc_iter + 1);
+  double eNonScc = 0.0, eScc = 0.0;
+  FillResult fr{};
+  bool conv = false;
+  int it = 0;
+  double ev_t[3] = {0, 0, 0};
+  hipEvent_t e0, e1, e2, e3;
+  DH_HIP(hipEventCreate(&e0));
+  DH_HIP(hipEventCreate(&e1));
+  DH_HIP(hipEventCreate(&e2));
+  DH_HIP(hipEventCreate(&e3));
+  for (it = 1; it <= o.max_scc_iter; ++it) {
+    const double tIt = now();
+    for (int a = 0; a < nat; ++a) dqa[a] = qin[a] - c.q0atom[a];
+    DH_HIP(hipEventRecord(e0, s));
+    DH_TRY(gamma_times(c, dqa, nullptr));
+    DH_TRY(launch_shift(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.h0.p, c.ov.p, c.vorb.p,
+                        c.hs.p));
+    DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p,
+                                 n, n));
+    DH_HIP(hipEventRecord(e1, s));
+    int info = 0;
+    DH_TRY(c.eig.solve(s, c.Hd.p, n, c.eigv, &info, ev_t));
+    if (info != 0) return info;
+    const double th = now();
+    DH_TRY(fermi_filling(c.eigv, nel, o.kT, c.fill, fr));
+    int nocc = 0;
+    for (int i = 0; i < n; ++i)
+      if (c.fill[i] > 0.0) nocc = i + 1;
+    c.nocc = nocc;
+    c.times[kTHost] += now() - th;
+    DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    DH_HIP(hipEventRecord(e2, s));
+    DH_TRY(c.eig.vectors(s, nocc, c.X.p, n));
+    DH_HIP(hipEventRecord(e3, s));
+    DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
+                        kKFull, c.f.p));
+    hipEvent_t e4, e5;
+    DH_HIP(hipEventCreate(&e4));
+    DH_HIP(hipEventCreate(&e5));
+    DH_HIP(hipEventRecord(e4, s));
+    DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n, c.rho.p));
+    DH_TRY(launch_mulliken(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.h0.p,
+                           c.rho.p, c.qorb.p, c.eorb.p, n));
+    DH_HIP(hipMemcpyAsync(qorbH.data(), c.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipMemcpyAsync(eorbH.data(), c.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipEventRecord(e5, s));
+    DH_HIP(hipStreamSynchronize(s));
+    float ms;
+    DH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    c.times[kTHam] += ms * 1e-3;
+    DH_HIP(hipEventElapsedTime(&ms, e2, e3));
+    c.times[kTBack] += ms * 1e-3;
+    DH_HIP(hipEventElapsedTime(&ms, e3, e4));
+    c.times[kTDM] += ms * 1e-3;
+    DH_HIP(hipEventElapsedTime(&ms, e4, e5));
+    c.times[kTPackMull] += ms * 1e-3;
+    (void)hipEventDestroy(e4);
+    (void)hipEventDestroy(e5);
+    const double th2 = now();
+    std::vector<double> qout(nat, 0.0);
+    eNonScc = 0.0;
+    for (int a = 0; a < nat; ++a)
+      for (long oi = c.atomStart[a]; oi < c.atomStart[a + 1]; ++oi) qout[a] += qorbH[oi];
+    for (int i = 0; i < n; ++i) eNonScc += eorbH[i];
+    std::vector<double> dqo(nat), diff(nat);
+    double err = 0.0;
+    for (int a = 0; a < nat; ++a) {
+      dqo[a] = qout[a] - c.q0atom[a];
+      diff[a] = qout[a] - qin[a];
+      err = std::max(err, std::fabs(diff[a]));
+    }
+    c.times[kTHost] += now() - th2;
+    DH_TRY(gamma_times(c, dqo, &vout));
+    eScc = 0.0;
+    for (int a = 0; a < nat; ++a) eScc += 0.5 * vout[a] * dqo[a];
+    c.qout = qout;
+    c.dqout = dqo;
+    conv = err < o.scc_tol && it >= o.min_scc_iter;
+    const double th3 = now();
+    if (!conv && it != o.max_scc_iter) mixer.mix(qin, diff);
+    c.times[kTHost] += now() - th3;
+    c.times[kTIter] = now() - tIt;
+    if (conv) break;
+  }
+  if (it > o.max_scc_iter) it = o.max_scc_iter;
+  c.times[kTReduce] = ev_t[0];
+  c.times[kTTridiag] = ev_t[1];
+  c.times[kTDC] = ev_t[2];
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  (void)hipEventDestroy(e3);
+  // repulsive energy
+  DH_TRY(fill<double>(s, c.eatom.p, nat, 0.0));
+  DH_TRY(fill<double>(s, c.grad.p, 3 * (size_t)nat, 0.0));
+  DH_TRY(launch_repulsive(s, c.nbdev, c.coordsAll.p, c.sk.dreps.p, c.sk.nsp, c.eatom.p, c.grad.p));
+  std::vector<double> ea(nat);
+  DH_HIP(hipMemcpyAsync(ea.data(), c.eatom.p, nat * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  double erep = 0.0;
+  for (double x : ea) erep += x;
+  const double etot = eNonScc + eScc + erep;
+  c.energies[0] = etot - fr.ts;
+  c.energies[1] = etot;
+  c.energies[2] = eNonScc;
+  c.energies[3] = eScc;
+  c.energies[4] = erep;
+  c.energies[5] = fr.ts;
+  c.energies[6] = fr.eband;
+  c.energies[7] = fr.ef;
+  if (energies) std::copy(c.energies, c.energies + 8, energies);
+  if (charges)
+    for (int a = 0; a < nat; ++a) charges[a] = c.q0atom[a] - c.qout[a];
+  if (niter) *niter = it;
+  if (converged) *converged = conv ? 1 : 0;
+  c.sccDone = true;
+  ++c.ncalls;
+  return 0;
+}
+
+static int ctx_gradients(Context& c, double* grad) {
+  DH_CHECK(c.sccDone, "gradients: run the SCC cycle first");
+  hipStream_t s = c.s;
+  const int nat = c.nat, n = c.norbTot;
+  const double t0 = now();
+  // energy-weighted density matrix
+  std::vector<double> fe(n, 0.0);
+  for (int i = 0; i < c.nocc; ++i) fe[i] = c.fill[i] * c.eigv[i];
+  DH_HIP(hipMemcpyAsync(c.feps.p, fe.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  DH_TRY(gemm<double>(s, false, true, n, n, c.nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
+                      kKFull, c.feps.p));
+  DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n, c.erho.p));
+  // output potential per orbital
+  std::vector<double> vout;
+  DH_TRY(gamma_times(c, c.dqout, &vout));
+  // repulsive gradient (energy recomputed, discarded)
+  DH_TRY(fill<double>(s, c.eatom.p, nat, 0.0));
+  DH_TRY(fill<double>(s, c.grad.p, 3 * (size_t)nat, 0.0));
+  DH_TRY(launch_repulsive(s, c.nbdev, c.coordsAll.p, c.sk.dreps.p, c.sk.nsp, c.eatom.p, c.grad.p));
+  DH_TRY(launch_pair_forces(s, c.sk.dev, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p,
+                            c.coordsAll.p, c.rho.p, c.erho.p, c.vorb.p, c.opt.force_mode,
+                            c.opt.fd_delta, c.grad.p));
+  const int ng = (int)c.ew.w.size();
+  if (c.periodic && ng > 0) {
+    // C_G = sum_j dq_j cos(G r_j), S_G = sum_j dq_j sin(G r_j)
+    DH_TRY(gemm<double>(s, true, false, 2 * ng, 1, nat, 1.0, c.ewCS.p, nat, c.dq.p, nat, 0.0,
+                        c.ewCSq.p, 2 * ng));
+  }
+  DH_TRY(launch_gamma_grad(s, c.nbdev, c.coordsAll.p, c.sk.dhubbu.p, c.sk.dgcut.p, c.sk.nsp, nat,
+                           c.periodic, (int)c.ew.tr.size() / 3, c.ewTr.p, c.ew.alpha, c.ew.maxr,
+                           ng, c.ewG.p, c.ewW.p, c.ewCSq.p, c.dq.p, c.grad.p));
+  DH_HIP(hipMemcpyAsync(grad, c.grad.p, 3 * (size_t)nat * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  c.times[kTForces] = now() - t0;
+  return 0;
+}
+
+}  // namespace dftbhip
+
+struct dftbhip_ctx {
+  dftbhip::Context c;
+};
+
+extern "C" int dftbhip_ctx_create(dftbhip_ctx** ctx, const dftbhip_params* p,
+                                  const dftbhip_options* opt) {
+  DH_CHECK(ctx && p && opt, "ctx_create: null pointer");
+  DH_CHECK(opt->precision == 0, "ctx_create: only precision = 0 (FP64) is available");
+  DH_CHECK(opt->max_scc_iter >= 1, "ctx_create: max_scc_iter < 1");
+  DH_TRY(ensure_runtime());
+  auto* h = new dftbhip_ctx();
+  h->c.opt = *opt;
+  h->c.s = g_rt.stream;
+  const int rc = sk_setup(h->c.sk, p, h->c.s);
+  if (rc != 0) {
+    delete h;
+    return rc;
+  }
+  *ctx = h;
+  return 0;
+}
+
+extern "C" int dftbhip_ctx_destroy(dftbhip_ctx* ctx) {
+  delete ctx;
+  return 0;
+}
+
+extern "C" int dftbhip_ctx_set_geometry(dftbhip_ctx* ctx, int nAtom, const double* coords,
+                                        const int* species, const double* latvecs) {
+  DH_CHECK(ctx && coords && species && nAtom > 0, "set_geometry: bad arguments");
+  return ctx_geometry(ctx->c, nAtom, coords, species, latvecs);
+}
+
+extern "C" int dftbhip_ctx_scc(dftbhip_ctx* ctx, double* energies, double* charges, int* niter,
+                               int* converged) {
+  DH_CHECK(ctx && ctx->c.nat > 0, "scc: no geometry");
+  return ctx_scc(ctx->c, energies, charges, niter, converged);
+}
+
+extern "C" int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad) {
+  DH_CHECK(ctx && grad, "gradients: null pointer");
+  return ctx_gradients(ctx->c, grad);
+}
+
+extern "C" int dftbhip_ctx_eigenvalues(dftbhip_ctx* ctx, double* eig, double* fill, int n) {
+  DH_CHECK(ctx && ctx->c.sccDone, "eigenvalues: run the SCC cycle first");
+  DH_CHECK(n == ctx->c.norbTot, "eigenvalues: size mismatch");
+  if (eig) std::copy(ctx->c.eigv.begin(), ctx->c.eigv.end(), eig);
+  if (fill) std::copy(ctx->c.fill.begin(), ctx->c.fill.end(), fill);
+  return 0;
+}
+
+extern "C" int dftbhip_ctx_nr_orbitals(dftbhip_ctx* ctx) { return ctx ? ctx->c.norbTot : -1; }
+
+extern "C" int dftbhip_ctx_timings(dftbhip_ctx* ctx, double* t, int n) {
+  DH_CHECK(ctx && t, "timings: null pointer");
+  for (int i = 0; i < n && i < kTCount; ++i) t[i] = ctx->c.times[i];
+  return ctx->c.ncalls;
+}
+
+// ------------------------------------------------------------------------------------------
+// BLAS/LAPACK-style drop-ins on host arrays
+// ------------------------------------------------------------------------------------------
+extern "C" int dftbhip_dgemm(char transa, char transb, int m, int n, int k, double alpha,
+                             const double* a, int lda, const double* b, int ldb, double beta,
+                             double* c, int ldc) {
+  const bool ta = (transa == 'T' || transa == 't'), tb = (transb == 'T' || transb == 't');
+  DH_CHECK(m >= 0 && n >= 0 && k >= 0, "dgemm: negative size");
+  DH_CHECK(a && b && c, "dgemm: null pointer");
+  const int ra = ta ? k : m, ca = ta ? m : k, rb = tb ? n : k, cb = tb ? k : n;
+  DH_CHECK(lda >= std::max(1, ra) && ldb >= std::max(1, rb) && ldc >= std::max(1, m),
+           "dgemm: bad leading dimension");
+  if (m == 0 || n == 0) return 0;
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  DevBuf<double> A, B, Cm;
+  DH_TRY(A.alloc((size_t)std::max(ra, 1) * std::max(ca, 1)));
+  DH_TRY(B.alloc((size_t)std::max(rb, 1) * std::max(cb, 1)));
+  DH_TRY(Cm.alloc((size_t)m * n));
+  if (ra > 0 && ca > 0)
+    DH_HIP(hipMemcpy2DAsync(A.p, ra * sizeof(double), a, lda * sizeof(double), ra * sizeof(double),
+                            ca, hipMemcpyHostToDevice, s));
+  if (rb > 0 && cb > 0)
+    DH_HIP(hipMemcpy2DAsync(B.p, rb * sizeof(double), b, ldb * sizeof(double), rb * sizeof(double),
+                            cb, hipMemcpyHostToDevice, s));
+  DH_HIP(hipMemcpy2DAsync(Cm.p, m * sizeof(double), c, ldc * sizeof(double), m * sizeof(double), n,
+                          hipMemcpyHostToDevice, s));
+  DH_TRY(gemm<double>(s, ta, tb, m, n, k, alpha, A.p, std::max(ra, 1), B.p, std::max(rb, 1), beta,
+                      Cm.p, m));
+  DH_HIP(hipMemcpy2DAsync(c, ldc * sizeof(double), Cm.p, m * sizeof(double), m * sizeof(double), n,
+                          hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz) {
+  DH_CHECK(n >= 0 && d && (n <= 1 || e) && z && ldz >= std::max(1, n), "dstedc: bad arguments");
+  if (n == 0) return 0;
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  DevBuf<double> w, Z, work;
+  DH_TRY(w.alloc(n));
+  DH_TRY(Z.alloc((size_t)n * n));
+  const size_t ws = stedc_work_size(n);
+  DH_TRY(work.alloc(ws));
+  int info = 0;
+  DH_TRY(stedc(s, n, d, e, w.p, Z.p, n, work.p, ws, &info));
+  if (info != 0) return info;
+  DH_HIP(hipMemcpyAsync(d, w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpy2DAsync(z, ldz * sizeof(double), Z.p, n * sizeof(double), n * sizeof(double), n,
+                          hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}

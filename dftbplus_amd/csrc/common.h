@@ -1,0 +1,158 @@
+// Shared declarations of the dftbhip library: error state, device buffers, launch helpers and
+// the internal (C++) interfaces between the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace dftbhip {
+
+// ------------------------------------------------------------------------------------------
+// Error handling: every entry point returns 0 on success, a negative code on argument/device
+// errors and a LAPACK-style positive info for solver failures. The message of the last error is
+// kept per thread and returned by dftbhip_last_error().
+// ------------------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+const std::string& last_error();
+
+constexpr int kErrArg = -1;
+constexpr int kErrDevice = -2;
+constexpr int kErrState = -3;
+
+#define DH_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t err_ = (call);                                                               \
+    if (err_ != hipSuccess) {                                                               \
+      (void)hipGetLastError();                                                              \
+      ::dftbhip::set_error(std::string(#call) + ": " + hipGetErrorString(err_) + " at " +    \
+                           __FILE__ + ":" + std::to_string(__LINE__));                     \
+      return ::dftbhip::kErrDevice;                                                         \
+    }                                                                                       \
+  } while (0)
+
+#define DH_CHECK(cond, msg)                                                                 \
+  do {                                                                                      \
+    if (!(cond)) {                                                                          \
+      ::dftbhip::set_error(msg);                                                            \
+      return ::dftbhip::kErrArg;                                                            \
+    }                                                                                       \
+  } while (0)
+
+#define DH_TRY(expr)                                                                        \
+  do {                                                                                      \
+    int rc_ = (expr);                                                                       \
+    if (rc_ != 0) return rc_;                                                               \
+  } while (0)
+
+#define DH_LAUNCH_CHECK()                                                                   \
+  do {                                                                                      \
+    hipError_t err_ = hipGetLastError();                                                    \
+    if (err_ != hipSuccess) {                                                               \
+      ::dftbhip::set_error(std::string("kernel launch: ") + hipGetErrorString(err_) + " at " + \
+                           __FILE__ + ":" + std::to_string(__LINE__));                     \
+      return ::dftbhip::kErrDevice;                                                         \
+    }                                                                                       \
+  } while (0)
+
+// Owning device allocation (hipMalloc / hipFree).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    if (count <= n && p) return 0;
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes: " +
+                hipGetErrorString(e));
+      return kErrDevice;
+    }
+    n = count;
+    return 0;
+  }
+  int upload(const T* h, size_t count, hipStream_t s) {
+    DH_TRY(alloc(count));
+    DH_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return 0;
+  }
+};
+
+inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------------------------------
+// GEMM core (gemm.hip):  C = alpha * op(A) * diag(w) * op(B) + beta * C, column-major.
+// ------------------------------------------------------------------------------------------
+enum GemmKLimit {
+  kKFull = 0,
+  kKALower = 1,   // op(A)(i,k) == 0 for k > i  -> k < row tile end
+  kKAUpper = 2,   // op(A)(i,k) == 0 for k < i  -> k >= row tile start
+  kKBUpper = 3,   // op(B)(k,j) == 0 for k > j  -> k < column tile end
+  kKBLower = 4,   // op(B)(k,j) == 0 for k < j  -> k >= column tile start
+};
+
+template <typename T>
+struct GemmDesc {
+  int M, N, K;
+  const T* A;
+  long lda;
+  const T* B;
+  long ldb;
+  T* C;
+  long ldc;
+};
+
+template <typename T>
+int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T* A, long lda,
+         const T* B, long ldb, T beta, T* C, long ldc, bool lowerOut = false,
+         GemmKLimit klim = kKFull, const T* w = nullptr);
+
+// Batched GEMM over device-resident descriptors; maxM/maxN bound the tile grid.
+template <typename T>
+int gemm_batched(hipStream_t s, bool ta, bool tb, const GemmDesc<T>* dDesc, int nBatch, int maxM,
+                 int maxN, T alpha, T beta);
+
+// ------------------------------------------------------------------------------------------
+// Dense helpers (dense.hip)
+// ------------------------------------------------------------------------------------------
+template <typename T>
+int fill(hipStream_t s, T* p, size_t n, T v);
+template <typename T>
+int fill2d(hipStream_t s, T* p, long ld, int m, int n, T v);
+template <typename T>
+int copy2d(hipStream_t s, const T* src, long lds, T* dst, long ldd, int m, int n);
+template <typename Tin, typename Tout>
+int convert2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n);
+int symmetrize_lower(hipStream_t s, double* a, long lda, int n);
+int zero_strict_upper(hipStream_t s, double* a, long lda, int n);
+
+
+// ------------------------------------------------------------------------------------------
+// Symmetric eigensolver pieces (sytrd.hip, stedc.hip)
+// ------------------------------------------------------------------------------------------
+int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e, double* tau,
+                double* work, size_t workSize);
+size_t sytrd_work_size(int n);
+int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* tau, double* z,
+                long ldz, int ncols, double* work, size_t workSize);
+size_t ormtr_work_size(int n, int ncols);
+int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double* w, double* z,
+          long ldz, double* work, size_t workSize, int* info);
+size_t stedc_work_size(int n);
+
+}  // namespace dftbhip

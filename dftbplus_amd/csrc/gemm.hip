@@ -1,0 +1,275 @@
+// MFMA GEMM core for CDNA4 (gfx950), FP64 (v_mfma_f64_16x16x4f64) and FP32
+// (v_mfma_f32_16x16x4f32).
+//
+//   C = alpha * op(A) * diag(w) * op(B) + beta * C        (column-major, LAPACK convention)
+//
+// Workgroup tile 128x128, 256 threads = 4 wave64s in a 2x2 arrangement, each wave owns a 64x64
+// sub-tile = 4x4 MFMA blocks of 16x16. K is staged through LDS in steps of 16 with a register
+// prefetch of the next step. Options used by the eigensolver and density-matrix builds:
+//  * lowerOut: only tiles on/below the diagonal are computed (SYRK/SYR2K-like outputs);
+//  * klim: skip the zero part of a triangular operand;
+//  * w: per-k weights folded into op(A) (C = X diag(f) X^T for the density matrices).
+#include "common.h"
+
+namespace dftbhip {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = v4d;
+  static constexpr bool kRowMajorQuad = false;
+  __device__ static inline v4d op(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = v4f;
+  static constexpr bool kRowMajorQuad = true;
+  __device__ static inline v4f op(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kBM = 128, kBN = 128, kBK = 16, kNT = 256, kPad = 1;
+
+template <typename T, bool TA, bool TB>
+__device__ __forceinline__ void gemm_tile(int tm, int tn, int M, int N, int K, T alpha,
+                                          const T* __restrict__ A, long lda,
+                                          const T* __restrict__ B, long ldb, T beta,
+                                          T* __restrict__ C, long ldc, int klim,
+                                          const T* __restrict__ w) {
+  using acc_t = typename Mfma<T>::acc_t;
+  __shared__ T As[kBK][kBM + kPad];
+  __shared__ T Bs[kBK][kBN + kPad];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int i0 = tm * kBM, j0 = tn * kBN;
+  int kbeg = 0, kend = K;
+  if (klim == kKALower) kend = min(K, i0 + kBM);
+  else if (klim == kKAUpper) kbeg = min(K, i0);
+  else if (klim == kKBUpper) kend = min(K, j0 + kBN);
+  else if (klim == kKBLower) kbeg = min(K, j0);
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+  T ra[8], rb[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + q * kNT;
+      int r, c;
+      if (!TA) { r = e % kBM; c = e / kBM; } else { c = e % kBK; r = e / kBK; }
+      const int gi = i0 + r, gk = k0 + c;
+      T v = T(0);
+      if (gi < M && gk < kend) {
+        v = TA ? A[gk + (long)gi * lda] : A[gi + (long)gk * lda];
+        if (w) v *= w[gk];
+      }
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + q * kNT;
+      int r, c;
+      if (!TB) { c = e % kBK; r = e / kBK; } else { r = e % kBN; c = e / kBN; }
+      const int gj = j0 + r, gk = k0 + c;
+      T v = T(0);
+      if (gj < N && gk < kend) v = TB ? B[gj + (long)gk * ldb] : B[gk + (long)gj * ldb];
+      rb[q] = v;
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + q * kNT;
+      int r, c;
+      if (!TA) { r = e % kBM; c = e / kBM; } else { c = e % kBK; r = e / kBK; }
+      As[c][r] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + q * kNT;
+      int r, c;
+      if (!TB) { c = e % kBK; r = e / kBK; } else { r = e % kBN; c = e / kBN; }
+      Bs[c][r] = rb[q];
+    }
+  };
+
+  if (kbeg < kend) {
+    gload(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+      __syncthreads();
+      sstore();
+      __syncthreads();
+      if (k0 + kBK < kend) gload(k0 + kBK);
+#pragma unroll
+      for (int kk = 0; kk < kBK / 4; ++kk) {
+        T fa[4], fb[4];
+        const int kr = kk * 4 + (lane >> 4);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) fa[m] = As[kr][wm * 64 + m * 16 + (lane & 15)];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) fb[n] = Bs[kr][wn * 64 + n * 16 + (lane & 15)];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int n = 0; n < 4; ++n) acc[m][n] = Mfma<T>::op(fa[m], fb[n], acc[m][n]);
+      }
+    }
+  }
+  // Output layout of the 16x16 accumulator, column l%16 for lane l; element r sits in row
+  // l/16 + 4r for FP64 and in row 4(l/16) + r for FP32 (pinned by the layout probes below).
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int gj = j0 + wn * 64 + n * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = Mfma<T>::kRowMajorQuad ? (lane >> 4) * 4 + r : (lane >> 4) + 4 * r;
+        const int gi = i0 + wm * 64 + m * 16 + rr;
+        if (gi < M && gj < N) {
+          T v = alpha * acc[m][n][r];
+          if (beta != T(0)) v += beta * C[gi + (long)gj * ldc];
+          C[gi + (long)gj * ldc] = v;
+        }
+      }
+    }
+}
+
+__device__ __forceinline__ void lower_tile_index(int x, int& tm, int& tn) {
+  int t = (int)((sqrt(8.0 * (double)x + 1.0) - 1.0) * 0.5);
+  while ((t + 1) * (t + 2) / 2 <= x) ++t;
+  while (t * (t + 1) / 2 > x) --t;
+  tm = t;
+  tn = x - t * (t + 1) / 2;
+}
+
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T* A,
+                                                   long lda, const T* B, long ldb, T beta, T* C,
+                                                   long ldc, int lowerOut, int klim, const T* w) {
+  int tm, tn;
+  if (lowerOut) {
+    lower_tile_index(blockIdx.x, tm, tn);
+  } else {
+    tm = blockIdx.x;
+    tn = blockIdx.y;
+  }
+  gemm_tile<T, TA, TB>(tm, tn, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, klim, w);
+}
+
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_batched_kernel(const GemmDesc<T>* desc, T alpha,
+                                                           T beta) {
+  const GemmDesc<T> d = desc[blockIdx.z];
+  const int tm = blockIdx.x, tn = blockIdx.y;
+  if (tm * kBM >= d.M || tn * kBN >= d.N) return;
+  gemm_tile<T, TA, TB>(tm, tn, d.M, d.N, d.K, alpha, d.A, d.lda, d.B, d.ldb, beta, d.C, d.ldc,
+                       kKFull, nullptr);
+}
+
+template <typename T>
+int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T* A, long lda,
+         const T* B, long ldb, T beta, T* C, long ldc, bool lowerOut, GemmKLimit klim,
+         const T* w) {
+  if (M <= 0 || N <= 0) return 0;
+  DH_CHECK(!lowerOut || M == N, "gemm: lower output needs a square C");
+  dim3 grid;
+  if (lowerOut) {
+    const long t = cdiv(M, kBM);
+    grid = dim3((unsigned)(t * (t + 1) / 2), 1, 1);
+  } else {
+    grid = dim3(cdiv(M, kBM), cdiv(N, kBN), 1);
+  }
+#define DH_GEMM_CASE(TA_, TB_)                                                           \
+  hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(kNT), 0, s, M, N, K, alpha, A, \
+                     lda, B, ldb, beta, C, ldc, (int)lowerOut, (int)klim, w)
+  if (!ta && !tb) DH_GEMM_CASE(false, false);
+  else if (!ta && tb) DH_GEMM_CASE(false, true);
+  else if (ta && !tb) DH_GEMM_CASE(true, false);
+  else DH_GEMM_CASE(true, true);
+#undef DH_GEMM_CASE
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+int gemm_batched(hipStream_t s, bool ta, bool tb, const GemmDesc<T>* dDesc, int nBatch, int maxM,
+                 int maxN, T alpha, T beta) {
+  if (nBatch <= 0 || maxM <= 0 || maxN <= 0) return 0;
+  dim3 grid(cdiv(maxM, kBM), cdiv(maxN, kBN), nBatch);
+#define DH_GEMMB_CASE(TA_, TB_)                                                            \
+  hipLaunchKernelGGL((gemm_batched_kernel<T, TA_, TB_>), grid, dim3(kNT), 0, s, dDesc, alpha, \
+                     beta)
+  if (!ta && !tb) DH_GEMMB_CASE(false, false);
+  else if (!ta && tb) DH_GEMMB_CASE(false, true);
+  else if (ta && !tb) DH_GEMMB_CASE(true, false);
+  else DH_GEMMB_CASE(true, true);
+#undef DH_GEMMB_CASE
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+template int gemm<double>(hipStream_t, bool, bool, int, int, int, double, const double*, long,
+                          const double*, long, double, double*, long, bool, GemmKLimit,
+                          const double*);
+template int gemm<float>(hipStream_t, bool, bool, int, int, int, float, const float*, long,
+                         const float*, long, float, float*, long, bool, GemmKLimit,
+                         const float*);
+template int gemm_batched<double>(hipStream_t, bool, bool, const GemmDesc<double>*, int, int, int,
+                                  double, double);
+template int gemm_batched<float>(hipStream_t, bool, bool, const GemmDesc<float>*, int, int, int,
+                                 float, float);
+
+}  // namespace dftbhip
+
+namespace dftbhip {
+// Lane-layout check of the FP64 MFMA accumulator: one wave computes D = A B with
+// A(i,k) = i [k==0] + [k==1], B(k,j) = [k==0] + 100 j [k==1], so D(i,j) = i + 100 j, and each
+// lane stores its four accumulator elements: out[4*lane + r].
+__global__ void mfma_layout_probe_kernel(double* out) {
+  const int lane = threadIdx.x;
+  const int i = lane & 15, k = lane >> 4;
+  const double a = (k == 0) ? (double)i : (k == 1 ? 1.0 : 0.0);
+  const int j = lane & 15;
+  const double b = (k == 0) ? 1.0 : (k == 1 ? 100.0 * j : 0.0);
+  v4d acc = {0, 0, 0, 0};
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[4 * lane + r] = acc[r];
+}
+
+__global__ void mfma_layout_probe_f32_kernel(double* out) {
+  const int lane = threadIdx.x;
+  const int i = lane & 15, k = lane >> 4;
+  const float a = (k == 0) ? (float)i : (k == 1 ? 1.0f : 0.0f);
+  const int j = lane & 15;
+  const float b = (k == 0) ? 1.0f : (k == 1 ? 100.0f * j : 0.0f);
+  v4f acc = {0, 0, 0, 0};
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[256 + 4 * lane + r] = acc[r];
+}
+}  // namespace dftbhip
+
+extern "C" int dftbhip_probe_mfma_f64(double* out) {
+  using namespace dftbhip;
+  DH_CHECK(out, "probe: null pointer");
+  DevBuf<double> d;
+  DH_TRY(d.alloc(512));
+  hipLaunchKernelGGL(mfma_layout_probe_kernel, dim3(1), dim3(64), 0, 0, d.p);
+  DH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(mfma_layout_probe_f32_kernel, dim3(1), dim3(64), 0, 0, d.p);
+  DH_LAUNCH_CHECK();
+  DH_HIP(hipMemcpy(out, d.p, 512 * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}

@@ -1,0 +1,89 @@
+// Device-side descriptions of the Slater-Koster tables, neighbour blocks and repulsives, and the
+// launch wrappers of the sparse kernels (sparse.hip).
+#pragma once
+
+#include "common.h"
+
+namespace dftbhip {
+
+constexpr int kMaxSpecies = 8;
+
+struct SpeciesDev {
+  int nshell;
+  int ang[2];
+  int norb;
+  double onsite[4];   // per orbital
+};
+
+struct SkDev {
+  int nsp;
+  int nrows;
+  double dist;
+  const double* htab;   // concatenated tables, row-major (row, integral)
+  const double* stab;
+  long tabOff[kMaxSpecies * kMaxSpecies];
+  int nint[kMaxSpecies * kMaxSpecies];
+  SpeciesDev sp[kMaxSpecies];
+};
+
+// Sparse blocks (atom, neighbour) within the SK cutoff.
+struct PairsDev {
+  int nblk;
+  const int* at1;       // central atom of the block column
+  const int* img2;      // image index of the neighbour (coordinates index)
+  const int* at2;       // central atom of the neighbour
+  const long* start;    // block offset in the sparse arrays
+  const int* species;   // species of central atoms
+};
+
+// Neighbour pairs (any cutoff) for gamma and repulsive.
+struct NeighDev {
+  int n;
+  const int* at1;
+  const int* img2;
+  const int* at2;
+  const int* species;
+};
+
+struct RepDev {
+  int nint;
+  double cutoff;
+  double expc[3];
+  const double* xs;     // nint starts
+  const double* coef;   // (nint-1) x 4
+  const double* last;   // 6
+};
+
+int launch_build_h0s(hipStream_t s, const SkDev& sk, const PairsDev& pr, const double* coords,
+                     double* h0, double* ov);
+template <typename T>
+int launch_unpack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                  const double* x, T* dense, long ld, int n);
+template <typename T>
+int launch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                const T* dense, long ld, double* x);
+int launch_shift(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                 const double* h0, const double* ov, const double* vorb, double* h);
+int launch_mulliken(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                    const double* ov, const double* h0, const double* rho, double* qorb,
+                    double* eorb, int nOrb);
+int launch_pair_forces(hipStream_t s, const SkDev& sk, const PairsDev& pr, const int* norb,
+                       const long* iAtomStart, const double* coords, const double* rho,
+                       const double* erho, const double* vorb, int mode, double delta,
+                       double* grad);
+int launch_invr_cluster(hipStream_t s, int nat, const double* coords, double* g);
+int launch_ewald_real(hipStream_t s, int nat, const double* coords, int ntr, const double* tr,
+                      double alpha, double maxr, double cst, double self, double* g);
+int launch_strucfac(hipStream_t s, int nat, const double* coords, int ng, const double* G,
+                    double* cs);
+int launch_short_gamma(hipStream_t s, const NeighDev& nb, const double* coords,
+                       const double* hubbu, const double* gcut, int nsp, int nat, double* g);
+int launch_gamma_grad(hipStream_t s, const NeighDev& nb, const double* coords,
+                      const double* hubbu, const double* gcut, int nsp, int nat, bool periodic,
+                      int ntr, const double* tr, double alpha, double maxr, int ng,
+                      const double* G, const double* w, const double* CSq, const double* dq,
+                      double* grad);
+int launch_repulsive(hipStream_t s, const NeighDev& nb, const double* coords, const RepDev* reps,
+                     int nsp, double* eatom, double* grad);
+
+}  // namespace dftbhip

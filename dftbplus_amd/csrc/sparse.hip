@@ -1,0 +1,729 @@
+// Sparse (atom-pair block) kernels of the SCC-DFTB path:
+//  * Slater-Koster H0/S assembly over the neighbour blocks (nonscc.F90 buildH0/buildS):
+//    8-point Neville interpolation on the equidistant grid, 5th-order tail to zero over the
+//    last bohr (slakoeqgrid.F90:183-244), s/p rotation to the bond direction (sk.F90:35-289);
+//  * sparse -> dense unpack (both triangles written, periodic images folded) and the dense ->
+//    sparse pack of the lower triangle (sparse2dense.F90:266-321, 1024-1097);
+//  * H = H0 + 1/2 S (V_mu + V_nu) (shift.F90:132-229), Mulliken charges and the non-SCC energy
+//    (populations.F90:120-193);
+//  * the pair part of the forces 2 sum(rho dH0 - W dS + rho (V_mu+V_nu)/2 dS)
+//    (forces.F90:548-672) with finite-difference (nonscc.F90:428-484) or analytic SK derivatives.
+//
+// Orbital order within an atom: s, p_y, p_z, p_x. Species carry at most two shells with l <= 1,
+// i.e. at most 4 orbitals per atom.
+#include "common.h"
+#include "sparse.h"
+
+namespace dftbhip {
+
+constexpr double kDistFudge = 1.0;
+constexpr double kDeltaR = 1.0e-5;
+
+// Neville interpolation as polyInterU2 (interpolation.F90:182-239) on 8 uniform points
+// xa_i = (i0 + i) * dx, values y_i = tab[(i0 - 1 + i) * nint + col].
+__device__ double neville8(const double* tab, int nint, int col, int i0, double dx, double xx,
+                           double* dydx) {
+  double xa[8], cc[8], dd[8], cp[8], dp[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    xa[i] = (double)(i0 + i) * dx;
+    cc[i] = dd[i] = tab[(long)(i0 - 1 + i) * nint + col];
+    cp[i] = dp[i] = 0.0;
+  }
+  double delta[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) delta[i] = 1.0 / (xa[i + 1] - xa[0]);
+  int icl = (int)ceil((xx - xa[0]) * delta[0]);
+  icl = min(max(icl, 1), 8);
+  double yy = cc[icl - 1];
+  double yp = 0.0;
+  icl -= 1;
+  for (int mm = 1; mm < 8; ++mm) {
+    for (int ii = 0; ii < 8 - mm; ++ii) {
+      const double r2 = (dd[ii] - cc[ii + 1]) * delta[mm - 1];
+      const double r2p = (dp[ii] - cp[ii + 1]) * delta[mm - 1];
+      // derivative of c = (x_i - x) r2 and d = (x_{i+m} - x) r2
+      cp[ii] = -r2 + (xa[ii] - xx) * r2p;
+      dp[ii] = -r2 + (xa[ii + mm] - xx) * r2p;
+      cc[ii] = (xa[ii] - xx) * r2;
+      dd[ii] = (xa[ii + mm] - xx) * r2;
+    }
+    if (2 * icl < 8 - mm) {
+      yy += cc[icl];
+      yp += cp[icl];
+    } else {
+      yy += dd[icl - 1];
+      yp += dp[icl - 1];
+      icl -= 1;
+    }
+  }
+  if (dydx) *dydx = yp;
+  return yy;
+}
+
+// SK integrals (and optionally their radial derivatives) at distance r.
+__device__ void sk_integrals(const double* tab, int nrows, int nint, double dx, double r,
+                             double* out, double* dout) {
+  const double rmax = nrows * dx + kDistFudge;
+  const int ind = (int)floor(r / dx);
+  for (int c = 0; c < nint; ++c) {
+    out[c] = 0.0;
+    if (dout) dout[c] = 0.0;
+  }
+  if (r >= rmax) return;
+  if (ind < nrows) {
+    int ilast = min(nrows, ind + 4);
+    ilast = max(ilast, 8);
+    for (int c = 0; c < nint; ++c) {
+      double d1;
+      out[c] = neville8(tab, nint, c, ilast - 7, dx, r, dout ? &d1 : nullptr);
+      if (dout) dout[c] = d1;
+    }
+  } else {
+    const int i0 = nrows - 7;
+    const double x8 = nrows * dx;
+    const double dr = r - rmax;
+    for (int c = 0; c < nint; ++c) {
+      const double y0 = neville8(tab, nint, c, i0, dx, x8 - kDeltaR, nullptr);
+      const double y2 = neville8(tab, nint, c, i0, dx, x8 + kDeltaR, nullptr);
+      const double y1 = tab[(long)(nrows - 1) * nint + c];
+      const double y1p = (y2 - y0) / (2.0 * kDeltaR);
+      const double y1pp = (y2 + y0 - 2.0 * y1) / (kDeltaR * kDeltaR);
+      // poly5ToZero (interpolation.F90:31-67) with dx = -distFudge
+      const double ddx = -kDistFudge;
+      const double dx1 = y1p * ddx, dx2 = y1pp * ddx * ddx;
+      const double a3 = 10.0 * y1 - 4.0 * dx1 + 0.5 * dx2;
+      const double a4 = -15.0 * y1 + 7.0 * dx1 - 1.0 * dx2;
+      const double a5 = 6.0 * y1 - 3.0 * dx1 + 0.5 * dx2;
+      const double xr = dr * (-1.0 / kDistFudge);
+      out[c] = ((a5 * xr + a4) * xr + a3) * xr * xr * xr;
+      if (dout) {
+        const double dpoly = ((5.0 * a5 * xr + 4.0 * a4) * xr + 3.0 * a3) * xr * xr;
+        dout[c] = dpoly * (-1.0 / kDistFudge);
+      }
+    }
+  }
+}
+
+// Rotated block hh[row * 4 + col] (row: orbital of atom 2, col: orbital of atom 1) for the bond
+// unit vector u = (l, m, n); optional derivative dh[c][row*4+col] w.r.t. the bond vector R.
+__device__ void sk_block(const SpeciesDev& s1, const SpeciesDev& s2, const double* sk,
+                         const double* dsk, double r, const double u[3], double* hh,
+                         double* dh /* 3 x 16 or null */) {
+  const double uv[3] = {u[1], u[2], u[0]};   // orbital order y, z, x
+  for (int i = 0; i < 16; ++i) hh[i] = 0.0;
+  if (dh)
+    for (int i = 0; i < 48; ++i) dh[i] = 0.0;
+  int ind = 0, icol = 0;
+  for (int a = 0; a < s1.nshell; ++a) {
+    const int l1 = s1.ang[a];
+    int irow = 0;
+    for (int b = 0; b < s2.nshell; ++b) {
+      const int l2 = s2.ang[b];
+      if (l1 == 0 && l2 == 0) {
+        hh[irow * 4 + icol] = sk[ind];
+        if (dh)
+          for (int c = 0; c < 3; ++c) dh[c * 16 + irow * 4 + icol] = dsk[ind] * u[c];
+      } else if (l1 + l2 == 1) {
+        const double sg = (l1 == 0) ? 1.0 : -1.0;   // (-1)^(l1+l2) transpose for p on atom 1
+        for (int p = 0; p < 3; ++p) {
+          const int rr = (l1 == 0) ? irow + p : irow;
+          const int cc = (l1 == 0) ? icol : icol + p;
+          hh[rr * 4 + cc] = sg * uv[p] * sk[ind];
+          if (dh) {
+            const int pc = (p == 0) ? 1 : (p == 1 ? 2 : 0);   // cartesian index of orbital p
+            for (int c = 0; c < 3; ++c) {
+              const double du = ((pc == c ? 1.0 : 0.0) - u[pc] * u[c]) / r;
+              dh[c * 16 + rr * 4 + cc] = sg * (du * sk[ind] + uv[p] * u[c] * dsk[ind]);
+            }
+          }
+        }
+      } else {
+        const double sg = sk[ind], pi = sk[ind + 1];
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 3; ++q) {
+            const double val = (sg - pi) * uv[p] * uv[q] + (p == q ? pi : 0.0);
+            hh[(irow + p) * 4 + icol + q] = val;
+            if (dh) {
+              const int pc = (p == 0) ? 1 : (p == 1 ? 2 : 0);
+              const int qc = (q == 0) ? 1 : (q == 1 ? 2 : 0);
+              for (int c = 0; c < 3; ++c) {
+                const double dup = ((pc == c ? 1.0 : 0.0) - u[pc] * u[c]) / r;
+                const double duq = ((qc == c ? 1.0 : 0.0) - u[qc] * u[c]) / r;
+                dh[c * 16 + (irow + p) * 4 + icol + q] =
+                    (dsk[ind] - dsk[ind + 1]) * u[c] * uv[p] * uv[q] +
+                    (p == q ? dsk[ind + 1] * u[c] : 0.0) + (sg - pi) * (dup * uv[q] + uv[p] * duq);
+              }
+            }
+          }
+      }
+      ind += min(l1, l2) + 1;
+      irow += 2 * l2 + 1;
+    }
+    icol += 2 * l1 + 1;
+  }
+}
+
+__device__ __forceinline__ void pair_block(const SkDev& sk, int sp1, int sp2, bool ham,
+                                           const double vec[3], double* hh, double* dh) {
+  const double r = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
+  const double u[3] = {vec[0] / r, vec[1] / r, vec[2] / r};
+  const int pair = sp1 * sk.nsp + sp2;
+  const double* tab = (ham ? sk.htab : sk.stab) + sk.tabOff[pair];
+  const int nint = sk.nint[pair];
+  double v[8], dv[8];
+  sk_integrals(tab, sk.nrows, nint, sk.dist, r, v, dh ? dv : nullptr);
+  sk_block(sk.sp[sp1], sk.sp[sp2], v, dv, r, u, hh, dh);
+}
+
+__global__ void build_h0s_kernel(SkDev sk, PairsDev pr, const double* coords, double* h0,
+                                 double* s) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr.nblk) return;
+  const int a1 = pr.at1[k], i2 = pr.img2[k], a2 = pr.at2[k];
+  const int sp1 = pr.species[a1], sp2 = pr.species[a2];
+  const SpeciesDev& S1 = sk.sp[sp1];
+  const SpeciesDev& S2 = sk.sp[sp2];
+  const int n1 = S1.norb, n2 = S2.norb;
+  const long st = pr.start[k];
+  if (i2 == a1) {
+    for (int c = 0; c < n1; ++c)
+      for (int r = 0; r < n2; ++r) {
+        h0[st + r + c * n2] = (r == c) ? S1.onsite[c] : 0.0;
+        s[st + r + c * n2] = (r == c) ? 1.0 : 0.0;
+      }
+    return;
+  }
+  double vec[3];
+  for (int c = 0; c < 3; ++c) vec[c] = coords[3 * (long)i2 + c] - coords[3 * (long)a1 + c];
+  double hh[16], ss[16];
+  pair_block(sk, sp1, sp2, true, vec, hh, nullptr);
+  pair_block(sk, sp1, sp2, false, vec, ss, nullptr);
+  for (int c = 0; c < n1; ++c)
+    for (int r = 0; r < n2; ++r) {
+      h0[st + r + c * n2] = hh[r * 4 + c];
+      s[st + r + c * n2] = ss[r * 4 + c];
+    }
+}
+
+// Dense fill of both triangles; blocks of periodic images of the same pair are summed.
+template <typename T>
+__global__ void unpack_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
+                              const double* x, T* dense, long ld) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr.nblk) return;
+  const int a1 = pr.at1[k], a2 = pr.at2[k];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
+  const long st = pr.start[k];
+  for (int c = 0; c < n1; ++c)
+    for (int r = 0; r < n2; ++r) {
+      const T v = (T)x[st + r + c * n2];
+      atomicAdd(&dense[(o2 + r) + (o1 + c) * ld], v);
+      if (a1 != a2) atomicAdd(&dense[(o1 + c) + (o2 + r) * ld], v);
+    }
+}
+
+template <typename T>
+__global__ void pack_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
+                            const T* dense, long ld, double* x) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr.nblk) return;
+  const int a1 = pr.at1[k], a2 = pr.at2[k];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
+  const long st = pr.start[k];
+  for (int c = 0; c < n1; ++c)
+    for (int r = 0; r < n2; ++r) {
+      long rr = r, cc = c;
+      if (a1 == a2 && r < c) {   // on-site block symmetrised from its lower triangle
+        rr = c;
+        cc = r;
+      }
+      x[st + r + c * n2] = (double)dense[(o2 + rr) + (o1 + cc) * ld];
+    }
+}
+
+__global__ void shift_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
+                             const double* h0, const double* s, const double* vorb, double* h) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr.nblk) return;
+  const int a1 = pr.at1[k], a2 = pr.at2[k];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
+  const long st = pr.start[k];
+  for (int c = 0; c < n1; ++c)
+    for (int r = 0; r < n2; ++r) {
+      const long e = st + r + c * n2;
+      h[e] = h0[e] + 0.5 * s[e] * (vorb[o2 + r] + vorb[o1 + c]);
+    }
+}
+
+// q_mu += sum_nu S rho (column sums to atom 1, row sums to atom 2 for off-site blocks);
+// the same contraction with H0 gives the non-SCC energy per orbital.
+__global__ void mulliken_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
+                                const double* s, const double* h0, const double* rho,
+                                double* qorb, double* eorb) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr.nblk) return;
+  const int a1 = pr.at1[k], a2 = pr.at2[k];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
+  const long st = pr.start[k];
+  double rq[4] = {0, 0, 0, 0}, re[4] = {0, 0, 0, 0};
+  for (int c = 0; c < n1; ++c) {
+    double cq = 0.0, ce = 0.0;
+    for (int r = 0; r < n2; ++r) {
+      const long e = st + r + c * n2;
+      const double pq = s[e] * rho[e];
+      const double pe = h0 ? h0[e] * rho[e] : 0.0;
+      cq += pq;
+      ce += pe;
+      rq[r] += pq;
+      re[r] += pe;
+    }
+    atomicAdd(&qorb[o1 + c], cq);
+    if (eorb) atomicAdd(&eorb[o1 + c], ce);
+  }
+  if (a1 != a2) {
+    for (int r = 0; r < n2; ++r) {
+      atomicAdd(&qorb[o2 + r], rq[r]);
+      if (eorb) atomicAdd(&eorb[o2 + r], re[r]);
+    }
+  }
+}
+
+// Pair forces: gradient contribution of every off-site block. mode 0: central finite
+// differences with step delta on the bond vector (the reference default); mode 1: analytic
+// derivative of the interpolated and rotated blocks (the adjoint of the assembly).
+__global__ void pair_force_kernel(SkDev sk, PairsDev pr, const int* norbAtom,
+                                  const long* iAtomStart, const double* coords, const double* rho,
+                                  const double* erho, const double* vorb, int mode, double delta,
+                                  double* grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr.nblk) return;
+  const int a1 = pr.at1[k], i2 = pr.img2[k], a2 = pr.at2[k];
+  if (a1 == a2) return;
+  const int sp1 = pr.species[a1], sp2 = pr.species[a2];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
+  const long st = pr.start[k];
+  double vec[3];
+  for (int c = 0; c < 3; ++c) vec[c] = coords[3 * (long)i2 + c] - coords[3 * (long)a1 + c];
+  double g[3] = {0, 0, 0};
+  if (mode == 0) {
+    for (int c = 0; c < 3; ++c) {
+      double hp[16], hm[16], sp[16], sm[16];
+      double vp[3] = {vec[0], vec[1], vec[2]}, vm[3] = {vec[0], vec[1], vec[2]};
+      vp[c] += delta;
+      vm[c] -= delta;
+      pair_block(sk, sp1, sp2, true, vp, hp, nullptr);
+      pair_block(sk, sp1, sp2, true, vm, hm, nullptr);
+      pair_block(sk, sp1, sp2, false, vp, sp, nullptr);
+      pair_block(sk, sp1, sp2, false, vm, sm, nullptr);
+      double acc = 0.0;
+      for (int cc = 0; cc < n1; ++cc)
+        for (int r = 0; r < n2; ++r) {
+          const long e = st + r + cc * n2;
+          const double dH = (hm[r * 4 + cc] - hp[r * 4 + cc]) / (2.0 * delta);
+          const double dS = (sm[r * 4 + cc] - sp[r * 4 + cc]) / (2.0 * delta);
+          const double vs = 0.5 * (vorb[o2 + r] + vorb[o1 + cc]);
+          acc += rho[e] * dH - erho[e] * dS + rho[e] * vs * dS;
+        }
+      g[c] = 2.0 * acc;
+    }
+  } else {
+    double hh[16], dh[48], ss[16], ds[48];
+    pair_block(sk, sp1, sp2, true, vec, hh, dh);
+    pair_block(sk, sp1, sp2, false, vec, ss, ds);
+    for (int c = 0; c < 3; ++c) {
+      double acc = 0.0;
+      for (int cc = 0; cc < n1; ++cc)
+        for (int r = 0; r < n2; ++r) {
+          const long e = st + r + cc * n2;
+          // derivative w.r.t. atom 1 = - d/dR
+          const double dH = -dh[c * 16 + r * 4 + cc];
+          const double dS = -ds[c * 16 + r * 4 + cc];
+          const double vs = 0.5 * (vorb[o2 + r] + vorb[o1 + cc]);
+          acc += rho[e] * dH - erho[e] * dS + rho[e] * vs * dS;
+        }
+      g[c] = 2.0 * acc;
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    atomicAdd(&grad[3 * (long)a1 + c], g[c]);
+    atomicAdd(&grad[3 * (long)a2 + c], -g[c]);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// SCC electrostatics: gamma = 1/R (or Ewald) - short-range exponential gamma (+U on-site)
+// ------------------------------------------------------------------------------------------
+__device__ double gsub(double r, double t1, double t2) {
+  return exp(-t1 * r) * ((0.5 * t2 * t2 * t2 * t2 * t1 / ((t1 * t1 - t2 * t2) * (t1 * t1 - t2 * t2))) -
+                         (t2 * t2 * t2 * t2 * t2 * t2 - 3.0 * t2 * t2 * t2 * t2 * t1 * t1) /
+                             (r * (t1 * t1 - t2 * t2) * (t1 * t1 - t2 * t2) * (t1 * t1 - t2 * t2)));
+}
+
+__device__ double gsubp(double r, double t1, double t2) {
+  const double d2 = t1 * t1 - t2 * t2;
+  const double a = 0.5 * t2 * t2 * t2 * t2 * t1 / (d2 * d2);
+  const double b = (t2 * t2 * t2 * t2 * t2 * t2 - 3.0 * t2 * t2 * t2 * t2 * t1 * t1) / (d2 * d2 * d2);
+  return -t1 * exp(-t1 * r) * (a - b / r) + exp(-t1 * r) * b / (r * r);
+}
+
+// expGamma (shortgammafuncs.F90:97-148)
+__device__ double expgamma(double r, double ua, double ub) {
+  const double ta = 3.2 * ua, tb = 3.2 * ub;
+  if (r < 1.0e-5) {
+    if (fabs(ua - ub) < 1.0e-10) return -0.5 * (ua + ub);
+    return -0.5 * ((ta * tb) / (ta + tb) + (ta * tb) * (ta * tb) / ((ta + tb) * (ta + tb) * (ta + tb)));
+  }
+  if (fabs(ua - ub) < 1.0e-10) {
+    const double tm = 0.5 * (ta + tb);
+    return exp(-tm * r) * (1.0 / r + 0.6875 * tm + 0.1875 * r * tm * tm +
+                           0.02083333333333333333 * r * r * tm * tm * tm);
+  }
+  return gsub(r, ta, tb) + gsub(r, tb, ta);
+}
+
+// expGammaPrime (shortgammafuncs.F90:152-196)
+__device__ double expgamma_prime(double r, double ua, double ub) {
+  if (r < 1.0e-5) return 0.0;
+  if (fabs(ua - ub) < 1.0e-10) {
+    const double tm = 3.2 * 0.5 * (ua + ub);
+    return -tm * exp(-tm * r) *
+               (1.0 / r + 0.6875 * tm + 0.1875 * r * tm * tm +
+                0.02083333333333333333 * r * r * tm * tm * tm) +
+           exp(-tm * r) * (-1.0 / (r * r) + 0.1875 * tm * tm +
+                           2.0 * 0.02083333333333333333 * r * tm * tm * tm);
+  }
+  const double ta = 3.2 * ua, tb = 3.2 * ub;
+  return gsubp(r, ta, tb) + gsubp(r, tb, ta);
+}
+
+__global__ void invr_cluster_kernel(int nat, const double* coords, double* g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nat) return;
+  double v = 0.0;
+  if (i != j) {
+    const double dx = coords[3 * i] - coords[3 * j], dy = coords[3 * i + 1] - coords[3 * j + 1],
+                 dz = coords[3 * i + 2] - coords[3 * j + 2];
+    v = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  g[i + (long)j * nat] = v;
+}
+
+// Ewald real-space sum over translations, plus the constant -pi/(V alpha^2) and the self term.
+__global__ void ewald_real_kernel(int nat, const double* coords, int ntr, const double* tr,
+                                  double alpha, double maxr, double cst, double self, double* g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nat) return;
+  const double dx = coords[3 * i] - coords[3 * j], dy = coords[3 * i + 1] - coords[3 * j + 1],
+               dz = coords[3 * i + 2] - coords[3 * j + 2];
+  double acc = 0.0;
+  for (int t = 0; t < ntr; ++t) {
+    const double x = dx - tr[3 * t], y = dy - tr[3 * t + 1], z = dz - tr[3 * t + 2];
+    const double r = sqrt(x * x + y * y + z * z);
+    if (r <= maxr && r > 1.0e-8) acc += erfc(alpha * r) / r;
+  }
+  acc += cst;
+  if (i == j) acc += self;
+  g[i + (long)j * nat] += acc;
+}
+
+// structure factor matrices C(i,g) = cos(G.r_i), S(i,g) = sin(G.r_i)
+__global__ void strucfac_kernel(int nat, const double* coords, int ng, const double* G,
+                                double* cs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (i >= nat) return;
+  const double ph =
+      G[3 * g] * coords[3 * i] + G[3 * g + 1] * coords[3 * i + 1] + G[3 * g + 2] * coords[3 * i + 2];
+  double sn, cn;
+  sincos(ph, &sn, &cn);
+  cs[i + (long)g * nat] = cn;
+  cs[i + (long)(g + ng) * nat] = sn;
+}
+
+// gamma(jc, i) and gamma(i, jc) -= expGamma for every neighbour within its cutoff
+__global__ void short_gamma_kernel(NeighDev nb, const double* coords, const double* hubbu,
+                                   const double* gcut, int nsp, int nat, double* g) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb.n) return;
+  const int i = nb.at1[k], j = nb.img2[k], jc = nb.at2[k];
+  const int si = nb.species[i], sj = nb.species[jc];
+  const double dx = coords[3 * (long)i] - coords[3 * (long)j],
+               dy = coords[3 * (long)i + 1] - coords[3 * (long)j + 1],
+               dz = coords[3 * (long)i + 2] - coords[3 * (long)j + 2];
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  if (j != i && r > gcut[sj * nsp + si]) return;
+  const double v = expgamma(r, hubbu[sj], hubbu[si]);
+  atomicAdd(&g[jc + (long)i * nat], -v);
+  if (jc != i) atomicAdd(&g[i + (long)jc * nat], -v);
+}
+
+// gradient of 1/2 dq^T gamma dq from the short-range part and (cluster) 1/R part per neighbour
+__global__ void short_gamma_grad_kernel(NeighDev nb, const double* coords, const double* hubbu,
+                                        const double* gcut, int nsp, const double* dq,
+                                        double* grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb.n) return;
+  const int i = nb.at1[k], j = nb.img2[k], jc = nb.at2[k];
+  if (jc == i) return;
+  const int si = nb.species[i], sj = nb.species[jc];
+  const double d[3] = {coords[3 * (long)i] - coords[3 * (long)j],
+                       coords[3 * (long)i + 1] - coords[3 * (long)j + 1],
+                       coords[3 * (long)i + 2] - coords[3 * (long)j + 2]};
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (r > gcut[sj * nsp + si]) return;
+  const double f = -expgamma_prime(r, hubbu[sj], hubbu[si]) * dq[i] * dq[jc] / r;
+  for (int c = 0; c < 3; ++c) {
+    atomicAdd(&grad[3 * (long)i + c], f * d[c]);
+    atomicAdd(&grad[3 * (long)jc + c], -f * d[c]);
+  }
+}
+
+// cluster 1/R gradient: one thread per atom i, loop over j
+__global__ void invr_cluster_grad_kernel(int nat, const double* coords, const double* dq,
+                                         double* grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nat) return;
+  double g[3] = {0, 0, 0};
+  for (int j = 0; j < nat; ++j) {
+    if (j == i) continue;
+    const double d[3] = {coords[3 * i] - coords[3 * j], coords[3 * i + 1] - coords[3 * j + 1],
+                         coords[3 * i + 2] - coords[3 * j + 2]};
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double f = -dq[i] * dq[j] / (r2 * sqrt(r2));
+    for (int c = 0; c < 3; ++c) g[c] += f * d[c];
+  }
+  for (int c = 0; c < 3; ++c) grad[3 * i + c] += g[c];
+}
+
+// Ewald gradient: real space over translations + reciprocal from structure factors
+__global__ void ewald_grad_kernel(int nat, const double* coords, int ntr, const double* tr,
+                                  double alpha, double maxr, int ng, const double* G,
+                                  const double* w, const double* CS, const double* dq,
+                                  double* grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nat) return;
+  double g[3] = {0, 0, 0};
+  const double ta = 2.0 * alpha / sqrt(M_PI);
+  for (int j = 0; j < nat; ++j) {
+    const double dx = coords[3 * i] - coords[3 * j], dy = coords[3 * i + 1] - coords[3 * j + 1],
+                 dz = coords[3 * i + 2] - coords[3 * j + 2];
+    double acc[3] = {0, 0, 0};
+    for (int t = 0; t < ntr; ++t) {
+      const double x = dx - tr[3 * t], y = dy - tr[3 * t + 1], z = dz - tr[3 * t + 2];
+      const double r2 = x * x + y * y + z * z;
+      const double r = sqrt(r2);
+      if (r <= maxr && r > 1.0e-8) {
+        const double dfdr = -(ta * exp(-alpha * alpha * r2) / r + erfc(alpha * r) / r2);
+        const double f = dfdr / r;
+        acc[0] += f * x;
+        acc[1] += f * y;
+        acc[2] += f * z;
+      }
+    }
+    for (int c = 0; c < 3; ++c) g[c] += acc[c] * dq[j];
+  }
+  // reciprocal: -dq_i sum_G w G (sin(G r_i) C_G - cos(G r_i) S_G), C_G = CS(ng+?) packed below
+  for (int q = 0; q < ng; ++q) {
+    const double ph = G[3 * q] * coords[3 * i] + G[3 * q + 1] * coords[3 * i + 1] +
+                      G[3 * q + 2] * coords[3 * i + 2];
+    double sn, cn;
+    sincos(ph, &sn, &cn);
+    const double fac = (sn * CS[q] - cn * CS[ng + q]) * w[q];
+    for (int c = 0; c < 3; ++c) g[c] -= fac * G[3 * q + c];
+  }
+  for (int c = 0; c < 3; ++c) grad[3 * i + c] += dq[i] * g[c];
+}
+
+// repulsive energy and gradient over neighbours within the repulsive cutoff
+__device__ void rep_value(const RepDev& rp, double r, double* e, double* de) {
+  *e = 0.0;
+  *de = 0.0;
+  if (r < 1.0e-2 || r >= rp.cutoff) return;
+  if (r < rp.xs[0]) {
+    const double ex = exp(-rp.expc[0] * r + rp.expc[1]);
+    *e = ex + rp.expc[2];
+    *de = -rp.expc[0] * ex;
+    return;
+  }
+  int im = 0;
+  while (im + 1 < rp.nint && rp.xs[im + 1] <= r) ++im;
+  const double dr = r - rp.xs[im];
+  const double* c = (im < rp.nint - 1) ? rp.coef + 4 * im : rp.last;
+  const int nc = (im < rp.nint - 1) ? 4 : 6;
+  double en = 0.0, dv = 0.0;
+  for (int q = nc - 1; q >= 0; --q) en = en * dr + c[q];
+  for (int q = nc - 1; q >= 1; --q) dv = dv * dr + q * c[q];
+  *e = en;
+  *de = dv;
+}
+
+__global__ void repulsive_kernel(NeighDev nb, const double* coords, const RepDev* reps, int nsp,
+                                 double* eatom, double* grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb.n) return;
+  const int i = nb.at1[k], j = nb.img2[k], jc = nb.at2[k];
+  if (j == i) return;
+  const double d[3] = {coords[3 * (long)i] - coords[3 * (long)j],
+                       coords[3 * (long)i + 1] - coords[3 * (long)j + 1],
+                       coords[3 * (long)i + 2] - coords[3 * (long)j + 2]};
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const RepDev& rp = reps[nb.species[i] * nsp + nb.species[jc]];
+  if (r >= rp.cutoff) return;
+  double e, de;
+  rep_value(rp, r, &e, &de);
+  atomicAdd(&eatom[i], 0.5 * e);
+  if (jc != i) {
+    atomicAdd(&eatom[jc], 0.5 * e);
+    for (int c = 0; c < 3; ++c) {
+      atomicAdd(&grad[3 * (long)i + c], de * d[c] / r);
+      atomicAdd(&grad[3 * (long)jc + c], -de * d[c] / r);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// launch wrappers
+// ------------------------------------------------------------------------------------------
+int launch_build_h0s(hipStream_t s, const SkDev& sk, const PairsDev& pr, const double* coords,
+                     double* h0, double* ov) {
+  if (pr.nblk == 0) return 0;
+  hipLaunchKernelGGL(build_h0s_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, sk, pr, coords,
+                     h0, ov);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+int launch_unpack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                  const double* x, T* dense, long ld, int n) {
+  DH_TRY(fill2d<T>(s, dense, ld, n, n, T(0)));
+  if (pr.nblk == 0) return 0;
+  hipLaunchKernelGGL((unpack_kernel<T>), dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+                     iAtomStart, x, dense, ld);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+int launch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                const T* dense, long ld, double* x) {
+  if (pr.nblk == 0) return 0;
+  hipLaunchKernelGGL((pack_kernel<T>), dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+                     iAtomStart, dense, ld, x);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_shift(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                 const double* h0, const double* ov, const double* vorb, double* h) {
+  if (pr.nblk == 0) return 0;
+  hipLaunchKernelGGL(shift_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+                     iAtomStart, h0, ov, vorb, h);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_mulliken(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                    const double* ov, const double* h0, const double* rho, double* qorb,
+                    double* eorb, int nOrb) {
+  DH_TRY(fill<double>(s, qorb, nOrb, 0.0));
+  if (eorb) DH_TRY(fill<double>(s, eorb, nOrb, 0.0));
+  if (pr.nblk == 0) return 0;
+  hipLaunchKernelGGL(mulliken_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+                     iAtomStart, ov, h0, rho, qorb, eorb);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_pair_forces(hipStream_t s, const SkDev& sk, const PairsDev& pr, const int* norb,
+                       const long* iAtomStart, const double* coords, const double* rho,
+                       const double* erho, const double* vorb, int mode, double delta,
+                       double* grad) {
+  if (pr.nblk == 0) return 0;
+  hipLaunchKernelGGL(pair_force_kernel, dim3(cdiv(pr.nblk, 64)), dim3(64), 0, s, sk, pr, norb,
+                     iAtomStart, coords, rho, erho, vorb, mode, delta, grad);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_invr_cluster(hipStream_t s, int nat, const double* coords, double* g) {
+  hipLaunchKernelGGL(invr_cluster_kernel, dim3(cdiv(nat, 256), nat), dim3(256), 0, s, nat, coords,
+                     g);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_ewald_real(hipStream_t s, int nat, const double* coords, int ntr, const double* tr,
+                      double alpha, double maxr, double cst, double self, double* g) {
+  hipLaunchKernelGGL(ewald_real_kernel, dim3(cdiv(nat, 256), nat), dim3(256), 0, s, nat, coords,
+                     ntr, tr, alpha, maxr, cst, self, g);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_strucfac(hipStream_t s, int nat, const double* coords, int ng, const double* G,
+                    double* cs) {
+  if (ng == 0) return 0;
+  hipLaunchKernelGGL(strucfac_kernel, dim3(cdiv(nat, 256), ng), dim3(256), 0, s, nat, coords, ng,
+                     G, cs);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_short_gamma(hipStream_t s, const NeighDev& nb, const double* coords,
+                       const double* hubbu, const double* gcut, int nsp, int nat, double* g) {
+  if (nb.n == 0) return 0;
+  hipLaunchKernelGGL(short_gamma_kernel, dim3(cdiv(nb.n, 256)), dim3(256), 0, s, nb, coords, hubbu,
+                     gcut, nsp, nat, g);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_gamma_grad(hipStream_t s, const NeighDev& nb, const double* coords,
+                      const double* hubbu, const double* gcut, int nsp, int nat, bool periodic,
+                      int ntr, const double* tr, double alpha, double maxr, int ng,
+                      const double* G, const double* w, const double* CSq, const double* dq,
+                      double* grad) {
+  if (nb.n > 0) {
+    hipLaunchKernelGGL(short_gamma_grad_kernel, dim3(cdiv(nb.n, 256)), dim3(256), 0, s, nb, coords,
+                       hubbu, gcut, nsp, dq, grad);
+    DH_LAUNCH_CHECK();
+  }
+  if (!periodic) {
+    hipLaunchKernelGGL(invr_cluster_grad_kernel, dim3(cdiv(nat, 64)), dim3(64), 0, s, nat, coords,
+                       dq, grad);
+  } else {
+    hipLaunchKernelGGL(ewald_grad_kernel, dim3(cdiv(nat, 64)), dim3(64), 0, s, nat, coords, ntr,
+                       tr, alpha, maxr, ng, G, w, CSq, dq, grad);
+  }
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_repulsive(hipStream_t s, const NeighDev& nb, const double* coords, const RepDev* reps,
+                     int nsp, double* eatom, double* grad) {
+  if (nb.n == 0) return 0;
+  hipLaunchKernelGGL(repulsive_kernel, dim3(cdiv(nb.n, 256)), dim3(256), 0, s, nb, coords, reps,
+                     nsp, eatom, grad);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+template int launch_unpack<double>(hipStream_t, const PairsDev&, const int*, const long*,
+                                   const double*, double*, long, int);
+template int launch_unpack<float>(hipStream_t, const PairsDev&, const int*, const long*,
+                                  const double*, float*, long, int);
+template int launch_pack<double>(hipStream_t, const PairsDev&, const int*, const long*,
+                                 const double*, long, double*);
+template int launch_pack<float>(hipStream_t, const PairsDev&, const int*, const long*,
+                                const float*, long, double*);
+
+}  // namespace dftbhip

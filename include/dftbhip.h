@@ -1,0 +1,140 @@
+/*
+ * dftbhip: MI355X (gfx950) backend for the SCC-DFTB hot path of DFTB+.
+ *
+ * C ABI, plain pointers and sizes. All matrices are column-major (Fortran order), all
+ * indices passed in neighbour-list arrays are the 1-based values DFTB+ stores. Every function
+ * returns 0 on success, a negative code on argument/device errors (message from
+ * dftbhip_last_error) and, for the eigensolver, a LAPACK-style positive info.
+ */
+#ifndef DFTBHIP_H
+#define DFTBHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- device management: replaces dftbp_extlibs_magma_get_gpus_available / magma_init
+ *      (src/dftbp/extlibs/magmac.c:1-46, src/dftbp/extlibs/magma.F90:30-60) ---------------- */
+int dftbhip_get_gpus_available(int* ngpus);
+int dftbhip_init(int device);
+int dftbhip_finalize(void);
+int dftbhip_last_error(char* buf, int len);
+
+/* ---- generalized symmetric-definite eigensolver: replaces magmaHegvd (dreal)
+ *      (src/dftbp/math/eigensolver.F90:1143-1245; contract of hegvd/hegvr :561-920).
+ * itype must be 1, uplo 'L'. On exit a holds the S-orthonormal eigenvectors (jobz = 'V'),
+ * b the Cholesky factor, w the ascending eigenvalues. info > n: leading minor info-n of b is
+ * not positive definite. */
+int dftbhip_sygvd(int itype, char jobz, char uplo, int n, double* a, int lda, double* b, int ldb,
+                  double* w);
+
+/* ---- BLAS / LAPACK style drop-ins (the dgemm and dstedc DFTB+ reaches through
+ *      src/dftbp/math/blasroutines.F90 and the divide-and-conquer solvers of
+ *      src/dftbp/math/eigensolver.F90:561-660). dstedc: on exit d holds the ascending
+ *      eigenvalues and z (n x n) the eigenvectors of the tridiagonal (d, e). */
+int dftbhip_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a,
+                  int lda, const double* b, int ldb, double beta, double* c, int ldc);
+int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz);
+
+/* ---- density matrices: replace makeDensityMtxRealGPU / makeEnergyDensityMtxRealGPU
+ *      (src/dftbp/dftb/densitymatrix.F90:1105-1395). Lower triangle of dm/edm is set:
+ *      dm = C diag(f) C^T, edm = C diag(f*eig) C^T over the first nlevels columns. */
+int dftbhip_dm_real(int n, int nlevels, const double* evec, int lde, const double* filling,
+                    double* dm, int lddm);
+int dftbhip_edm_real(int n, int nlevels, const double* evec, int lde, const double* filling,
+                     const double* eigen, double* edm, int lddm);
+
+/* ---- Slater-Koster parameters and repulsives (the content of the .skf files after
+ *      getFullTable, src/dftbp/dftbplus/parser.F90:3668-4130). Species have at most two
+ *      shells with l <= 1. Tables are row-major (nrows x ninteg[sp1*nspecies+sp2]), row i at
+ *      r = (i+1)*dist. ------------------------------------------------------------------- */
+#define DFTBHIP_MAX_SPECIES 8
+typedef struct {
+  int nspecies;
+  int nshell[DFTBHIP_MAX_SPECIES];
+  int angshell[DFTBHIP_MAX_SPECIES][2];
+  double onsite[DFTBHIP_MAX_SPECIES][2];  /* per shell (Ha) */
+  double hubbu[DFTBHIP_MAX_SPECIES];      /* first-shell U (non shell-resolved SCC) */
+  double occ[DFTBHIP_MAX_SPECIES][2];     /* neutral shell occupations */
+  int nrows;
+  double dist;
+  int ninteg[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES];
+  const double* htab[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES];
+  const double* stab[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES];
+  int rep_nint[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES];
+  double rep_cutoff[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES];
+  double rep_expc[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES][3];
+  const double* rep_xs[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES];   /* rep_nint */
+  const double* rep_coef[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES]; /* (rep_nint-1) x 4 */
+  const double* rep_last[DFTBHIP_MAX_SPECIES * DFTBHIP_MAX_SPECIES]; /* 6 */
+} dftbhip_params;
+
+typedef struct dftbhip_sk dftbhip_sk;
+int dftbhip_sk_create(dftbhip_sk** sk, const dftbhip_params* p);
+int dftbhip_sk_destroy(dftbhip_sk* sk);
+
+/* ---- sparse kernels with the reference's array conventions ------------------------------
+ * coords(3, nAll); species(nAll) 1-based; img2CentCell(nAll) 1-based;
+ * iNeighbour(0:mNeighbour, nAtom) 1-based image indices; nNeighbourSK(nAtom);
+ * iPair(0:mNeighbour, nAtom) 0-based sparse offsets; iAtomStart(nAtom+1) 1-based. */
+
+/* replaces buildH0 / buildS (src/dftbp/dftb/nonscc.F90:66-181) */
+int dftbhip_build_h0_s(const dftbhip_sk* sk, int nAtom, int nAll, const double* coords,
+                       const int* species, const int* img2CentCell, int mNeighbour,
+                       const int* iNeighbour, const int* nNeighbourSK, const int* iPair,
+                       int sparseSize, double* h0, double* over);
+/* replaces unpackHS_real (src/dftbp/dftb/sparse2dense.F90:266-321); writes both triangles */
+int dftbhip_unpack_hs_real(int nAtom, const int* iAtomStart, const int* img2CentCell,
+                           int mNeighbour, const int* iNeighbour, const int* nNeighbourSK,
+                           const int* iPair, int sparseSize, const double* orig, double* square,
+                           int ldsq);
+/* replaces packHS_real (src/dftbp/dftb/sparse2dense.F90:1024-1097); primitive is overwritten */
+int dftbhip_pack_hs_real(int nAtom, const int* iAtomStart, const int* img2CentCell,
+                         int mNeighbour, const int* iNeighbour, const int* nNeighbourSK,
+                         const int* iPair, int sparseSize, const double* square, int ldsq,
+                         double* primitive);
+/* replaces mullikenPerOrbital (src/dftbp/dftb/populations.F90:120-193); qorb(nOrb) */
+int dftbhip_mulliken(int nAtom, const int* iAtomStart, const int* img2CentCell, int mNeighbour,
+                     const int* iNeighbour, const int* nNeighbourSK, const int* iPair,
+                     int sparseSize, const double* over, const double* rho, double* qorb);
+
+/* ---- resident SCC context: the full hot path on the device -------------------------------
+ * (SCC loop of src/dftbp/dftbplus/main.F90:1429-1563, gradients main.F90:7058-7440) */
+typedef struct {
+  double kT;            /* electron temperature (Ha), floored at 1e-8 */
+  double scc_tol;       /* max |q_out - q_in| */
+  int max_scc_iter;
+  int min_scc_iter;
+  double mix_param, omega0, min_weight, max_weight, weight_fac;   /* Broyden */
+  double fd_delta;      /* finite-difference step of the SK derivatives */
+  double ewald_tol;
+  int force_mode;       /* 0: finite differences (reference default), 1: analytic adjoint */
+  int precision;        /* 0: FP64, 1: mixed (FP32 MFMA bulk + FP64 refinement) */
+} dftbhip_options;
+
+typedef struct dftbhip_ctx dftbhip_ctx;
+int dftbhip_ctx_create(dftbhip_ctx** ctx, const dftbhip_params* p, const dftbhip_options* opt);
+int dftbhip_ctx_destroy(dftbhip_ctx* ctx);
+/* coords(3, nAtom) bohr, species 0-based, latvecs(3,3) rows = lattice vectors or NULL */
+int dftbhip_ctx_set_geometry(dftbhip_ctx* ctx, int nAtom, const double* coords,
+                             const int* species, const double* latvecs);
+/* energies: mermin, total, nonscc, scc, repulsive, TS, Eband, Ef; gross charges q0 - q */
+int dftbhip_ctx_scc(dftbhip_ctx* ctx, double* energies, double* charges, int* niter,
+                    int* converged);
+/* gradient dE/dR (3, nAtom); needs a converged dftbhip_ctx_scc */
+int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad);
+int dftbhip_ctx_eigenvalues(dftbhip_ctx* ctx, double* eig, double* fill, int n);
+int dftbhip_ctx_nr_orbitals(dftbhip_ctx* ctx);
+/* per-phase wall times (s) of the last SCC/gradient call and the number of calls */
+int dftbhip_ctx_timings(dftbhip_ctx* ctx, double* t, int n);
+
+/* ---- diagnostics: accumulator lane layouts of the FP64 and FP32 16x16x4 MFMAs (one wave
+ *      each), out[512]: FP64 in out[0:256], FP32 in out[256:512] ----------------------------
+ * (no counterpart in the reference; used by the tests to pin the MFMA tiling) */
+int dftbhip_probe_mfma_f64(double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -134,6 +134,33 @@ class Sparse:
     start: np.ndarray     # offset of the block (column-major nOrb2 x nOrb1)
     nnz: int
     nneigh_sk: np.ndarray  # per atom, neighbours within the SK cutoff (self excluded)
+    rows: Optional[np.ndarray] = None     # dense row of every sparse element
+    cols: Optional[np.ndarray] = None     # dense column of every sparse element
+    offsite: Optional[np.ndarray] = None  # element belongs to a block with atom1 != atom2
+    prow: Optional[np.ndarray] = None     # row/column read by pack (on-site blocks: lower part)
+    pcol: Optional[np.ndarray] = None
+
+    def index_maps(self, off: np.ndarray) -> "Sparse":
+        rows, cols, offs, prow, pcol = (np.zeros(self.nnz, dtype=np.int64) for _ in range(5))
+        offs = np.zeros(self.nnz, dtype=bool)
+        for k in range(len(self.at1)):
+            a, b = self.at1[k], self.at2[k]
+            n1, n2 = off[a + 1] - off[a], off[b + 1] - off[b]
+            r = np.tile(np.arange(n2), n1)
+            c = np.repeat(np.arange(n1), n2)
+            sl = slice(self.start[k], self.start[k] + n1 * n2)
+            rows[sl] = off[b] + r
+            cols[sl] = off[a] + c
+            offs[sl] = a != b
+            if a == b:
+                lo = r >= c
+                prow[sl] = np.where(lo, off[b] + r, off[b] + c)
+                pcol[sl] = np.where(lo, off[a] + c, off[a] + r)
+            else:
+                prow[sl] = off[b] + r
+                pcol[sl] = off[a] + c
+        self.rows, self.cols, self.offsite, self.prow, self.pcol = rows, cols, offs, prow, pcol
+        return self
 
 
 def sparse_descriptor(nl: NeighbourList, norb_atom: np.ndarray, sk_cutoff: float) -> Sparse:
@@ -319,6 +346,9 @@ def unpack(x: np.ndarray, params, species, sp: Sparse, off: np.ndarray) -> np.nd
     """Dense lower triangle (plus full diagonal blocks), image blocks accumulated."""
     N = int(off[-1])
     out = np.zeros((N, N))
+    if sp.rows is not None:
+        np.add.at(out, (sp.rows, sp.cols), x)
+        return out
     for k in range(len(sp.at1)):
         a, b = sp.at1[k], sp.at2[k]
         n1, n2 = off[a + 1] - off[a], off[b + 1] - off[b]
@@ -333,6 +363,8 @@ def symmetrize_lower(m: np.ndarray) -> np.ndarray:
 
 def pack(dense: np.ndarray, params, species, sp: Sparse, off: np.ndarray) -> np.ndarray:
     """Gather the half-list blocks from the dense lower triangle, on-site blocks symmetrised."""
+    if sp.prow is not None:
+        return dense[sp.prow, sp.pcol].copy()
     out = np.zeros(sp.nnz)
     for k in range(len(sp.at1)):
         a, b = sp.at1[k], sp.at2[k]
@@ -348,6 +380,11 @@ def mulliken(over: np.ndarray, rho: np.ndarray, species, sp: Sparse, off: np.nda
     """``mullikenPerOrbital`` (populations.F90:120-193): q per orbital (length N)."""
     N = int(off[-1])
     q = np.zeros(N)
+    if sp.rows is not None:
+        prod = over * rho
+        np.add.at(q, sp.cols, prod)
+        np.add.at(q, sp.rows[sp.offsite], prod[sp.offsite])
+        return q
     for k in range(len(sp.at1)):
         a, b = sp.at1[k], sp.at2[k]
         n1, n2 = off[a + 1] - off[a], off[b + 1] - off[b]
@@ -361,6 +398,8 @@ def mulliken(over: np.ndarray, rho: np.ndarray, species, sp: Sparse, off: np.nda
 def add_shift(h0, over, vorb, species, sp: Sparse, off):
     """H = H0 + 1/2 S (V_mu + V_nu) for a diagonal per-orbital potential
     (``addShift_block``, shift.F90:132-229)."""
+    if sp.rows is not None:
+        return h0 + 0.5 * over * (vorb[sp.rows] + vorb[sp.cols])
     h = h0.copy()
     for k in range(len(sp.at1)):
         a, b = sp.at1[k], sp.at2[k]
@@ -808,7 +847,7 @@ def repulsive(params, species, nl: NeighbourList):
             rep = params.rep[(species[i], species[jc])]
             r = math.sqrt(nl.dist2[i][k])
             en, de = rep.value(r)
-            e += en
+            e += en if jc != i else 0.5 * en      # twobodyrep.F90:160-166
             if jc != i:
                 vec = nl.coords_all[i] - nl.coords_all[j]
                 g = de * vec / r
@@ -858,7 +897,7 @@ class Calculator:
         gcut = max(expgamma_cutoff(a, b) for a in u for b in u)
         self.cutoff = max(cutoff, gcut)
         self.nl = neighbour_list(geo.coords, geo.latvecs, self.cutoff)
-        self.sp = sparse_descriptor(self.nl, self.norb_atom, params.sk_cutoff())
+        self.sp = sparse_descriptor(self.nl, self.norb_atom, params.sk_cutoff()).index_maps(self.off)
         self.h0, self.s = build_h0_s(params, sp, self.nl, self.sp)
         self.gamma = gamma_matrix(params, sp, self.nl, geo.coords, geo.latvecs, opts.ewald_tol)
         self.q0 = np.array([sum(params.species[s].occ) for s in sp])

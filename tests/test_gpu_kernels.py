@@ -1,0 +1,143 @@
+"""GPU parity of the individual kernels against numpy/scipy and the CPU oracle, through the C ABI."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import reference_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _api():
+    from dftbplus_amd import api
+    api.lib()
+    return api
+
+
+def test_mfma_layouts():
+    """Accumulator element r of lane l of the 16x16x4 MFMA holds D(i, l%16) with
+    i = l//16 + 4r for FP64 and i = 4(l//16) + r for FP32 (D(i, j) = i + 100 j in the probe)."""
+    api = _api()
+    out = np.zeros(512)
+    api._check(api.lib().dftbhip_probe_mfma_f64(api._d(out)), "probe")
+    f64 = out[:256].reshape(64, 4)
+    f32 = out[256:].reshape(64, 4)
+    e64 = np.array([[l // 16 + 4 * r + 100 * (l % 16) for r in range(4)] for l in range(64)], float)
+    e32 = np.array([[4 * (l // 16) + r + 100 * (l % 16) for r in range(4)] for l in range(64)], float)
+    assert np.array_equal(f64, e64)
+    assert np.array_equal(f32, e32)
+
+
+@pytest.mark.parametrize("m,n,k,ta,tb", [(1, 1, 1, "N", "N"), (37, 53, 29, "N", "N"),
+                                         (130, 257, 200, "T", "N"), (128, 128, 16, "N", "T"),
+                                         (300, 129, 77, "T", "T")])
+def test_dgemm(m, n, k, ta, tb):
+    api = _api()
+    rng = np.random.default_rng(m + n + k)
+    a = rng.standard_normal((k, m) if ta == "T" else (m, k))
+    b = rng.standard_normal((n, k) if tb == "T" else (k, n))
+    c0 = rng.standard_normal((m, n))
+    ref = 0.7 * ((a.T if ta == "T" else a) @ (b.T if tb == "T" else b)) - 0.3 * c0
+    got = api.dgemm(a, b, ta, tb, alpha=0.7, beta=-0.3, c=c0)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def _tridiag_check(d, e, w, z):
+    n = len(d)
+    ref = scipy.linalg.eigh_tridiagonal(d, e, eigvals_only=True) if n > 1 else d
+    scale = max(1.0, np.abs(ref).max())
+    assert np.max(np.abs(w - ref)) < 1e-12 * scale * max(1, np.sqrt(n))
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    assert np.max(np.abs(T @ z - z * w)) < 1e-11 * scale * max(1, np.sqrt(n))
+    assert np.max(np.abs(z.T @ z - np.eye(n))) < 1e-11 * max(1, np.sqrt(n))
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 64, 65, 129, 300, 1000])
+def test_dstedc_random(n):
+    api = _api()
+    rng = np.random.default_rng(n)
+    d = rng.uniform(-1, 1, n)
+    e = rng.uniform(-1, 1, max(n - 1, 0))
+    w, z = api.dstedc(d, e)
+    _tridiag_check(d, e, w, z)
+
+
+def test_dstedc_clustered():
+    """Glued Wilkinson matrices: many pairs of eigenvalues equal to working precision (heavy
+    deflation) plus exact degeneracies from a decoupled block."""
+    api = _api()
+    m = 21
+    dw = np.abs(np.arange(m) - m // 2).astype(float)
+    d = np.concatenate([dw] * 10)
+    e = np.concatenate([np.concatenate([np.ones(m - 1), [1e-8]])] * 10)[:-1]
+    e[m * 5 - 1] = 0.0
+    w, z = api.dstedc(d, e)
+    _tridiag_check(d, e, w, z)
+
+
+@pytest.mark.parametrize("n", [10, 100, 333])
+def test_sygvd_random(n):
+    api = _api()
+    rng = np.random.default_rng(7 * n)
+    x = rng.standard_normal((n, n))
+    s = x @ x.T / n + np.eye(n)
+    h = rng.standard_normal((n, n))
+    h = h + h.T
+    w, c, l = api.sygvd(h, s)
+    ref = scipy.linalg.eigh(h, s, eigvals_only=True)
+    assert np.max(np.abs(w - ref)) < 1e-11 * np.abs(ref).max()
+    assert np.max(np.abs(h @ c - s @ c * w)) < 1e-10 * np.abs(ref).max()
+    assert np.max(np.abs(c.T @ s @ c - np.eye(n))) < 1e-11
+    assert np.allclose(np.tril(l), np.linalg.cholesky(s), atol=1e-12)
+
+
+def test_sygvd_not_positive_definite():
+    api = _api()
+    s = np.eye(4)
+    s[2, 2] = -1.0
+    with pytest.raises(api.DftbHipError, match="info=7"):
+        api.sygvd(np.eye(4), s)
+
+
+def test_density_matrices():
+    api = _api()
+    rng = np.random.default_rng(3)
+    n, nl = 200, 120
+    c = rng.standard_normal((n, n))
+    f = np.sort(rng.uniform(0, 2, n))[::-1]
+    eig = rng.uniform(-1, 1, n)
+    dm = api.dm_real(c, f, nl)
+    edm = api.edm_real(c, f, eig, nl)
+    ref = (c[:, :nl] * f[:nl]) @ c[:, :nl].T
+    refe = (c[:, :nl] * (f[:nl] * eig[:nl])) @ c[:, :nl].T
+    lo = np.tril_indices(n)
+    assert np.allclose(dm[lo], ref[lo], rtol=1e-12, atol=1e-11)
+    assert np.allclose(edm[lo], refe[lo], rtol=1e-12, atol=1e-11)
+
+
+@pytest.mark.parametrize("system", ["cluster216", "diamond2"])
+def test_sparse_kernels(system, request):
+    """H0/S assembly, unpack, pack and Mulliken against the oracle on the same neighbour list."""
+    api = _api()
+    geo, params, opts, calc, res = request.getfixturevalue(system)
+    ra = reference_arrays(calc)
+    sk = api.SkHandle(params)
+    h0, s = api.build_h0_s(sk, ra)
+    # the 5th-order tail beyond the grid uses a second difference with step 1e-5 bohr
+    # (slakoeqgrid.F90:233-239), which turns last-bit differences of the Neville values into
+    # ~1e-11; inside the grid the blocks agree to rounding.
+    assert np.max(np.abs(h0 - calc.h0)) < 1e-10
+    assert np.max(np.abs(s - calc.s)) < 1e-10
+    from oracle import dftb_ref as R
+    dense = api.unpack_hs(ra, calc.s)
+    ref = R.unpack(calc.s, params, geo.species, calc.sp, calc.off)
+    full = R.symmetrize_lower(ref)
+    assert np.max(np.abs(dense - full)) < 1e-14
+    rng = np.random.default_rng(1)
+    m = rng.standard_normal(dense.shape)
+    m = m + m.T
+    packed = api.pack_hs(ra, m)
+    assert np.max(np.abs(packed - R.pack(m, params, geo.species, calc.sp, calc.off))) == 0.0
+    q = api.mulliken(ra, calc.s, res.rho)
+    qref = R.mulliken(calc.s, res.rho, geo.species, calc.sp, calc.off)
+    assert np.max(np.abs(q - qref)) < 1e-13
