@@ -92,6 +92,8 @@ EXPORTS = {
                       _dp, C.c_int, C.c_double, _dp, C.c_int],
     "dftbhip_dstedc": [C.c_int, _dp, _dp, _dp, C.c_int],
     "dftbhip_probe_mfma_f64": [_dp],
+    "dftbhip_kernel_profile": [C.c_int],
+    "dftbhip_kernel_stats": [_dp],
     "dftbhip_dm_real": [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int],
     "dftbhip_edm_real": [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int],
     "dftbhip_sk_create": [C.POINTER(C.c_void_p), C.POINTER(Params)],
@@ -107,6 +109,9 @@ EXPORTS = {
     "dftbhip_ctx_destroy": [C.c_void_p],
     "dftbhip_ctx_set_geometry": [C.c_void_p, C.c_int, _dp, _ip, _dp],
     "dftbhip_ctx_scc": [C.c_void_p, _dp, _dp, _ip, _ip],
+    "dftbhip_ctx_scc_begin": [C.c_void_p],
+    "dftbhip_ctx_scc_step": [C.c_void_p, _ip, _dp],
+    "dftbhip_ctx_scc_end": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_ctx_gradients": [C.c_void_p, _dp],
     "dftbhip_ctx_eigenvalues": [C.c_void_p, _dp, _dp, C.c_int],
     "dftbhip_ctx_nr_orbitals": [C.c_void_p],
@@ -386,6 +391,30 @@ class DftbPlus:
         conv = C.c_int()
         _check(lib().dftbhip_ctx_scc(self.h, _d(e), _d(q), C.byref(nit), C.byref(conv)),
                "dftbhip_ctx_scc")
+        self.energies = dict(zip(ENERGY_KEYS, e))
+        self.charges = q
+        self.n_iter = nit.value
+        self.converged = bool(conv.value)
+        self._grad = None
+        self._dirty = False
+
+    # one-iteration control of the SCC cycle (used to time single iterations)
+    def scc_begin(self):
+        _check(lib().dftbhip_ctx_scc_begin(self.h), "dftbhip_ctx_scc_begin")
+
+    def scc_step(self):
+        conv = C.c_int()
+        err = C.c_double()
+        _check(lib().dftbhip_ctx_scc_step(self.h, C.byref(conv), C.byref(err)), "dftbhip_ctx_scc_step")
+        return bool(conv.value), err.value
+
+    def scc_end(self):
+        e = np.zeros(8)
+        q = np.zeros(self.natom)
+        nit = C.c_int()
+        conv = C.c_int()
+        _check(lib().dftbhip_ctx_scc_end(self.h, _d(e), _d(q), C.byref(nit), C.byref(conv)),
+               "dftbhip_ctx_scc_end")
         self.energies = dict(zip(ENERGY_KEYS, e))
         self.charges = q
         self.n_iter = nit.value

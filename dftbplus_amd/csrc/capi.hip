@@ -1040,8 +1040,18 @@ enum Timer {
   kTIter, kTCount
 };
 
+struct SccState {
+  std::vector<double> qin;
+  Broyden mixer;
+  int it = 0;
+  bool conv = false, active = false;
+  double err = 0.0, nel = 0.0, eNonScc = 0.0, eScc = 0.0;
+  FillResult fr{};
+};
+
 struct Context {
   dftbhip_options opt{};
+  SccState st;
   SkTables sk;
   hipStream_t s = nullptr;
   int nat = 0, norbTot = 0;
@@ -1214,109 +1224,115 @@ static int gamma_times(Context& c, const std::vector<double>& dqa, std::vector<d
   return 0;
 }
 
-static int ctx_scc(Context& c, double* energies, double* charges, int* niter, int* converged) {
+static int ctx_scc_begin(Context& c) {
+  for (int t = 1; t < kTCount; ++t) c.times[t] = 0.0;
+  SccState& st = c.st;
+  st.nel = 0.0;
+  for (double q : c.q0atom) st.nel += q;
+  st.qin = c.q0atom;
+  st.mixer.reset(c.nat, c.opt, c.opt.max_scc_iter + 1);
+  st.it = 0;
+  st.conv = false;
+  st.err = 0.0;
+  st.active = true;
+  c.sccDone = false;
+  return 0;
+}
+
+// One SCC iteration (main.F90:1429-1563): potential from the input charges, H, generalized
+// eigenproblem, Fermi filling, density matrix, Mulliken charges, energies with the output
+// potential, convergence test and Broyden mixing.
+static int ctx_scc_step(Context& c) {
+  DH_CHECK(c.st.active, "scc_step: call scc_begin first");
   hipStream_t s = c.s;
   const int nat = c.nat, n = c.norbTot;
   const dftbhip_options& o = c.opt;
-  for (int t = 1; t < kTCount; ++t) c.times[t] = 0.0;
-  double nel = 0.0;
-  for (double q : c.q0atom) nel += q;
-  std::vector<double> qin = c.q0atom, dqa(nat), qorbH(n), eorbH(n), vout;
-  Broyden mixer;
-  mixer.reset(nat, o, o.max_scc_iter + 1);
-  double eNonScc = 0.0, eScc = 0.0;
-  FillResult fr{};
-  bool conv = false;
-  int it = 0;
-  double ev_t[3] = {0, 0, 0};
-  hipEvent_t e0, e1, e2, e3;
-  DH_HIP(hipEventCreate(&e0));
-  DH_HIP(hipEventCreate(&e1));
-  DH_HIP(hipEventCreate(&e2));
-  DH_HIP(hipEventCreate(&e3));
-  for (it = 1; it <= o.max_scc_iter; ++it) {
-    const double tIt = now();
-    for (int a = 0; a < nat; ++a) dqa[a] = qin[a] - c.q0atom[a];
-    DH_HIP(hipEventRecord(e0, s));
-    DH_TRY(gamma_times(c, dqa, nullptr));
-    DH_TRY(launch_shift(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.h0.p, c.ov.p, c.vorb.p,
-                        c.hs.p));
-    DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p,
-                                 n, n));
-    DH_HIP(hipEventRecord(e1, s));
-    int info = 0;
-    DH_TRY(c.eig.solve(s, c.Hd.p, n, c.eigv, &info, ev_t));
-    if (info != 0) return info;
-    const double th = now();
-    DH_TRY(fermi_filling(c.eigv, nel, o.kT, c.fill, fr));
-    int nocc = 0;
-    for (int i = 0; i < n; ++i)
-      if (c.fill[i] > 0.0) nocc = i + 1;
-    c.nocc = nocc;
-    c.times[kTHost] += now() - th;
-    DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-    DH_HIP(hipEventRecord(e2, s));
-    DH_TRY(c.eig.vectors(s, nocc, c.X.p, n));
-    DH_HIP(hipEventRecord(e3, s));
-    DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
-                        kKFull, c.f.p));
-    hipEvent_t e4, e5;
-    DH_HIP(hipEventCreate(&e4));
-    DH_HIP(hipEventCreate(&e5));
-    DH_HIP(hipEventRecord(e4, s));
-    DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n, c.rho.p));
-    DH_TRY(launch_mulliken(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.h0.p,
-                           c.rho.p, c.qorb.p, c.eorb.p, n));
-    DH_HIP(hipMemcpyAsync(qorbH.data(), c.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    DH_HIP(hipMemcpyAsync(eorbH.data(), c.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    DH_HIP(hipEventRecord(e5, s));
-    DH_HIP(hipStreamSynchronize(s));
-    float ms;
-    DH_HIP(hipEventElapsedTime(&ms, e0, e1));
-    c.times[kTHam] += ms * 1e-3;
-    DH_HIP(hipEventElapsedTime(&ms, e2, e3));
-    c.times[kTBack] += ms * 1e-3;
-    DH_HIP(hipEventElapsedTime(&ms, e3, e4));
-    c.times[kTDM] += ms * 1e-3;
-    DH_HIP(hipEventElapsedTime(&ms, e4, e5));
-    c.times[kTPackMull] += ms * 1e-3;
-    (void)hipEventDestroy(e4);
-    (void)hipEventDestroy(e5);
-    const double th2 = now();
-    std::vector<double> qout(nat, 0.0);
-    eNonScc = 0.0;
-    for (int a = 0; a < nat; ++a)
-      for (long oi = c.atomStart[a]; oi < c.atomStart[a + 1]; ++oi) qout[a] += qorbH[oi];
-    for (int i = 0; i < n; ++i) eNonScc += eorbH[i];
-    std::vector<double> dqo(nat), diff(nat);
-    double err = 0.0;
-    for (int a = 0; a < nat; ++a) {
-      dqo[a] = qout[a] - c.q0atom[a];
-      diff[a] = qout[a] - qin[a];
-      err = std::max(err, std::fabs(diff[a]));
-    }
-    c.times[kTHost] += now() - th2;
-    DH_TRY(gamma_times(c, dqo, &vout));
-    eScc = 0.0;
-    for (int a = 0; a < nat; ++a) eScc += 0.5 * vout[a] * dqo[a];
-    c.qout = qout;
-    c.dqout = dqo;
-    conv = err < o.scc_tol && it >= o.min_scc_iter;
-    const double th3 = now();
-    if (!conv && it != o.max_scc_iter) mixer.mix(qin, diff);
-    c.times[kTHost] += now() - th3;
-    c.times[kTIter] = now() - tIt;
-    if (conv) break;
+  SccState& st = c.st;
+  st.it += 1;
+  const double tIt = now();
+  std::vector<double> dqa(nat), qorbH(n), eorbH(n), vout;
+  for (int a = 0; a < nat; ++a) dqa[a] = st.qin[a] - c.q0atom[a];
+  hipEvent_t ev[6];
+  for (auto& x : ev) DH_HIP(hipEventCreate(&x));
+  DH_HIP(hipEventRecord(ev[0], s));
+  DH_TRY(gamma_times(c, dqa, nullptr));
+  DH_TRY(launch_shift(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.h0.p, c.ov.p, c.vorb.p,
+                      c.hs.p));
+  DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p, n,
+                               n));
+  DH_HIP(hipEventRecord(ev[1], s));
+  int info = 0;
+  double evt[3] = {0, 0, 0};
+  DH_TRY(c.eig.solve(s, c.Hd.p, n, c.eigv, &info, evt));
+  if (info != 0) return info;
+  const double th = now();
+  DH_TRY(fermi_filling(c.eigv, st.nel, o.kT, c.fill, st.fr));
+  int nocc = 0;
+  for (int i = 0; i < n; ++i)
+    if (c.fill[i] > 0.0) nocc = i + 1;
+  c.nocc = nocc;
+  c.times[kTHost] += now() - th;
+  DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  DH_HIP(hipEventRecord(ev[2], s));
+  DH_TRY(c.eig.vectors(s, nocc, c.X.p, n));
+  DH_HIP(hipEventRecord(ev[3], s));
+  DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
+                      kKFull, c.f.p));
+  DH_HIP(hipEventRecord(ev[4], s));
+  DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n, c.rho.p));
+  DH_TRY(launch_mulliken(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.h0.p, c.rho.p,
+                         c.qorb.p, c.eorb.p, n));
+  DH_HIP(hipMemcpyAsync(qorbH.data(), c.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(eorbH.data(), c.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipEventRecord(ev[5], s));
+  DH_HIP(hipStreamSynchronize(s));
+  float ms;
+  DH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  c.times[kTHam] += ms * 1e-3;
+  DH_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+  c.times[kTBack] += ms * 1e-3;
+  DH_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
+  c.times[kTDM] += ms * 1e-3;
+  DH_HIP(hipEventElapsedTime(&ms, ev[4], ev[5]));
+  c.times[kTPackMull] += ms * 1e-3;
+  c.times[kTReduce] += evt[0];
+  c.times[kTTridiag] += evt[1];
+  c.times[kTDC] += evt[2];
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  const double th2 = now();
+  std::vector<double> qout(nat, 0.0);
+  st.eNonScc = 0.0;
+  for (int a = 0; a < nat; ++a)
+    for (long oi = c.atomStart[a]; oi < c.atomStart[a + 1]; ++oi) qout[a] += qorbH[oi];
+  for (int i = 0; i < n; ++i) st.eNonScc += eorbH[i];
+  std::vector<double> dqo(nat), diff(nat);
+  double err = 0.0;
+  for (int a = 0; a < nat; ++a) {
+    dqo[a] = qout[a] - c.q0atom[a];
+    diff[a] = qout[a] - st.qin[a];
+    err = std::max(err, std::fabs(diff[a]));
   }
-  if (it > o.max_scc_iter) it = o.max_scc_iter;
-  c.times[kTReduce] = ev_t[0];
-  c.times[kTTridiag] = ev_t[1];
-  c.times[kTDC] = ev_t[2];
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(e2);
-  (void)hipEventDestroy(e3);
-  // repulsive energy
+  c.times[kTHost] += now() - th2;
+  DH_TRY(gamma_times(c, dqo, &vout));
+  st.eScc = 0.0;
+  for (int a = 0; a < nat; ++a) st.eScc += 0.5 * vout[a] * dqo[a];
+  c.qout = qout;
+  c.dqout = dqo;
+  st.err = err;
+  st.conv = err < o.scc_tol && st.it >= o.min_scc_iter;
+  const double th3 = now();
+  if (!st.conv && st.it != o.max_scc_iter) st.mixer.mix(st.qin, diff);
+  c.times[kTHost] += now() - th3;
+  c.times[kTIter] = now() - tIt;
+  c.sccDone = true;   // a density exists (gradients need convergence for physical meaning)
+  return 0;
+}
+
+static int ctx_scc_end(Context& c, double* energies, double* charges, int* niter, int* converged) {
+  DH_CHECK(c.st.it > 0, "scc_end: no SCC iteration was run");
+  hipStream_t s = c.s;
+  const int nat = c.nat;
+  SccState& st = c.st;
   DH_TRY(fill<double>(s, c.eatom.p, nat, 0.0));
   DH_TRY(fill<double>(s, c.grad.p, 3 * (size_t)nat, 0.0));
   DH_TRY(launch_repulsive(s, c.nbdev, c.coordsAll.p, c.sk.dreps.p, c.sk.nsp, c.eatom.p, c.grad.p));
@@ -1325,23 +1341,29 @@ static int ctx_scc(Context& c, double* energies, double* charges, int* niter, in
   DH_HIP(hipStreamSynchronize(s));
   double erep = 0.0;
   for (double x : ea) erep += x;
-  const double etot = eNonScc + eScc + erep;
-  c.energies[0] = etot - fr.ts;
+  const double etot = st.eNonScc + st.eScc + erep;
+  c.energies[0] = etot - st.fr.ts;
   c.energies[1] = etot;
-  c.energies[2] = eNonScc;
-  c.energies[3] = eScc;
+  c.energies[2] = st.eNonScc;
+  c.energies[3] = st.eScc;
   c.energies[4] = erep;
-  c.energies[5] = fr.ts;
-  c.energies[6] = fr.eband;
-  c.energies[7] = fr.ef;
+  c.energies[5] = st.fr.ts;
+  c.energies[6] = st.fr.eband;
+  c.energies[7] = st.fr.ef;
   if (energies) std::copy(c.energies, c.energies + 8, energies);
   if (charges)
     for (int a = 0; a < nat; ++a) charges[a] = c.q0atom[a] - c.qout[a];
-  if (niter) *niter = it;
-  if (converged) *converged = conv ? 1 : 0;
-  c.sccDone = true;
+  if (niter) *niter = st.it;
+  if (converged) *converged = st.conv ? 1 : 0;
+  st.active = false;
   ++c.ncalls;
   return 0;
+}
+
+static int ctx_scc(Context& c, double* energies, double* charges, int* niter, int* converged) {
+  DH_TRY(ctx_scc_begin(c));
+  while (!c.st.conv && c.st.it < c.opt.max_scc_iter) DH_TRY(ctx_scc_step(c));
+  return ctx_scc_end(c, energies, charges, niter, converged);
 }
 
 static int ctx_gradients(Context& c, double* grad) {
@@ -1422,6 +1444,25 @@ extern "C" int dftbhip_ctx_scc(dftbhip_ctx* ctx, double* energies, double* charg
   return ctx_scc(ctx->c, energies, charges, niter, converged);
 }
 
+extern "C" int dftbhip_ctx_scc_begin(dftbhip_ctx* ctx) {
+  DH_CHECK(ctx && ctx->c.nat > 0, "scc_begin: no geometry");
+  return ctx_scc_begin(ctx->c);
+}
+
+extern "C" int dftbhip_ctx_scc_step(dftbhip_ctx* ctx, int* converged, double* qerr) {
+  DH_CHECK(ctx && ctx->c.nat > 0, "scc_step: no geometry");
+  DH_TRY(ctx_scc_step(ctx->c));
+  if (converged) *converged = ctx->c.st.conv ? 1 : 0;
+  if (qerr) *qerr = ctx->c.st.err;
+  return 0;
+}
+
+extern "C" int dftbhip_ctx_scc_end(dftbhip_ctx* ctx, double* energies, double* charges,
+                                   int* niter, int* converged) {
+  DH_CHECK(ctx, "scc_end: null context");
+  return ctx_scc_end(ctx->c, energies, charges, niter, converged);
+}
+
 extern "C" int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad) {
   DH_CHECK(ctx && grad, "gradients: null pointer");
   return ctx_gradients(ctx->c, grad);
@@ -1495,5 +1536,16 @@ extern "C" int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz) {
   DH_HIP(hipMemcpy2DAsync(z, ldz * sizeof(double), Z.p, n * sizeof(double), n * sizeof(double), n,
                           hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_kernel_profile(int enable) {
+  symv_profiling(enable != 0);
+  return 0;
+}
+
+extern "C" int dftbhip_kernel_stats(double* out) {
+  DH_CHECK(out, "kernel_stats: null pointer");
+  symv_profile_stats(out);
   return 0;
 }

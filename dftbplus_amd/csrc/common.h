@@ -148,6 +148,9 @@ int zero_strict_upper(hipStream_t s, double* a, long lda, int n);
 int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e, double* tau,
                 double* work, size_t workSize);
 size_t sytrd_work_size(int n);
+// per-launch timing of the tridiagonalisation's symmetric product: launches, seconds, bytes
+void symv_profiling(bool on);
+void symv_profile_stats(double* out);
 int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* tau, double* z,
                 long ldz, int ncols, double* work, size_t workSize);
 size_t ormtr_work_size(int n, int ncols);

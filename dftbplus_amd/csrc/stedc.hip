@@ -332,16 +332,18 @@ struct Merge {
   int s, n1, n2;
 };
 
-static void build_tree(int s, int n, int depth, std::vector<Merge>* levels, int& maxDepth,
-                       std::vector<int>& ls, std::vector<int>& lz) {
-  if (n <= kLeaf) {
+// Every branch is split to the same depth, so each tree level covers the whole matrix and the
+// two Q buffers alternate consistently between levels.
+static void build_tree(int s, int n, int depth, int leafDepth, std::vector<Merge>* levels,
+                       int& maxDepth, std::vector<int>& ls, std::vector<int>& lz) {
+  if (depth == leafDepth) {
     ls.push_back(s);
     lz.push_back(n);
     return;
   }
   const int n1 = n / 2;
-  build_tree(s, n1, depth + 1, levels, maxDepth, ls, lz);
-  build_tree(s + n1, n - n1, depth + 1, levels, maxDepth, ls, lz);
+  build_tree(s, n1, depth + 1, leafDepth, levels, maxDepth, ls, lz);
+  build_tree(s + n1, n - n1, depth + 1, leafDepth, levels, maxDepth, ls, lz);
   levels[depth].push_back({s, n1, n - n1});
   maxDepth = std::max(maxDepth, depth);
 }
@@ -367,7 +369,9 @@ int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double
   std::vector<Merge> levels[64];
   int maxDepth = -1;
   std::vector<int> ls, lz;
-  build_tree(0, n, 0, levels, maxDepth, ls, lz);
+  int leafDepth = 0;
+  while (((n + (1 << leafDepth) - 1) >> leafDepth) > kLeaf) ++leafDepth;
+  build_tree(0, n, 0, leafDepth, levels, maxDepth, ls, lz);
   // tearing
   for (int L = 0; L <= maxDepth; ++L)
     for (const Merge& mg : levels[L]) {

@@ -16,6 +16,43 @@
 namespace dftbhip {
 
 constexpr int kNB = 32;      // panel width
+
+// Optional per-launch timing of the symmetric product kernel (HIP events on the launch stream):
+// launches, summed duration and the algorithmic bytes (lower triangle of the trailing matrix).
+struct SymvProfile {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<double> bytes;
+  size_t used = 0;
+  double launches = 0, seconds = 0, algBytes = 0;
+};
+static SymvProfile g_symvProf;
+
+void symv_profiling(bool on) {
+  g_symvProf.on = on;
+  g_symvProf.launches = g_symvProf.seconds = g_symvProf.algBytes = 0;
+}
+
+void symv_profile_stats(double* out) {
+  out[0] = g_symvProf.launches;
+  out[1] = g_symvProf.seconds;
+  out[2] = g_symvProf.algBytes;
+}
+
+static int symv_prof_collect(hipStream_t s) {
+  SymvProfile& p = g_symvProf;
+  if (!p.on || p.used == 0) return 0;
+  DH_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < p.used; ++i) {
+    float ms = 0.f;
+    DH_HIP(hipEventElapsedTime(&ms, p.ev[2 * i], p.ev[2 * i + 1]));
+    p.seconds += ms * 1e-3;
+    p.algBytes += p.bytes[i];
+    p.launches += 1;
+  }
+  p.used = 0;
+  return 0;
+}
 constexpr int kTB = 64;      // SYMV tile
 constexpr int kCH = 8;       // column tiles per SYMV workgroup
 
@@ -296,9 +333,25 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
       if (m <= 0) continue;
       const int nRB = (m + kTB - 1) / kTB;
       const int nCh = (nRB + kCH - 1) / kCH;
+      SymvProfile& pf = g_symvProf;
+      if (pf.on) {
+        if (pf.ev.size() < 2 * (pf.used + 1)) {
+          const size_t old = pf.ev.size();
+          pf.ev.resize(2 * (pf.used + 1) + 2 * 1024);
+          for (size_t q = old; q < pf.ev.size(); ++q) DH_HIP(hipEventCreate(&pf.ev[q]));
+          pf.bytes.resize(pf.ev.size() / 2);
+        }
+        DH_HIP(hipEventRecord(pf.ev[2 * pf.used], s));
+      }
       hipLaunchKernelGGL(symv_kernel, dim3(nRB, nCh), dim3(256), 0, s, a, lda, n, c, j0, ws.W,
                          ldw, ws.nrm2, ws.Y, ws.Z, ws.vbuf, ws.t12, ws.scal, e, tau);
       DH_LAUNCH_CHECK();
+      if (pf.on) {
+        DH_HIP(hipEventRecord(pf.ev[2 * pf.used + 1], s));
+        // lower triangle of the m x m trailing matrix plus v in and the partial sums out
+        pf.bytes[pf.used] = 8.0 * ((double)m * (m + 1) / 2 + 2.0 * m);
+        ++pf.used;
+      }
       hipLaunchKernelGGL(finalize_w_kernel, dim3(cdiv(m, 256)), dim3(256), 0, s, a, lda, n, c, j0,
                          ws.W, ldw, ws.Y, ws.Z, ws.vbuf, ws.t12, tau, ws.dot);
       DH_LAUNCH_CHECK();
@@ -319,6 +372,7 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
     hipLaunchKernelGGL(panel_restore_kernel, dim3(1), dim3(64), 0, s, a, lda, n, j0, pw, d, e);
     DH_LAUNCH_CHECK();
   }
+  DH_TRY(symv_prof_collect(s));
   return 0;
 }
 

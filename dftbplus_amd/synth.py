@@ -149,11 +149,11 @@ def _assign(rng, n, fractions):
     return rng.permutation(lab)
 
 
-def cluster_216(seed: int = 0) -> Tuple[Geometry, List[str]]:
+def cluster_216(seed: int = 0, nside: int = 6) -> Tuple[Geometry, List[str]]:
     """216-atom C/H/O cluster: jittered 6x6x6 cubic lattice, 2.6 bohr, +-0.3 bohr noise;
-    0.45 H, 0.35 C, 0.20 O."""
+    0.45 H, 0.35 C, 0.20 O (``nside`` gives smaller clusters of the same recipe)."""
     rng = np.random.default_rng(seed)
-    g = np.arange(6) * 2.6
+    g = np.arange(nside) * 2.6
     pos = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T
     pos = pos + rng.uniform(-0.3, 0.3, pos.shape)
     elements = ["H", "C", "O"]
@@ -268,6 +268,10 @@ def workload(name: str):
     """(geometry, parameters, options) of a named workload."""
     if name == "cluster216":
         geo, el = cluster_216(0)
+        params, _ = make_parameters(el, 100)
+        return geo, params, Options(kT=1.0e-8, scc_tol=1.0e-8)
+    if name == "cluster27":
+        geo, el = cluster_216(0, 3)
         params, _ = make_parameters(el, 100)
         return geo, params, Options(kT=1.0e-8, scc_tol=1.0e-8)
     if name.startswith("diamond"):

@@ -121,12 +121,26 @@ int dftbhip_ctx_set_geometry(dftbhip_ctx* ctx, int nAtom, const double* coords,
 /* energies: mermin, total, nonscc, scc, repulsive, TS, Eband, Ef; gross charges q0 - q */
 int dftbhip_ctx_scc(dftbhip_ctx* ctx, double* energies, double* charges, int* niter,
                     int* converged);
+/* the same cycle one iteration at a time: begin (input charges = reference charges), step
+ * (one full iteration incl. mixing; converged flag and max |q_out - q_in|), end (energies and
+ * charges of the last iteration, as dftbhip_ctx_scc) */
+int dftbhip_ctx_scc_begin(dftbhip_ctx* ctx);
+int dftbhip_ctx_scc_step(dftbhip_ctx* ctx, int* converged, double* qerr);
+int dftbhip_ctx_scc_end(dftbhip_ctx* ctx, double* energies, double* charges, int* niter,
+                        int* converged);
 /* gradient dE/dR (3, nAtom); needs a converged dftbhip_ctx_scc */
 int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad);
 int dftbhip_ctx_eigenvalues(dftbhip_ctx* ctx, double* eig, double* fill, int n);
 int dftbhip_ctx_nr_orbitals(dftbhip_ctx* ctx);
 /* per-phase wall times (s) of the last SCC/gradient call and the number of calls */
 int dftbhip_ctx_timings(dftbhip_ctx* ctx, double* t, int n);
+
+/* ---- measurement: per-launch HIP-event timing of the tridiagonalisation's symmetric
+ *      product (the HBM-bound kernel of the eigensolver). stats: out[0] launches, out[1] summed
+ *      duration (s), out[2] algorithmic bytes (lower triangle of each trailing matrix).
+ *      (no counterpart in the reference) */
+int dftbhip_kernel_profile(int enable);
+int dftbhip_kernel_stats(double* out);
 
 /* ---- diagnostics: accumulator lane layouts of the FP64 and FP32 16x16x4 MFMAs (one wave
  *      each), out[512]: FP64 in out[0:256], FP32 in out[256:512] ----------------------------
