@@ -120,7 +120,7 @@ struct GemmDesc {
 template <typename T>
 int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T* A, long lda,
          const T* B, long ldb, T beta, T* C, long ldc, bool lowerOut = false,
-         GemmKLimit klim = kKFull, const T* w = nullptr);
+         GemmKLimit klim = kKFull, const T* w = nullptr, int splitK = 1);
 
 // Batched GEMM over device-resident descriptors; maxM/maxN bound the tile grid.
 template <typename T>

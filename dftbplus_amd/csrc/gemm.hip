@@ -42,10 +42,14 @@ __device__ __forceinline__ void gemm_tile(int tm, int tn, int M, int N, int K, T
                                           const T* __restrict__ A, long lda,
                                           const T* __restrict__ B, long ldb, T beta,
                                           T* __restrict__ C, long ldc, int klim,
-                                          const T* __restrict__ w) {
+                                          const T* __restrict__ w, int kz = 0, int nkz = 1) {
   using acc_t = typename Mfma<T>::acc_t;
-  __shared__ T As[kBK][kBM + kPad];
-  __shared__ T Bs[kBK][kBN + kPad];
+  // LDS: A and B staging during the K loop, reused for the output staging afterwards
+  constexpr int kStage = 2 * kBK * (kBM + kPad);
+  constexpr int kOut = 4 * 16 * (64 + 1);
+  __shared__ T smem[kStage > kOut ? kStage : kOut];
+  T (*As)[kBM + kPad] = reinterpret_cast<T (*)[kBM + kPad]>(smem);
+  T (*Bs)[kBN + kPad] = reinterpret_cast<T (*)[kBN + kPad]>(smem + kBK * (kBM + kPad));
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -55,6 +59,14 @@ __device__ __forceinline__ void gemm_tile(int tm, int tn, int M, int N, int K, T
   else if (klim == kKAUpper) kbeg = min(K, i0);
   else if (klim == kKBUpper) kend = min(K, j0 + kBN);
   else if (klim == kKBLower) kbeg = min(K, j0);
+  // split-K: slice kz of nkz (slices are multiples of the K step); partial sums added atomically
+  if (nkz > 1) {
+    const int span = kend - kbeg;
+    const int chunk = ((span + nkz - 1) / nkz + kBK - 1) / kBK * kBK;
+    const int b = kbeg + kz * chunk;
+    kend = min(kend, b + chunk);
+    kbeg = b;
+  }
 
   acc_t acc[4][4];
 #pragma unroll
@@ -129,22 +141,39 @@ __device__ __forceinline__ void gemm_tile(int tm, int tn, int M, int N, int K, T
   }
   // Output layout of the 16x16 accumulator, column l%16 for lane l; element r sits in row
   // l/16 + 4r for FP64 and in row 4(l/16) + r for FP32 (pinned by the layout probes below).
+  // Each wave stages 16 columns x 64 rows of its sub-tile in LDS and writes them back one
+  // column per store instruction (64 consecutive rows).
+  __syncthreads();
+  T* Cs = smem + wave * 16 * 65;
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int n = 0; n < 4; ++n) {
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      const int gj = j0 + wn * 64 + n * 16 + (lane & 15);
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int rr = Mfma<T>::kRowMajorQuad ? (lane >> 4) * 4 + r : (lane >> 4) + 4 * r;
-        const int gi = i0 + wm * 64 + m * 16 + rr;
-        if (gi < M && gj < N) {
-          T v = alpha * acc[m][n][r];
-          if (beta != T(0)) v += beta * C[gi + (long)gj * ldc];
-          C[gi + (long)gj * ldc] = v;
+        Cs[(lane & 15) * 65 + m * 16 + rr] = acc[m][n][r];
+      }
+    __syncthreads();
+    const int gi = i0 + wm * 64 + lane;
+    if (gi < M) {
+#pragma unroll 4
+      for (int cc = 0; cc < 16; ++cc) {
+        const int gj = j0 + wn * 64 + n * 16 + cc;
+        if (gj < N) {
+          T v = alpha * Cs[cc * 65 + lane];
+          T* dst = C + gi + (long)gj * ldc;
+          if (nkz > 1) {
+            atomicAdd(dst, v);
+          } else {
+            if (beta != T(0)) v += beta * *dst;
+            *dst = v;
+          }
         }
       }
     }
+    __syncthreads();
+  }
 }
 
 __device__ __forceinline__ void lower_tile_index(int x, int& tm, int& tn) {
@@ -166,7 +195,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
     tm = blockIdx.x;
     tn = blockIdx.y;
   }
-  gemm_tile<T, TA, TB>(tm, tn, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, klim, w);
+  gemm_tile<T, TA, TB>(tm, tn, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, klim, w,
+                       (int)blockIdx.z, (int)gridDim.z);
 }
 
 template <typename T, bool TA, bool TB>
@@ -182,15 +212,21 @@ __global__ __launch_bounds__(256) void gemm_batched_kernel(const GemmDesc<T>* de
 template <typename T>
 int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T* A, long lda,
          const T* B, long ldb, T beta, T* C, long ldc, bool lowerOut, GemmKLimit klim,
-         const T* w) {
+         const T* w, int splitK) {
   if (M <= 0 || N <= 0) return 0;
   DH_CHECK(!lowerOut || M == N, "gemm: lower output needs a square C");
+  if (splitK > 1) {
+    DH_CHECK(beta == T(0) && !lowerOut, "gemm: split-K needs beta = 0 and a full output");
+    splitK = std::min<int>(splitK, std::max(1, K / kBK));
+    DH_TRY(fill2d<T>(s, C, ldc, M, N, T(0)));
+  }
+  splitK = std::max(splitK, 1);
   dim3 grid;
   if (lowerOut) {
     const long t = cdiv(M, kBM);
     grid = dim3((unsigned)(t * (t + 1) / 2), 1, 1);
   } else {
-    grid = dim3(cdiv(M, kBM), cdiv(N, kBN), 1);
+    grid = dim3(cdiv(M, kBM), cdiv(N, kBN), splitK);
   }
 #define DH_GEMM_CASE(TA_, TB_)                                                           \
   hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(kNT), 0, s, M, N, K, alpha, A, \
@@ -223,10 +259,10 @@ int gemm_batched(hipStream_t s, bool ta, bool tb, const GemmDesc<T>* dDesc, int 
 
 template int gemm<double>(hipStream_t, bool, bool, int, int, int, double, const double*, long,
                           const double*, long, double, double*, long, bool, GemmKLimit,
-                          const double*);
+                          const double*, int);
 template int gemm<float>(hipStream_t, bool, bool, int, int, int, float, const float*, long,
                          const float*, long, float, float*, long, bool, GemmKLimit,
-                         const float*);
+                         const float*, int);
 template int gemm_batched<double>(hipStream_t, bool, bool, const GemmDesc<double>*, int, int, int,
                                   double, double);
 template int gemm_batched<float>(hipStream_t, bool, bool, const GemmDesc<float>*, int, int, int,

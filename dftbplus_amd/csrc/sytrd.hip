@@ -58,20 +58,23 @@ constexpr int kCH = 8;       // column tiles per SYMV workgroup
 
 struct SytrdWs {
   double* W;       // n x kNB
-  double* Y;       // maxChunk x n   row-block partial sums
-  double* Z;       // nRB x n        transposed partial sums
+  double* Y;       // n x nSlot   partial sums of A22 v, one row of slots per matrix row
   double* vbuf;    // n
   double* nrm2;    // n   ||x||^2 per column
-  double* dot;     // n   w.v per column
+  double* dot;     // n   w.v per column (before the rank-one correction of w)
   double* t12;     // n x 2kNB
   double* scal;    // n
   double* P;       // n x 2kNB
   double* Q;       // n x 2kNB
+  int nSlot;
 };
 
+static int sytrd_slots(int n) {
+  const int nRB = (n + kTB - 1) / kTB;
+  return nRB + (nRB + kCH - 1) / kCH;
+}
+
 static size_t sytrd_layout(int n, SytrdWs* ws, double* base) {
-  const size_t nRB = (n + kTB - 1) / kTB;
-  const size_t nCh = (nRB + kCH - 1) / kCH;
   size_t off = 0;
   auto take = [&](double** p, size_t cnt) {
     if (ws) *p = base + off;
@@ -79,9 +82,9 @@ static size_t sytrd_layout(int n, SytrdWs* ws, double* base) {
   };
   SytrdWs dummy;
   SytrdWs* w = ws ? ws : &dummy;
+  w->nSlot = sytrd_slots(n);
   take(&w->W, (size_t)n * kNB);
-  take(&w->Y, nCh * n);
-  take(&w->Z, nRB * n);
+  take(&w->Y, (size_t)n * w->nSlot);
   take(&w->vbuf, n);
   take(&w->nrm2, n);
   take(&w->dot, n);
@@ -94,9 +97,14 @@ static size_t sytrd_layout(int n, SytrdWs* ws, double* base) {
 
 size_t sytrd_work_size(int n) { return sytrd_layout(n, nullptr, nullptr); }
 
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
 __device__ __forceinline__ double block_reduce_sum(double v, double* sh) {
   // 256 threads
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  v = wave_sum(v);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) sh[wv] = v;
   __syncthreads();
@@ -123,23 +131,35 @@ __device__ __forceinline__ void reflector(double alpha, double xn2, double& beta
   scal = 1.0 / (alpha - beta);
 }
 
+// W stores w before its rank-one correction: the true column p of the panel is
+// W(:,p) + alpha_p V(:,p) with alpha_p = -tau_p (w_p . v_p) / 2, applied by every reader.
+__device__ __forceinline__ double wcorr(int cp, const double* tau, const double* dot) {
+  return -0.5 * tau[cp] * dot[cp];
+}
+
 // A(r,c) -= sum_p V(r,p) W(c,p) + W(r,p) V(c,p), rows r >= c; accumulate ||A(c+2:n,c)||^2.
 __global__ __launch_bounds__(256) void col_update_kernel(double* a, long lda, int n, int c, int j0,
                                                          const double* W, long ldw,
+                                                         const double* tau, const double* dot,
                                                          double* nrm2) {
-  __shared__ double vc[kNB], wc[kNB], sh[4];
+  __shared__ double vc[kNB], wc[kNB], al[kNB], sh[4];
   const int i = c - j0;
   if (threadIdx.x < i) {
-    vc[threadIdx.x] = a[c + (long)(j0 + threadIdx.x) * lda];
-    wc[threadIdx.x] = W[c + (long)threadIdx.x * ldw];
+    const int p = threadIdx.x;
+    al[p] = wcorr(j0 + p, tau, dot);
+    vc[p] = a[c + (long)(j0 + p) * lda];
+    wc[p] = W[c + (long)p * ldw] + al[p] * vc[p];
   }
   __syncthreads();
   const int r = c + blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0;
   if (r < n) {
     double v = a[r + (long)c * lda];
-    for (int p = 0; p < i; ++p)
-      v -= a[r + (long)(j0 + p) * lda] * wc[p] + W[r + (long)p * ldw] * vc[p];
+    for (int p = 0; p < i; ++p) {
+      const double vr = a[r + (long)(j0 + p) * lda];
+      const double wr = W[r + (long)p * ldw] + al[p] * vr;
+      v -= vr * wc[p] + wr * vc[p];
+    }
     a[r + (long)c * lda] = v;
     if (r >= c + 2) sq = v * v;
   }
@@ -147,155 +167,186 @@ __global__ __launch_bounds__(256) void col_update_kernel(double* a, long lda, in
   if (threadIdx.x == 0 && tot != 0.0) atomicAdd(&nrm2[c], tot);
 }
 
-// Partial sums of y = A22 v (A22 = a(c+1:n, c+1:n), lower) and of t1 = W^T v, t2 = V^T v.
+// Partial sums of y = A22 v (A22 = a(c+1:n, c+1:n), lower triangle read once).
+// Workgroup (J, chunk) owns column block J and row blocks J + chunk*kCH ... (kCH of them):
+//   y partial of row block I  -> Y(row, J)            (one per tile, reduced over 4 waves)
+//   transposed sums (A^T v)_J -> Y(col, nRB + chunk)  (accumulated in registers over the tiles)
+// Thread (ii = tid % 64, cg = tid / 64) holds rows ii of 16 columns cg*16 .. cg*16+15.
+// The chunk-0 workgroup of each column block publishes v and the panel products t1, t2.
 __global__ __launch_bounds__(256) void symv_kernel(const double* a, long lda, int n, int c, int j0,
                                                    const double* W, long ldw, const double* nrm2,
-                                                   double* Y, double* Z, double* vbuf,
+                                                   const double* tauA, const double* dotA,
+                                                   double* Y, int nSlot, double* vbuf,
                                                    double* t12, double* scalOut, double* eOut,
                                                    double* tauOut) {
-  __shared__ double T[kTB][kTB + 1];    // T[jj][ii]
-  __shared__ double vI[kTB], vJ[kTB];
-  __shared__ double part[4][kTB];
+  __shared__ double vJs[kTB];
+  __shared__ double red[2][4][kTB];
   const int m = n - c - 1;
-  const int I = blockIdx.x, ch = blockIdx.y;
-  const int jb0 = ch * kCH;
-  if (jb0 > I) return;
-  const int jb1 = min(jb0 + kCH, I + 1);
+  const int nRB = (m + kTB - 1) / kTB;
+  const int J = blockIdx.x, ch = blockIdx.y;
+  const int Ib = J + ch * kCH;
+  if (J >= nRB || Ib >= nRB) return;
+  const int Ie = min(Ib + kCH, nRB);
   const int tid = threadIdx.x;
+  const int ii = tid & 63, cg = tid >> 6;
   const double alpha = a[(c + 1) + (long)c * lda];
   double beta, tau, scal;
   reflector(alpha, nrm2[c], beta, tau, scal);
-  if (I == 0 && ch == 0 && tid == 0) {
+  if (J == 0 && ch == 0 && tid == 0) {
     scalOut[c] = scal;
     eOut[c] = beta;
     tauOut[c] = tau;
   }
-  auto vload = [&](int blk, double* dst) {
-    if (tid < kTB) {
-      const int k = blk * kTB + tid;
-      double v = 0.0;
-      if (k < m) v = (k == 0) ? 1.0 : a[(c + 1 + k) + (long)c * lda] * scal;
-      dst[tid] = v;
+  auto vval = [&](int k) -> double {
+    if (k >= m) return 0.0;
+    return (k == 0) ? 1.0 : a[(c + 1 + k) + (long)c * lda] * scal;
+  };
+  if (tid < kTB) vJs[tid] = vval(J * kTB + tid);
+  __syncthreads();
+  const int cols0 = c + 1 + J * kTB;
+  if (ch == 0) {
+    if (tid < kTB && J * kTB + tid < m) vbuf[cols0 + tid] = vJs[tid];
+    const int i = c - j0;
+    for (int p = cg; p < 2 * i; p += 4) {
+      double prod = 0.0;
+      if (J * kTB + ii < m) {
+        const int r = cols0 + ii;
+        double x;
+        if (p < i) {
+          const double vr = a[r + (long)(j0 + p) * lda];
+          x = W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dotA) * vr;
+        } else {
+          x = a[r + (long)(j0 + p - i) * lda];
+        }
+        prod = x * vJs[ii];
+      }
+      prod = wave_sum(prod);
+      if (ii == 0 && prod != 0.0) atomicAdd(&t12[(long)c * 2 * kNB + p], prod);
+    }
+  }
+  double vj[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) vj[e] = vJs[cg * 16 + e];
+  double b0[16], b1[16], b2[16], zacc[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) zacc[e] = 0.0;
+  auto load = [&](int I, double* dst) {
+    const int gr = I * kTB + ii;
+    const bool rowOk = gr < m;
+    const double* src = a + (c + 1 + gr) + (long)cols0 * lda;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int jj = cg * 16 + e;
+      const int gc = J * kTB + jj;
+      dst[e] = (rowOk && gc < m && (I > J || ii >= jj)) ? src[(long)jj * lda] : 0.0;
     }
   };
-  vload(I, vI);
-  const int rows0 = c + 1 + I * kTB;
-  // the ch == 0 workgroup of each row block publishes v and the panel products t1, t2
-  if (ch == 0) {
-    __syncthreads();
-    if (tid < kTB && I * kTB + tid < m) vbuf[rows0 + tid] = vI[tid];
-    const int i = c - j0;
-    // t1[p] = sum_r W(r,p) v(r), t2[p] = sum_r V(r,p) v(r) over this row block
-    for (int p = tid / kTB; p < 2 * i; p += 4) {
-      const int rr = tid % kTB;
-      double prod = 0.0;
-      if (I * kTB + rr < m) {
-        const int r = rows0 + rr;
-        const double x = (p < i) ? W[r + (long)p * ldw] : a[r + (long)(j0 + p - i) * lda];
-        prod = x * vI[rr];
+  // one tile: row partials (reduced over the 4 waves through LDS) and transposed sums
+  auto tile = [&](int I, const double* t) {
+    const double vi = vval(I * kTB + ii);
+    double yp = 0.0;
+    if (I > J) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        yp += t[e] * vj[e];
+        zacc[e] += t[e] * vi;
       }
-      for (int o = 32; o > 0; o >>= 1) prod += __shfl_down(prod, o, 64);
-      if ((tid & 63) == 0 && prod != 0.0) atomicAdd(&t12[(long)c * 2 * kNB + p], prod);
-    }
-  }
-  double yacc = 0.0;   // row ii = tid & 63, quarter q = tid >> 6
-  const int ii = tid & 63, q = tid >> 6;
-  for (int J = jb0; J < jb1; ++J) {
-    __syncthreads();
-    vload(J, vJ);
-    const int cols0 = c + 1 + J * kTB;
-#pragma unroll 4
-    for (int e = 0; e < 16; ++e) {
-      const int jj = q * 16 + e;
-      const int gr = I * kTB + ii, gc = J * kTB + jj;
-      double v = 0.0;
-      if (gr < m && gc < m && (I > J || ii >= jj)) v = a[(rows0 + ii) + (long)(cols0 + jj) * lda];
-      T[jj][ii] = v;
-    }
-    __syncthreads();
-    // row sums over this thread's quarter of columns
+    } else {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) yacc += T[q * 16 + e][ii] * vJ[q * 16 + e];
-    // column sums (transposed product) over this thread's quarter of rows
-    const int jj = tid & 63;
-    double zc = 0.0;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int r = q * 16 + e;
-      if (I > J || r > jj) zc += T[jj][r] * vI[r];
+      for (int e = 0; e < 16; ++e) {
+        yp += t[e] * vj[e];
+        if (ii > cg * 16 + e) zacc[e] += t[e] * vi;
+      }
     }
-    part[q][jj] = zc;
+    red[I & 1][cg][ii] = yp;
     __syncthreads();
-    if (tid < kTB && J * kTB + tid < m)
-      Z[(long)I * n + cols0 + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    if (cg == 0) {
+      const int k = I * kTB + ii;
+      if (k < m)
+        Y[(long)k * nSlot + J] = red[I & 1][0][ii] + red[I & 1][1][ii] + red[I & 1][2][ii] +
+                                 red[I & 1][3][ii];
+    }
+  };
+  // two tiles in flight ahead of the one being reduced
+  load(Ib, b0);
+  if (Ib + 1 < Ie) load(Ib + 1, b1);
+  for (int I = Ib; I < Ie; I += 3) {
+    if (I + 2 < Ie) load(I + 2, b2);
+    tile(I, b0);
+    if (I + 1 >= Ie) break;
+    if (I + 3 < Ie) load(I + 3, b0);
+    tile(I + 1, b1);
+    if (I + 2 >= Ie) break;
+    if (I + 4 < Ie) load(I + 4, b1);
+    tile(I + 2, b2);
   }
-  __syncthreads();
-  part[q][ii] = yacc;
-  __syncthreads();
-  if (tid < kTB && I * kTB + tid < m)
-    Y[(long)ch * n + rows0 + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const double z = wave_sum(zacc[e]);
+    const int k = J * kTB + cg * 16 + e;
+    if (ii == 0 && k < m) Y[(long)k * nSlot + nRB + ch] = z;
+  }
 }
 
-// w = tau * (y - V t1 - W t2) with y reduced from the partial sums; store v into column c.
+// y(r) from its slots (one wave per row), w = tau * (y - V t1 - W t2); store v into column c
+// and accumulate w.v. 16 rows per 256-thread block.
 __global__ __launch_bounds__(256) void finalize_w_kernel(double* a, long lda, int n, int c, int j0,
                                                          double* W, long ldw, const double* Y,
-                                                         const double* Z, const double* vbuf,
+                                                         int nSlot, const double* vbuf,
                                                          const double* t12, const double* tauA,
                                                          double* dot) {
-  __shared__ double t1[kNB], t2[kNB], sh[4];
+  __shared__ double sh[16];
+  const int m = n - c - 1;
+  const int nRB = (m + kTB - 1) / kTB;
   const int i = c - j0;
-  if (threadIdx.x < i) {
-    t1[threadIdx.x] = t12[(long)c * 2 * kNB + threadIdx.x];
-    t2[threadIdx.x] = t12[(long)c * 2 * kNB + i + threadIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double tau = tauA[c];
+  for (int q = 0; q < 4; ++q) {
+    const int k = blockIdx.x * 16 + wv * 4 + q;
+    double prod = 0.0;
+    if (k < m) {
+      const int r = c + 1 + k;
+      const int Ir = k / kTB;
+      const int nch = (nRB - Ir + kCH - 1) / kCH;
+      const double* yr = Y + (long)k * nSlot;
+      double acc = 0.0;
+      for (int sl = lane; sl <= Ir; sl += 64) acc += yr[sl];
+      for (int sl = lane; sl < nch; sl += 64) acc += yr[nRB + sl];
+      if (lane < i) {
+        const int p = lane;
+        const double vr = a[r + (long)(j0 + p) * lda];
+        const double wr = W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dot) * vr;
+        acc -= vr * t12[(long)c * 2 * kNB + p] + wr * t12[(long)c * 2 * kNB + i + p];
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) {
+        const double w = tau * acc;
+        W[r + (long)i * ldw] = w;
+        const double v = vbuf[r];
+        a[r + (long)c * lda] = v;
+        prod = w * v;
+      }
+    }
+    if (lane == 0) sh[wv * 4 + q] = prod;
   }
   __syncthreads();
-  const int m = n - c - 1;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double prod = 0.0;
-  if (k < m) {
-    const int r = c + 1 + k;
-    const int Jr = k / kTB;
-    const int nRB = (m + kTB - 1) / kTB;
-    double y = 0.0;
-    for (int ch = 0; ch * kCH <= nRB - 1; ++ch)
-      if (ch * kCH <= Jr || true) {
-        // chunk ch contributes to row block I only if ch*kCH <= I; rows of block Jr
-        if (ch * kCH <= Jr) y += Y[(long)ch * n + r];
-      }
-    for (int I = Jr; I < nRB; ++I) y += Z[(long)I * n + r];
-    for (int p = 0; p < i; ++p)
-      y -= a[r + (long)(j0 + p) * lda] * t1[p] + W[r + (long)p * ldw] * t2[p];
-    const double tau = tauA[c];
-    const double w = tau * y;
-    W[r + (long)i * ldw] = w;
-    const double v = vbuf[r];
-    a[r + (long)c * lda] = v;
-    prod = w * v;
-  }
-  const double tot = block_reduce_sum(prod, sh);
-  if (threadIdx.x == 0 && tot != 0.0) atomicAdd(&dot[c], tot);
-}
-
-__global__ void finish_w_kernel(int n, int c, int i, double* W, long ldw, const double* vbuf,
-                                const double* tauA, const double* dot) {
-  const int m = n - c - 1;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < m) {
-    const int r = c + 1 + k;
-    const double al = -0.5 * tauA[c] * dot[c];
-    W[r + (long)i * ldw] += al * vbuf[r];
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < 16; ++q) t += sh[q];
+    if (t != 0.0) atomicAdd(&dot[c], t);
   }
 }
 
-// P = [V W], Q = [W V] for the trailing update, rows r0..n-1.
+// P = [V W], Q = [W V] for the trailing update, rows r0..n-1 (W with its rank-one correction).
 __global__ void panel_pack_kernel(const double* a, long lda, int n, int r0, int j0, int pw,
-                                  const double* W, long ldw, double* P, double* Q, long ldp) {
+                                  const double* W, long ldw, const double* tau, const double* dot,
+                                  double* P, double* Q, long ldp) {
   const int p = blockIdx.y;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = r0 + k;
   if (r >= n) return;
   const double v = a[r + (long)(j0 + p) * lda];
-  const double w = W[r + (long)p * ldw];
+  const double w = W[r + (long)p * ldw] + wcorr(j0 + p, tau, dot) * v;
   P[k + (long)p * ldp] = v;
   P[k + (long)(p + pw) * ldp] = w;
   Q[k + (long)p * ldp] = w;
@@ -320,14 +371,14 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
   DH_TRY(fill<double>(s, ws.dot, n, 0.0));
   DH_TRY(fill<double>(s, ws.t12, (size_t)n * 2 * kNB, 0.0));
   DH_TRY(fill<double>(s, e, std::max(n - 1, 1), 0.0));
-  DH_TRY(fill<double>(s, tau, std::max(n - 1, 1), 0.0));
+  DH_TRY(fill<double>(s, tau, std::max(n, 1), 0.0));
   const long ldw = n;
   for (int j0 = 0; j0 < n; j0 += kNB) {
     const int pw = std::min(kNB, n - j0);
     for (int i = 0; i < pw; ++i) {
       const int c = j0 + i;
       hipLaunchKernelGGL(col_update_kernel, dim3(cdiv(n - c, 256)), dim3(256), 0, s, a, lda, n, c,
-                         j0, ws.W, ldw, ws.nrm2);
+                         j0, ws.W, ldw, tau, ws.dot, ws.nrm2);
       DH_LAUNCH_CHECK();
       const int m = n - c - 1;
       if (m <= 0) continue;
@@ -344,7 +395,8 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used], s));
       }
       hipLaunchKernelGGL(symv_kernel, dim3(nRB, nCh), dim3(256), 0, s, a, lda, n, c, j0, ws.W,
-                         ldw, ws.nrm2, ws.Y, ws.Z, ws.vbuf, ws.t12, ws.scal, e, tau);
+                         ldw, ws.nrm2, tau, ws.dot, ws.Y, ws.nSlot, ws.vbuf, ws.t12, ws.scal, e,
+                         tau);
       DH_LAUNCH_CHECK();
       if (pf.on) {
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used + 1], s));
@@ -352,18 +404,15 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
         pf.bytes[pf.used] = 8.0 * ((double)m * (m + 1) / 2 + 2.0 * m);
         ++pf.used;
       }
-      hipLaunchKernelGGL(finalize_w_kernel, dim3(cdiv(m, 256)), dim3(256), 0, s, a, lda, n, c, j0,
-                         ws.W, ldw, ws.Y, ws.Z, ws.vbuf, ws.t12, tau, ws.dot);
-      DH_LAUNCH_CHECK();
-      hipLaunchKernelGGL(finish_w_kernel, dim3(cdiv(m, 256)), dim3(256), 0, s, n, c, i, ws.W, ldw,
-                         ws.vbuf, tau, ws.dot);
+      hipLaunchKernelGGL(finalize_w_kernel, dim3(cdiv(m, 16)), dim3(256), 0, s, a, lda, n, c, j0,
+                         ws.W, ldw, ws.Y, ws.nSlot, ws.vbuf, ws.t12, tau, ws.dot);
       DH_LAUNCH_CHECK();
     }
     const int r0 = j0 + pw;
     if (r0 < n) {
       const int mt = n - r0;
       hipLaunchKernelGGL(panel_pack_kernel, dim3(cdiv(mt, 256), pw), dim3(256), 0, s, a, lda, n,
-                         r0, j0, pw, ws.W, ldw, ws.P, ws.Q, (long)mt);
+                         r0, j0, pw, ws.W, ldw, tau, ws.dot, ws.P, ws.Q, (long)mt);
       DH_LAUNCH_CHECK();
       double* a22 = a + r0 + (long)r0 * lda;
       DH_TRY(gemm<double>(s, false, true, mt, mt, 2 * pw, -1.0, ws.P, mt, ws.Q, mt, 1.0, a22, lda,
@@ -381,12 +430,14 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
 // compact WY form I - V T V^T (T upper triangular, LAPACK dlarft 'F','C').
 // ---------------------------------------------------------------------------------------------
 struct OrmWs {
-  double* V;    // n x kNB
-  double* G;    // kNB x kNB
-  double* T;    // kNB x kNB
-  double* W1;   // kNB x ncols
-  double* W2;   // kNB x ncols
+  double* V;    // n x kOB
+  double* G;    // kOB x kOB
+  double* T;    // kOB x kOB
+  double* W1;   // kOB x ncols
+  double* W2;   // kOB x ncols
 };
+
+constexpr int kOB = 128;   // reflectors per block of the back-transformation
 
 static size_t ormtr_layout(int n, int ncols, OrmWs* ws, double* base) {
   size_t off = 0;
@@ -396,17 +447,17 @@ static size_t ormtr_layout(int n, int ncols, OrmWs* ws, double* base) {
     if (ws) *p = base + off;
     off += (cnt + 63) / 64 * 64;
   };
-  take(&w->V, (size_t)n * kNB);
-  take(&w->G, kNB * kNB);
-  take(&w->T, kNB * kNB);
-  take(&w->W1, (size_t)kNB * std::max(ncols, 1));
-  take(&w->W2, (size_t)kNB * std::max(ncols, 1));
+  take(&w->V, (size_t)n * kOB);
+  take(&w->G, kOB * kOB);
+  take(&w->T, kOB * kOB);
+  take(&w->W1, (size_t)kOB * std::max(ncols, 1));
+  take(&w->W2, (size_t)kOB * std::max(ncols, 1));
   return off;
 }
 
 size_t ormtr_work_size(int n, int ncols) { return ormtr_layout(n, ncols, nullptr, nullptr); }
 
-// explicit V (rows j0+1..n-1) of the panel's reflectors
+// explicit V (rows j0+1..n-1) of the block's reflectors
 __global__ void form_v_kernel(const double* a, long lda, int n, int j0, int pw, double* V,
                               long ldv) {
   const int p = blockIdx.y;
@@ -421,23 +472,24 @@ __global__ void form_v_kernel(const double* a, long lda, int n, int j0, int pw, 
   V[k + (long)p * ldv] = v;
 }
 
-// T(i,i) = tau_i, T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), G = V^T V
-__global__ void larft_kernel(const double* G, const double* tau, int j0, int pw, double* T) {
-  __shared__ double Ts[kNB][kNB + 1];
+// T(i,i) = tau_i, T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), G = V^T V   (LAPACK dlarft 'F','C')
+__global__ __launch_bounds__(128) void larft_kernel(const double* G, const double* tau, int j0,
+                                                    int pw, double* T) {
+  __shared__ double Ts[kOB][kOB + 1];
   const int t = threadIdx.x;
-  for (int e = t; e < kNB * kNB; e += blockDim.x) Ts[e % kNB][e / kNB] = 0.0;
+  for (int e = t; e < kOB * kOB; e += blockDim.x) Ts[e % kOB][e / kOB] = 0.0;
   __syncthreads();
   for (int i = 0; i < pw; ++i) {
     const double ti = tau[j0 + i];
     if (t < i) {
       double acc = 0.0;
-      for (int k = t; k < i; ++k) acc += Ts[t][k] * G[k + i * kNB];
+      for (int k = t; k < i; ++k) acc += Ts[t][k] * G[k + i * kOB];
       Ts[t][i] = -ti * acc;
     }
     if (t == 0) Ts[i][i] = ti;
     __syncthreads();
   }
-  for (int e = t; e < kNB * kNB; e += blockDim.x) T[e] = Ts[e % kNB][e / kNB];
+  for (int e = t; e < kOB * kOB; e += blockDim.x) T[e] = Ts[e % kOB][e / kOB];
 }
 
 int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* tau, double* z,
@@ -447,21 +499,25 @@ int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* t
   OrmWs ws;
   ormtr_layout(n, ncols, &ws, work);
   const int nref = n - 1;
-  int lastPanel = ((nref - 1) / kNB) * kNB;
-  for (int j0 = lastPanel; j0 >= 0; j0 -= kNB) {
-    const int pw = std::min(kNB, nref - j0);
+  const int lastBlock = ((nref - 1) / kOB) * kOB;
+  for (int j0 = lastBlock; j0 >= 0; j0 -= kOB) {
+    const int pw = std::min(kOB, nref - j0);
     const int m = n - j0 - 1;
     hipLaunchKernelGGL(form_v_kernel, dim3(cdiv(m, 256), pw), dim3(256), 0, s, a, lda, n, j0, pw,
                        ws.V, (long)m);
     DH_LAUNCH_CHECK();
-    DH_TRY(gemm<double>(s, true, false, pw, pw, m, 1.0, ws.V, m, ws.V, m, 0.0, ws.G, kNB));
-    hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(64), 0, s, ws.G, tau, j0, pw, ws.T);
+    const int sk = std::max(1, std::min(64, m / 512));
+    DH_TRY(gemm<double>(s, true, false, pw, pw, m, 1.0, ws.V, m, ws.V, m, 0.0, ws.G, kOB, false,
+                        kKFull, nullptr, sk));
+    hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(128), 0, s, ws.G, tau, j0, pw, ws.T);
     DH_LAUNCH_CHECK();
     double* zr = z + (j0 + 1);
-    DH_TRY(gemm<double>(s, true, false, pw, ncols, m, 1.0, ws.V, m, zr, ldz, 0.0, ws.W1, kNB));
-    DH_TRY(gemm<double>(s, false, false, pw, ncols, pw, 1.0, ws.T, kNB, ws.W1, kNB, 0.0, ws.W2,
-                        kNB));
-    DH_TRY(gemm<double>(s, false, false, m, ncols, pw, -1.0, ws.V, m, ws.W2, kNB, 1.0, zr, ldz));
+    const int skz = std::max(1, std::min(16, m / 1024));
+    DH_TRY(gemm<double>(s, true, false, pw, ncols, m, 1.0, ws.V, m, zr, ldz, 0.0, ws.W1, kOB,
+                        false, kKFull, nullptr, skz));
+    DH_TRY(gemm<double>(s, false, false, pw, ncols, pw, 1.0, ws.T, kOB, ws.W1, kOB, 0.0, ws.W2,
+                        kOB));
+    DH_TRY(gemm<double>(s, false, false, m, ncols, pw, -1.0, ws.V, m, ws.W2, kOB, 1.0, zr, ldz));
   }
   return 0;
 }

@@ -1,0 +1,254 @@
+"""Pins of the CPU oracle against fixtures of the reference's unit tests, closed forms and
+textbook values (CPU only)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from dftbplus_amd import skf, synth
+from oracle import dftb_ref as R
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- SK interpolation and tail
+def test_neville_reproduces_degree7_polynomials():
+    """An 8-point Neville interpolant is exact for polynomials of degree <= 7."""
+    rng = np.random.default_rng(0)
+    coef = rng.standard_normal((8, 3))
+    dist, nrows = 0.02, 500
+    r = dist * np.arange(1, nrows + 1)
+    x = r / 10.0
+    table = np.stack([np.polyval(coef[:, k], x) for k in range(3)], axis=1)
+    rr = rng.uniform(0.2, dist * nrows - 1e-6, 200)
+    got = R.sk_interp(table, dist, rr)
+    ref = np.stack([np.polyval(coef[:, k], rr / 10.0) for k in range(3)], axis=1)
+    assert np.max(np.abs(got - ref)) < 1e-10 * np.abs(ref).max()
+    # at a grid point the table value itself
+    assert np.allclose(R.sk_interp(table, dist, np.array([r[100]]))[0], table[100], rtol=1e-13)
+
+
+def test_tail_polynomial_closed_form():
+    """poly5ToZero: equals the last table value at the grid end, is 0 at grid end + 1 bohr with
+    vanishing first and second derivatives there, and is 0 beyond."""
+    dist, nrows = 0.02, 500
+    r = dist * np.arange(1, nrows + 1)
+    table = (0.3 * (1 + 0.4 * r) * np.exp(-r))[:, None]
+    end = nrows * dist
+    assert abs(R.sk_interp(table, dist, np.array([end]))[0, 0] - table[-1, 0]) < 1e-15
+    rmax = end + 1.0
+    h = 1e-3
+    v = [R.sk_interp(table, dist, np.array([rmax - k * h]))[0, 0] for k in (1, 2, 3)]
+    assert abs(v[0]) < 1e-9 * abs(table[-1, 0]) * 1e3
+    assert R.sk_interp(table, dist, np.array([rmax, rmax + 0.5]))[:, 0].tolist() == [0.0, 0.0]
+    # cubic onset: f(rmax - h) ~ c h^3
+    assert abs(v[1] / v[0] - 8.0) < 0.05 and abs(v[2] / v[0] - 27.0) < 0.2
+
+
+# ---------------------------------------------------------------- Slater-Koster rotations
+def _unit(v):
+    return v / np.linalg.norm(v)
+
+
+def test_rotation_textbook_table():
+    """Slater & Koster (1954): E_s,x = l V_sp_sigma; E_x,y = l m (V_pp_sigma - V_pp_pi);
+    E_x,x = l^2 V_pp_sigma + (1 - l^2) V_pp_pi; orbitals ordered s, p_y, p_z, p_x."""
+    u = _unit(np.array([0.3, -0.5, 0.8]))
+    l, m, n = u
+    ss, sp, ps, pps, ppp = 0.7, -0.4, 0.25, 0.9, -0.3
+    sk = np.array([[ss, sp, ps, pps, ppp]])
+    hh = R.rotate_blocks(sk, [0, 1], [0, 1], np.array([l]), np.array([m]), np.array([n]))[0]
+    cart = {"y": m, "z": n, "x": l}
+    order = ["y", "z", "x"]
+    assert hh[0, 0] == pytest.approx(ss)
+    for a, ca in enumerate(order):
+        # p on atom 2 (rows), s on atom 1 (column): direction cosine times sp(sigma)
+        assert hh[1 + a, 0] == pytest.approx(cart[ca] * sp)
+        # p on atom 1, s on atom 2: opposite direction, integral of the B-A table
+        assert hh[0, 1 + a] == pytest.approx(-cart[ca] * ps)
+        for b, cb in enumerate(order):
+            exp = cart[ca] * cart[cb] * (pps - ppp) + (ppp if a == b else 0.0)
+            assert hh[1 + a, 1 + b] == pytest.approx(exp)
+
+
+def test_rotation_invariance_of_pp_block():
+    rng = np.random.default_rng(3)
+    u = _unit(rng.standard_normal(3))
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    sk = np.array([[0.8, -0.35]])
+    blk = lambda v: R.rotate_blocks(sk, [1], [1], np.array([v[0]]), np.array([v[1]]), np.array([v[2]]))[0]
+    P = np.array([[0, 1, 0], [0, 0, 1], [1, 0, 0]])     # (x,y,z) -> (y,z,x)
+    D = P @ q @ P.T
+    assert np.allclose(blk(q @ u), D @ blk(u) @ D.T, atol=1e-14)
+
+
+# ---------------------------------------------------------------- neighbour list, sparse format
+def test_neighbour_list_half_list_invariants():
+    geo, params, opts = synth.workload("diamond2")
+    cut = 9.0
+    nl = R.neighbour_list(geo.coords, geo.latvecs, cut)
+    nat = geo.natom
+    # brute force number of (atom, image) pairs with distance <= cut and central index >= atom
+    count = 0
+    for T in R.cell_translations(geo.latvecs, cut):
+        d = geo.coords[:, None, :] - (geo.coords[None, :, :] + T)
+        r = np.sqrt(np.sum(d * d, axis=2))
+        ok = (r <= cut) & (r > 1e-8)
+        count += int(np.sum(np.triu(ok)))
+    assert sum(len(x) - 1 for x in nl.neigh) == count
+    for a in range(nat):
+        assert np.all(np.diff(nl.dist2[a]) >= 0)
+        assert np.all(nl.img2cent[nl.neigh[a]] >= a)
+
+
+def test_unpack_pack_round_trip_and_brute_force(cluster216):
+    geo, params, opts, calc, res = cluster216
+    off = calc.off
+    dense = R.symmetrize_lower(calc.dense(calc.s))
+    # brute-force overlap from pairwise blocks of the geometry
+    N = int(off[-1])
+    ref = np.eye(N)
+    for i in range(geo.natom):
+        for j in range(i + 1, geo.natom):
+            vec = geo.coords[j] - geo.coords[i]
+            if np.linalg.norm(vec) > params.sk_cutoff():
+                continue
+            b = R.diatomic_blocks(params, params.over, geo.species[i], geo.species[j], vec[None])[0]
+            ref[off[j]:off[j + 1], off[i]:off[i + 1]] = b
+            ref[off[i]:off[i + 1], off[j]:off[j + 1]] = b.T
+    assert np.max(np.abs(dense - ref)) < 1e-14
+    assert np.array_equal(R.pack(dense, params, geo.species, calc.sp, off), calc.s)
+
+
+def test_mulliken_sum_rules(cluster216):
+    geo, params, opts, calc, res = cluster216
+    q = R.mulliken(calc.s, res.rho, geo.species, calc.sp, calc.off)
+    P = calc.dense(res.rho)
+    S = calc.dense(calc.s)
+    assert abs(q.sum() - np.trace(P @ S)) < 1e-10
+    assert abs(q.sum() - calc.nel) < 1e-9
+    assert np.allclose(q, np.sum(P * S, axis=1), atol=1e-12)
+
+
+def test_density_matrix_traces(cluster216):
+    geo, params, opts, calc, res = cluster216
+    S = calc.dense(calc.s)
+    W = calc.dense(res.erho)
+    assert abs(np.trace(W @ S) - res.eband) < 1e-10 * abs(res.eband)
+
+
+# ---------------------------------------------------------------- electrostatics
+def test_expgamma_limits_and_symmetry():
+    for ua, ub in ((0.4, 0.4), (0.3, 0.47)):
+        assert R.expgamma(0.0, ua, ub) < 0
+        assert R.expgamma(3.0, ua, ub) == pytest.approx(R.expgamma(3.0, ub, ua), rel=1e-13)
+        assert R.expgamma(40.0, ua, ub) < 1e-10 * (1 / 40.0)
+    # gamma = 1/r - S(r) tends to the Hubbard U for equal U
+    u = 0.42
+    r = 1e-3
+    assert abs((1.0 / r - R.expgamma(r, u, u)) - u) < 2e-3 * u
+    assert R.expgamma(0.0, u, u) == -u
+    # derivative consistent with the value
+    h = 1e-6
+    for ua, ub in ((0.4, 0.4), (0.3, 0.47)):
+        fd = (R.expgamma(2.5 + h, ua, ub) - R.expgamma(2.5 - h, ua, ub)) / (2 * h)
+        assert R.expgamma_prime(2.5, ua, ub) == pytest.approx(fd, rel=1e-7)
+
+
+def test_coulomb_derivative_fixture():
+    """Reference unit test coulombderivcluster: dV/dR of a +1/-1 pair at 1 bohr."""
+    fx = json.load(open(os.path.join(GOLDEN, "coulomb_cluster.json")))
+    c = np.array(fx["coords"])
+    q = np.array(fx["charges"])
+    nat = len(q)
+    out = np.zeros((nat, 3, nat))
+    h = 1e-6
+    for at in range(nat):
+        for k in range(3):
+            cp, cm = c.copy(), c.copy()
+            cp[at, k] += h
+            cm[at, k] -= h
+            out[:, k, at] = (R.invr_cluster(cp) @ q - R.invr_cluster(cm) @ q) / (2 * h)
+    ref = np.array(fx["invr_prime"], dtype=float).reshape((nat, 3, nat), order="F")
+    assert np.max(np.abs(out - ref)) < 1e-8
+
+
+def test_ewald_madelung_nacl_and_alpha_independence():
+    """Rock salt: E per ion pair = -M / r0 with the Madelung constant M = 1.747565."""
+    a = 5.0
+    lat = np.eye(3) * a
+    fcc = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    coords = np.concatenate([fcc, fcc + np.array([.5, 0, 0])]) * a
+    q = np.array([1.0] * 4 + [-1.0] * 4)
+    G = R.invr_ewald(coords, lat, 1e-12)
+    e = 0.5 * q @ G @ q
+    assert e / 4 == pytest.approx(-1.747564594633 / (a / 2), rel=1e-9)
+    G9 = R.invr_ewald(coords, lat, 1e-9)
+    assert np.max(np.abs(G9 - G)) < 1e-7
+
+
+# ---------------------------------------------------------------- Fermi filling
+def test_fermi_filling_counts_and_two_level_closed_form():
+    rng = np.random.default_rng(2)
+    eig = np.sort(rng.uniform(-1, 1, 50))
+    for kT, nel in ((1e-8, 20.0), (0.01, 31.0), (0.05, 17.3)):
+        f, ef, eb, ts, e0 = R.fermi_filling(eig, nel, kT)
+        assert abs(f.sum() - nel) < 1e-7
+        assert np.all(np.diff(f) <= 1e-15)
+        assert eb == pytest.approx(np.sum(f * eig), abs=1e-12)
+    # one electron in two levels e1 < e2: f1 + f2 = 1 per spin channel, f = Fermi(E_F)
+    e = np.array([-0.1, 0.1])
+    kT = 0.05
+    f, ef, eb, ts, e0 = R.fermi_filling(e, 2.0, kT)
+    assert ef == pytest.approx(0.0, abs=1e-12)
+    assert f[0] / 2 == pytest.approx(1.0 / (1.0 + math.exp(-0.1 / kT)), rel=1e-12)
+    x = f[0] / 2
+    # two spin channels x two levels, each with entropy -(x ln x + (1-x) ln(1-x))
+    assert ts == pytest.approx(4 * kT * -(x * math.log(x) + (1 - x) * math.log(1 - x)), rel=1e-10)
+
+
+# ---------------------------------------------------------------- Broyden mixer
+def test_broyden_golden_sequence():
+    """Reference regression sequence test/src/dftbp/unit/mixer/broyden.F90:24-58."""
+    fx = json.load(open(os.path.join(GOLDEN, "broyden_h2o.json")))
+    ch = np.array(fx["charges"])
+    df = np.array(fx["differences"])
+    mix = R.BroydenMixer(3, fx["mix_param"], fx["omega0"], fx["min_weight"], fx["max_weight"],
+                         fx["weight_fac"], 101)
+    q = ch[0].copy()
+    for i in range(4):
+        q = mix.mix(q, df[i])
+        assert np.max(np.abs(q - ch[i + 1])) < 100 * np.finfo(float).eps * 10
+
+
+# ---------------------------------------------------------------- SCC energy and forces
+def test_forces_match_energy_finite_differences():
+    geo, params, opts = synth.workload("cluster27")
+    calc = R.Calculator(params, geo, opts)
+    res = calc.scc()
+    g = calc.gradients(res)
+    assert np.max(np.abs(g.sum(axis=0))) < 1e-10
+    h = 1e-4
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        at, k = int(rng.integers(geo.natom)), int(rng.integers(3))
+        e = []
+        for sgn in (1, -1):
+            c = geo.coords.copy()
+            c[at, k] += sgn * h
+            g2 = type(geo)(c, geo.species, None)
+            e.append(R.Calculator(params, g2, opts).scc().energy_mermin)
+        fd = (e[0] - e[1]) / (2 * h)
+        assert abs(fd - g[at, k]) < 2e-6, (at, k, fd, g[at, k])
+
+
+def test_skf_round_trip(tmp_path):
+    params, files = synth.make_parameters(["H", "C"], 7)
+    paths = synth.write_skfiles(str(tmp_path), ["H", "C"], files)
+    for (i, j), p in paths.items():
+        back = skf.read_skf(p, homo=(i == j))
+        assert np.array_equal(back.ham, files[(i, j)].ham)
+        assert np.array_equal(back.over, files[(i, j)].over)
+        assert back.rep.value(2.1) == files[(i, j)].rep.value(2.1)
