@@ -43,7 +43,32 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
+    build_fortran(objdir, verbose)
     return LIB
+
+
+FORTRAN_TEST = os.path.join(HERE, "fortran", "test_dftbhip")
+
+
+def build_fortran(objdir: str, verbose: bool = True) -> None:
+    """Compile the ISO_C_BINDING module and its test program with amdflang when present."""
+    import shutil
+    fc = shutil.which("amdflang") or (os.path.exists("/opt/rocm/bin/amdflang") and "/opt/rocm/bin/amdflang")
+    if not fc:
+        return
+    fdir = os.path.join(HERE, "fortran")
+    mod_o = os.path.join(objdir, "dftbhip_mod.o")
+    r = subprocess.run([fc, "-O2", "-c", os.path.join(fdir, "dftbhip.F90"), "-o", mod_o, "-J", objdir],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed for dftbhip.F90:\n{r.stderr}")
+    r = subprocess.run([fc, "-O2", os.path.join(fdir, "test_dftbhip.F90"), mod_o, "-I", objdir,
+                        "-L" + HERE, "-ldftbhip", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_TEST],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed for test_dftbhip.F90:\n{r.stderr}")
+    if verbose:
+        print(f"built {FORTRAN_TEST}", file=sys.stderr)
 
 
 if __name__ == "__main__":

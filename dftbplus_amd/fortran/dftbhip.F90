@@ -1,0 +1,124 @@
+!> ISO_C_BINDING interface to libdftbhip, the MI355X backend of the SCC-DFTB hot path.
+!>
+!> Mirrors the pattern of DFTB+'s MAGMA binding (src/dftbp/extlibs/magma.F90:30-60): thin
+!> bind(C) interfaces plus Fortran wrappers with the argument lists of the routines they
+!> replace (magmaHegvd in src/dftbp/math/eigensolver.F90:1143-1245, makeDensityMtxRealGPU /
+!> makeEnergyDensityMtxRealGPU in src/dftbp/dftb/densitymatrix.F90:1105-1395).
+module dftbhip
+  use iso_c_binding, only : c_int, c_double, c_char, c_ptr, c_null_char
+  implicit none
+  private
+  public :: dftbhip_gpus_available, dftbhip_hegvd_dreal, dftbhip_dm_dreal, dftbhip_edm_dreal
+  public :: dftbhip_error_message
+
+  interface
+    function c_gpus(ngpus) result(rc) bind(C, name="dftbhip_get_gpus_available")
+      import :: c_int
+      integer(c_int), intent(out) :: ngpus
+      integer(c_int) :: rc
+    end function c_gpus
+
+    function c_sygvd(itype, jobz, uplo, n, a, lda, b, ldb, w) result(rc)&
+        & bind(C, name="dftbhip_sygvd")
+      import :: c_int, c_double, c_char
+      integer(c_int), value :: itype, n, lda, ldb
+      character(kind=c_char), value :: jobz, uplo
+      real(c_double), intent(inout) :: a(lda, *), b(ldb, *)
+      real(c_double), intent(out) :: w(*)
+      integer(c_int) :: rc
+    end function c_sygvd
+
+    function c_dm(n, nlevels, evec, lde, filling, dm, lddm) result(rc)&
+        & bind(C, name="dftbhip_dm_real")
+      import :: c_int, c_double
+      integer(c_int), value :: n, nlevels, lde, lddm
+      real(c_double), intent(in) :: evec(lde, *), filling(*)
+      real(c_double), intent(inout) :: dm(lddm, *)
+      integer(c_int) :: rc
+    end function c_dm
+
+    function c_edm(n, nlevels, evec, lde, filling, eigen, edm, lddm) result(rc)&
+        & bind(C, name="dftbhip_edm_real")
+      import :: c_int, c_double
+      integer(c_int), value :: n, nlevels, lde, lddm
+      real(c_double), intent(in) :: evec(lde, *), filling(*), eigen(*)
+      real(c_double), intent(inout) :: edm(lddm, *)
+      integer(c_int) :: rc
+    end function c_edm
+
+    function c_last_error(buf, blen) result(rc) bind(C, name="dftbhip_last_error")
+      import :: c_int, c_char
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_int), value :: blen
+      integer(c_int) :: rc
+    end function c_last_error
+  end interface
+
+contains
+
+  !> Number of visible GPUs (replaces getGpusAvailable).
+  subroutine dftbhip_gpus_available(ngpus)
+    integer, intent(out) :: ngpus
+    integer(c_int) :: n, rc
+    rc = c_gpus(n)
+    ngpus = n
+  end subroutine dftbhip_gpus_available
+
+  !> Generalized symmetric-definite eigenproblem, itype 1, lower storage (replaces
+  !> magmaHegvd for real matrices). info follows LAPACK: > n means a non-positive-definite
+  !> leading minor (info - n) of b; < 0 is an argument or device error.
+  subroutine dftbhip_hegvd_dreal(a, b, w, jobz, info)
+    real(c_double), intent(inout) :: a(:,:), b(:,:)
+    real(c_double), intent(out) :: w(:)
+    character, intent(in) :: jobz
+    integer, intent(out) :: info
+    integer(c_int) :: n
+    n = size(a, dim=1)
+    info = c_sygvd(1_c_int, jobz, "L", n, a, n, b, size(b, dim=1), w)
+  end subroutine dftbhip_hegvd_dreal
+
+  !> Density matrix from eigenvectors and fillings (lower triangle).
+  subroutine dftbhip_dm_dreal(dm, eigenvecs, filling, info)
+    real(c_double), intent(inout) :: dm(:,:)
+    real(c_double), intent(in) :: eigenvecs(:,:), filling(:)
+    integer, intent(out) :: info
+    integer :: nLevels
+    nLevels = size(filling)
+    do while (nLevels > 0)
+      if (filling(nLevels) > epsilon(1.0_c_double)) exit
+      nLevels = nLevels - 1
+    end do
+    info = c_dm(int(size(eigenvecs, dim=1), c_int), int(nLevels, c_int), eigenvecs,&
+        & int(size(eigenvecs, dim=1), c_int), filling, dm, int(size(dm, dim=1), c_int))
+  end subroutine dftbhip_dm_dreal
+
+  !> Energy-weighted density matrix (lower triangle).
+  subroutine dftbhip_edm_dreal(edm, eigenvecs, filling, eigen, info)
+    real(c_double), intent(inout) :: edm(:,:)
+    real(c_double), intent(in) :: eigenvecs(:,:), filling(:), eigen(:)
+    integer, intent(out) :: info
+    integer :: nLevels
+    nLevels = size(filling)
+    do while (nLevels > 0)
+      if (filling(nLevels) > epsilon(1.0_c_double)) exit
+      nLevels = nLevels - 1
+    end do
+    info = c_edm(int(size(eigenvecs, dim=1), c_int), int(nLevels, c_int), eigenvecs,&
+        & int(size(eigenvecs, dim=1), c_int), filling, eigen, edm, int(size(edm, dim=1), c_int))
+  end subroutine dftbhip_edm_dreal
+
+  !> Message of the last failing call.
+  function dftbhip_error_message() result(msg)
+    character(len=512) :: msg
+    character(kind=c_char) :: buf(512)
+    integer(c_int) :: rc
+    integer :: ii
+    rc = c_last_error(buf, 512_c_int)
+    msg = ""
+    do ii = 1, 512
+      if (buf(ii) == c_null_char) exit
+      msg(ii:ii) = buf(ii)
+    end do
+  end function dftbhip_error_message
+
+end module dftbhip
