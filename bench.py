@@ -5,13 +5,24 @@ A step is one SCC iteration of the periodic 4096-atom diamond-like C/H/O superce
 (15769 orbitals, Gamma point, Fermi kT = 0.01 Ha): charge potential, H = H0 + 1/2 S(V+V),
 unpack, generalized eigensolve, Fermi filling, density matrix, pack, Mulliken, energies and
 Broyden mixing, everything of O(N^2) or more on the GPU with the inputs resident in HBM.
-After the timed steps one force evaluation (energy-weighted density matrix + pair, gamma and
-repulsive gradients) is timed as well.
+A force evaluation is the energy-weighted density matrix plus the pair, gamma and repulsive
+gradients.
+
+Three legs run on the same input:
+  fp64          the plain FP64 path;
+  mixed_fp32    FP32 MFMA reduction, tridiagonalisation, back-transformation and density
+                matrices; FP64 Cholesky, divide and conquer, Mulliken and energy reductions;
+  mixed_refined the FP32 eigenpairs of every iteration refined in FP64 (two Ogita-Aishima
+                steps) and FP64 density matrices.
+Each leg first runs one probe iteration from neutral charges followed by a force evaluation
+(timed); its energy, charges and gradients are compared with the fp64 leg's probe. Then the
+timed iterations run. `value` is the SCC iterations/s of the fastest leg whose probe meets
+|dE| < 1e-6 Ha/atom and max |dF| < 1e-5 Ha/bohr against the fp64 leg; every leg is reported.
 
 The dominant kernel is the symmetric product of the Householder tridiagonalisation (HBM bound);
 its launches are timed live with HIP events on the library's stream during the timed steps.
 The CPU baseline is the numpy/LAPACK oracle timed on the host for SCC iterations of the 1000-atom
-cell of the same recipe (3853 orbitals), scaled by N^3 (the dense eigensolve) to 15769 orbitals.
+cell of the same recipe (3850 orbitals), scaled by N^3 (the dense eigensolve) to 15769 orbitals.
 """
 import argparse
 import json
@@ -24,11 +35,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SCC iter time + force evals/s @16k orbs: mixed vs FP64 GPU at ΔE<1e-6 Ha/atom"
 HBM_PEAK_GBS = 8000.0
+LEGS = {"fp64": dict(precision=0),
+        "mixed_fp32": dict(precision=1, refine_below=0.0),
+        "mixed_refined": dict(precision=1, refine_below=float("inf"), refine_steps=2)}
+E_BOUND, F_BOUND = 1e-6, 1e-5
 
 
 def cpu_baseline(sample_cells=5, iters=2):
     """Oracle SCC iterations on the host, N^3-scaled to the benchmark size."""
-    import numpy as np
     from dftbplus_amd import synth
     from oracle import dftb_ref as R
     geo, params, opts = synth.workload(f"diamond{sample_cells}")
@@ -38,8 +52,62 @@ def cpu_baseline(sample_cells=5, iters=2):
     t0 = time.perf_counter()
     calc.scc()
     dt = (time.perf_counter() - t0) / iters
-    threads = int(os.environ.get("OPENBLAS_NUM_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    threads = int(os.environ.get("OPENBLAS_NUM_THREADS",
+                                 os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     return n, dt, threads
+
+
+def run_leg(api, geo, params, opts, leg, steps, warmup):
+    """Setup, a probe iteration with a force evaluation, warmup iterations, timed iterations
+    with the SYMV launch profile. Every library call ends with a synchronisation of the
+    library's stream."""
+    import numpy as np
+    L = api.lib()
+    tag = f"[bench:{leg}]"
+    t0 = time.perf_counter()
+    dp = api.DftbPlus(params, geo, opts, **LEGS[leg])
+    setup_s = time.perf_counter() - t0
+    n = dp.get_nr_orbitals()
+    print(f"{tag} {geo.natom} atoms, {n} orbitals, setup {setup_s:.2f}s", file=sys.stderr,
+          flush=True)
+    dp.scc_begin()
+    dp.scc_step()
+    dp.scc_end()
+    energy = dp.get_energy()
+    charges = dp.get_gross_charges()
+    t1 = time.perf_counter()
+    grad = dp.get_gradients()
+    force_s = time.perf_counter() - t1
+    print(f"{tag} probe iteration done, force evaluation {1e3 * force_s:.1f} ms",
+          file=sys.stderr, flush=True)
+    dp.scc_begin()
+    for i in range(max(warmup - 1, 0)):
+        dp.scc_step()
+        print(f"{tag} warmup step {i + 2} done", file=sys.stderr, flush=True)
+    ph0 = dp.timings()
+    L.dftbhip_kernel_profile(1)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        conv, err = dp.scc_step()
+        print(f"{tag} step {i + 1}: charge error {err:.3e}", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    st = np.zeros(3)
+    L.dftbhip_kernel_stats(api._d(st))
+    L.dftbhip_kernel_profile(0)
+    ph1 = dp.timings()
+    dp.scc_end()
+    print(f"{tag} {1e3 * elapsed / steps:.1f} ms/step", file=sys.stderr, flush=True)
+    dp.close()
+    launches, ksec, kbytes = st
+    achieved = kbytes / ksec / 1e9 if ksec > 0 else None
+    return {
+        "n": n, "elapsed": elapsed, "force_s": force_s, "setup_s": setup_s,
+        "phases": {k: (ph1[k] - ph0[k]) / steps for k in ph1
+                   if k not in ("setup", "last_iteration", "forces")},
+        "energy": energy, "charges": charges, "grad": grad,
+        "symv": {"launches": int(launches), "seconds": ksec, "bytes": kbytes,
+                 "achieved": achieved},
+    }
 
 
 def main():
@@ -48,89 +116,79 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="diamond8")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--legs", default="fp64,mixed_fp32,mixed_refined",
+                    help="comma-separated subset of " + ",".join(LEGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCC path of a single Gamma-point system runs on one "
               "device (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-
-    from dftbplus_amd import api, synth
-    L = api.lib()
-    # every library call ends with a synchronisation of the library's HIP stream; torch is not
-    # loaded here (its bundled HIP runtime would be a second runtime in the process)
-    sync = lambda: None
-
-    geo, params, opts = synth.workload(args.workload)
-    t0 = time.perf_counter()
-    dp = api.DftbPlus(params, geo, opts)
-    setup_s = time.perf_counter() - t0
-    n = dp.get_nr_orbitals()
-    print(f"[bench] {args.workload}: {geo.natom} atoms, {n} orbitals, setup {setup_s:.2f}s",
-          file=sys.stderr, flush=True)
-    dp.scc_begin()
-    for i in range(args.warmup):
-        dp.scc_step()
-        print(f"[bench] warmup step {i + 1} done", file=sys.stderr, flush=True)
-    L.dftbhip_kernel_profile(1)
-    sync()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        conv, err = dp.scc_step()
-        print(f"[bench] step {i + 1}: charge error {err:.3e}", file=sys.stderr, flush=True)
-    sync()
-    elapsed = time.perf_counter() - t0
     import numpy as np
-    st = np.zeros(3)
-    L.dftbhip_kernel_stats(api._d(st))
-    L.dftbhip_kernel_profile(0)
-    phases = dp.timings()
-    dp.scc_end()
-    sync()
-    t1 = time.perf_counter()
-    dp.get_gradients()
-    sync()
-    force_s = time.perf_counter() - t1
-    ms = 1e3 * elapsed / args.steps
-    launches, ksec, kbytes = st
-    achieved = kbytes / ksec / 1e9 if ksec > 0 else None
-    avg_us = 1e6 * ksec / launches if launches else None
+    from dftbplus_amd import api, synth
+    # torch is not loaded here: its bundled HIP runtime would be a second runtime in the process
+    geo, params, opts = synth.workload(args.workload)
+    legs = [x for x in args.legs.split(",") if x]
+    if "fp64" not in legs:
+        legs.insert(0, "fp64")   # the reference leg of the error comparison
+    res = {p: run_leg(api, geo, params, opts, p, args.steps, args.warmup) for p in legs}
+    nat = geo.natom
+    ref = res["fp64"]
+    for p, r in res.items():
+        r["error"] = {"dE_per_atom": abs(r["energy"] - ref["energy"]) / nat,
+                      "dq_max": float(np.max(np.abs(r["charges"] - ref["charges"]))),
+                      "dF_max": float(np.max(np.abs(r["grad"] - ref["grad"])))}
+        r["meets_bound"] = (r["error"]["dE_per_atom"] < E_BOUND
+                            and r["error"]["dF_max"] < F_BOUND)
+    head = min((p for p in res if res[p]["meets_bound"]), key=lambda p: res[p]["elapsed"])
+    r = res[head]
+    n = r["n"]
+    sv = r["symv"]
+    achieved = sv["achieved"]
+    fp32 = head != "fp64"
     rec = {
         "metric": METRIC,
-        "value": args.steps / elapsed,
+        "value": args.steps / r["elapsed"],
         "unit": "SCC iterations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if fp32 else "f64",
         "data": "synthetic",
-        "config": {"workload": f"periodic diamond-like C/H/O supercell, {geo.natom} atoms, {n} orbitals, "
-                               f"Gamma point, Fermi kT=0.01 Ha ({args.workload})",
-                   "precision": args.precision, "orbitals": n, "atoms": geo.natom},
-        "force_evals_per_s": 1.0 / force_s,
-        "force_eval_ms": 1e3 * force_s,
-        "phase_s_per_step": {k: v / args.steps for k, v in phases.items()
-                             if k not in ("setup", "last_iteration", "forces")},
-        "roofline": {"kernel": "symv_kernel (tridiagonalisation, lower-triangle symmetric product)",
+        "config": {"workload": f"periodic diamond-like C/H/O supercell, {geo.natom} atoms, "
+                               f"{n} orbitals, Gamma point, Fermi kT=0.01 Ha ({args.workload})",
+                   "leg": head, "orbitals": n, "atoms": geo.natom},
+        "force_evals_per_s": 1.0 / r["force_s"],
+        "force_eval_ms": 1e3 * r["force_s"],
+        "phase_s_per_step": r["phases"],
+        "roofline": {"kernel": "symv_kernel (tridiagonalisation, lower-triangle symmetric product, "
+                               + ("FP32" if fp32 else "FP64") + ")",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "launches": int(launches), "avg_launch_us": avg_us,
-                     "alg_bytes_per_launch": kbytes / launches if launches else None},
+                     "launches": sv["launches"],
+                     "avg_launch_us": 1e6 * sv["seconds"] / sv["launches"] if sv["launches"] else None,
+                     "alg_bytes_per_launch": sv["bytes"] / sv["launches"] if sv["launches"] else None},
+        "legs": {p: {"scc_iter_per_s": args.steps / x["elapsed"],
+                     "ms_per_step": 1e3 * x["elapsed"] / args.steps,
+                     "force_evals_per_s": 1.0 / x["force_s"],
+                     "speedup_vs_fp64": ref["elapsed"] / x["elapsed"],
+                     "probe_error_vs_fp64": x["error"], "meets_bound": x["meets_bound"],
+                     "phase_s_per_step": x["phases"],
+                     "symv_achieved_GBs": x["symv"]["achieved"]} for p, x in res.items()},
     }
     if not args.no_cpu_baseline:
         ns, dt, threads = cpu_baseline()
         scale = (n / ns) ** 3
         rec["cpu_baseline"] = {"value": 1.0 / (dt * scale), "unit": "SCC iterations/s",
                                "cores": threads, "kind": "port",
-                               "sample": f"numpy/LAPACK oracle, 2 SCC iterations of the 1000-atom cell "
-                                         f"({ns} orbitals, {dt:.2f} s/iteration) scaled by (N/{ns})^3 to N={n}"}
+                               "sample": f"numpy/LAPACK oracle, 2 SCC iterations of the 1000-atom "
+                                         f"cell ({ns} orbitals, {dt:.2f} s/iteration) scaled by "
+                                         f"(N/{ns})^3 to N={n}"}
     print(json.dumps(rec), flush=True)
-    dp.close()
 
 
 if __name__ == "__main__":

@@ -79,6 +79,8 @@ class COptions(C.Structure):
         ("ewald_tol", C.c_double),
         ("force_mode", C.c_int),
         ("precision", C.c_int),
+        ("refine_below", C.c_double),
+        ("refine_steps", C.c_int),
     ]
 
 
@@ -190,7 +192,8 @@ def make_params(params: Parameters):
     return p, keep
 
 
-def make_options(opts: Options, force_mode: int = 0, precision: int = 0) -> COptions:
+def make_options(opts: Options, force_mode: int = 0, precision: int = 0,
+                 refine_below: float = 1e-4, refine_steps: int = 2) -> COptions:
     o = COptions()
     o.kT = opts.kT
     o.scc_tol = opts.scc_tol
@@ -205,6 +208,8 @@ def make_options(opts: Options, force_mode: int = 0, precision: int = 0) -> COpt
     o.ewald_tol = opts.ewald_tol
     o.force_mode = force_mode
     o.precision = precision
+    o.refine_below = refine_below
+    o.refine_steps = refine_steps
     return o
 
 
@@ -348,7 +353,7 @@ def mulliken(ra: RefArrays, over: np.ndarray, rho: np.ndarray) -> np.ndarray:
 # ---------------------------------------------------------------------------------------------
 ENERGY_KEYS = ("mermin", "total", "nonscc", "scc", "repulsive", "ts", "eband", "ef")
 TIMER_KEYS = ("setup", "hamiltonian", "reduce", "tridiag", "divconq", "backtransform", "density",
-              "pack_mulliken", "host", "forces", "last_iteration")
+              "pack_mulliken", "host", "forces", "last_iteration", "refine")
 
 
 class DftbPlus:
@@ -360,10 +365,11 @@ class DftbPlus:
     """
 
     def __init__(self, params: Parameters, geometry: Geometry, options: Options = Options(),
-                 force_mode: int = 0, precision: int = 0):
+                 force_mode: int = 0, precision: int = 0, refine_below: float = 1e-4,
+                 refine_steps: int = 2):
         self.params = params
         self._p, self._keep = make_params(params)
-        self._o = make_options(options, force_mode, precision)
+        self._o = make_options(options, force_mode, precision, refine_below, refine_steps)
         self.h = C.c_void_p()
         _check(lib().dftbhip_ctx_create(C.byref(self.h), C.byref(self._p), C.byref(self._o)),
                "dftbhip_ctx_create")

@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <numeric>
 #include <string>
@@ -1037,7 +1038,7 @@ struct Broyden {
 
 enum Timer {
   kTSetup = 0, kTHam, kTReduce, kTTridiag, kTDC, kTBack, kTDM, kTPackMull, kTHost, kTForces,
-  kTIter, kTCount
+  kTIter, kTRefine, kTCount
 };
 
 struct SccState {
@@ -1046,6 +1047,7 @@ struct SccState {
   int it = 0;
   bool conv = false, active = false;
   double err = 0.0, nel = 0.0, eNonScc = 0.0, eScc = 0.0;
+  double prevErr = HUGE_VAL;   // charge error of the previous iteration (mixed-path switch)
   FillResult fr{};
 };
 
@@ -1069,10 +1071,12 @@ struct Context {
   EwaldSetup ew;
   DevBuf<double> coordsAll, coordsC, h0, ov, hs, rho, erho, gamma, vorb, vatom, dq, qorb, eorb;
   DevBuf<double> Hd, Sd, X, P, f, feps, grad, eatom, ewG, ewW, ewTr, ewCS, ewCSq, ww2;
+  DevBuf<float> f32;   // FP32 filling weights of the mixed path
   EigenSolver eig;
   std::vector<double> q0atom, eigv, fill, qout, dqout;
   double energies[8]{};
   int nocc = 0;
+  bool xf64 = true;     // the eigenvectors in X are FP64 (else FP32, mixed path)
   bool sccDone = false;
   double times[kTCount]{};
   int ncalls = 0;
@@ -1201,6 +1205,10 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
               std::to_string(info) + ")");
     return n + info;
   }
+  if (c.opt.precision == 1) {
+    DH_TRY(c.eig.prepare_mixed(s));
+    DH_TRY(c.f32.alloc(n));
+  }
   DH_HIP(hipStreamSynchronize(s));
   c.sccDone = false;
   c.times[kTSetup] = now() - t0;
@@ -1224,6 +1232,18 @@ static int gamma_times(Context& c, const std::vector<double>& dqa, std::vector<d
   return 0;
 }
 
+// Levels entering the density matrices: up to the last one with a filling of at least
+// machine epsilon (fullDensityMatrix_real, densitymatrix.F90:549-554).
+static int occupied_levels(const std::vector<double>& f) {
+  for (int i = (int)f.size() - 1; i >= 0; --i)
+    if (std::fabs(f[i]) >= std::numeric_limits<double>::epsilon()) return i + 1;
+  return 0;
+}
+
+// Levels beyond the occupied ones kept in the refined density matrix: refined eigenvalues
+// move by the FP32 error of the tridiagonalisation, which can reorder levels at the cut.
+constexpr int kRefineMargin = 64;
+
 static int ctx_scc_begin(Context& c) {
   for (int t = 1; t < kTCount; ++t) c.times[t] = 0.0;
   SccState& st = c.st;
@@ -1234,6 +1254,7 @@ static int ctx_scc_begin(Context& c) {
   st.it = 0;
   st.conv = false;
   st.err = 0.0;
+  st.prevErr = HUGE_VAL;
   st.active = true;
   c.sccDone = false;
   return 0;
@@ -1252,34 +1273,89 @@ static int ctx_scc_step(Context& c) {
   const double tIt = now();
   std::vector<double> dqa(nat), qorbH(n), eorbH(n), vout;
   for (int a = 0; a < nat; ++a) dqa[a] = st.qin[a] - c.q0atom[a];
-  hipEvent_t ev[6];
+  hipEvent_t ev[8];
   for (auto& x : ev) DH_HIP(hipEventCreate(&x));
   DH_HIP(hipEventRecord(ev[0], s));
   DH_TRY(gamma_times(c, dqa, nullptr));
   DH_TRY(launch_shift(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.h0.p, c.ov.p, c.vorb.p,
                       c.hs.p));
-  DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p, n,
-                               n));
+  // FP64: the plain path. Mixed: FP32 reduction, tridiagonalisation, back-transformation and
+  // density matrix. Mixed + refined: the FP32 eigenvectors of all levels are refined in FP64
+  // (Ogita-Aishima) and the density matrix is built in FP64.
+  const bool mixed = o.precision == 1;
+  const bool refined = mixed && st.prevErr <= o.refine_below;
+  c.xf64 = !mixed || refined;
+  float* X32 = reinterpret_cast<float*>(c.X.p);
+  float* P32 = reinterpret_cast<float*>(c.P.p);
+  float* H32 = refined ? c.eig.h32() : reinterpret_cast<float*>(c.Hd.p);
+  if (mixed && !refined) {
+    DH_TRY(launch_unpack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, H32, n,
+                                n));
+  } else {
+    DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p,
+                                 n, n));
+    if (refined) DH_TRY((convert2d<double, float>(s, c.Hd.p, n, H32, n, n, n)));
+  }
   DH_HIP(hipEventRecord(ev[1], s));
   int info = 0;
   double evt[3] = {0, 0, 0};
-  DH_TRY(c.eig.solve(s, c.Hd.p, n, c.eigv, &info, evt));
+  if (mixed)
+    DH_TRY(c.eig.solve_mixed(s, H32, n, c.eigv, &info, evt));
+  else
+    DH_TRY(c.eig.solve(s, c.Hd.p, n, c.eigv, &info, evt));
   if (info != 0) return info;
   const double th = now();
   DH_TRY(fermi_filling(c.eigv, st.nel, o.kT, c.fill, st.fr));
-  int nocc = 0;
-  for (int i = 0; i < n; ++i)
-    if (c.fill[i] > 0.0) nocc = i + 1;
-  c.nocc = nocc;
+  int nocc = occupied_levels(c.fill);
   c.times[kTHost] += now() - th;
-  DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  std::vector<float> fill32;
+  if (mixed && !refined) {
+    fill32.assign(c.fill.begin(), c.fill.end());
+    DH_HIP(hipMemcpyAsync(c.f32.p, fill32.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+  } else if (!mixed) {
+    DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  }
   DH_HIP(hipEventRecord(ev[2], s));
-  DH_TRY(c.eig.vectors(s, nocc, c.X.p, n));
+  if (refined) {
+    // all levels form the refinement basis; the leading nocc + margin refined vectors enter
+    // the density matrix with fillings recomputed from the refined eigenvalues
+    DH_TRY(c.eig.vectors_mixed(s, n, P32, n));
+    DH_HIP(hipEventRecord(ev[6], s));
+    const int nvec = std::min(n, nocc + kRefineMargin);
+    std::vector<double> lam;
+    DH_TRY(c.eig.refine(s, c.Hd.p, c.Sd.p, P32, o.refine_steps, nvec, c.X.p, n, lam));
+    DH_HIP(hipEventRecord(ev[7], s));
+    const double th3 = now();
+    std::vector<int> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return lam[x] < lam[y]; });
+    std::vector<double> ls(n), fs;
+    for (int k = 0; k < n; ++k) ls[k] = lam[perm[k]];
+    DH_TRY(fermi_filling(ls, st.nel, o.kT, fs, st.fr));
+    for (int k = 0; k < n; ++k) c.fill[perm[k]] = fs[k];
+    c.eigv = lam;
+    nocc = nvec;
+    DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    c.times[kTHost] += now() - th3;
+  } else if (mixed) {
+    DH_TRY(c.eig.vectors_mixed(s, nocc, X32, n));
+  } else {
+    DH_TRY(c.eig.vectors(s, nocc, c.X.p, n));
+  }
+  c.nocc = nocc;
   DH_HIP(hipEventRecord(ev[3], s));
-  DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
-                      kKFull, c.f.p));
+  if (c.xf64)
+    DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
+                        kKFull, c.f.p));
+  else
+    DH_TRY(gemm<float>(s, false, true, n, n, nocc, 1.0f, X32, n, X32, n, 0.0f, P32, n, true,
+                       kKFull, c.f32.p));
   DH_HIP(hipEventRecord(ev[4], s));
-  DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n, c.rho.p));
+  if (c.xf64)
+    DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
+                               c.rho.p));
+  else
+    DH_TRY(launch_pack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, P32, n, c.rho.p));
   DH_TRY(launch_mulliken(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.h0.p, c.rho.p,
                          c.qorb.p, c.eorb.p, n));
   DH_HIP(hipMemcpyAsync(qorbH.data(), c.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1291,6 +1367,11 @@ static int ctx_scc_step(Context& c) {
   c.times[kTHam] += ms * 1e-3;
   DH_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
   c.times[kTBack] += ms * 1e-3;
+  if (refined) {
+    DH_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
+    c.times[kTRefine] += ms * 1e-3;
+    c.times[kTBack] -= ms * 1e-3;
+  }
   DH_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
   c.times[kTDM] += ms * 1e-3;
   DH_HIP(hipEventElapsedTime(&ms, ev[4], ev[5]));
@@ -1319,6 +1400,7 @@ static int ctx_scc_step(Context& c) {
   c.qout = qout;
   c.dqout = dqo;
   st.err = err;
+  st.prevErr = err;
   st.conv = err < o.scc_tol && st.it >= o.min_scc_iter;
   const double th3 = now();
   if (!st.conv && st.it != o.max_scc_iter) st.mixer.mix(st.qin, diff);
@@ -1374,10 +1456,22 @@ static int ctx_gradients(Context& c, double* grad) {
   // energy-weighted density matrix
   std::vector<double> fe(n, 0.0);
   for (int i = 0; i < c.nocc; ++i) fe[i] = c.fill[i] * c.eigv[i];
-  DH_HIP(hipMemcpyAsync(c.feps.p, fe.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-  DH_TRY(gemm<double>(s, false, true, n, n, c.nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
-                      kKFull, c.feps.p));
-  DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n, c.erho.p));
+  if (!c.xf64) {
+    std::vector<float> fe32(fe.begin(), fe.end());
+    float* X32 = reinterpret_cast<float*>(c.X.p);
+    float* P32 = reinterpret_cast<float*>(c.P.p);
+    DH_HIP(hipMemcpyAsync(c.f32.p, fe32.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    DH_TRY(gemm<float>(s, false, true, n, n, c.nocc, 1.0f, X32, n, X32, n, 0.0f, P32, n, true,
+                       kKFull, c.f32.p));
+    DH_TRY(launch_pack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, P32, n, c.erho.p));
+    DH_HIP(hipStreamSynchronize(s));   // fe32 leaves scope
+  } else {
+    DH_HIP(hipMemcpyAsync(c.feps.p, fe.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    DH_TRY(gemm<double>(s, false, true, n, n, c.nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n,
+                        true, kKFull, c.feps.p));
+    DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
+                               c.erho.p));
+  }
   // output potential per orbital
   std::vector<double> vout;
   DH_TRY(gamma_times(c, c.dqout, &vout));
@@ -1412,7 +1506,8 @@ struct dftbhip_ctx {
 extern "C" int dftbhip_ctx_create(dftbhip_ctx** ctx, const dftbhip_params* p,
                                   const dftbhip_options* opt) {
   DH_CHECK(ctx && p && opt, "ctx_create: null pointer");
-  DH_CHECK(opt->precision == 0, "ctx_create: only precision = 0 (FP64) is available");
+  DH_CHECK(opt->precision == 0 || opt->precision == 1,
+           "ctx_create: precision must be 0 (FP64) or 1 (mixed FP32/FP64)");
   DH_CHECK(opt->max_scc_iter >= 1, "ctx_create: max_scc_iter < 1");
   DH_TRY(ensure_runtime());
   auto* h = new dftbhip_ctx();

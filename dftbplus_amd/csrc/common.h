@@ -154,6 +154,13 @@ void symv_profile_stats(double* out);
 int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* tau, double* z,
                 long ldz, int ncols, double* work, size_t workSize);
 size_t ormtr_work_size(int n, int ncols);
+// FP32 variants (workspace in bytes)
+int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e, float* tau,
+                    void* work, size_t workBytes);
+int ormtr_lower_f32(hipStream_t s, const float* a, long lda, int n, const float* tau, float* z,
+                    long ldz, int ncols, void* work, size_t workBytes);
+size_t sytrd_work_bytes(int n, int elemSize);
+size_t ormtr_work_bytes(int n, int ncols, int elemSize);
 int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double* w, double* z,
           long ldz, double* work, size_t workSize, int* info);
 size_t stedc_work_size(int n);

@@ -12,11 +12,13 @@ struct EigenSolver {
   bool factored = false;
   DevBuf<double> L, Linv, C, Zd, T, d, e, tau, w, work;
   DevBuf<int> dinfo;
+  DevBuf<float> Linv32, d32, e32, tau32;
   size_t workSize = 0;
 
   int ensure(int n_);
   void release() {
     for (DevBuf<double>* b : {&L, &Linv, &C, &Zd, &T, &d, &e, &tau, &w, &work}) b->release();
+    for (DevBuf<float>* b : {&Linv32, &d32, &e32, &tau32}) b->release();
     dinfo.release();
     n = 0;
     factored = false;
@@ -29,6 +31,23 @@ struct EigenSolver {
             double* times = nullptr);
   // The first nvec generalized eigenvectors into X (n x nvec).
   int vectors(hipStream_t s, int nvec, double* X, long ldx);
+
+  // Mixed precision: the reduction, tridiagonalisation and back-transformation run in FP32
+  // (FP32 MFMA, FP32 accumulation); the Cholesky factor comes from the FP64 factor_overlap and
+  // the tridiagonal eigenproblem is solved in FP64. prepare_mixed rounds Linv once per geometry.
+  int prepare_mixed(hipStream_t s);
+  // FP32 staging area for H in refined iterations (second half of the FP64 buffer T)
+  float* h32() { return reinterpret_cast<float*>(T.p) + (size_t)n * n; }
+  int solve_mixed(hipStream_t s, const float* H, long ldh, std::vector<double>& wHost, int* info,
+                  double* times = nullptr);
+  int vectors_mixed(hipStream_t s, int nvec, float* X, long ldx);
+
+  // Ogita-Aishima refinement in FP64 of all n approximate generalized eigenpairs X0 (FP32,
+  // n x n, from vectors_mixed) against the FP64 matrices H and S (both triangles stored):
+  // steps refinement steps; the first nvec refined vectors go to X (FP64), the refined
+  // eigenvalues (Rayleigh quotients of the last input basis, unsorted) to lamHost.
+  int refine(hipStream_t s, const double* H, const double* S, const float* X0, int steps,
+             int nvec, double* X, long ldx, std::vector<double>& lamHost);
 };
 
 }  // namespace dftbhip

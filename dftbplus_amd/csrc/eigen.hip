@@ -94,4 +94,136 @@ int EigenSolver::vectors(hipStream_t s, int nvec, double* X, long ldx) {
   return 0;
 }
 
+// Ogita-Aishima correction E (n x nv) from G = X^T S X and M = X^T H X (both full):
+//   lam_j = M_jj / G_jj,  R = I - G,
+//   E_jj = R_jj / 2,
+//   E_ij = (M_ij + lam_j R_ij) / (lam_j - lam_i)   where that is small (|E_ij| < kOaMax),
+//   E_ij = R_ij / 2                                 otherwise (near-degenerate pairs: the pair
+//                                                   is only re-orthonormalised).
+constexpr double kOaMax = 0.25;
+
+__global__ void oa_lambda_kernel(const double* M, const double* G, int n, double* lam) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lam[i] = M[i + (long)i * n] / G[i + (long)i * n];
+}
+
+__global__ void oa_correction_kernel(const double* M, const double* G, const double* lam, int n,
+                                     int nv, double* E) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || j >= nv) return;
+  const long ij = i + (long)j * n;
+  double e;
+  if (i == j) {
+    e = 0.5 * (1.0 - G[ij]);
+  } else {
+    const double r = -G[ij];
+    const double num = M[ij] + lam[j] * r, den = lam[j] - lam[i];
+    e = (std::fabs(num) < kOaMax * std::fabs(den)) ? num / den : 0.5 * r;
+  }
+  E[ij] = e;
+}
+
+int EigenSolver::refine(hipStream_t s, const double* H, const double* S, const float* X0,
+                        int steps, int nvec, double* X, long ldx, std::vector<double>& lamHost) {
+  DH_CHECK(workSize >= 2 * (size_t)n * n, "refine: workspace too small");
+  steps = std::max(steps, 1);
+  double* Y = C.p;                  // H X or S X
+  double* M = work.p;               // X^T H X
+  double* G = T.p;                  // X^T S X
+  double* E = work.p + (size_t)n * n;
+  double* Xa = Zd.p;                // basis of the current step
+  DH_TRY((convert2d<float, double>(s, X0, n, Xa, n, n, n)));
+  for (int st = 0; st < steps; ++st) {
+    const bool last = st == steps - 1;
+    const int nv = last ? nvec : n;
+    DH_TRY(gemm<double>(s, false, false, n, n, n, 1.0, H, n, Xa, n, 0.0, Y, n));
+    DH_TRY(gemm<double>(s, true, false, n, n, n, 1.0, Xa, n, Y, n, 0.0, M, n, true));
+    DH_TRY(symmetrize_lower(s, M, n, n));
+    DH_TRY(gemm<double>(s, false, false, n, n, n, 1.0, S, n, Xa, n, 0.0, Y, n));
+    DH_TRY(gemm<double>(s, true, false, n, n, n, 1.0, Xa, n, Y, n, 0.0, G, n, true));
+    DH_TRY(symmetrize_lower(s, G, n, n));
+    hipLaunchKernelGGL(oa_lambda_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, M, G, n, w.p);
+    DH_LAUNCH_CHECK();
+    if (nv > 0) {
+      hipLaunchKernelGGL(oa_correction_kernel, dim3(cdiv(n, 256), nv), dim3(256), 0, s, M, G,
+                         w.p, n, nv, E);
+      DH_LAUNCH_CHECK();
+    }
+    // X' = X (I + E): into X (last step) or into Y (the next basis, then swapped with Xa)
+    double* out = last ? X : Y;
+    const long ldo = last ? ldx : n;
+    if (nv > 0) {
+      DH_TRY(copy2d<double>(s, Xa, n, out, ldo, n, nv));
+      DH_TRY(gemm<double>(s, false, false, n, nv, n, 1.0, Xa, n, E, n, 1.0, out, ldo));
+    }
+    if (!last) DH_TRY(copy2d<double>(s, Y, n, Xa, n, n, n));
+  }
+  lamHost.resize(n);
+  DH_HIP(hipMemcpyAsync(lamHost.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int EigenSolver::prepare_mixed(hipStream_t s) {
+  DH_CHECK(factored, "eigensolver: overlap not factorised");
+  DH_TRY(Linv32.alloc((size_t)n * n));
+  DH_TRY(d32.alloc(n));
+  DH_TRY(e32.alloc(std::max(n, 2)));
+  DH_TRY(tau32.alloc(std::max(n, 1)));
+  DH_TRY((convert2d<double, float>(s, Linv.p, n, Linv32.p, n, n, n)));
+  return 0;
+}
+
+// FP32 work arrays share the FP64 buffers: the reduced matrix and the Householder vectors live
+// in C, the GEMM temporary and the FP32 copy of the tridiagonal eigenvectors in T.
+int EigenSolver::solve_mixed(hipStream_t s, const float* H, long ldh, std::vector<double>& wHost,
+                             int* info, double* times) {
+  DH_CHECK(factored && Linv32.p, "eigensolver: mixed path not prepared");
+  *info = 0;
+  float* A32 = reinterpret_cast<float*>(C.p);
+  float* T32 = reinterpret_cast<float*>(T.p);
+  hipEvent_t ev[4];
+  for (auto& x : ev) DH_HIP(hipEventCreate(&x));
+  DH_HIP(hipEventRecord(ev[0], s));
+  DH_TRY(gemm<float>(s, false, false, n, n, n, 1.0f, Linv32.p, n, H, ldh, 0.0f, T32, n, false,
+                     kKALower));
+  DH_TRY(gemm<float>(s, false, true, n, n, n, 1.0f, T32, n, Linv32.p, n, 0.0f, A32, n, true,
+                     kKBUpper));
+  DH_HIP(hipEventRecord(ev[1], s));
+  DH_TRY(sytrd_lower_f32(s, A32, n, n, d32.p, e32.p, tau32.p, work.p, workSize * sizeof(double)));
+  DH_HIP(hipEventRecord(ev[2], s));
+  std::vector<float> df(n), ef(std::max(n - 1, 1));
+  DH_HIP(hipMemcpyAsync(df.data(), d32.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (n > 1)
+    DH_HIP(hipMemcpyAsync(ef.data(), e32.p, (n - 1) * sizeof(float), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  std::vector<double> dh(df.begin(), df.end()), eh(ef.begin(), ef.end());
+  DH_TRY(stedc(s, n, dh.data(), eh.data(), w.p, Zd.p, n, work.p, workSize, info));
+  DH_HIP(hipEventRecord(ev[3], s));
+  wHost.resize(n);
+  DH_HIP(hipMemcpyAsync(wHost.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  if (times) {
+    float ms;
+    for (int k = 0; k < 3; ++k) {
+      DH_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      times[k] += ms * 1e-3;
+    }
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  return 0;
+}
+
+int EigenSolver::vectors_mixed(hipStream_t s, int nvec, float* X, long ldx) {
+  if (nvec <= 0) return 0;
+  float* A32 = reinterpret_cast<float*>(C.p);
+  float* Z32 = reinterpret_cast<float*>(T.p);
+  DH_TRY((convert2d<double, float>(s, Zd.p, n, Z32, n, n, nvec)));
+  DH_TRY(ormtr_lower_f32(s, A32, n, n, tau32.p, Z32, n, nvec, work.p, workSize * sizeof(double)));
+  DH_TRY(gemm<float>(s, true, false, n, nvec, n, 1.0f, Linv32.p, n, Z32, n, 0.0f, X, ldx, false,
+                     kKAUpper));
+  return 0;
+}
+
 }  // namespace dftbhip

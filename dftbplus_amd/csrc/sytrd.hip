@@ -11,11 +11,13 @@
 // and a chunk of <= 8 column blocks; row-block partial sums and transposed-tile partial sums go
 // to workspaces that the next kernel reduces (no atomics on the vector). This product is the
 // HBM-bound part of the eigensolver (it streams the trailing matrix once per column).
+#include <type_traits>
+
 #include "common.h"
 
 namespace dftbhip {
 
-constexpr int kNB = 32;      // panel width
+constexpr int kNB = 64;      // panel width
 
 // Optional per-launch timing of the symmetric product kernel (HIP events on the launch stream):
 // launches, summed duration and the algorithmic bytes (lower triangle of the trailing matrix).
@@ -56,16 +58,17 @@ static int symv_prof_collect(hipStream_t s) {
 constexpr int kTB = 64;      // SYMV tile
 constexpr int kCH = 8;       // column tiles per SYMV workgroup
 
+template <typename T>
 struct SytrdWs {
-  double* W;       // n x kNB
-  double* Y;       // n x nSlot   partial sums of A22 v, one row of slots per matrix row
-  double* vbuf;    // n
+  T* W;            // n x kNB
+  T* Y;            // n x nSlot   partial sums of A22 v, one row of slots per matrix row
+  T* vbuf;         // n
   double* nrm2;    // n   ||x||^2 per column
   double* dot;     // n   w.v per column (before the rank-one correction of w)
   double* t12;     // n x 2kNB
   double* scal;    // n
-  double* P;       // n x 2kNB
-  double* Q;       // n x 2kNB
+  T* P;            // n x 2kNB
+  T* Q;            // n x 2kNB
   int nSlot;
 };
 
@@ -74,30 +77,38 @@ static int sytrd_slots(int n) {
   return nRB + (nRB + kCH - 1) / kCH;
 }
 
-static size_t sytrd_layout(int n, SytrdWs* ws, double* base) {
+// byte layout of the workspace (each array 256-byte aligned)
+template <typename T>
+static size_t sytrd_layout(int n, SytrdWs<T>* ws, char* base) {
   size_t off = 0;
-  auto take = [&](double** p, size_t cnt) {
-    if (ws) *p = base + off;
-    off += (cnt + 63) / 64 * 64;
+  auto take = [&](auto** p, size_t bytes) {
+    if (ws) *p = reinterpret_cast<std::remove_reference_t<decltype(**p)>*>(base + off);
+    off += (bytes + 255) / 256 * 256;
   };
-  SytrdWs dummy;
-  SytrdWs* w = ws ? ws : &dummy;
+  SytrdWs<T> dummy;
+  SytrdWs<T>* w = ws ? ws : &dummy;
   w->nSlot = sytrd_slots(n);
-  take(&w->W, (size_t)n * kNB);
-  take(&w->Y, (size_t)n * w->nSlot);
-  take(&w->vbuf, n);
-  take(&w->nrm2, n);
-  take(&w->dot, n);
-  take(&w->t12, (size_t)n * 2 * kNB);
-  take(&w->scal, n);
-  take(&w->P, (size_t)n * 2 * kNB);
-  take(&w->Q, (size_t)n * 2 * kNB);
+  take(&w->W, sizeof(T) * n * kNB);
+  take(&w->Y, sizeof(T) * n * w->nSlot);
+  take(&w->vbuf, sizeof(T) * n);
+  take(&w->nrm2, sizeof(double) * n);
+  take(&w->dot, sizeof(double) * n);
+  take(&w->t12, sizeof(double) * n * 2 * kNB);
+  take(&w->scal, sizeof(double) * n);
+  take(&w->P, sizeof(T) * n * 2 * kNB);
+  take(&w->Q, sizeof(T) * n * 2 * kNB);
   return off;
 }
 
-size_t sytrd_work_size(int n) { return sytrd_layout(n, nullptr, nullptr); }
+// workspace bytes
+size_t sytrd_work_bytes(int n, int elemSize) {
+  return elemSize == 8 ? sytrd_layout<double>(n, nullptr, nullptr)
+                       : sytrd_layout<float>(n, nullptr, nullptr);
+}
+size_t sytrd_work_size(int n) { return (sytrd_work_bytes(n, 8) + 7) / 8; }
 
-__device__ __forceinline__ double wave_sum(double v) {
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
 }
@@ -133,15 +144,16 @@ __device__ __forceinline__ void reflector(double alpha, double xn2, double& beta
 
 // W stores w before its rank-one correction: the true column p of the panel is
 // W(:,p) + alpha_p V(:,p) with alpha_p = -tau_p (w_p . v_p) / 2, applied by every reader.
-__device__ __forceinline__ double wcorr(int cp, const double* tau, const double* dot) {
-  return -0.5 * tau[cp] * dot[cp];
+template <typename T>
+__device__ __forceinline__ double wcorr(int cp, const T* tau, const double* dot) {
+  return -0.5 * (double)tau[cp] * dot[cp];
 }
 
 // A(r,c) -= sum_p V(r,p) W(c,p) + W(r,p) V(c,p), rows r >= c; accumulate ||A(c+2:n,c)||^2.
-__global__ __launch_bounds__(256) void col_update_kernel(double* a, long lda, int n, int c, int j0,
-                                                         const double* W, long ldw,
-                                                         const double* tau, const double* dot,
-                                                         double* nrm2) {
+template <typename T>
+__global__ __launch_bounds__(256) void col_update_kernel(T* a, long lda, int n, int c, int j0,
+                                                         const T* W, long ldw, const T* tau,
+                                                         const double* dot, double* nrm2) {
   __shared__ double vc[kNB], wc[kNB], al[kNB], sh[4];
   const int i = c - j0;
   if (threadIdx.x < i) {
@@ -160,7 +172,8 @@ __global__ __launch_bounds__(256) void col_update_kernel(double* a, long lda, in
       const double wr = W[r + (long)p * ldw] + al[p] * vr;
       v -= vr * wc[p] + wr * vc[p];
     }
-    a[r + (long)c * lda] = v;
+    a[r + (long)c * lda] = (T)v;
+    v = (double)(T)v;
     if (r >= c + 2) sq = v * v;
   }
   const double tot = block_reduce_sum(sq, sh);
@@ -173,14 +186,14 @@ __global__ __launch_bounds__(256) void col_update_kernel(double* a, long lda, in
 //   transposed sums (A^T v)_J -> Y(col, nRB + chunk)  (accumulated in registers over the tiles)
 // Thread (ii = tid % 64, cg = tid / 64) holds rows ii of 16 columns cg*16 .. cg*16+15.
 // The chunk-0 workgroup of each column block publishes v and the panel products t1, t2.
-__global__ __launch_bounds__(256) void symv_kernel(const double* a, long lda, int n, int c, int j0,
-                                                   const double* W, long ldw, const double* nrm2,
-                                                   const double* tauA, const double* dotA,
-                                                   double* Y, int nSlot, double* vbuf,
-                                                   double* t12, double* scalOut, double* eOut,
-                                                   double* tauOut) {
-  __shared__ double vJs[kTB];
-  __shared__ double red[2][4][kTB];
+template <typename T>
+__global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, int c, int j0,
+                                                   const T* W, long ldw, const double* nrm2,
+                                                   const T* tauA, const double* dotA, T* Y,
+                                                   int nSlot, T* vbuf, double* t12,
+                                                   double* scalOut, T* eOut, T* tauOut) {
+  __shared__ T vJs[kTB];
+  __shared__ T red[2][4][kTB];
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
   const int J = blockIdx.x, ch = blockIdx.y;
@@ -194,12 +207,12 @@ __global__ __launch_bounds__(256) void symv_kernel(const double* a, long lda, in
   reflector(alpha, nrm2[c], beta, tau, scal);
   if (J == 0 && ch == 0 && tid == 0) {
     scalOut[c] = scal;
-    eOut[c] = beta;
-    tauOut[c] = tau;
+    eOut[c] = (T)beta;
+    tauOut[c] = (T)tau;
   }
-  auto vval = [&](int k) -> double {
-    if (k >= m) return 0.0;
-    return (k == 0) ? 1.0 : a[(c + 1 + k) + (long)c * lda] * scal;
+  auto vval = [&](int k) -> T {
+    if (k >= m) return T(0);
+    return (k == 0) ? T(1) : (T)((double)a[(c + 1 + k) + (long)c * lda] * scal);
   };
   if (tid < kTB) vJs[tid] = vval(J * kTB + tid);
   __syncthreads();
@@ -214,37 +227,37 @@ __global__ __launch_bounds__(256) void symv_kernel(const double* a, long lda, in
         double x;
         if (p < i) {
           const double vr = a[r + (long)(j0 + p) * lda];
-          x = W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dotA) * vr;
+          x = (double)W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dotA) * vr;
         } else {
           x = a[r + (long)(j0 + p - i) * lda];
         }
-        prod = x * vJs[ii];
+        prod = x * (double)vJs[ii];
       }
       prod = wave_sum(prod);
       if (ii == 0 && prod != 0.0) atomicAdd(&t12[(long)c * 2 * kNB + p], prod);
     }
   }
-  double vj[16];
+  T vj[16];
 #pragma unroll
   for (int e = 0; e < 16; ++e) vj[e] = vJs[cg * 16 + e];
-  double b0[16], b1[16], b2[16], zacc[16];
+  T b0[16], b1[16], b2[16], zacc[16];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) zacc[e] = 0.0;
-  auto load = [&](int I, double* dst) {
+  for (int e = 0; e < 16; ++e) zacc[e] = T(0);
+  auto load = [&](int I, T* dst) {
     const int gr = I * kTB + ii;
     const bool rowOk = gr < m;
-    const double* src = a + (c + 1 + gr) + (long)cols0 * lda;
+    const T* src = a + (c + 1 + gr) + (long)cols0 * lda;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int jj = cg * 16 + e;
       const int gc = J * kTB + jj;
-      dst[e] = (rowOk && gc < m && (I > J || ii >= jj)) ? src[(long)jj * lda] : 0.0;
+      dst[e] = (rowOk && gc < m && (I > J || ii >= jj)) ? src[(long)jj * lda] : T(0);
     }
   };
   // one tile: row partials (reduced over the 4 waves through LDS) and transposed sums
-  auto tile = [&](int I, const double* t) {
-    const double vi = vval(I * kTB + ii);
-    double yp = 0.0;
+  auto tile = [&](int I, const T* t) {
+    const T vi = vval(I * kTB + ii);
+    T yp = T(0);
     if (I > J) {
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
@@ -282,7 +295,7 @@ __global__ __launch_bounds__(256) void symv_kernel(const double* a, long lda, in
   }
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
-    const double z = wave_sum(zacc[e]);
+    const T z = wave_sum(zacc[e]);
     const int k = J * kTB + cg * 16 + e;
     if (ii == 0 && k < m) Y[(long)k * nSlot + nRB + ch] = z;
   }
@@ -290,11 +303,11 @@ __global__ __launch_bounds__(256) void symv_kernel(const double* a, long lda, in
 
 // y(r) from its slots (one wave per row), w = tau * (y - V t1 - W t2); store v into column c
 // and accumulate w.v. 16 rows per 256-thread block.
-__global__ __launch_bounds__(256) void finalize_w_kernel(double* a, long lda, int n, int c, int j0,
-                                                         double* W, long ldw, const double* Y,
-                                                         int nSlot, const double* vbuf,
-                                                         const double* t12, const double* tauA,
-                                                         double* dot) {
+template <typename T>
+__global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, int c, int j0,
+                                                         T* W, long ldw, const T* Y, int nSlot,
+                                                         const T* vbuf, const double* t12,
+                                                         const T* tauA, double* dot) {
   __shared__ double sh[16];
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
@@ -308,23 +321,23 @@ __global__ __launch_bounds__(256) void finalize_w_kernel(double* a, long lda, in
       const int r = c + 1 + k;
       const int Ir = k / kTB;
       const int nch = (nRB - Ir + kCH - 1) / kCH;
-      const double* yr = Y + (long)k * nSlot;
+      const T* yr = Y + (long)k * nSlot;
       double acc = 0.0;
       for (int sl = lane; sl <= Ir; sl += 64) acc += yr[sl];
       for (int sl = lane; sl < nch; sl += 64) acc += yr[nRB + sl];
       if (lane < i) {
         const int p = lane;
         const double vr = a[r + (long)(j0 + p) * lda];
-        const double wr = W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dot) * vr;
+        const double wr = (double)W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dot) * vr;
         acc -= vr * t12[(long)c * 2 * kNB + p] + wr * t12[(long)c * 2 * kNB + i + p];
       }
       acc = wave_sum(acc);
       if (lane == 0) {
-        const double w = tau * acc;
+        const T w = (T)(tau * acc);
         W[r + (long)i * ldw] = w;
-        const double v = vbuf[r];
+        const T v = vbuf[r];
         a[r + (long)c * lda] = v;
-        prod = w * v;
+        prod = (double)w * (double)v;
       }
     }
     if (lane == 0) sh[wv * 4 + q] = prod;
@@ -338,23 +351,24 @@ __global__ __launch_bounds__(256) void finalize_w_kernel(double* a, long lda, in
 }
 
 // P = [V W], Q = [W V] for the trailing update, rows r0..n-1 (W with its rank-one correction).
-__global__ void panel_pack_kernel(const double* a, long lda, int n, int r0, int j0, int pw,
-                                  const double* W, long ldw, const double* tau, const double* dot,
-                                  double* P, double* Q, long ldp) {
+template <typename T>
+__global__ void panel_pack_kernel(const T* a, long lda, int n, int r0, int j0, int pw,
+                                  const T* W, long ldw, const T* tau, const double* dot, T* P,
+                                  T* Q, long ldp) {
   const int p = blockIdx.y;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = r0 + k;
   if (r >= n) return;
-  const double v = a[r + (long)(j0 + p) * lda];
-  const double w = W[r + (long)p * ldw] + wcorr(j0 + p, tau, dot) * v;
+  const T v = a[r + (long)(j0 + p) * lda];
+  const T w = (T)((double)W[r + (long)p * ldw] + wcorr(j0 + p, tau, dot) * (double)v);
   P[k + (long)p * ldp] = v;
   P[k + (long)(p + pw) * ldp] = w;
   Q[k + (long)p * ldp] = w;
   Q[k + (long)(p + pw) * ldp] = v;
 }
 
-__global__ void panel_restore_kernel(double* a, long lda, int n, int j0, int pw, double* d,
-                                     const double* e) {
+template <typename T>
+__global__ void panel_restore_kernel(T* a, long lda, int n, int j0, int pw, T* d, const T* e) {
   const int p = threadIdx.x;
   if (p >= pw) return;
   const int c = j0 + p;
@@ -362,23 +376,24 @@ __global__ void panel_restore_kernel(double* a, long lda, int n, int j0, int pw,
   if (c + 1 < n) a[(c + 1) + (long)c * lda] = e[c];
 }
 
-int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e, double* tau,
-                double* work, size_t workSize) {
-  DH_CHECK(workSize >= sytrd_work_size(n), "sytrd: workspace too small");
-  SytrdWs ws;
-  sytrd_layout(n, &ws, work);
+template <typename T>
+int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void* work,
+                  size_t workBytes) {
+  DH_CHECK(workBytes >= sytrd_layout<T>(n, nullptr, nullptr), "sytrd: workspace too small");
+  SytrdWs<T> ws;
+  sytrd_layout<T>(n, &ws, static_cast<char*>(work));
   DH_TRY(fill<double>(s, ws.nrm2, n, 0.0));
   DH_TRY(fill<double>(s, ws.dot, n, 0.0));
   DH_TRY(fill<double>(s, ws.t12, (size_t)n * 2 * kNB, 0.0));
-  DH_TRY(fill<double>(s, e, std::max(n - 1, 1), 0.0));
-  DH_TRY(fill<double>(s, tau, std::max(n, 1), 0.0));
+  DH_TRY(fill<T>(s, e, std::max(n - 1, 1), T(0)));
+  DH_TRY(fill<T>(s, tau, std::max(n, 1), T(0)));
   const long ldw = n;
   for (int j0 = 0; j0 < n; j0 += kNB) {
     const int pw = std::min(kNB, n - j0);
     for (int i = 0; i < pw; ++i) {
       const int c = j0 + i;
-      hipLaunchKernelGGL(col_update_kernel, dim3(cdiv(n - c, 256)), dim3(256), 0, s, a, lda, n, c,
-                         j0, ws.W, ldw, tau, ws.dot, ws.nrm2);
+      hipLaunchKernelGGL(col_update_kernel<T>, dim3(cdiv(n - c, 256)), dim3(256), 0, s, a, lda, n,
+                         c, j0, ws.W, ldw, tau, ws.dot, ws.nrm2);
       DH_LAUNCH_CHECK();
       const int m = n - c - 1;
       if (m <= 0) continue;
@@ -394,31 +409,31 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
         }
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used], s));
       }
-      hipLaunchKernelGGL(symv_kernel, dim3(nRB, nCh), dim3(256), 0, s, a, lda, n, c, j0, ws.W,
+      hipLaunchKernelGGL(symv_kernel<T>, dim3(nRB, nCh), dim3(256), 0, s, a, lda, n, c, j0, ws.W,
                          ldw, ws.nrm2, tau, ws.dot, ws.Y, ws.nSlot, ws.vbuf, ws.t12, ws.scal, e,
                          tau);
       DH_LAUNCH_CHECK();
       if (pf.on) {
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used + 1], s));
         // lower triangle of the m x m trailing matrix plus v in and the partial sums out
-        pf.bytes[pf.used] = 8.0 * ((double)m * (m + 1) / 2 + 2.0 * m);
+        pf.bytes[pf.used] = sizeof(T) * ((double)m * (m + 1) / 2 + 2.0 * m);
         ++pf.used;
       }
-      hipLaunchKernelGGL(finalize_w_kernel, dim3(cdiv(m, 16)), dim3(256), 0, s, a, lda, n, c, j0,
+      hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 16)), dim3(256), 0, s, a, lda, n, c, j0,
                          ws.W, ldw, ws.Y, ws.nSlot, ws.vbuf, ws.t12, tau, ws.dot);
       DH_LAUNCH_CHECK();
     }
     const int r0 = j0 + pw;
     if (r0 < n) {
       const int mt = n - r0;
-      hipLaunchKernelGGL(panel_pack_kernel, dim3(cdiv(mt, 256), pw), dim3(256), 0, s, a, lda, n,
+      hipLaunchKernelGGL(panel_pack_kernel<T>, dim3(cdiv(mt, 256), pw), dim3(256), 0, s, a, lda, n,
                          r0, j0, pw, ws.W, ldw, tau, ws.dot, ws.P, ws.Q, (long)mt);
       DH_LAUNCH_CHECK();
-      double* a22 = a + r0 + (long)r0 * lda;
-      DH_TRY(gemm<double>(s, false, true, mt, mt, 2 * pw, -1.0, ws.P, mt, ws.Q, mt, 1.0, a22, lda,
-                          true));
+      T* a22 = a + r0 + (long)r0 * lda;
+      DH_TRY(gemm<T>(s, false, true, mt, mt, 2 * pw, T(-1), ws.P, mt, ws.Q, mt, T(1), a22, lda,
+                     true));
     }
-    hipLaunchKernelGGL(panel_restore_kernel, dim3(1), dim3(64), 0, s, a, lda, n, j0, pw, d, e);
+    hipLaunchKernelGGL(panel_restore_kernel<T>, dim3(1), dim3(64), 0, s, a, lda, n, j0, pw, d, e);
     DH_LAUNCH_CHECK();
   }
   DH_TRY(symv_prof_collect(s));
@@ -429,97 +444,123 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
 // Back-transformation Z <- Q Z, Q = H(0) H(1) ... H(n-2), panels applied last to first with the
 // compact WY form I - V T V^T (T upper triangular, LAPACK dlarft 'F','C').
 // ---------------------------------------------------------------------------------------------
+template <typename T>
 struct OrmWs {
-  double* V;    // n x kOB
-  double* G;    // kOB x kOB
-  double* T;    // kOB x kOB
-  double* W1;   // kOB x ncols
-  double* W2;   // kOB x ncols
+  T* V;         // n x kOB
+  T* G;         // kOB x kOB
+  T* Tm;        // kOB x kOB
+  T* W1;        // kOB x ncols
+  T* W2;        // kOB x ncols
 };
 
 constexpr int kOB = 128;   // reflectors per block of the back-transformation
 
-static size_t ormtr_layout(int n, int ncols, OrmWs* ws, double* base) {
+template <typename T>
+static size_t ormtr_layout(int n, int ncols, OrmWs<T>* ws, char* base) {
   size_t off = 0;
-  OrmWs dummy;
-  OrmWs* w = ws ? ws : &dummy;
-  auto take = [&](double** p, size_t cnt) {
-    if (ws) *p = base + off;
-    off += (cnt + 63) / 64 * 64;
+  OrmWs<T> dummy;
+  OrmWs<T>* w = ws ? ws : &dummy;
+  auto take = [&](T** p, size_t cnt) {
+    if (ws) *p = reinterpret_cast<T*>(base + off);
+    off += (cnt * sizeof(T) + 255) / 256 * 256;
   };
   take(&w->V, (size_t)n * kOB);
   take(&w->G, kOB * kOB);
-  take(&w->T, kOB * kOB);
+  take(&w->Tm, kOB * kOB);
   take(&w->W1, (size_t)kOB * std::max(ncols, 1));
   take(&w->W2, (size_t)kOB * std::max(ncols, 1));
   return off;
 }
 
-size_t ormtr_work_size(int n, int ncols) { return ormtr_layout(n, ncols, nullptr, nullptr); }
+size_t ormtr_work_bytes(int n, int ncols, int elemSize) {
+  return elemSize == 8 ? ormtr_layout<double>(n, ncols, nullptr, nullptr)
+                       : ormtr_layout<float>(n, ncols, nullptr, nullptr);
+}
+size_t ormtr_work_size(int n, int ncols) { return (ormtr_work_bytes(n, ncols, 8) + 7) / 8; }
 
 // explicit V (rows j0+1..n-1) of the block's reflectors
-__global__ void form_v_kernel(const double* a, long lda, int n, int j0, int pw, double* V,
-                              long ldv) {
+template <typename T>
+__global__ void form_v_kernel(const T* a, long lda, int n, int j0, int pw, T* V, long ldv) {
   const int p = blockIdx.y;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = j0 + 1 + k;
   if (r >= n) return;
   const int c = j0 + p;
-  double v;
-  if (r < c + 1) v = 0.0;
-  else if (r == c + 1) v = 1.0;
+  T v;
+  if (r < c + 1) v = T(0);
+  else if (r == c + 1) v = T(1);
   else v = a[r + (long)c * lda];
   V[k + (long)p * ldv] = v;
 }
 
 // T(i,i) = tau_i, T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), G = V^T V   (LAPACK dlarft 'F','C')
-__global__ __launch_bounds__(128) void larft_kernel(const double* G, const double* tau, int j0,
-                                                    int pw, double* T) {
-  __shared__ double Ts[kOB][kOB + 1];
+template <typename T>
+__global__ __launch_bounds__(128) void larft_kernel(const T* G, const T* tau, int j0, int pw,
+                                                    T* Tout) {
+  __shared__ T Ts[kOB][kOB + 1];
   const int t = threadIdx.x;
   for (int e = t; e < kOB * kOB; e += blockDim.x) Ts[e % kOB][e / kOB] = 0.0;
   __syncthreads();
   for (int i = 0; i < pw; ++i) {
-    const double ti = tau[j0 + i];
+    const T ti = tau[j0 + i];
     if (t < i) {
-      double acc = 0.0;
+      T acc = T(0);
       for (int k = t; k < i; ++k) acc += Ts[t][k] * G[k + i * kOB];
       Ts[t][i] = -ti * acc;
     }
     if (t == 0) Ts[i][i] = ti;
     __syncthreads();
   }
-  for (int e = t; e < kOB * kOB; e += blockDim.x) T[e] = Ts[e % kOB][e / kOB];
+  for (int e = t; e < kOB * kOB; e += blockDim.x) Tout[e] = Ts[e % kOB][e / kOB];
 }
 
-int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* tau, double* z,
-                long ldz, int ncols, double* work, size_t workSize) {
-  DH_CHECK(workSize >= ormtr_work_size(n, ncols), "ormtr: workspace too small");
+template <typename T>
+int ormtr_lower_t(hipStream_t s, const T* a, long lda, int n, const T* tau, T* z, long ldz,
+                  int ncols, void* work, size_t workBytes) {
+  DH_CHECK(workBytes >= ormtr_layout<T>(n, ncols, nullptr, nullptr), "ormtr: workspace too small");
   if (n <= 1 || ncols <= 0) return 0;
-  OrmWs ws;
-  ormtr_layout(n, ncols, &ws, work);
+  OrmWs<T> ws;
+  ormtr_layout<T>(n, ncols, &ws, static_cast<char*>(work));
   const int nref = n - 1;
   const int lastBlock = ((nref - 1) / kOB) * kOB;
   for (int j0 = lastBlock; j0 >= 0; j0 -= kOB) {
     const int pw = std::min(kOB, nref - j0);
     const int m = n - j0 - 1;
-    hipLaunchKernelGGL(form_v_kernel, dim3(cdiv(m, 256), pw), dim3(256), 0, s, a, lda, n, j0, pw,
-                       ws.V, (long)m);
+    hipLaunchKernelGGL(form_v_kernel<T>, dim3(cdiv(m, 256), pw), dim3(256), 0, s, a, lda, n, j0,
+                       pw, ws.V, (long)m);
     DH_LAUNCH_CHECK();
     const int sk = std::max(1, std::min(64, m / 512));
-    DH_TRY(gemm<double>(s, true, false, pw, pw, m, 1.0, ws.V, m, ws.V, m, 0.0, ws.G, kOB, false,
-                        kKFull, nullptr, sk));
-    hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(128), 0, s, ws.G, tau, j0, pw, ws.T);
+    DH_TRY(gemm<T>(s, true, false, pw, pw, m, T(1), ws.V, m, ws.V, m, T(0), ws.G, kOB, false,
+                   kKFull, nullptr, sk));
+    hipLaunchKernelGGL(larft_kernel<T>, dim3(1), dim3(128), 0, s, ws.G, tau, j0, pw, ws.Tm);
     DH_LAUNCH_CHECK();
-    double* zr = z + (j0 + 1);
+    T* zr = z + (j0 + 1);
     const int skz = std::max(1, std::min(16, m / 1024));
-    DH_TRY(gemm<double>(s, true, false, pw, ncols, m, 1.0, ws.V, m, zr, ldz, 0.0, ws.W1, kOB,
-                        false, kKFull, nullptr, skz));
-    DH_TRY(gemm<double>(s, false, false, pw, ncols, pw, 1.0, ws.T, kOB, ws.W1, kOB, 0.0, ws.W2,
-                        kOB));
-    DH_TRY(gemm<double>(s, false, false, m, ncols, pw, -1.0, ws.V, m, ws.W2, kOB, 1.0, zr, ldz));
+    DH_TRY(gemm<T>(s, true, false, pw, ncols, m, T(1), ws.V, m, zr, ldz, T(0), ws.W1, kOB, false,
+                   kKFull, nullptr, skz));
+    DH_TRY(gemm<T>(s, false, false, pw, ncols, pw, T(1), ws.Tm, kOB, ws.W1, kOB, T(0), ws.W2,
+                   kOB));
+    DH_TRY(gemm<T>(s, false, false, m, ncols, pw, T(-1), ws.V, m, ws.W2, kOB, T(1), zr, ldz));
   }
   return 0;
+}
+
+
+int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e, double* tau,
+                double* work, size_t workSize) {
+  return sytrd_lower_t<double>(s, a, lda, n, d, e, tau, work, workSize * sizeof(double));
+}
+int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e, float* tau,
+                    void* work, size_t workBytes) {
+  return sytrd_lower_t<float>(s, a, lda, n, d, e, tau, work, workBytes);
+}
+int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* tau, double* z,
+                long ldz, int ncols, double* work, size_t workSize) {
+  return ormtr_lower_t<double>(s, a, lda, n, tau, z, ldz, ncols, work, workSize * sizeof(double));
+}
+int ormtr_lower_f32(hipStream_t s, const float* a, long lda, int n, const float* tau, float* z,
+                    long ldz, int ncols, void* work, size_t workBytes) {
+  return ormtr_lower_t<float>(s, a, lda, n, tau, z, ldz, ncols, work, workBytes);
 }
 
 }  // namespace dftbhip

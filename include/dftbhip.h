@@ -110,6 +110,11 @@ typedef struct {
   double ewald_tol;
   int force_mode;       /* 0: finite differences (reference default), 1: analytic adjoint */
   int precision;        /* 0: FP64, 1: mixed (FP32 MFMA bulk + FP64 refinement) */
+  /* mixed precision only: an SCC iteration refines its eigenpairs in FP64 (Ogita-Aishima,
+   * refine_steps steps, then FP64 density matrices) when the charge error of the previous
+   * iteration is below refine_below (0: never, a huge value: every iteration) */
+  double refine_below;
+  int refine_steps;
 } dftbhip_options;
 
 typedef struct dftbhip_ctx dftbhip_ctx;

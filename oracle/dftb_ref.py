@@ -722,6 +722,13 @@ def _electron_fill(ef, eig, kT):
     return f, eband, ts
 
 
+def levels(f: np.ndarray) -> slice:
+    """Levels entering the density matrices: up to the last filling >= machine epsilon
+    (densitymatrix.F90:549-554, 688-692)."""
+    big = np.nonzero(np.abs(f) >= np.finfo(float).eps)[0]
+    return slice(0, int(big[-1]) + 1 if big.size else len(f))
+
+
 def fermi_filling(eig: np.ndarray, nel: float, kT: float):
     """Spin-unpolarised Fermi filling of one k-point (``getFillingsAndBandEnergies``,
     main.F90:4544-4651 -> ``Efilling``): returns (f, Ef, Eband, TS, E0) with the factor 2."""
@@ -932,7 +939,7 @@ class Calculator:
             h = add_shift(self.h0, self.s, vorb, self.geo.species, self.sp, self.off)
             eig, C = self.solve(h, S)
             f, ef, eband, ts, e0 = fermi_filling(eig, self.nel, o.kT)
-            occ = f > 0
+            occ = levels(f)
             Cf = C[:, occ] * f[occ]
             P = Cf @ C[:, occ].T
             rho = pack(P, self.params, self.geo.species, self.sp, self.off)
@@ -959,7 +966,7 @@ class Calculator:
         """Total energy gradient: 2 Sum(rho dH0 - W dS) + shift dS term (forces.F90:548-672),
         gamma derivative (shortgamma.F90:349-411, coulomb.F90:1140-1187/1380-1494), repulsive."""
         C, eig, f = res._C, res._eig, res._f
-        occ = f > 0
+        occ = levels(f)
         W = (C[:, occ] * (f[occ] * eig[occ])) @ C[:, occ].T
         erho = pack(W, self.params, self.geo.species, self.sp, self.off)
         res.erho = erho
