@@ -81,6 +81,7 @@ class COptions(C.Structure):
         ("precision", C.c_int),
         ("refine_below", C.c_double),
         ("refine_steps", C.c_int),
+        ("tridiag", C.c_int),
     ]
 
 
@@ -193,7 +194,7 @@ def make_params(params: Parameters):
 
 
 def make_options(opts: Options, force_mode: int = 0, precision: int = 0,
-                 refine_below: float = 1e-4, refine_steps: int = 2) -> COptions:
+                 refine_below: float = 1e-4, refine_steps: int = 2, tridiag: int = 1) -> COptions:
     o = COptions()
     o.kT = opts.kT
     o.scc_tol = opts.scc_tol
@@ -210,6 +211,7 @@ def make_options(opts: Options, force_mode: int = 0, precision: int = 0,
     o.precision = precision
     o.refine_below = refine_below
     o.refine_steps = refine_steps
+    o.tridiag = tridiag
     return o
 
 
@@ -366,10 +368,11 @@ class DftbPlus:
 
     def __init__(self, params: Parameters, geometry: Geometry, options: Options = Options(),
                  force_mode: int = 0, precision: int = 0, refine_below: float = 1e-4,
-                 refine_steps: int = 2):
+                 refine_steps: int = 2, tridiag: int = 1):
         self.params = params
         self._p, self._keep = make_params(params)
-        self._o = make_options(options, force_mode, precision, refine_below, refine_steps)
+        self._o = make_options(options, force_mode, precision, refine_below, refine_steps,
+                               tridiag)
         self.h = C.c_void_p()
         _check(lib().dftbhip_ctx_create(C.byref(self.h), C.byref(self._p), C.byref(self._o)),
                "dftbhip_ctx_create")

@@ -1198,6 +1198,7 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
                                n));
   DH_TRY(c.eig.ensure(n));
   c.eig.factored = false;
+  c.eig.twoStage = c.opt.tridiag == 0;
   int info = 0;
   DH_TRY(c.eig.factor_overlap(s, c.Sd.p, n, &info));
   if (info != 0) {
@@ -1509,6 +1510,7 @@ extern "C" int dftbhip_ctx_create(dftbhip_ctx** ctx, const dftbhip_params* p,
   DH_CHECK(opt->precision == 0 || opt->precision == 1,
            "ctx_create: precision must be 0 (FP64) or 1 (mixed FP32/FP64)");
   DH_CHECK(opt->max_scc_iter >= 1, "ctx_create: max_scc_iter < 1");
+  DH_CHECK(opt->tridiag == 0 || opt->tridiag == 1, "ctx_create: tridiag must be 0 or 1");
   DH_TRY(ensure_runtime());
   auto* h = new dftbhip_ctx();
   h->c.opt = *opt;

@@ -139,6 +139,7 @@ int copy2d(hipStream_t s, const T* src, long lds, T* dst, long ldd, int m, int n
 template <typename Tin, typename Tout>
 int convert2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n);
 int symmetrize_lower(hipStream_t s, double* a, long lda, int n);
+int symmetrize_lower(hipStream_t s, float* a, long lda, int n);
 int zero_strict_upper(hipStream_t s, double* a, long lda, int n);
 
 
@@ -160,6 +161,17 @@ int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e
 int ormtr_lower_f32(hipStream_t s, const float* a, long lda, int n, const float* tau, float* z,
                     long ldz, int ncols, void* work, size_t workBytes);
 size_t sytrd_work_bytes(int n, int elemSize);
+// two-stage tridiagonalisation (band.hip): full symmetric input, workspace in bytes
+size_t band_work_bytes(int n, int elemSize);
+int sytrd_2stage(hipStream_t s, double* a, long lda, int n, double* d, double* e, void* work,
+                 size_t workBytes);
+int sytrd_2stage_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e, void* work,
+                     size_t workBytes);
+// Z <- Q Z; work is the tridiagonalisation's workspace, work2 that of ormtr (n, ncols)
+int ormtr_2stage(hipStream_t s, const double* a, long lda, int n, double* z, long ldz, int ncols,
+                 void* work, size_t workBytes, void* work2, size_t work2Bytes);
+int ormtr_2stage_f32(hipStream_t s, const float* a, long lda, int n, float* z, long ldz,
+                     int ncols, void* work, size_t workBytes, void* work2, size_t work2Bytes);
 size_t ormtr_work_bytes(int n, int ncols, int elemSize);
 int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double* w, double* z,
           long ldz, double* work, size_t workSize, int* info);

@@ -65,14 +65,15 @@ int copy2d(hipStream_t s, const T* src, long lds, T* dst, long ldd, int m, int n
 }
 
 // a(j, i) = a(i, j) for i > j, 32x32 tiles through LDS.
-__global__ void symmetrize_lower_kernel(double* a, long lda, int n) {
-  __shared__ double t[32][33];
+template <typename T>
+__global__ void symmetrize_lower_kernel(T* a, long lda, int n) {
+  __shared__ T t[32][33];
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bj > bi) return;
   const int tx = threadIdx.x, ty = threadIdx.y;   // 32 x 8
   for (int k = ty; k < 32; k += 8) {
     const int i = bi * 32 + tx, j = bj * 32 + k;
-    t[k][tx] = (i < n && j < n) ? a[i + (long)j * lda] : 0.0;
+    t[k][tx] = (i < n && j < n) ? a[i + (long)j * lda] : T(0);
   }
   __syncthreads();
   for (int k = ty; k < 32; k += 8) {
@@ -87,12 +88,19 @@ __global__ void zero_upper_kernel(double* a, long lda, int n) {
     a[i + (long)j * lda] = 0.0;
 }
 
-int symmetrize_lower(hipStream_t s, double* a, long lda, int n) {
+template <typename T>
+static int symmetrize_lower_t(hipStream_t s, T* a, long lda, int n) {
   if (n <= 1) return 0;
   dim3 grid(cdiv(n, 32), cdiv(n, 32));
-  hipLaunchKernelGGL(symmetrize_lower_kernel, grid, dim3(32, 8), 0, s, a, lda, n);
+  hipLaunchKernelGGL(symmetrize_lower_kernel<T>, grid, dim3(32, 8), 0, s, a, lda, n);
   DH_LAUNCH_CHECK();
   return 0;
+}
+int symmetrize_lower(hipStream_t s, double* a, long lda, int n) {
+  return symmetrize_lower_t<double>(s, a, lda, n);
+}
+int symmetrize_lower(hipStream_t s, float* a, long lda, int n) {
+  return symmetrize_lower_t<float>(s, a, lda, n);
 }
 
 int zero_strict_upper(hipStream_t s, double* a, long lda, int n) {

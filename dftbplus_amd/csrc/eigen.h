@@ -13,11 +13,14 @@ struct EigenSolver {
   DevBuf<double> L, Linv, C, Zd, T, d, e, tau, w, work;
   DevBuf<int> dinfo;
   DevBuf<float> Linv32, d32, e32, tau32;
-  size_t workSize = 0;
+  DevBuf<double> band;          // two-stage tridiagonalisation workspace (reflectors of stage 2)
+  size_t workSize = 0, bandBytes = 0;
+  bool twoStage = true;         // two-stage (band) tridiagonalisation, else one-stage
 
   int ensure(int n_);
   void release() {
-    for (DevBuf<double>* b : {&L, &Linv, &C, &Zd, &T, &d, &e, &tau, &w, &work}) b->release();
+    for (DevBuf<double>* b : {&L, &Linv, &C, &Zd, &T, &d, &e, &tau, &w, &work, &band})
+      b->release();
     for (DevBuf<float>* b : {&Linv32, &d32, &e32, &tau32}) b->release();
     dinfo.release();
     n = 0;

@@ -33,6 +33,8 @@ int EigenSolver::ensure(int n_) {
   workSize = std::max(workSize, ormtr_work_size(n, n));
   workSize = std::max(workSize, nn / 2 + 64 * (size_t)n + 4096);
   DH_TRY(work.alloc(workSize));
+  bandBytes = band_work_bytes(n, 8);
+  DH_TRY(band.alloc((bandBytes + 7) / 8));
   return 0;
 }
 
@@ -61,7 +63,12 @@ int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<dou
   DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, T.p, n, Linv.p, n, 0.0, C.p, n, true,
                       kKBUpper));
   DH_HIP(hipEventRecord(ev[1], s));
-  DH_TRY(sytrd_lower(s, C.p, n, n, d.p, e.p, tau.p, work.p, workSize));
+  if (twoStage) {
+    DH_TRY(symmetrize_lower(s, C.p, n, n));
+    DH_TRY(sytrd_2stage(s, C.p, n, n, d.p, e.p, band.p, bandBytes));
+  } else {
+    DH_TRY(sytrd_lower(s, C.p, n, n, d.p, e.p, tau.p, work.p, workSize));
+  }
   DH_HIP(hipEventRecord(ev[2], s));
   std::vector<double> dh(n), eh(std::max(n - 1, 1));
   DH_HIP(hipMemcpyAsync(dh.data(), d.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -88,7 +95,11 @@ int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<dou
 
 int EigenSolver::vectors(hipStream_t s, int nvec, double* X, long ldx) {
   if (nvec <= 0) return 0;
-  DH_TRY(ormtr_lower(s, C.p, n, n, tau.p, Zd.p, n, nvec, work.p, workSize));
+  if (twoStage)
+    DH_TRY(ormtr_2stage(s, C.p, n, n, Zd.p, n, nvec, band.p, bandBytes, work.p,
+                        workSize * sizeof(double)));
+  else
+    DH_TRY(ormtr_lower(s, C.p, n, n, tau.p, Zd.p, n, nvec, work.p, workSize));
   DH_TRY(gemm<double>(s, true, false, n, nvec, n, 1.0, Linv.p, n, Zd.p, n, 0.0, X, ldx, false,
                       kKAUpper));
   return 0;
@@ -191,7 +202,12 @@ int EigenSolver::solve_mixed(hipStream_t s, const float* H, long ldh, std::vecto
   DH_TRY(gemm<float>(s, false, true, n, n, n, 1.0f, T32, n, Linv32.p, n, 0.0f, A32, n, true,
                      kKBUpper));
   DH_HIP(hipEventRecord(ev[1], s));
-  DH_TRY(sytrd_lower_f32(s, A32, n, n, d32.p, e32.p, tau32.p, work.p, workSize * sizeof(double)));
+  if (twoStage) {
+    DH_TRY(symmetrize_lower(s, A32, n, n));
+    DH_TRY(sytrd_2stage_f32(s, A32, n, n, d32.p, e32.p, band.p, bandBytes));
+  } else {
+    DH_TRY(sytrd_lower_f32(s, A32, n, n, d32.p, e32.p, tau32.p, work.p, workSize * sizeof(double)));
+  }
   DH_HIP(hipEventRecord(ev[2], s));
   std::vector<float> df(n), ef(std::max(n - 1, 1));
   DH_HIP(hipMemcpyAsync(df.data(), d32.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -220,7 +236,12 @@ int EigenSolver::vectors_mixed(hipStream_t s, int nvec, float* X, long ldx) {
   float* A32 = reinterpret_cast<float*>(C.p);
   float* Z32 = reinterpret_cast<float*>(T.p);
   DH_TRY((convert2d<double, float>(s, Zd.p, n, Z32, n, n, nvec)));
-  DH_TRY(ormtr_lower_f32(s, A32, n, n, tau32.p, Z32, n, nvec, work.p, workSize * sizeof(double)));
+  if (twoStage)
+    DH_TRY(ormtr_2stage_f32(s, A32, n, n, Z32, n, nvec, band.p, bandBytes, work.p,
+                            workSize * sizeof(double)));
+  else
+    DH_TRY(ormtr_lower_f32(s, A32, n, n, tau32.p, Z32, n, nvec, work.p,
+                           workSize * sizeof(double)));
   DH_TRY(gemm<float>(s, true, false, n, nvec, n, 1.0f, Linv32.p, n, Z32, n, 0.0f, X, ldx, false,
                      kKAUpper));
   return 0;

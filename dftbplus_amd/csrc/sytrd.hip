@@ -478,17 +478,19 @@ size_t ormtr_work_bytes(int n, int ncols, int elemSize) {
 }
 size_t ormtr_work_size(int n, int ncols) { return (ormtr_work_bytes(n, ncols, 8) + 7) / 8; }
 
-// explicit V (rows j0+1..n-1) of the block's reflectors
+// explicit V (rows j0+shift..n-1) of the block's reflectors; reflector c has its unit element
+// at row c + shift and its other elements below it in column c of a
 template <typename T>
-__global__ void form_v_kernel(const T* a, long lda, int n, int j0, int pw, T* V, long ldv) {
+__global__ void form_v_kernel(const T* a, long lda, int n, int j0, int pw, int shift, T* V,
+                              long ldv) {
   const int p = blockIdx.y;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = j0 + 1 + k;
+  const int r = j0 + shift + k;
   if (r >= n) return;
   const int c = j0 + p;
   T v;
-  if (r < c + 1) v = T(0);
-  else if (r == c + 1) v = T(1);
+  if (r < c + shift) v = T(0);
+  else if (r == c + shift) v = T(1);
   else v = a[r + (long)c * lda];
   V[k + (long)p * ldv] = v;
 }
@@ -516,25 +518,25 @@ __global__ __launch_bounds__(128) void larft_kernel(const T* G, const T* tau, in
 
 template <typename T>
 int ormtr_lower_t(hipStream_t s, const T* a, long lda, int n, const T* tau, T* z, long ldz,
-                  int ncols, void* work, size_t workBytes) {
+                  int ncols, void* work, size_t workBytes, int shift, int nref) {
   DH_CHECK(workBytes >= ormtr_layout<T>(n, ncols, nullptr, nullptr), "ormtr: workspace too small");
-  if (n <= 1 || ncols <= 0) return 0;
+  if (nref <= 0 || ncols <= 0) return 0;
   OrmWs<T> ws;
   ormtr_layout<T>(n, ncols, &ws, static_cast<char*>(work));
-  const int nref = n - 1;
   const int lastBlock = ((nref - 1) / kOB) * kOB;
   for (int j0 = lastBlock; j0 >= 0; j0 -= kOB) {
     const int pw = std::min(kOB, nref - j0);
-    const int m = n - j0 - 1;
+    const int m = n - j0 - shift;
+    if (m <= 0) continue;
     hipLaunchKernelGGL(form_v_kernel<T>, dim3(cdiv(m, 256), pw), dim3(256), 0, s, a, lda, n, j0,
-                       pw, ws.V, (long)m);
+                       pw, shift, ws.V, (long)m);
     DH_LAUNCH_CHECK();
     const int sk = std::max(1, std::min(64, m / 512));
     DH_TRY(gemm<T>(s, true, false, pw, pw, m, T(1), ws.V, m, ws.V, m, T(0), ws.G, kOB, false,
                    kKFull, nullptr, sk));
     hipLaunchKernelGGL(larft_kernel<T>, dim3(1), dim3(128), 0, s, ws.G, tau, j0, pw, ws.Tm);
     DH_LAUNCH_CHECK();
-    T* zr = z + (j0 + 1);
+    T* zr = z + (j0 + shift);
     const int skz = std::max(1, std::min(16, m / 1024));
     DH_TRY(gemm<T>(s, true, false, pw, ncols, m, T(1), ws.V, m, zr, ldz, T(0), ws.W1, kOB, false,
                    kKFull, nullptr, skz));
@@ -544,7 +546,6 @@ int ormtr_lower_t(hipStream_t s, const T* a, long lda, int n, const T* tau, T* z
   }
   return 0;
 }
-
 
 int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e, double* tau,
                 double* work, size_t workSize) {
@@ -556,11 +557,23 @@ int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e
 }
 int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* tau, double* z,
                 long ldz, int ncols, double* work, size_t workSize) {
-  return ormtr_lower_t<double>(s, a, lda, n, tau, z, ldz, ncols, work, workSize * sizeof(double));
+  return ormtr_lower_t<double>(s, a, lda, n, tau, z, ldz, ncols, work, workSize * sizeof(double),
+                               1, n - 1);
 }
 int ormtr_lower_f32(hipStream_t s, const float* a, long lda, int n, const float* tau, float* z,
                     long ldz, int ncols, void* work, size_t workBytes) {
-  return ormtr_lower_t<float>(s, a, lda, n, tau, z, ldz, ncols, work, workBytes);
+  return ormtr_lower_t<float>(s, a, lda, n, tau, z, ldz, ncols, work, workBytes, 1, n - 1);
 }
+// back-transformation with the reflectors of the band reduction (unit element `shift` rows
+// below the column, nref reflectors)
+template <typename T>
+int ormqr_shift(hipStream_t s, const T* a, long lda, int n, const T* tau, T* z, long ldz,
+                int ncols, void* work, size_t workBytes, int shift, int nref) {
+  return ormtr_lower_t<T>(s, a, lda, n, tau, z, ldz, ncols, work, workBytes, shift, nref);
+}
+template int ormqr_shift<double>(hipStream_t, const double*, long, int, const double*, double*,
+                                 long, int, void*, size_t, int, int);
+template int ormqr_shift<float>(hipStream_t, const float*, long, int, const float*, float*, long,
+                                int, void*, size_t, int, int);
 
 }  // namespace dftbhip

@@ -8,11 +8,12 @@ pytestmark = pytest.mark.gpu
 E_TOL, Q_TOL, F_TOL = 1e-8, 1e-8, 1e-7
 
 
+@pytest.mark.parametrize("tridiag", [0, 1], ids=["two_stage", "one_stage"])
 @pytest.mark.parametrize("system", ["cluster216", "diamond2"])
-def test_scc_energy_charges_forces(system, request):
+def test_scc_energy_charges_forces(system, tridiag, request):
     from dftbplus_amd import api
     geo, params, opts, calc, res = request.getfixturevalue(system)
-    dp = api.DftbPlus(params, geo, opts)
+    dp = api.DftbPlus(params, geo, opts, tridiag=tridiag)
     e = dp.get_energy()
     q = dp.get_gross_charges()
     g = dp.get_gradients()
