@@ -24,6 +24,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "mfma.h"
 
 namespace dftbhip {
 
@@ -220,7 +221,9 @@ __global__ __launch_bounds__(256) void sb2st_kernel(T* AB, int n, int nsweep, in
           const int need = min(k + 3, chase_steps(n, s - 1));
           long spins = 0;
           int ab = 0;
-          while (__hip_atomic_load(&prog[s - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+          // relaxed polls (an acquire load would invalidate the L2 on every poll); one
+          // acquire fence after the wait
+          while (__hip_atomic_load(&prog[s - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
                  need) {
             ab = __hip_atomic_load(abortFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (ab != 0) break;
@@ -378,9 +381,11 @@ __device__ void q2_load_v(const T* V2, const T* tau2, int n, int nsweep, int kma
   }
 }
 
+// T of block (J, k) (forward, columnwise, LAPACK dlarft) and its explicit V, zero padded to
+// 2 kBW rows, into Vx (column-major 2 kBW x kBW per block)
 template <typename T>
 __global__ __launch_bounds__(256) void q2_larft_kernel(const T* V2, const T* tau2, int n,
-                                                       int nsweep, int kmax, T* T2) {
+                                                       int nsweep, int kmax, T* T2, T* Vx) {
   __shared__ double Vs[2 * kBW][kBW + 1];
   __shared__ double G[kBW][kBW + 1];
   __shared__ double Ts[kBW][kBW + 1];
@@ -391,7 +396,13 @@ __global__ __launch_bounds__(256) void q2_larft_kernel(const T* V2, const T* tau
   if (k >= chase_steps(n, s0)) return;
   const int R = r1 - r0;
   q2_load_v(V2, tau2, n, nsweep, kmax, J, k, Vs, taus, R);
+  for (int e = threadIdx.x; e < (2 * kBW - R) * kBW; e += 256) {
+    const int i = R + e % (2 * kBW - R), t = e / (2 * kBW - R);
+    Vs[i][t] = 0.0;
+  }
   __syncthreads();
+  T* vout = Vx + ((long)J * kmax + k) * 2 * kBW * kBW;
+  for (int e = threadIdx.x; e < 2 * kBW * kBW; e += 256) vout[e] = (T)Vs[e % (2 * kBW)][e / (2 * kBW)];
   for (int e = threadIdx.x; e < kBW * kBW; e += 256) {
     const int a = e % kBW, c = e / kBW;
     double g = 0.0;
@@ -414,69 +425,101 @@ __global__ __launch_bounds__(256) void q2_larft_kernel(const T* V2, const T* tau
   for (int e = threadIdx.x; e < kBW * kBW; e += 256) out[e] = (T)Ts[e % kBW][e / kBW];
 }
 
-// Z <- Q2 Z for the columns c0 .. c0 + kQ2Cols - 1 of one workgroup
+// Z <- Q2 Z for the columns c0 .. c0 + kQ2Cols - 1 of one workgroup (4 wave64s, MFMA 16x16x4).
+// For each block: W = V^T Z (32 x 32), W2 = T W, Z -= V W2 (64 x 32). Inside a sweep group the
+// row windows of consecutive step blocks overlap by kBW rows: Z rows live in a 2 kBW-row ring in
+// LDS, the half that leaves the window is stored and the half that enters it is loaded.
 template <typename T>
-__global__ __launch_bounds__(256) void q2_apply_kernel(const T* V2, const T* tau2, const T* T2,
-                                                       int n, int nsweep, int kmax, T* Z, long ldz,
+__global__ __launch_bounds__(256) void q2_apply_kernel(const T* Vx, const T* T2, int n,
+                                                       int nsweep, int kmax, T* Z, long ldz,
                                                        int ncols) {
-  __shared__ double Vs[2 * kBW][kBW + 1];
-  __shared__ double Ts[kBW][kBW + 1];
-  __shared__ double Zs[2 * kBW][kQ2Cols + 1];
-  __shared__ double Ws[kBW][kQ2Cols + 1];
-  __shared__ double taus[kBW];
-  const int c0 = blockIdx.x * kQ2Cols;
-  const int nc = min(kQ2Cols, ncols - c0);
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  constexpr int R2 = 2 * kBW;        // 64 rows of a block window
+  constexpr int NC = kQ2Cols;        // 32 columns
+  __shared__ T Vs[R2][kBW + 1];
+  __shared__ T Ts[kBW][kBW + 1];
+  __shared__ T Zs[R2][NC + 1];       // ring: logical row i at physical (i + ofs) % R2
+  __shared__ T Ws[kBW][NC + 1];
+  __shared__ T W2s[kBW][NC + 1];
+  const int c0 = blockIdx.x * NC;
+  const int nc = min(NC, ncols - c0);
   if (nc <= 0) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
   const int nJ = (nsweep + kBW - 1) / kBW;
+  auto zload = [&](int grow0, int lrow0, int nrows, int ofs) {
+    for (int e = tid; e < nrows * NC; e += 256) {
+      const int i = e % nrows, c = e / nrows;
+      const int gr = grow0 + i;
+      Zs[(lrow0 + i + ofs) % R2][c] = (gr < n && c < nc) ? Z[gr + (long)(c0 + c) * ldz] : T(0);
+    }
+  };
+  auto zstore = [&](int grow0, int lrow0, int nrows, int ofs) {
+    for (int e = tid; e < nrows * NC; e += 256) {
+      const int i = e % nrows, c = e / nrows;
+      const int gr = grow0 + i;
+      if (gr < n && c < nc) Z[gr + (long)(c0 + c) * ldz] = Zs[(lrow0 + i + ofs) % R2][c];
+    }
+  };
   for (int J = nJ - 1; J >= 0; --J) {
     const int kn = chase_steps(n, J * kBW);
     for (int k = 0; k < kn; ++k) {
       int s0, s1, r0, r1;
       q2_rows(n, nsweep, J, k, s0, s1, r0, r1);
-      const int R = r1 - r0;
-      q2_load_v(V2, tau2, n, nsweep, kmax, J, k, Vs, taus, R);
-      const T* Tb = T2 + ((long)J * kmax + k) * kBW * kBW;
-      for (int e = tid; e < kBW * kBW; e += 256) Ts[e % kBW][e / kBW] = (double)Tb[e];
-      for (int e = tid; e < R * kQ2Cols; e += 256) {
-        const int i = e % R, c = e / R;
-        Zs[i][c] = (c < nc) ? (double)Z[(r0 + i) + (long)(c0 + c) * ldz] : 0.0;
-      }
+      const int ofs = (k & 1) * kBW;
+      if (k == 0) zload(r0, 0, R2, ofs);
+      else zload(r0 + kBW, kBW, kBW, ofs);
+      const T* vb = Vx + ((long)J * kmax + k) * R2 * kBW;
+      for (int e = tid; e < R2 * kBW; e += 256) Vs[e % R2][e / R2] = vb[e];
+      const T* tb = T2 + ((long)J * kmax + k) * kBW * kBW;
+      for (int e = tid; e < kBW * kBW; e += 256) Ts[e % kBW][e / kBW] = tb[e];
       __syncthreads();
-      // W = V^T Z
-      for (int e = tid; e < kBW * kQ2Cols; e += 256) {
-        const int a = e % kBW, c = e / kBW;
-        double w = 0.0;
-        for (int i = a; i < R; ++i) w += Vs[i][a] * Zs[i][c];
-        Ws[a][c] = w;
-      }
-      __syncthreads();
-      // W <- T W (T upper triangular); each thread owns whole columns of W
-      double wt[kBW / 8];
+      // W = V^T Z: wave wv owns the 16 x 16 tile (ti, tj)
       {
-        const int c = tid % kQ2Cols, a0 = tid / kQ2Cols;   // 8 row groups of 4 rows
-#pragma unroll
-        for (int q = 0; q < kBW / 8; ++q) {
-          const int a = a0 + 8 * q;
-          double acc = 0.0;
-          for (int p = a; p < kBW; ++p) acc += Ts[a][p] * Ws[p][c];
-          wt[q] = acc;
+        const int ti = wv & 1, tj = wv >> 1;
+        acc_t acc = {0, 0, 0, 0};
+        for (int k0 = 0; k0 < R2; k0 += 4) {
+          const T a = Vs[k0 + lk][ti * 16 + lr];
+          const T b = Zs[(k0 + lk + ofs) % R2][tj * 16 + lr];
+          acc = M::op(a, b, acc);
         }
-        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < kBW / 8; ++q) Ws[a0 + 8 * q][c] = wt[q];
+        for (int r = 0; r < 4; ++r) Ws[ti * 16 + M::row(lane, r)][tj * 16 + lr] = acc[r];
       }
       __syncthreads();
-      // Z -= V W
-      for (int e = tid; e < R * kQ2Cols; e += 256) {
-        const int i = e % R, c = e / R;
-        if (c < nc) {
-          double acc = 0.0;
-          const int amax = min(i + 1, kBW);
-          for (int a = 0; a < amax; ++a) acc += Vs[i][a] * Ws[a][c];
-          Z[(r0 + i) + (long)(c0 + c) * ldz] = (T)(Zs[i][c] - acc);
+      // W2 = T W
+      {
+        const int ti = wv & 1, tj = wv >> 1;
+        acc_t acc = {0, 0, 0, 0};
+        for (int p0 = 0; p0 < kBW; p0 += 4) {
+          const T a = Ts[ti * 16 + lr][p0 + lk];
+          const T b = Ws[p0 + lk][tj * 16 + lr];
+          acc = M::op(a, b, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) W2s[ti * 16 + M::row(lane, r)][tj * 16 + lr] = acc[r];
+      }
+      __syncthreads();
+      // Z -= V W2: wave wv owns rows wv*16 .. wv*16+15, both column tiles
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        acc_t acc = {0, 0, 0, 0};
+        for (int a0 = 0; a0 < kBW; a0 += 4) {
+          const T a = Vs[wv * 16 + lr][a0 + lk];
+          const T b = W2s[a0 + lk][ct * 16 + lr];
+          acc = M::op(a, b, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = (wv * 16 + M::row(lane, r) + ofs) % R2;
+          Zs[row][ct * 16 + lr] -= acc[r];
         }
       }
+      __syncthreads();
+      // the first kBW rows leave the window (all of it after the last step of the group)
+      if (k + 1 < kn) zstore(r0, 0, kBW, ofs);
+      else zstore(r0, 0, R2, ofs);
       __syncthreads();
     }
   }
@@ -490,6 +533,7 @@ struct BandWs {
   T* V2;          // nsweep x kmax x kBW
   T* tau2;        // nsweep x kmax
   T* T2;          // nJ x kmax x kBW x kBW
+  T* Vx;          // nJ x kmax x 2kBW x kBW  explicit block reflectors
   T* tau1;        // n
   T* P1;          // n x 2kBW   [V W]
   T* P2;          // n x 2kBW   [W V]
@@ -521,6 +565,7 @@ static size_t band_layout(int n, BandWs<T>* ws, char* base) {
   take(&w->V2, sizeof(T) * nsweep * kmax * kBW);
   take(&w->tau2, sizeof(T) * nsweep * kmax);
   take(&w->T2, sizeof(T) * nJ * kmax * kBW * kBW);
+  take(&w->Vx, sizeof(T) * nJ * kmax * 2 * kBW * kBW);
   take(&w->tau1, sizeof(T) * std::max(n, 1));
   take(&w->P1, sizeof(T) * std::max(n, 1) * 2 * kBW);
   take(&w->P2, sizeof(T) * std::max(n, 1) * 2 * kBW);
@@ -583,7 +628,8 @@ int sytrd_2stage_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, void* work,
     hipLaunchKernelGGL(larft32_kernel<T>, dim3(1), dim3(64), 0, s, ws.G, ws.tau1 + j0, kr, ws.Tm);
     DH_LAUNCH_CHECK();
     T* a22 = a + (j0 + kBW) + (long)(j0 + kBW) * lda;
-    DH_TRY(gemm<T>(s, false, false, m, kr, m, T(1), a22, lda, V, m, T(0), ws.X, m));
+    DH_TRY(gemm<T>(s, false, false, m, kr, m, T(1), a22, lda, V, m, T(0), ws.X, m, false, kKFull,
+                   nullptr, std::max(1, std::min(16, m / 1024))));
     DH_TRY(gemm<T>(s, false, false, m, kr, kr, T(1), ws.X, m, ws.Tm, kBW, T(0), W, m));
     DH_TRY(gemm<T>(s, true, false, kr, kr, m, T(1), V, m, W, m, T(0), ws.Zs, kBW, false, kKFull,
                    nullptr, sk));
@@ -623,7 +669,7 @@ int sytrd_2stage_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, void* work,
     DH_CHECK(hAbort == 0, "sb2st: a progress wait ran out (bulge chasing aborted)");
     const int nJ = (nsweep + kBW - 1) / kBW;
     hipLaunchKernelGGL(q2_larft_kernel<T>, dim3(nJ, kmax), dim3(256), 0, s, ws.V2, ws.tau2, n,
-                       nsweep, kmax, ws.T2);
+                       nsweep, kmax, ws.T2, ws.Vx);
     DH_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(band_diag_kernel<T>, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, ws.AB,
@@ -642,8 +688,8 @@ int ormtr_2stage_t(hipStream_t s, const T* a, long lda, int n, T* z, long ldz, i
   band_layout<T>(n, &ws, static_cast<char*>(work));
   const int nsweep = n - 2;
   const int kmax = std::max(chase_steps(n, 0), 1);
-  hipLaunchKernelGGL(q2_apply_kernel<T>, dim3(cdiv(ncols, kQ2Cols)), dim3(256), 0, s, ws.V2,
-                     ws.tau2, ws.T2, n, nsweep, kmax, z, ldz, ncols);
+  hipLaunchKernelGGL(q2_apply_kernel<T>, dim3(cdiv(ncols, kQ2Cols)), dim3(256), 0, s, ws.Vx,
+                     ws.T2, n, nsweep, kmax, z, ldz, ncols);
   DH_LAUNCH_CHECK();
   const int nref = n - kBW - 1;
   if (nref > 0)

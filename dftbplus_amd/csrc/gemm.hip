@@ -10,30 +10,9 @@
 //  * klim: skip the zero part of a triangular operand;
 //  * w: per-k weights folded into op(A) (C = X diag(f) X^T for the density matrices).
 #include "common.h"
+#include "mfma.h"
 
 namespace dftbhip {
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  using acc_t = v4d;
-  static constexpr bool kRowMajorQuad = false;
-  __device__ static inline v4d op(double a, double b, v4d c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Mfma<float> {
-  using acc_t = v4f;
-  static constexpr bool kRowMajorQuad = true;
-  __device__ static inline v4f op(float a, float b, v4f c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-};
 
 constexpr int kBM = 128, kBN = 128, kBK = 16, kNT = 256, kPad = 1;
 

@@ -504,42 +504,64 @@ __global__ void invr_cluster_grad_kernel(int nat, const double* coords, const do
 }
 
 // Ewald gradient: real space over translations + reciprocal from structure factors
-__global__ void ewald_grad_kernel(int nat, const double* coords, int ntr, const double* tr,
-                                  double alpha, double maxr, int ng, const double* G,
-                                  const double* w, const double* CS, const double* dq,
-                                  double* grad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Ewald gradient of atom i (one 256-thread workgroup per atom): real-space sum over all atoms
+// and lattice translations within maxr, reciprocal sum over the G vectors with the structure
+// factors C_G = sum_j dq_j cos(G r_j), S_G = sum_j dq_j sin(G r_j) (CS = [C; S])
+__global__ __launch_bounds__(256) void ewald_grad_kernel(int nat, const double* coords, int ntr,
+                                                         const double* tr, double alpha,
+                                                         double maxr, int ng, const double* G,
+                                                         const double* w, const double* CS,
+                                                         const double* dq, double* grad) {
+  __shared__ double red[3][4];
+  const int i = blockIdx.x;
   if (i >= nat) return;
-  double g[3] = {0, 0, 0};
+  const int tid = threadIdx.x;
+  const double xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
   const double ta = 2.0 * alpha / sqrt(M_PI);
-  for (int j = 0; j < nat; ++j) {
-    const double dx = coords[3 * i] - coords[3 * j], dy = coords[3 * i + 1] - coords[3 * j + 1],
-                 dz = coords[3 * i + 2] - coords[3 * j + 2];
-    double acc[3] = {0, 0, 0};
+  const double maxr2 = maxr * maxr;
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  for (int j = tid; j < nat; j += 256) {
+    const double dx = xi - coords[3 * j], dy = yi - coords[3 * j + 1], dz = zi - coords[3 * j + 2];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int t = 0; t < ntr; ++t) {
       const double x = dx - tr[3 * t], y = dy - tr[3 * t + 1], z = dz - tr[3 * t + 2];
       const double r2 = x * x + y * y + z * z;
-      const double r = sqrt(r2);
-      if (r <= maxr && r > 1.0e-8) {
+      if (r2 <= maxr2 && r2 > 1.0e-16) {
+        const double r = sqrt(r2);
         const double dfdr = -(ta * exp(-alpha * alpha * r2) / r + erfc(alpha * r) / r2);
         const double f = dfdr / r;
-        acc[0] += f * x;
-        acc[1] += f * y;
-        acc[2] += f * z;
+        a0 += f * x;
+        a1 += f * y;
+        a2 += f * z;
       }
     }
-    for (int c = 0; c < 3; ++c) g[c] += acc[c] * dq[j];
+    g0 += a0 * dq[j];
+    g1 += a1 * dq[j];
+    g2 += a2 * dq[j];
   }
-  // reciprocal: -dq_i sum_G w G (sin(G r_i) C_G - cos(G r_i) S_G), C_G = CS(ng+?) packed below
-  for (int q = 0; q < ng; ++q) {
-    const double ph = G[3 * q] * coords[3 * i] + G[3 * q + 1] * coords[3 * i + 1] +
-                      G[3 * q + 2] * coords[3 * i + 2];
+  for (int q = tid; q < ng; q += 256) {
+    const double ph = G[3 * q] * xi + G[3 * q + 1] * yi + G[3 * q + 2] * zi;
     double sn, cn;
     sincos(ph, &sn, &cn);
     const double fac = (sn * CS[q] - cn * CS[ng + q]) * w[q];
-    for (int c = 0; c < 3; ++c) g[c] -= fac * G[3 * q + c];
+    g0 -= fac * G[3 * q];
+    g1 -= fac * G[3 * q + 1];
+    g2 -= fac * G[3 * q + 2];
   }
-  for (int c = 0; c < 3; ++c) grad[3 * i + c] += dq[i] * g[c];
+  for (int o = 32; o > 0; o >>= 1) {
+    g0 += __shfl_down(g0, o, 64);
+    g1 += __shfl_down(g1, o, 64);
+    g2 += __shfl_down(g2, o, 64);
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+  if (lane == 0) {
+    red[0][wv] = g0;
+    red[1][wv] = g1;
+    red[2][wv] = g2;
+  }
+  __syncthreads();
+  if (tid < 3)
+    grad[3 * i + tid] += dq[i] * (red[tid][0] + red[tid][1] + red[tid][2] + red[tid][3]);
 }
 
 // repulsive energy and gradient over neighbours within the repulsive cutoff
@@ -701,8 +723,8 @@ int launch_gamma_grad(hipStream_t s, const NeighDev& nb, const double* coords,
     hipLaunchKernelGGL(invr_cluster_grad_kernel, dim3(cdiv(nat, 64)), dim3(64), 0, s, nat, coords,
                        dq, grad);
   } else {
-    hipLaunchKernelGGL(ewald_grad_kernel, dim3(cdiv(nat, 64)), dim3(64), 0, s, nat, coords, ntr,
-                       tr, alpha, maxr, ng, G, w, CSq, dq, grad);
+    hipLaunchKernelGGL(ewald_grad_kernel, dim3(nat), dim3(256), 0, s, nat, coords, ntr, tr, alpha,
+                       maxr, ng, G, w, CSq, dq, grad);
   }
   DH_LAUNCH_CHECK();
   return 0;
