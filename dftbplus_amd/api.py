@@ -119,6 +119,11 @@ EXPORTS = {
     "dftbhip_ctx_eigenvalues": [C.c_void_p, _dp, _dp, C.c_int],
     "dftbhip_ctx_nr_orbitals": [C.c_void_p],
     "dftbhip_ctx_timings": [C.c_void_p, _dp, C.c_int],
+    "dftbhip_batch_create": [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p],
+    "dftbhip_batch_destroy": [C.c_void_p],
+    "dftbhip_batch_set_geometries": [C.c_void_p, C.c_int, _ip, _dp, _ip],
+    "dftbhip_batch_scc": [C.c_void_p, _dp, _dp, _ip, _ip],
+    "dftbhip_batch_gradients": [C.c_void_p, _dp],
 }
 
 
@@ -469,6 +474,60 @@ class DftbPlus:
     def close(self):
         if self.h:
             lib().dftbhip_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MoleculeBatch:
+    """Many independent non-periodic molecules solved together (``dftbhip_batch_*``): the
+    result for each molecule is that of a separate single-molecule calculation."""
+
+    def __init__(self, params: Parameters, geometries, options: Options = Options(),
+                 force_mode: int = 0):
+        self._p, self._keep = make_params(params)
+        self._o = make_options(options, force_mode, 0)
+        self.h = C.c_void_p()
+        _check(lib().dftbhip_batch_create(C.byref(self.h), C.byref(self._p), C.byref(self._o)),
+               "dftbhip_batch_create")
+        self.nat = np.array([g.natom for g in geometries], dtype=np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(self.nat)])
+        coords = _f64(np.concatenate([g.coords.reshape(-1, 3) for g in geometries]).reshape(-1))
+        species = _i32(np.concatenate([g.species for g in geometries]))
+        self.nmol = len(geometries)
+        _check(lib().dftbhip_batch_set_geometries(self.h, self.nmol, _i(self.nat), _d(coords),
+                                                  _i(species)), "dftbhip_batch_set_geometries")
+        self._done = False
+
+    def scc(self):
+        e = np.zeros((self.nmol, 8))
+        q = np.zeros(int(self.offsets[-1]))
+        nit = np.zeros(self.nmol, dtype=np.int32)
+        conv = np.zeros(self.nmol, dtype=np.int32)
+        _check(lib().dftbhip_batch_scc(self.h, _d(e), _d(q), _i(nit), _i(conv)),
+               "dftbhip_batch_scc")
+        self.energies = {k: e[:, i].copy() for i, k in enumerate(ENERGY_KEYS)}
+        self.charges = [q[self.offsets[m]:self.offsets[m + 1]] for m in range(self.nmol)]
+        self.n_iter = nit
+        self.converged = conv.astype(bool)
+        self._done = True
+        return self.energies["mermin"]
+
+    def gradients(self):
+        if not self._done:
+            self.scc()
+        g = np.zeros(3 * int(self.offsets[-1]))
+        _check(lib().dftbhip_batch_gradients(self.h, _d(g)), "dftbhip_batch_gradients")
+        g = g.reshape(-1, 3)
+        return [g[self.offsets[m]:self.offsets[m + 1]] for m in range(self.nmol)]
+
+    def close(self):
+        if self.h:
+            lib().dftbhip_batch_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -1,0 +1,38 @@
+// Per-molecule layout of a molecule batch and the batched kernels (batch.hip).
+#pragma once
+
+#include "sparse.h"
+
+namespace dftbhip {
+
+constexpr int kBatchMaxOrb = 512;   // orbitals per molecule in a batch
+
+struct BatchLayout {
+  int nmol;
+  const int* atom0;     // nmol + 1: first atom of each molecule
+  const long* orb0;     // nmol + 1: first orbital
+  const long* dOff;     // nmol + 1: offsets of the packed n_m x n_m dense matrices
+  const long* gOff;     // nmol + 1: offsets of the packed nat_m x nat_m gamma matrices
+  const int* atomMol;   // molecule of each atom
+};
+
+int launch_batch_unpack(hipStream_t s, const PairsDev& pr, const int* norb,
+                        const long* iAtomStart, const BatchLayout& L, const double* x,
+                        double* dense, long denseSize);
+int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                      const BatchLayout& L, const double* dense, double* x);
+// eigenvalues w (ascending per molecule, at orb0), eigenvectors C (packed), info per molecule
+// (0, or j+1 when the leading minor j+1 of S is not positive definite); work 3 x packed size
+int launch_batch_sygv(hipStream_t s, const BatchLayout& L, const double* S, const double* H,
+                      double* work, double* w, double* C, int* info);
+int launch_batch_density(hipStream_t s, const BatchLayout& L, const double* C, const double* f,
+                         const int* nocc, double* P);
+int launch_batch_gamma(hipStream_t s, const BatchLayout& L, const double* coords,
+                       const int* species, const double* hubbu, const double* gcut, int nsp,
+                       double* g);
+int launch_batch_gemv(hipStream_t s, const BatchLayout& L, int nat, const double* g,
+                      const double* dq, double* v);
+int launch_batch_invr_grad(hipStream_t s, const BatchLayout& L, int nat, const double* coords,
+                           const double* dq, double* grad);
+
+}  // namespace dftbhip

@@ -18,6 +18,7 @@
 
 #include "../../include/dftbhip.h"
 #include "common.h"
+#include "batch.h"
 #include "eigen.h"
 #include "sparse.h"
 
@@ -1498,7 +1499,357 @@ static int ctx_gradients(Context& c, double* grad) {
   return 0;
 }
 
+// ------------------------------------------------------------------------------------------
+// Molecule batch: many independent non-periodic molecules solved together. The union of the
+// molecules is one block-diagonal system; every molecule keeps its own electron count, Fermi
+// level, Broyden mixer and convergence state, as in a loop over single-molecule calculations.
+// ------------------------------------------------------------------------------------------
+struct BatchContext {
+  dftbhip_options opt{};
+  SkTables sk;
+  hipStream_t s = nullptr;
+  int nmol = 0, nat = 0, norbTot = 0;
+  std::vector<int> atom0, atomMol, species, norbAtom, nocc, iters, conv;
+  std::vector<long> orb0, dOff, gOff, atomStart;
+  std::vector<double> coords, q0atom, nel, eigv, fill, qin, qout, dqout;
+  std::vector<Broyden> mixers;
+  std::vector<FillResult> fr;
+  std::vector<double> eNonScc, eScc, erep, err;
+  HostBlocks blk, nbp;
+  DevBlocks dblk, dnb;
+  NeighDev nbdev{};
+  long nnz = 0, denseSize = 0, gammaSize = 0;
+  DevBuf<double> coordsD, h0, ov, hs, rho, erho, vorb, vatom, dq, qorb, eorb, grad, eatom, gamma;
+  DevBuf<double> Sd, Hd, Cv, P, work, w, f;
+  DevBuf<int> atom0D, atomMolD, infoD, noccD;
+  DevBuf<long> orb0D, dOffD, gOffD;
+  BatchLayout L{};
+  bool sccDone = false;
+};
+
+static int batch_geometry(BatchContext& b, int nmol, const int* natMol, const double* coords,
+                          const int* species) {
+  hipStream_t s = b.s;
+  b.nmol = nmol;
+  b.atom0.assign(nmol + 1, 0);
+  for (int m = 0; m < nmol; ++m) {
+    DH_CHECK(natMol[m] >= 1, "batch: a molecule without atoms");
+    b.atom0[m + 1] = b.atom0[m] + natMol[m];
+  }
+  const int nat = b.atom0[nmol];
+  b.nat = nat;
+  b.coords.assign(coords, coords + 3 * (size_t)nat);
+  b.species.assign(species, species + nat);
+  b.atomMol.resize(nat);
+  b.norbAtom.resize(nat);
+  b.atomStart.assign(nat + 1, 0);
+  b.q0atom.resize(nat);
+  for (int m = 0; m < nmol; ++m)
+    for (int a = b.atom0[m]; a < b.atom0[m + 1]; ++a) b.atomMol[a] = m;
+  for (int a = 0; a < nat; ++a) {
+    DH_CHECK(species[a] >= 0 && species[a] < b.sk.nsp, "batch: species out of range");
+    b.norbAtom[a] = b.sk.norb[species[a]];
+    b.atomStart[a + 1] = b.atomStart[a] + b.norbAtom[a];
+    b.q0atom[a] = b.sk.q0[species[a]];
+  }
+  b.norbTot = (int)b.atomStart[nat];
+  b.orb0.assign(nmol + 1, 0);
+  b.dOff.assign(nmol + 1, 0);
+  b.gOff.assign(nmol + 1, 0);
+  b.nel.assign(nmol, 0.0);
+  for (int m = 0; m < nmol; ++m) {
+    b.orb0[m + 1] = b.atomStart[b.atom0[m + 1]];
+    const long n = b.orb0[m + 1] - b.orb0[m];
+    DH_CHECK(n <= kBatchMaxOrb, "batch: a molecule has more than 512 orbitals");
+    b.dOff[m + 1] = b.dOff[m] + n * n;
+    const long na = natMol[m];
+    b.gOff[m + 1] = b.gOff[m] + na * na;
+    for (int a = b.atom0[m]; a < b.atom0[m + 1]; ++a) b.nel[m] += b.q0atom[a];
+  }
+  b.denseSize = b.dOff[nmol];
+  b.gammaSize = b.gOff[nmol];
+  // neighbour lists per molecule, concatenated (image index = atom index: no periodicity)
+  double cutoff = b.sk.skcut;
+  for (double r : b.sk.repcut) cutoff = std::max(cutoff, r);
+  for (double g : b.sk.gcut) cutoff = std::max(cutoff, g);
+  b.blk = HostBlocks();
+  b.nbp = HostBlocks();
+  const double sk2 = b.sk.skcut * b.sk.skcut;
+  long st = 0;
+  for (int m = 0; m < nmol; ++m) {
+    const int a0 = b.atom0[m];
+    NeighbourList nl;
+    DH_TRY(build_neighbours(natMol[m], coords + 3 * (size_t)a0, nullptr, cutoff, nl));
+    for (int a = 0; a < natMol[m]; ++a) {
+      const auto& nb = nl.nb[a];
+      const auto& d2 = nl.d2[a];
+      for (size_t k = 0; k < nb.size(); ++k) {
+        const int ga = a0 + a, gj = a0 + nl.img2cent[nb[k]];
+        b.nbp.at1.push_back(ga);
+        b.nbp.img2.push_back(gj);
+        b.nbp.at2.push_back(gj);
+        b.nbp.start.push_back(0);
+        if (d2[k] <= sk2) {
+          b.blk.at1.push_back(ga);
+          b.blk.img2.push_back(gj);
+          b.blk.at2.push_back(gj);
+          b.blk.start.push_back(st);
+          st += (long)b.norbAtom[ga] * b.norbAtom[gj];
+        }
+      }
+    }
+  }
+  b.nnz = st;
+  DH_TRY(b.dblk.upload(b.blk, b.species, b.norbAtom, b.atomStart, s));
+  DH_TRY(b.dnb.upload(b.nbp, b.species, b.norbAtom, b.atomStart, s));
+  b.nbdev.n = b.dnb.pr.nblk;
+  b.nbdev.at1 = b.dnb.pr.at1;
+  b.nbdev.img2 = b.dnb.pr.img2;
+  b.nbdev.at2 = b.dnb.pr.at2;
+  b.nbdev.species = b.dnb.pr.species;
+  DH_TRY(b.coordsD.upload(coords, 3 * (size_t)nat, s));
+  DH_TRY(b.atom0D.upload(b.atom0.data(), b.atom0.size(), s));
+  DH_TRY(b.atomMolD.upload(b.atomMol.data(), b.atomMol.size(), s));
+  DH_TRY(b.orb0D.upload(b.orb0.data(), b.orb0.size(), s));
+  DH_TRY(b.dOffD.upload(b.dOff.data(), b.dOff.size(), s));
+  DH_TRY(b.gOffD.upload(b.gOff.data(), b.gOff.size(), s));
+  b.L.nmol = nmol;
+  b.L.atom0 = b.atom0D.p;
+  b.L.orb0 = b.orb0D.p;
+  b.L.dOff = b.dOffD.p;
+  b.L.gOff = b.gOffD.p;
+  b.L.atomMol = b.atomMolD.p;
+  const int n = b.norbTot;
+  for (DevBuf<double>* x : {&b.h0, &b.ov, &b.hs, &b.rho, &b.erho}) DH_TRY(x->alloc(std::max(b.nnz, 1L)));
+  for (DevBuf<double>* x : {&b.vorb, &b.qorb, &b.eorb, &b.w, &b.f}) DH_TRY(x->alloc(n));
+  for (DevBuf<double>* x : {&b.vatom, &b.dq, &b.eatom}) DH_TRY(x->alloc(nat));
+  DH_TRY(b.grad.alloc(3 * (size_t)nat));
+  for (DevBuf<double>* x : {&b.Sd, &b.Hd, &b.Cv, &b.P}) DH_TRY(x->alloc(b.denseSize));
+  DH_TRY(b.work.alloc(3 * b.denseSize));
+  DH_TRY(b.gamma.alloc(b.gammaSize));
+  DH_TRY(b.infoD.alloc(nmol));
+  DH_TRY(b.noccD.alloc(nmol));
+  DH_TRY(launch_build_h0s(s, b.sk.dev, b.dblk.pr, b.coordsD.p, b.h0.p, b.ov.p));
+  DH_TRY(launch_batch_unpack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.ov.p,
+                             b.Sd.p, b.denseSize));
+  DH_TRY(launch_batch_gamma(s, b.L, b.coordsD.p, b.dblk.species.p, b.sk.dhubbu.p, b.sk.dgcut.p,
+                            b.sk.nsp, b.gamma.p));
+  DH_HIP(hipStreamSynchronize(s));
+  b.sccDone = false;
+  return 0;
+}
+
+// atom potentials of the charges dqa (device dq, vatom, vorb); optionally back to the host
+static int batch_potential(BatchContext& b, const std::vector<double>& dqa,
+                           std::vector<double>* vout) {
+  hipStream_t s = b.s;
+  DH_HIP(hipMemcpyAsync(b.dq.p, dqa.data(), b.nat * sizeof(double), hipMemcpyHostToDevice, s));
+  DH_TRY(launch_batch_gemv(s, b.L, b.nat, b.gamma.p, b.dq.p, b.vatom.p));
+  hipLaunchKernelGGL(expand_kernel, dim3(cdiv(b.nat, 256)), dim3(256), 0, s, b.nat,
+                     b.dblk.atomStart.p, b.vatom.p, b.vorb.p);
+  DH_LAUNCH_CHECK();
+  if (vout) {
+    vout->resize(b.nat);
+    DH_HIP(hipMemcpyAsync(vout->data(), b.vatom.p, b.nat * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  }
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// The SCC cycle of every molecule (main.F90:1429-1563 per molecule); molecules that have
+// converged keep their input charges, so their state stays that of their last iteration.
+static int batch_scc(BatchContext& b, double* energies, double* charges, int* niter,
+                     int* converged) {
+  hipStream_t s = b.s;
+  const dftbhip_options& o = b.opt;
+  const int nmol = b.nmol, nat = b.nat, n = b.norbTot;
+  b.qin = b.q0atom;
+  b.mixers.assign(nmol, Broyden());
+  for (int m = 0; m < nmol; ++m)
+    b.mixers[m].reset(b.atom0[m + 1] - b.atom0[m], o, o.max_scc_iter + 1);
+  b.iters.assign(nmol, 0);
+  b.conv.assign(nmol, 0);
+  b.err.assign(nmol, 0.0);
+  b.fr.assign(nmol, FillResult{});
+  b.eNonScc.assign(nmol, 0.0);
+  b.eScc.assign(nmol, 0.0);
+  b.nocc.assign(nmol, 0);
+  b.fill.assign(n, 0.0);
+  b.qout.assign(nat, 0.0);
+  b.dqout.assign(nat, 0.0);
+  std::vector<double> dqa(nat), qorbH(n), eorbH(n), vout;
+  std::vector<int> info(nmol);
+  int nactive = nmol;
+  for (int it = 1; it <= o.max_scc_iter && nactive > 0; ++it) {
+    for (int a = 0; a < nat; ++a) dqa[a] = b.qin[a] - b.q0atom[a];
+    DH_TRY(batch_potential(b, dqa, nullptr));
+    DH_TRY(launch_shift(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.h0.p, b.ov.p,
+                        b.vorb.p, b.hs.p));
+    DH_TRY(launch_batch_unpack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.hs.p,
+                               b.Hd.p, b.denseSize));
+    DH_TRY(launch_batch_sygv(s, b.L, b.Sd.p, b.Hd.p, b.work.p, b.w.p, b.Cv.p, b.infoD.p));
+    b.eigv.resize(n);
+    DH_HIP(hipMemcpyAsync(b.eigv.data(), b.w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipMemcpyAsync(info.data(), b.infoD.p, nmol * sizeof(int), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipStreamSynchronize(s));
+    for (int m = 0; m < nmol; ++m)
+      if (info[m] != 0) {
+        set_error("batch: overlap of molecule " + std::to_string(m) +
+                  " not positive definite (minor " + std::to_string(info[m]) + ")");
+        return kErrState;
+      }
+    for (int m = 0; m < nmol; ++m) {
+      const long o0 = b.orb0[m], nm = b.orb0[m + 1] - o0;
+      std::vector<double> e(b.eigv.begin() + o0, b.eigv.begin() + o0 + nm), fm;
+      DH_TRY(fermi_filling(e, b.nel[m], o.kT, fm, b.fr[m]));
+      std::copy(fm.begin(), fm.end(), b.fill.begin() + o0);
+      b.nocc[m] = occupied_levels(fm);
+    }
+    DH_HIP(hipMemcpyAsync(b.f.p, b.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    DH_HIP(hipMemcpyAsync(b.noccD.p, b.nocc.data(), nmol * sizeof(int), hipMemcpyHostToDevice, s));
+    DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p));
+    DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p,
+                             b.rho.p));
+    DH_TRY(launch_mulliken(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.ov.p, b.h0.p,
+                           b.rho.p, b.qorb.p, b.eorb.p, n));
+    DH_HIP(hipMemcpyAsync(qorbH.data(), b.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipMemcpyAsync(eorbH.data(), b.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipStreamSynchronize(s));
+    std::vector<double> qo(nat, 0.0);
+    for (int a = 0; a < nat; ++a)
+      for (long oi = b.atomStart[a]; oi < b.atomStart[a + 1]; ++oi) qo[a] += qorbH[oi];
+    std::vector<double> dqo(nat);
+    for (int a = 0; a < nat; ++a) dqo[a] = qo[a] - b.q0atom[a];
+    DH_TRY(batch_potential(b, dqo, &vout));
+    for (int m = 0; m < nmol; ++m) {
+      if (b.conv[m]) continue;
+      b.iters[m] = it;
+      double en = 0.0, es = 0.0, er = 0.0;
+      for (long oi = b.orb0[m]; oi < b.orb0[m + 1]; ++oi) en += eorbH[oi];
+      std::vector<double> q(b.qin.begin() + b.atom0[m], b.qin.begin() + b.atom0[m + 1]);
+      std::vector<double> diff(q.size());
+      for (int a = b.atom0[m]; a < b.atom0[m + 1]; ++a) {
+        es += 0.5 * vout[a] * dqo[a];
+        diff[a - b.atom0[m]] = qo[a] - b.qin[a];
+        er = std::max(er, std::fabs(diff[a - b.atom0[m]]));
+        b.qout[a] = qo[a];
+        b.dqout[a] = dqo[a];
+      }
+      b.eNonScc[m] = en;
+      b.eScc[m] = es;
+      b.err[m] = er;
+      if (er < o.scc_tol && it >= o.min_scc_iter) {
+        b.conv[m] = 1;
+        --nactive;
+      } else if (it != o.max_scc_iter) {
+        b.mixers[m].mix(q, diff);
+        std::copy(q.begin(), q.end(), b.qin.begin() + b.atom0[m]);
+      }
+    }
+  }
+  // repulsive energies and the final energies per molecule
+  DH_TRY(fill<double>(s, b.eatom.p, nat, 0.0));
+  DH_TRY(fill<double>(s, b.grad.p, 3 * (size_t)nat, 0.0));
+  DH_TRY(launch_repulsive(s, b.nbdev, b.coordsD.p, b.sk.dreps.p, b.sk.nsp, b.eatom.p, b.grad.p));
+  std::vector<double> ea(nat);
+  DH_HIP(hipMemcpyAsync(ea.data(), b.eatom.p, nat * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  for (int m = 0; m < nmol; ++m) {
+    double erep = 0.0;
+    for (int a = b.atom0[m]; a < b.atom0[m + 1]; ++a) erep += ea[a];
+    const double etot = b.eNonScc[m] + b.eScc[m] + erep;
+    double* e = energies ? energies + 8 * (size_t)m : nullptr;
+    if (e) {
+      e[0] = etot - b.fr[m].ts;
+      e[1] = etot;
+      e[2] = b.eNonScc[m];
+      e[3] = b.eScc[m];
+      e[4] = erep;
+      e[5] = b.fr[m].ts;
+      e[6] = b.fr[m].eband;
+      e[7] = b.fr[m].ef;
+    }
+    if (niter) niter[m] = b.iters[m];
+    if (converged) converged[m] = b.conv[m];
+  }
+  if (charges)
+    for (int a = 0; a < nat; ++a) charges[a] = b.q0atom[a] - b.qout[a];
+  b.sccDone = true;
+  return 0;
+}
+
+static int batch_gradients(BatchContext& b, double* grad) {
+  DH_CHECK(b.sccDone, "batch gradients: run the SCC cycle first");
+  hipStream_t s = b.s;
+  const int nat = b.nat, n = b.norbTot;
+  std::vector<double> fe(n, 0.0);
+  for (int m = 0; m < b.nmol; ++m)
+    for (long k = 0; k < b.nocc[m]; ++k) fe[b.orb0[m] + k] = b.fill[b.orb0[m] + k] * b.eigv[b.orb0[m] + k];
+  DH_HIP(hipMemcpyAsync(b.f.p, fe.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p));
+  DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p, b.erho.p));
+  DH_TRY(batch_potential(b, b.dqout, nullptr));
+  DH_TRY(fill<double>(s, b.eatom.p, nat, 0.0));
+  DH_TRY(fill<double>(s, b.grad.p, 3 * (size_t)nat, 0.0));
+  DH_TRY(launch_repulsive(s, b.nbdev, b.coordsD.p, b.sk.dreps.p, b.sk.nsp, b.eatom.p, b.grad.p));
+  DH_TRY(launch_pair_forces(s, b.sk.dev, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p,
+                            b.coordsD.p, b.rho.p, b.erho.p, b.vorb.p, b.opt.force_mode,
+                            b.opt.fd_delta, b.grad.p));
+  DH_TRY(launch_short_gamma_grad(s, b.nbdev, b.coordsD.p, b.sk.dhubbu.p, b.sk.dgcut.p, b.sk.nsp,
+                                 b.dq.p, b.grad.p));
+  DH_TRY(launch_batch_invr_grad(s, b.L, nat, b.coordsD.p, b.dq.p, b.grad.p));
+  DH_HIP(hipMemcpyAsync(grad, b.grad.p, 3 * (size_t)nat * sizeof(double), hipMemcpyDeviceToHost,
+                        s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 }  // namespace dftbhip
+
+struct dftbhip_batch {
+  dftbhip::BatchContext b;
+};
+
+extern "C" int dftbhip_batch_create(dftbhip_batch** bp, const dftbhip_params* p,
+                                    const dftbhip_options* opt) {
+  DH_CHECK(bp && p && opt, "batch_create: null pointer");
+  DH_CHECK(opt->precision == 0, "batch_create: the molecule batch runs in FP64 (precision 0)");
+  DH_CHECK(opt->max_scc_iter >= 1, "batch_create: max_scc_iter < 1");
+  DH_TRY(dftbhip::ensure_runtime());
+  auto* h = new dftbhip_batch();
+  h->b.opt = *opt;
+  h->b.s = dftbhip::g_rt.stream;
+  const int rc = dftbhip::sk_setup(h->b.sk, p, h->b.s);
+  if (rc != 0) {
+    delete h;
+    return rc;
+  }
+  *bp = h;
+  return 0;
+}
+
+extern "C" int dftbhip_batch_destroy(dftbhip_batch* b) {
+  delete b;
+  return 0;
+}
+
+extern "C" int dftbhip_batch_set_geometries(dftbhip_batch* b, int nMol, const int* nAtomMol,
+                                            const double* coords, const int* species) {
+  DH_CHECK(b && nAtomMol && coords && species && nMol > 0, "batch_set_geometries: bad arguments");
+  return dftbhip::batch_geometry(b->b, nMol, nAtomMol, coords, species);
+}
+
+extern "C" int dftbhip_batch_scc(dftbhip_batch* b, double* energies, double* charges, int* niter,
+                                 int* converged) {
+  DH_CHECK(b && b->b.nmol > 0, "batch_scc: no geometries");
+  return dftbhip::batch_scc(b->b, energies, charges, niter, converged);
+}
+
+extern "C" int dftbhip_batch_gradients(dftbhip_batch* b, double* grad) {
+  DH_CHECK(b && grad, "batch_gradients: null pointer");
+  return dftbhip::batch_gradients(b->b, grad);
+}
 
 struct dftbhip_ctx {
   dftbhip::Context c;

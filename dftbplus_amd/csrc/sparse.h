@@ -78,6 +78,9 @@ int launch_strucfac(hipStream_t s, int nat, const double* coords, int ng, const 
                     double* cs);
 int launch_short_gamma(hipStream_t s, const NeighDev& nb, const double* coords,
                        const double* hubbu, const double* gcut, int nsp, int nat, double* g);
+int launch_short_gamma_grad(hipStream_t s, const NeighDev& nb, const double* coords,
+                            const double* hubbu, const double* gcut, int nsp, const double* dq,
+                            double* grad);
 int launch_gamma_grad(hipStream_t s, const NeighDev& nb, const double* coords,
                       const double* hubbu, const double* gcut, int nsp, int nat, bool periodic,
                       int ntr, const double* tr, double alpha, double maxr, int ng,

@@ -13,6 +13,7 @@
 // i.e. at most 4 orbitals per atom.
 #include "common.h"
 #include "sparse.h"
+#include "gamma_dev.h"
 
 namespace dftbhip {
 
@@ -356,52 +357,6 @@ __global__ void pair_force_kernel(SkDev sk, PairsDev pr, const int* norbAtom,
   }
 }
 
-// ------------------------------------------------------------------------------------------
-// SCC electrostatics: gamma = 1/R (or Ewald) - short-range exponential gamma (+U on-site)
-// ------------------------------------------------------------------------------------------
-__device__ double gsub(double r, double t1, double t2) {
-  return exp(-t1 * r) * ((0.5 * t2 * t2 * t2 * t2 * t1 / ((t1 * t1 - t2 * t2) * (t1 * t1 - t2 * t2))) -
-                         (t2 * t2 * t2 * t2 * t2 * t2 - 3.0 * t2 * t2 * t2 * t2 * t1 * t1) /
-                             (r * (t1 * t1 - t2 * t2) * (t1 * t1 - t2 * t2) * (t1 * t1 - t2 * t2)));
-}
-
-__device__ double gsubp(double r, double t1, double t2) {
-  const double d2 = t1 * t1 - t2 * t2;
-  const double a = 0.5 * t2 * t2 * t2 * t2 * t1 / (d2 * d2);
-  const double b = (t2 * t2 * t2 * t2 * t2 * t2 - 3.0 * t2 * t2 * t2 * t2 * t1 * t1) / (d2 * d2 * d2);
-  return -t1 * exp(-t1 * r) * (a - b / r) + exp(-t1 * r) * b / (r * r);
-}
-
-// expGamma (shortgammafuncs.F90:97-148)
-__device__ double expgamma(double r, double ua, double ub) {
-  const double ta = 3.2 * ua, tb = 3.2 * ub;
-  if (r < 1.0e-5) {
-    if (fabs(ua - ub) < 1.0e-10) return -0.5 * (ua + ub);
-    return -0.5 * ((ta * tb) / (ta + tb) + (ta * tb) * (ta * tb) / ((ta + tb) * (ta + tb) * (ta + tb)));
-  }
-  if (fabs(ua - ub) < 1.0e-10) {
-    const double tm = 0.5 * (ta + tb);
-    return exp(-tm * r) * (1.0 / r + 0.6875 * tm + 0.1875 * r * tm * tm +
-                           0.02083333333333333333 * r * r * tm * tm * tm);
-  }
-  return gsub(r, ta, tb) + gsub(r, tb, ta);
-}
-
-// expGammaPrime (shortgammafuncs.F90:152-196)
-__device__ double expgamma_prime(double r, double ua, double ub) {
-  if (r < 1.0e-5) return 0.0;
-  if (fabs(ua - ub) < 1.0e-10) {
-    const double tm = 3.2 * 0.5 * (ua + ub);
-    return -tm * exp(-tm * r) *
-               (1.0 / r + 0.6875 * tm + 0.1875 * r * tm * tm +
-                0.02083333333333333333 * r * r * tm * tm * tm) +
-           exp(-tm * r) * (-1.0 / (r * r) + 0.1875 * tm * tm +
-                           2.0 * 0.02083333333333333333 * r * tm * tm * tm);
-  }
-  const double ta = 3.2 * ua, tb = 3.2 * ub;
-  return gsubp(r, ta, tb) + gsubp(r, tb, ta);
-}
-
 __global__ void invr_cluster_kernel(int nat, const double* coords, double* g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
@@ -705,6 +660,16 @@ int launch_short_gamma(hipStream_t s, const NeighDev& nb, const double* coords,
   if (nb.n == 0) return 0;
   hipLaunchKernelGGL(short_gamma_kernel, dim3(cdiv(nb.n, 256)), dim3(256), 0, s, nb, coords, hubbu,
                      gcut, nsp, nat, g);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_short_gamma_grad(hipStream_t s, const NeighDev& nb, const double* coords,
+                            const double* hubbu, const double* gcut, int nsp, const double* dq,
+                            double* grad) {
+  if (nb.n == 0) return 0;
+  hipLaunchKernelGGL(short_gamma_grad_kernel, dim3(cdiv(nb.n, 256)), dim3(256), 0, s, nb, coords,
+                     hubbu, gcut, nsp, dq, grad);
   DH_LAUNCH_CHECK();
   return 0;
 }

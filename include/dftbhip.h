@@ -141,6 +141,22 @@ int dftbhip_ctx_nr_orbitals(dftbhip_ctx* ctx);
 /* per-phase wall times (s) of the last SCC/gradient call and the number of calls */
 int dftbhip_ctx_timings(dftbhip_ctx* ctx, double* t, int n);
 
+/* ---- molecule batch: many independent non-periodic molecules in one set of launches -------
+ * Replaces a loop over single-geometry calculations (one dftbp_* instance per molecule,
+ * src/dftbp/api/mm/mmapi.F90 / api/dftbplus.h) with one block-diagonal system: per molecule its
+ * own electron count, Fermi level, Broyden mixer and convergence. FP64 only; at most 512
+ * orbitals per molecule. coords(3, sum nAtomMol) bohr, species 0-based, molecules one after
+ * the other. energies(8, nMol) as dftbhip_ctx_scc; charges(sum nAtomMol) = q0 - q. */
+typedef struct dftbhip_batch dftbhip_batch;
+int dftbhip_batch_create(dftbhip_batch** batch, const dftbhip_params* p,
+                         const dftbhip_options* opt);
+int dftbhip_batch_destroy(dftbhip_batch* batch);
+int dftbhip_batch_set_geometries(dftbhip_batch* batch, int nMol, const int* nAtomMol,
+                                 const double* coords, const int* species);
+int dftbhip_batch_scc(dftbhip_batch* batch, double* energies, double* charges, int* niter,
+                      int* converged);
+int dftbhip_batch_gradients(dftbhip_batch* batch, double* grad);
+
 /* ---- measurement: per-launch HIP-event timing of the tridiagonalisation's symmetric
  *      product (the HBM-bound kernel of the eigensolver). stats: out[0] launches, out[1] summed
  *      duration (s), out[2] algorithmic bytes (lower triangle of each trailing matrix).
