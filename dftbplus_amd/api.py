@@ -119,6 +119,7 @@ EXPORTS = {
     "dftbhip_ctx_eigenvalues": [C.c_void_p, _dp, _dp, C.c_int],
     "dftbhip_ctx_nr_orbitals": [C.c_void_p],
     "dftbhip_ctx_timings": [C.c_void_p, _dp, C.c_int],
+    "dftbhip_ctx_set_kpoints": [C.c_void_p, C.c_int, _dp, _dp],
     "dftbhip_batch_create": [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p],
     "dftbhip_batch_destroy": [C.c_void_p],
     "dftbhip_batch_set_geometries": [C.c_void_p, C.c_int, _ip, _dp, _ip],
@@ -384,6 +385,14 @@ class DftbPlus:
         self.species = _i32(geometry.species)
         self.natom = len(self.species)
         self._dirty = True
+        kp = getattr(options, "kpoints", None)
+        if kp is not None:
+            k = _f64(np.asarray(kp, dtype=np.float64).reshape(-1))
+            kw = getattr(options, "kweights", None)
+            w = None if kw is None else _f64(np.asarray(kw, dtype=np.float64))
+            _check(lib().dftbhip_ctx_set_kpoints(self.h, len(k) // 3, _d(k),
+                                                 None if w is None else _d(w)),
+                   "dftbhip_ctx_set_kpoints")
         self.set_geometry(geometry.coords, geometry.latvecs)
 
     def set_geometry(self, coords, latvecs=None):

@@ -947,6 +947,114 @@ static int fermi_filling(const std::vector<double>& e, double nel, double kT,
   return 0;
 }
 
+// Fermi filling over k-points with normalised weights kw (Efilling with kWeight,
+// etemp.F90:59-194; middleGap :481-520): e[k] ascending levels of k-point k. Fillings carry the
+// factor 2 of the spin-unpolarised case; Eband and TS are k-weighted.
+static int fermi_filling_k(const std::vector<std::vector<double>>& e,
+                           const std::vector<double>& kw, double nel, double kT,
+                           std::vector<std::vector<double>>& f, FillResult& r) {
+  kT = std::max(kT, 1.0e-8);
+  const double ne = 0.5 * nel;
+  const int nk = (int)e.size();
+  const int nlev = nk ? (int)e[0].size() : 0;
+  const double eps2 = 2.0 * 2.220446049250313e-16;
+  auto count = [&](double ef) {
+    double s = 0.0, comp = 0.0;
+    for (int k = 0; k < nk; ++k)
+      for (double x : e[k]) {
+        const double v = kw[k] / (1.0 + std::exp((x - ef) / kT));
+        const double y = v - comp;
+        const double t = s + y;
+        comp = (t - s) - y;
+        s = t;
+      }
+    return s;
+  };
+  double ef = 0.0;
+  auto finish = [&](double scale) {
+    f.assign(nk, std::vector<double>());
+    double eb = 0.0, ts = 0.0, cb = 0.0, ct = 0.0;
+    auto kahan = [](double& sum, double v, double& c) {
+      const double y = v - c;
+      const double t = sum + y;
+      c = (t - sum) - y;
+      sum = t;
+    };
+    for (int k = 0; k < nk; ++k) {
+      double ebk, tsk;
+      electron_fill(ef, e[k], kT, f[k], ebk, tsk);
+      kahan(eb, kw[k] * ebk, cb);
+      kahan(ts, kw[k] * tsk, ct);
+      for (double& x : f[k]) x *= 2.0 * scale;
+    }
+    r.ef = ef;
+    r.eband = 2.0 * eb;
+    r.ts = 2.0 * ts;
+    r.e0 = 2.0 * (eb - 0.5 * ts);
+  };
+  double emin = HUGE_VAL, emax = -HUGE_VAL;
+  for (int k = 0; k < nk; ++k)
+    for (double x : e[k]) {
+      emin = std::min(emin, x);
+      emax = std::max(emax, x);
+    }
+  if (nlev <= ne) {
+    ef = emax + 1000.0 * (kT + 1.1920928955078125e-07);
+    finish(1.0);
+    return 0;
+  }
+  if (std::fabs(ne - std::round(ne)) <= 1.0e-15) {
+    // all levels sorted, weights accumulated until the electron count is reached
+    std::vector<std::pair<double, int>> all;
+    all.reserve((size_t)nk * nlev);
+    for (int k = 0; k < nk; ++k)
+      for (int j = 0; j < nlev; ++j) all.push_back({e[k][j], k});
+    std::stable_sort(all.begin(), all.end(),
+                     [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+                       return a.first < b.first;
+                     });
+    double acc = 0.0;
+    size_t ind = 0;
+    while (acc < ne && ind < all.size()) acc += kw[all[ind++].second];
+    ind = std::min(all.size() - 1, ind);
+    ef = 0.5 * (all[ind].first + all[ind - 1].first);
+    if (std::fabs(ne - count(ef)) <= 1.0e-7) {
+      finish(1.0);
+      return 0;
+    }
+  }
+  double upper = emax + 0.01, lower = emin - 0.01;
+  double nmax = count(upper), nmin = count(lower);
+  while (nmin > ne) {
+    lower = 2.0 * (lower - upper) + lower;
+    nmin = count(lower);
+  }
+  while (nmax < ne) {
+    upper = 2.0 * (upper - lower) + lower;
+    nmax = count(upper);
+  }
+  ef = 0.5 * (upper + lower);
+  double nc = count(ef);
+  while (std::fabs(ne - nc) > 1.0e-15 &&
+         std::fabs(upper - lower) >= std::max(std::fabs(ef) * eps2, eps2)) {
+    if ((nmax >= nmin) == (ne >= nc)) {
+      lower = ef;
+      nmin = nc;
+    } else {
+      upper = ef;
+      nmax = nc;
+    }
+    ef = 0.5 * (upper + lower);
+    nc = count(ef);
+  }
+  if (std::fabs(ne - nc) > 1.0e-7) {
+    set_error("Fermi level search did not converge.");
+    return kErrState;
+  }
+  finish((ne > eps2 && std::fabs(ne - nc) > eps2) ? ne / nc : 1.0);
+  return 0;
+}
+
 // Modified Broyden mixing (src/dftbp/mixer/broydenmixer.F90:204-342).
 struct Broyden {
   int n = 0, it = 0;
@@ -1073,6 +1181,14 @@ struct Context {
   DevBuf<double> coordsAll, coordsC, h0, ov, hs, rho, erho, gamma, vorb, vatom, dq, qorb, eorb;
   DevBuf<double> Hd, Sd, X, P, f, feps, grad, eatom, ewG, ewW, ewTr, ewCS, ewCSq, ww2;
   DevBuf<float> f32;   // FP32 filling weights of the mixed path
+  // k-points: fractional k (3 nk), normalised weights; per block fractional image translation;
+  // per k-point the inverse Cholesky factor of the embedded S(k) and all eigenvectors
+  int nk = 0;
+  std::vector<double> kpts, kw;
+  DevBuf<double> blkT;
+  std::vector<DevBuf<double>> LinvK, XK;
+  std::vector<std::vector<double>> eigK, fillK;
+  std::vector<int> noccK;
   EigenSolver eig;
   std::vector<double> q0atom, eigv, fill, qout, dqout;
   double energies[8]{};
@@ -1159,10 +1275,39 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
   DH_TRY(c.feps.alloc(n));
   DH_TRY(c.grad.alloc(3 * (size_t)nat));
   DH_TRY(c.eatom.alloc(nat));
-  DH_TRY(c.Hd.alloc((size_t)n * n));
-  DH_TRY(c.Sd.alloc((size_t)n * n));
-  DH_TRY(c.X.alloc((size_t)n * n));
-  DH_TRY(c.P.alloc((size_t)n * n));
+  // k-points: complex problems as their real embedding of size 2n
+  const int nd = c.nk > 0 ? 2 * n : n;
+  DH_CHECK(c.nk == 0 || c.periodic, "set_geometry: k-points need lattice vectors");
+  DH_TRY(c.Hd.alloc((size_t)nd * nd));
+  DH_TRY(c.Sd.alloc((size_t)nd * nd));
+  DH_TRY(c.X.alloc((size_t)nd * nd));
+  DH_TRY(c.P.alloc((size_t)nd * nd));
+  if (c.nk > 0) {
+    // fractional cell translation of each block's image: T = (r_img - r_centre) L^-1
+    const double* L = c.lat;
+    const double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
+                       L[2] * (L[3] * L[7] - L[4] * L[6]);
+    double inv[9];
+    inv[0] = (L[4] * L[8] - L[5] * L[7]) / det;
+    inv[1] = -(L[1] * L[8] - L[2] * L[7]) / det;
+    inv[2] = (L[1] * L[5] - L[2] * L[4]) / det;
+    inv[3] = -(L[3] * L[8] - L[5] * L[6]) / det;
+    inv[4] = (L[0] * L[8] - L[2] * L[6]) / det;
+    inv[5] = -(L[0] * L[5] - L[2] * L[3]) / det;
+    inv[6] = (L[3] * L[7] - L[4] * L[6]) / det;
+    inv[7] = -(L[0] * L[7] - L[1] * L[6]) / det;
+    inv[8] = (L[0] * L[4] - L[1] * L[3]) / det;
+    const size_t nb = c.blk.at1.size();
+    std::vector<double> T(3 * std::max<size_t>(nb, 1), 0.0);
+    for (size_t k = 0; k < nb; ++k) {
+      const int img = c.blk.img2[k], jc = c.blk.at2[k];
+      double d[3];
+      for (int q = 0; q < 3; ++q) d[q] = c.nl.coordsAll[3 * img + q] - coords[3 * jc + q];
+      for (int q = 0; q < 3; ++q)
+        T[3 * k + q] = std::nearbyint(d[0] * inv[q] + d[1] * inv[3 + q] + d[2] * inv[6 + q]);
+    }
+    DH_TRY(c.blkT.upload(T.data(), T.size(), s));
+  }
   DH_TRY(c.gamma.alloc((size_t)nat * nat));
   // H0, S
   DH_TRY(launch_build_h0s(s, c.sk.dev, c.dblk.pr, c.coordsAll.p, c.h0.p, c.ov.p));
@@ -1194,23 +1339,42 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
   }
   DH_TRY(launch_short_gamma(s, c.nbdev, c.coordsAll.p, c.sk.dhubbu.p, c.sk.dgcut.p, ns, nat,
                             c.gamma.p));
-  // overlap factorisation (S fixed during the SCC cycle)
-  DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.Sd.p, n,
-                               n));
-  DH_TRY(c.eig.ensure(n));
+  // overlap factorisation (S fixed during the SCC cycle); per k-point with k-points
+  DH_TRY(c.eig.ensure(nd));
   c.eig.factored = false;
   c.eig.twoStage = c.opt.tridiag == 0;
   int info = 0;
-  DH_TRY(c.eig.factor_overlap(s, c.Sd.p, n, &info));
+  if (c.nk == 0) {
+    DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.Sd.p,
+                                 n, n));
+    DH_TRY(c.eig.factor_overlap(s, c.Sd.p, n, &info));
+  } else {
+    c.LinvK.resize(c.nk);
+    c.XK.resize(c.nk);
+    c.eigK.assign(c.nk, std::vector<double>());
+    c.fillK.assign(c.nk, std::vector<double>());
+    c.noccK.assign(c.nk, 0);
+    for (int k = 0; k < c.nk && info == 0; ++k) {
+      DH_TRY(launch_unpack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
+                               &c.kpts[3 * k], c.ov.p, c.Sd.p, nd, n));
+      DH_TRY(c.eig.factor_overlap(s, c.Sd.p, nd, &info));
+      if (info != 0) break;
+      DH_TRY(c.LinvK[k].alloc((size_t)nd * nd));
+      DH_TRY(c.XK[k].alloc((size_t)nd * nd));
+      DH_TRY(copy2d<double>(s, c.eig.Linv.p, nd, c.LinvK[k].p, nd, nd, nd));
+    }
+  }
   if (info != 0) {
     set_error("set_geometry: overlap matrix not positive definite (minor " +
               std::to_string(info) + ")");
-    return n + info;
+    return nd + info;
   }
   if (c.opt.precision == 1) {
+    DH_CHECK(c.nk == 0, "set_geometry: k-points run in FP64 (precision 0)");
     DH_TRY(c.eig.prepare_mixed(s));
     DH_TRY(c.f32.alloc(n));
   }
+  if (c.nk > 0) DH_TRY(c.f.alloc(nd));
   DH_HIP(hipStreamSynchronize(s));
   c.sccDone = false;
   c.times[kTSetup] = now() - t0;
@@ -1281,83 +1445,123 @@ static int ctx_scc_step(Context& c) {
   DH_TRY(gamma_times(c, dqa, nullptr));
   DH_TRY(launch_shift(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.h0.p, c.ov.p, c.vorb.p,
                       c.hs.p));
-  // FP64: the plain path. Mixed: FP32 reduction, tridiagonalisation, back-transformation and
-  // density matrix. Mixed + refined: the FP32 eigenvectors of all levels are refined in FP64
-  // (Ogita-Aishima) and the density matrix is built in FP64.
+  double evt[3] = {0, 0, 0};
   const bool mixed = o.precision == 1;
   const bool refined = mixed && st.prevErr <= o.refine_below;
-  c.xf64 = !mixed || refined;
-  float* X32 = reinterpret_cast<float*>(c.X.p);
-  float* P32 = reinterpret_cast<float*>(c.P.p);
-  float* H32 = refined ? c.eig.h32() : reinterpret_cast<float*>(c.Hd.p);
-  if (mixed && !refined) {
-    DH_TRY(launch_unpack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, H32, n,
-                                n));
+  if (c.nk > 0) {
+    // k-points: per k the embedded H(k), eigenpairs and all eigenvectors; then the k-weighted
+    // Fermi level and the density matrix k by k (packHS_cmplx_kpts)
+    const int nd = 2 * n;
+    c.xf64 = true;
+    DH_HIP(hipEventRecord(ev[1], s));
+    std::vector<std::vector<double>> ec(c.nk);
+    for (int k = 0; k < c.nk; ++k) {
+      DH_TRY(launch_unpack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
+                               &c.kpts[3 * k], c.hs.p, c.Hd.p, nd, n));
+      DH_TRY(copy2d<double>(s, c.LinvK[k].p, nd, c.eig.Linv.p, nd, nd, nd));
+      int info = 0;
+      DH_TRY(c.eig.solve(s, c.Hd.p, nd, c.eigK[k], &info, evt));
+      if (info != 0) return info;
+      DH_TRY(c.eig.vectors(s, nd, c.XK[k].p, nd));
+      // each complex level appears twice in the embedding
+      ec[k].resize(n);
+      for (int j = 0; j < n; ++j) ec[k][j] = c.eigK[k][2 * j];
+    }
+    DH_HIP(hipEventRecord(ev[2], s));
+    const double th = now();
+    DH_TRY(fermi_filling_k(ec, c.kw, st.nel, o.kT, c.fillK, st.fr));
+    c.times[kTHost] += now() - th;
+    DH_HIP(hipEventRecord(ev[3], s));
+    DH_TRY(fill<double>(s, c.rho.p, c.nnz, 0.0));
+    std::vector<double> fe(nd);
+    for (int k = 0; k < c.nk; ++k) {
+      c.noccK[k] = occupied_levels(c.fillK[k]);
+      for (int j = 0; j < n; ++j) fe[2 * j] = fe[2 * j + 1] = c.fillK[k][j];
+      DH_HIP(hipMemcpyAsync(c.f.p, fe.data(), nd * sizeof(double), hipMemcpyHostToDevice, s));
+      DH_TRY(gemm<double>(s, false, true, nd, nd, 2 * c.noccK[k], 1.0, c.XK[k].p, nd, c.XK[k].p,
+                          nd, 0.0, c.P.p, nd, true, kKFull, c.f.p));
+      DH_TRY(launch_pack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
+                             &c.kpts[3 * k], c.kw[k], c.P.p, nd, n, c.rho.p));
+      DH_HIP(hipStreamSynchronize(s));   // fe is reused for the next k-point
+    }
+    c.eigv.clear();
+    DH_HIP(hipEventRecord(ev[4], s));
   } else {
-    DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p,
-                                 n, n));
-    if (refined) DH_TRY((convert2d<double, float>(s, c.Hd.p, n, H32, n, n, n)));
+    // FP64: the plain path. Mixed: FP32 reduction, tridiagonalisation, back-transformation and
+    // density matrix. Mixed + refined: the FP32 eigenvectors of all levels are refined in FP64
+    // (Ogita-Aishima) and the density matrix is built in FP64.
+    c.xf64 = !mixed || refined;
+    float* X32 = reinterpret_cast<float*>(c.X.p);
+    float* P32 = reinterpret_cast<float*>(c.P.p);
+    float* H32 = refined ? c.eig.h32() : reinterpret_cast<float*>(c.Hd.p);
+    if (mixed && !refined) {
+      DH_TRY(launch_unpack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, H32, n,
+                                  n));
+    } else {
+      DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p,
+                                   n, n));
+      if (refined) DH_TRY((convert2d<double, float>(s, c.Hd.p, n, H32, n, n, n)));
+    }
+    DH_HIP(hipEventRecord(ev[1], s));
+    int info = 0;
+    if (mixed)
+      DH_TRY(c.eig.solve_mixed(s, H32, n, c.eigv, &info, evt));
+    else
+      DH_TRY(c.eig.solve(s, c.Hd.p, n, c.eigv, &info, evt));
+    if (info != 0) return info;
+    const double th = now();
+    DH_TRY(fermi_filling(c.eigv, st.nel, o.kT, c.fill, st.fr));
+    int nocc = occupied_levels(c.fill);
+    c.times[kTHost] += now() - th;
+    std::vector<float> fill32;
+    if (mixed && !refined) {
+      fill32.assign(c.fill.begin(), c.fill.end());
+      DH_HIP(hipMemcpyAsync(c.f32.p, fill32.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    } else if (!mixed) {
+      DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    DH_HIP(hipEventRecord(ev[2], s));
+    if (refined) {
+      // all levels form the refinement basis; the leading nocc + margin refined vectors enter
+      // the density matrix with fillings recomputed from the refined eigenvalues
+      DH_TRY(c.eig.vectors_mixed(s, n, P32, n));
+      DH_HIP(hipEventRecord(ev[6], s));
+      const int nvec = std::min(n, nocc + kRefineMargin);
+      std::vector<double> lam;
+      DH_TRY(c.eig.refine(s, c.Hd.p, c.Sd.p, P32, o.refine_steps, nvec, c.X.p, n, lam));
+      DH_HIP(hipEventRecord(ev[7], s));
+      const double th3 = now();
+      std::vector<int> perm(n);
+      std::iota(perm.begin(), perm.end(), 0);
+      std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return lam[x] < lam[y]; });
+      std::vector<double> ls(n), fs;
+      for (int k = 0; k < n; ++k) ls[k] = lam[perm[k]];
+      DH_TRY(fermi_filling(ls, st.nel, o.kT, fs, st.fr));
+      for (int k = 0; k < n; ++k) c.fill[perm[k]] = fs[k];
+      c.eigv = lam;
+      nocc = nvec;
+      DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+      c.times[kTHost] += now() - th3;
+    } else if (mixed) {
+      DH_TRY(c.eig.vectors_mixed(s, nocc, X32, n));
+    } else {
+      DH_TRY(c.eig.vectors(s, nocc, c.X.p, n));
+    }
+    c.nocc = nocc;
+    DH_HIP(hipEventRecord(ev[3], s));
+    if (c.xf64)
+      DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
+                          kKFull, c.f.p));
+    else
+      DH_TRY(gemm<float>(s, false, true, n, n, nocc, 1.0f, X32, n, X32, n, 0.0f, P32, n, true,
+                         kKFull, c.f32.p));
+    DH_HIP(hipEventRecord(ev[4], s));
+    if (c.xf64)
+      DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
+                                 c.rho.p));
+    else
+      DH_TRY(launch_pack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, P32, n, c.rho.p));
   }
-  DH_HIP(hipEventRecord(ev[1], s));
-  int info = 0;
-  double evt[3] = {0, 0, 0};
-  if (mixed)
-    DH_TRY(c.eig.solve_mixed(s, H32, n, c.eigv, &info, evt));
-  else
-    DH_TRY(c.eig.solve(s, c.Hd.p, n, c.eigv, &info, evt));
-  if (info != 0) return info;
-  const double th = now();
-  DH_TRY(fermi_filling(c.eigv, st.nel, o.kT, c.fill, st.fr));
-  int nocc = occupied_levels(c.fill);
-  c.times[kTHost] += now() - th;
-  std::vector<float> fill32;
-  if (mixed && !refined) {
-    fill32.assign(c.fill.begin(), c.fill.end());
-    DH_HIP(hipMemcpyAsync(c.f32.p, fill32.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
-  } else if (!mixed) {
-    DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-  }
-  DH_HIP(hipEventRecord(ev[2], s));
-  if (refined) {
-    // all levels form the refinement basis; the leading nocc + margin refined vectors enter
-    // the density matrix with fillings recomputed from the refined eigenvalues
-    DH_TRY(c.eig.vectors_mixed(s, n, P32, n));
-    DH_HIP(hipEventRecord(ev[6], s));
-    const int nvec = std::min(n, nocc + kRefineMargin);
-    std::vector<double> lam;
-    DH_TRY(c.eig.refine(s, c.Hd.p, c.Sd.p, P32, o.refine_steps, nvec, c.X.p, n, lam));
-    DH_HIP(hipEventRecord(ev[7], s));
-    const double th3 = now();
-    std::vector<int> perm(n);
-    std::iota(perm.begin(), perm.end(), 0);
-    std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return lam[x] < lam[y]; });
-    std::vector<double> ls(n), fs;
-    for (int k = 0; k < n; ++k) ls[k] = lam[perm[k]];
-    DH_TRY(fermi_filling(ls, st.nel, o.kT, fs, st.fr));
-    for (int k = 0; k < n; ++k) c.fill[perm[k]] = fs[k];
-    c.eigv = lam;
-    nocc = nvec;
-    DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-    c.times[kTHost] += now() - th3;
-  } else if (mixed) {
-    DH_TRY(c.eig.vectors_mixed(s, nocc, X32, n));
-  } else {
-    DH_TRY(c.eig.vectors(s, nocc, c.X.p, n));
-  }
-  c.nocc = nocc;
-  DH_HIP(hipEventRecord(ev[3], s));
-  if (c.xf64)
-    DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
-                        kKFull, c.f.p));
-  else
-    DH_TRY(gemm<float>(s, false, true, n, n, nocc, 1.0f, X32, n, X32, n, 0.0f, P32, n, true,
-                       kKFull, c.f32.p));
-  DH_HIP(hipEventRecord(ev[4], s));
-  if (c.xf64)
-    DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
-                               c.rho.p));
-  else
-    DH_TRY(launch_pack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, P32, n, c.rho.p));
   DH_TRY(launch_mulliken(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.h0.p, c.rho.p,
                          c.qorb.p, c.eorb.p, n));
   DH_HIP(hipMemcpyAsync(qorbH.data(), c.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1457,8 +1661,21 @@ static int ctx_gradients(Context& c, double* grad) {
   const double t0 = now();
   // energy-weighted density matrix
   std::vector<double> fe(n, 0.0);
-  for (int i = 0; i < c.nocc; ++i) fe[i] = c.fill[i] * c.eigv[i];
-  if (!c.xf64) {
+  for (int i = 0; i < c.nocc && i < (int)c.eigv.size(); ++i) fe[i] = c.fill[i] * c.eigv[i];
+  if (c.nk > 0) {
+    const int nd = 2 * n;
+    std::vector<double> fk(nd);
+    DH_TRY(fill<double>(s, c.erho.p, c.nnz, 0.0));
+    for (int k = 0; k < c.nk; ++k) {
+      for (int j = 0; j < n; ++j) fk[2 * j] = fk[2 * j + 1] = c.fillK[k][j] * c.eigK[k][2 * j];
+      DH_HIP(hipMemcpyAsync(c.f.p, fk.data(), nd * sizeof(double), hipMemcpyHostToDevice, s));
+      DH_TRY(gemm<double>(s, false, true, nd, nd, 2 * c.noccK[k], 1.0, c.XK[k].p, nd, c.XK[k].p,
+                          nd, 0.0, c.P.p, nd, true, kKFull, c.f.p));
+      DH_TRY(launch_pack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
+                             &c.kpts[3 * k], c.kw[k], c.P.p, nd, n, c.erho.p));
+      DH_HIP(hipStreamSynchronize(s));
+    }
+  } else if (!c.xf64) {
     std::vector<float> fe32(fe.begin(), fe.end());
     float* X32 = reinterpret_cast<float*>(c.X.p);
     float* P32 = reinterpret_cast<float*>(c.P.p);
@@ -1877,6 +2094,27 @@ extern "C" int dftbhip_ctx_create(dftbhip_ctx** ctx, const dftbhip_params* p,
 
 extern "C" int dftbhip_ctx_destroy(dftbhip_ctx* ctx) {
   delete ctx;
+  return 0;
+}
+
+extern "C" int dftbhip_ctx_set_kpoints(dftbhip_ctx* ctx, int nK, const double* kpts,
+                                       const double* weights) {
+  DH_CHECK(ctx && nK >= 0 && (nK == 0 || kpts), "set_kpoints: bad arguments");
+  dftbhip::Context& c = ctx->c;
+  c.nk = nK;
+  c.kpts.assign(kpts, kpts + 3 * (size_t)nK);
+  c.kw.assign(nK, 1.0);
+  if (weights) c.kw.assign(weights, weights + nK);
+  double sum = 0.0;
+  for (double w : c.kw) {
+    DH_CHECK(w >= 0.0, "set_kpoints: negative weight");
+    sum += w;
+  }
+  DH_CHECK(nK == 0 || sum > 0.0, "set_kpoints: weights sum to zero");
+  for (double& w : c.kw) w /= sum;
+  if (c.nat > 0)   // re-derive the per-k-point overlaps of the current geometry
+    return dftbhip::ctx_geometry(c, c.nat, c.coords.data(), c.species.data(),
+                                 c.periodic ? c.lat : nullptr);
   return 0;
 }
 

@@ -62,6 +62,14 @@ int launch_unpack(hipStream_t s, const PairsDev& pr, const int* norb, const long
 template <typename T>
 int launch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                 const T* dense, long ld, double* x);
+// k-point matrices as their real 2n x 2n embedding (blkT: fractional cell translation of the
+// image of each block, kf: fractional k-point); pack_kpt accumulates into x
+int launch_unpack_kpt(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                      const double* blkT, const double* kf, const double* x, double* dense,
+                      long ld, int n);
+int launch_pack_kpt(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                    const double* blkT, const double* kf, double w, const double* dense, long ld,
+                    int n, double* x);
 int launch_shift(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                  const double* h0, const double* ov, const double* vorb, double* h);
 int launch_mulliken(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,

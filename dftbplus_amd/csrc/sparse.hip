@@ -245,6 +245,70 @@ __global__ void pack_kernel(PairsDev pr, const int* norbAtom, const long* iAtomS
     }
 }
 
+// Complex matrix at the k-point kf (fractional) into its real 2n x 2n embedding
+// [[Re, -Im], [Im, Re]]: blocks times exp(i 2 pi kf.T) at (atom2 rows, atom1 columns) and the
+// Hermitian counterpart; on-site blocks contribute their lower triangle with a real diagonal
+// ('L' storage of unpackHS_cmplx_kpts, sparse2dense.F90:185-262). The dense array is zeroed
+// beforehand.
+__global__ void unpack_kpt_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
+                                  const double* blkT, double kx, double ky, double kz,
+                                  const double* x, double* dense, long ld, long n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr.nblk) return;
+  const int a1 = pr.at1[k], a2 = pr.at2[k];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
+  const long st = pr.start[k];
+  double sn, cs;
+  sincos(2.0 * M_PI * (kx * blkT[3 * k] + ky * blkT[3 * k + 1] + kz * blkT[3 * k + 2]), &sn, &cs);
+  for (int c = 0; c < n1; ++c)
+    for (int r = 0; r < n2; ++r) {
+      if (a1 == a2 && r < c) continue;
+      const double v = x[st + r + c * n2];
+      const double re = v * cs, im = v * sn;
+      const long i = o2 + r, j = o1 + c;
+      atomicAdd(&dense[i + j * ld], re);
+      atomicAdd(&dense[(n + i) + (n + j) * ld], re);
+      if (i == j) continue;
+      atomicAdd(&dense[(n + i) + j * ld], im);
+      atomicAdd(&dense[i + (n + j) * ld], -im);
+      atomicAdd(&dense[j + i * ld], re);
+      atomicAdd(&dense[(n + j) + (n + i) * ld], re);
+      atomicAdd(&dense[(n + j) + i * ld], -im);
+      atomicAdd(&dense[j + (n + i) * ld], im);
+    }
+}
+
+// x += w Re(exp(-i 2 pi kf.T) P) per block, P = TL + i BL read from the lower triangle of the
+// embedded 2n x 2n matrix; on-site blocks completed from their lower triangle by conjugation
+// (packHS_cmplx_kpts, sparse2dense.F90:820-918)
+__global__ void pack_kpt_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
+                                const double* blkT, double kx, double ky, double kz, double w,
+                                const double* dense, long ld, long n, double* x) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr.nblk) return;
+  const int a1 = pr.at1[k], a2 = pr.at2[k];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
+  const long st = pr.start[k];
+  double sn, cs;
+  sincos(2.0 * M_PI * (kx * blkT[3 * k] + ky * blkT[3 * k + 1] + kz * blkT[3 * k + 2]), &sn, &cs);
+  for (int c = 0; c < n1; ++c)
+    for (int r = 0; r < n2; ++r) {
+      double re, im;
+      if (a1 == a2 && r < c) {
+        const long i = o2 + c, j = o1 + r;
+        re = dense[i + j * ld];
+        im = -dense[(n + i) + j * ld];
+      } else {
+        const long i = o2 + r, j = o1 + c;
+        re = dense[i + j * ld];
+        im = dense[(n + i) + j * ld];
+      }
+      x[st + r + c * n2] += w * (re * cs + im * sn);
+    }
+}
+
 __global__ void shift_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
                              const double* h0, const double* s, const double* vorb, double* h) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -595,6 +659,27 @@ int launch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* 
   if (pr.nblk == 0) return 0;
   hipLaunchKernelGGL((pack_kernel<T>), dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
                      iAtomStart, dense, ld, x);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_unpack_kpt(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                      const double* blkT, const double* kf, const double* x, double* dense,
+                      long ld, int n) {
+  DH_TRY(fill2d<double>(s, dense, ld, 2 * n, 2 * n, 0.0));
+  if (pr.nblk == 0) return 0;
+  hipLaunchKernelGGL(unpack_kpt_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+                     iAtomStart, blkT, kf[0], kf[1], kf[2], x, dense, ld, (long)n);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_pack_kpt(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
+                    const double* blkT, const double* kf, double w, const double* dense, long ld,
+                    int n, double* x) {
+  if (pr.nblk == 0) return 0;
+  hipLaunchKernelGGL(pack_kpt_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+                     iAtomStart, blkT, kf[0], kf[1], kf[2], w, dense, ld, (long)n, x);
   DH_LAUNCH_CHECK();
   return 0;
 }

@@ -106,6 +106,10 @@ class Options:
     weight_fac: float = 1.0e-2
     fd_delta: float = float(np.finfo(float).eps) ** 0.25
     ewald_tol: float = 1.0e-9
+    # k-points of a periodic system: fractional coordinates (nK x 3) in units of the reciprocal
+    # lattice vectors and weights (normalised to 1); None = Gamma point with real arithmetic
+    kpoints: Optional[np.ndarray] = None
+    kweights: Optional[np.ndarray] = None
 
 
 def orbital_offsets(params: Parameters, species: Sequence[int]) -> np.ndarray:

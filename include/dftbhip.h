@@ -121,6 +121,13 @@ typedef struct {
 typedef struct dftbhip_ctx dftbhip_ctx;
 int dftbhip_ctx_create(dftbhip_ctx** ctx, const dftbhip_params* p, const dftbhip_options* opt);
 int dftbhip_ctx_destroy(dftbhip_ctx* ctx);
+/* k-points (periodic systems): kpts(3, nK) in units of the reciprocal lattice vectors, weights
+ * (normalised here; NULL = equal). nK = 0 returns to the Gamma point in real arithmetic. Each
+ * k-point's complex Hermitian problem is solved through its real 2n x 2n embedding; FP64 only.
+ * Replaces the k-point loop of the reference (kPoint/kWeight, unpackHS_cmplx_kpts,
+ * packHS_cmplx_kpts: src/dftbp/dftb/sparse2dense.F90:185-262, 820-918; Efilling with kWeight:
+ * src/dftbp/dftb/etemp.F90:59-194). Call before or after set_geometry. */
+int dftbhip_ctx_set_kpoints(dftbhip_ctx* ctx, int nK, const double* kpts, const double* weights);
 /* coords(3, nAtom) bohr, species 0-based, latvecs(3,3) rows = lattice vectors or NULL */
 int dftbhip_ctx_set_geometry(dftbhip_ctx* ctx, int nAtom, const double* coords,
                              const int* species, const double* latvecs);

@@ -143,6 +143,8 @@ class Sparse:
     def index_maps(self, off: np.ndarray) -> "Sparse":
         rows, cols, offs, prow, pcol = (np.zeros(self.nnz, dtype=np.int64) for _ in range(5))
         offs = np.zeros(self.nnz, dtype=bool)
+        blk = np.zeros(self.nnz, dtype=np.int64)
+        swap = np.zeros(self.nnz, dtype=bool)
         for k in range(len(self.at1)):
             a, b = self.at1[k], self.at2[k]
             n1, n2 = off[a + 1] - off[a], off[b + 1] - off[b]
@@ -152,14 +154,17 @@ class Sparse:
             rows[sl] = off[b] + r
             cols[sl] = off[a] + c
             offs[sl] = a != b
+            blk[sl] = k
             if a == b:
                 lo = r >= c
+                swap[sl] = ~lo
                 prow[sl] = np.where(lo, off[b] + r, off[b] + c)
                 pcol[sl] = np.where(lo, off[a] + c, off[a] + r)
             else:
                 prow[sl] = off[b] + r
                 pcol[sl] = off[a] + c
         self.rows, self.cols, self.offsite, self.prow, self.pcol = rows, cols, offs, prow, pcol
+        self.blk, self.swap = blk, swap
         return self
 
 
@@ -355,6 +360,37 @@ def unpack(x: np.ndarray, params, species, sp: Sparse, off: np.ndarray) -> np.nd
         blk = x[sp.start[k]: sp.start[k] + n1 * n2].reshape(n1, n2).T
         out[off[b]:off[b] + n2, off[a]:off[a] + n1] += blk
     return out
+
+
+def block_phases(nl, sp: Sparse, coords, latvecs, kpt) -> np.ndarray:
+    """Bloch phase exp(i 2 pi k.T) of every sparse block, T the fractional cell translation of
+    the block's image atom (unpackHS_cmplx_kpts, sparse2dense.F90:185-262)."""
+    T = nl.coords_all[sp.img2] - coords[nl.img2cent[sp.img2]]
+    frac = np.rint(T @ np.linalg.inv(latvecs))
+    return np.exp(2j * np.pi * (frac @ np.asarray(kpt, dtype=float)))
+
+
+def unpack_k(x: np.ndarray, sp: Sparse, off: np.ndarray, phase: np.ndarray) -> np.ndarray:
+    """Complex square matrix at one k-point: blocks times their phase added at (atom2 rows,
+    atom1 columns), i.e. the lower triangle plus full on-site blocks (unpackHS_cmplx_kpts)."""
+    N = int(off[-1])
+    out = np.zeros((N, N), dtype=complex)
+    np.add.at(out, (sp.rows, sp.cols), x * phase[sp.blk])
+    return out
+
+
+def hermitian_from_lower(m: np.ndarray) -> np.ndarray:
+    """The Hermitian matrix whose lower triangle (real diagonal) is that of m ('L' storage)."""
+    lo = np.tril(m, -1)
+    return lo + lo.conj().T + np.diag(np.diag(m).real)
+
+
+def pack_k(dense: np.ndarray, sp: Sparse, phase: np.ndarray, weight: float) -> np.ndarray:
+    """kWeight * Re(exp(-i 2 pi k.T) block), on-site blocks completed from their lower triangle
+    by conjugation (packHS_cmplx_kpts, sparse2dense.F90:820-918)."""
+    vals = dense[sp.prow, sp.pcol]
+    vals = np.where(sp.swap, np.conj(vals), vals)
+    return weight * np.real(np.conj(phase[sp.blk]) * vals)
 
 
 def symmetrize_lower(m: np.ndarray) -> np.ndarray:
@@ -722,6 +758,74 @@ def _electron_fill(ef, eig, kT):
     return f, eband, ts
 
 
+def fermi_filling_k(eig: np.ndarray, nel: float, kT: float, kw: np.ndarray):
+    """Spin-unpolarised Fermi filling over k-points (``Efilling`` with kWeight, etemp.F90:59-194,
+    ``electronCount``, ``electronFill``, ``middleGap`` :481-520): eig (nK, nLev), normalised
+    weights kw. Returns (f (nK, nLev) with the factor 2, Ef, Eband, TS, E0), Eband and TS
+    k-weighted."""
+    kT = max(kT, MIN_TEMP)
+    ne = 0.5 * nel
+    eig = np.asarray(eig)
+    nk, nlev = eig.shape
+
+    def count(ef):
+        x = (eig - ef) / kT
+        with np.errstate(over="ignore"):
+            occ = 1.0 / (1.0 + np.exp(x))
+        return math.fsum((occ * kw[:, None]).ravel())
+
+    def fill(ef):
+        f = np.zeros_like(eig)
+        eb, ts = [], []
+        for k in range(nk):
+            fk, ebk, tsk = _electron_fill(ef, eig[k], kT)
+            f[k] = fk
+            eb.append(kw[k] * ebk)
+            ts.append(kw[k] * tsk)
+        return f, math.fsum(eb), math.fsum(ts)
+
+    def done(ef, scale=1.0):
+        f, eb, ts = fill(ef)
+        return 2 * f * scale, ef, 2 * eb, 2 * ts, 2 * (eb - 0.5 * ts)
+
+    if nlev <= ne:
+        return done(eig.max() + 1000.0 * (kT + float(np.finfo(np.float32).eps)))
+    if abs(ne - round(ne)) <= ELEC_TOL:
+        flat = eig.T.ravel(order="F")           # (level, k) column-major, as reshape(eigenvals)
+        order = np.argsort(flat, kind="stable")
+        acc, ind = 0.0, 0
+        while acc < ne:
+            acc += kw[order[ind] // nlev]
+            ind += 1
+        ind = min(len(flat) - 1, ind)
+        ef = 0.5 * (flat[order[ind]] + flat[order[ind - 1]])
+        if abs(ne - count(ef)) <= ELEC_TOL_MAX:
+            return done(ef)
+    upper = eig.max() + 0.01
+    lower = eig.min() - 0.01
+    nmax, nmin = count(upper), count(lower)
+    while nmin > ne:
+        lower = 2.0 * (lower - upper) + lower
+        nmin = count(lower)
+    while nmax < ne:
+        upper = 2.0 * (upper - lower) + lower
+        nmax = count(upper)
+    ef = 0.5 * (upper + lower)
+    ncur = count(ef)
+    eps2 = 2.0 * EPS
+    while abs(ne - ncur) > ELEC_TOL and abs(upper - lower) >= max(abs(ef) * eps2, eps2):
+        if (nmax >= nmin) == (ne >= ncur):
+            lower, nmin = ef, ncur
+        else:
+            upper, nmax = ef, ncur
+        ef = 0.5 * (upper + lower)
+        ncur = count(ef)
+    if abs(ne - ncur) > ELEC_TOL_MAX:
+        raise RuntimeError("Fermi level search did not converge.")
+    scale = ne / ncur if (ne > eps2 and abs(ne - ncur) > eps2) else 1.0
+    return done(ef, scale)
+
+
 def levels(f: np.ndarray) -> slice:
     """Levels entering the density matrices: up to the last filling >= machine epsilon
     (densitymatrix.F90:549-554, 688-692)."""
@@ -910,6 +1014,56 @@ class Calculator:
         self.q0 = np.array([sum(params.species[s].occ) for s in sp])
         self.nel = float(self.q0.sum())
         self.erep, self.grad_rep = repulsive(params, sp, self.nl)
+        self.kpts = None
+        if getattr(opts, "kpoints", None) is not None:
+            if geo.latvecs is None:
+                raise ValueError("k-points need a periodic geometry")
+            self.kpts = np.asarray(opts.kpoints, dtype=float).reshape(-1, 3)
+            w = (np.ones(len(self.kpts)) if opts.kweights is None
+                 else np.asarray(opts.kweights, dtype=float))
+            self.kw = w / w.sum()
+            self.phases = [block_phases(self.nl, self.sp, geo.coords, geo.latvecs, k)
+                           for k in self.kpts]
+            self.Sk = [hermitian_from_lower(unpack_k(self.s, self.sp, self.off, ph))
+                       for ph in self.phases]
+
+    def _density(self, h, S):
+        """Eigenpairs, Fermi filling and the sparse density matrix of the Hamiltonian h (sparse):
+        Gamma point in real arithmetic, else one complex Hermitian problem per k-point and the
+        k-weighted density (fullDensityMatrix_cmplx + packHS_cmplx_kpts)."""
+        if self.kpts is None:
+            eig, C = self.solve(h, S)
+            f, ef, eband, ts, e0 = fermi_filling(eig, self.nel, self.opts.kT)
+            occ = levels(f)
+            P = (C[:, occ] * f[occ]) @ C[:, occ].T
+            return eig, C, f, ef, eband, ts, pack(P, self.params, self.geo.species, self.sp, self.off)
+        eigs, Cs = [], []
+        for ph, Sk in zip(self.phases, self.Sk):
+            Hk = hermitian_from_lower(unpack_k(h, self.sp, self.off, ph))
+            e, C = scipy.linalg.eigh(Hk, Sk, lower=True)
+            eigs.append(e)
+            Cs.append(C)
+        eig = np.array(eigs)
+        f, ef, eband, ts, e0 = fermi_filling_k(eig, self.nel, self.opts.kT, self.kw)
+        rho = np.zeros(self.sp.nnz)
+        for k in range(len(self.kpts)):
+            occ = levels(f[k])
+            P = (Cs[k][:, occ] * f[k][occ]) @ Cs[k][:, occ].conj().T
+            rho += pack_k(P, self.sp, self.phases[k], self.kw[k])
+        return eig, Cs, f, ef, eband, ts, rho
+
+    def _energy_density(self, C, eig, f):
+        """Sparse energy-weighted density matrix (sum_i f_i e_i c_i c_i^+, k-weighted)."""
+        if self.kpts is None:
+            occ = levels(f)
+            W = (C[:, occ] * (f[occ] * eig[occ])) @ C[:, occ].T
+            return pack(W, self.params, self.geo.species, self.sp, self.off)
+        erho = np.zeros(self.sp.nnz)
+        for k in range(len(self.kpts)):
+            occ = levels(f[k])
+            W = (C[k][:, occ] * (f[k][occ] * eig[k][occ])) @ C[k][:, occ].conj().T
+            erho += pack_k(W, self.sp, self.phases[k], self.kw[k])
+        return erho
 
     def dense(self, x):
         return symmetrize_lower(unpack(x, self.params, self.geo.species, self.sp, self.off))
@@ -931,18 +1085,13 @@ class Calculator:
         qin = self.q0.copy()
         mixer = BroydenMixer(nat, o.mix_param, o.omega0, o.min_weight, o.max_weight, o.weight_fac,
                              o.max_scc_iter + 1)
-        S = self.dense(self.s)
+        S = self.dense(self.s) if self.kpts is None else None
         converged = False
         for it in range(1, o.max_scc_iter + 1):
             dqin = qin - self.q0
             vorb, _ = self.orbital_potential(dqin)
             h = add_shift(self.h0, self.s, vorb, self.geo.species, self.sp, self.off)
-            eig, C = self.solve(h, S)
-            f, ef, eband, ts, e0 = fermi_filling(eig, self.nel, o.kT)
-            occ = levels(f)
-            Cf = C[:, occ] * f[occ]
-            P = Cf @ C[:, occ].T
-            rho = pack(P, self.params, self.geo.species, self.sp, self.off)
+            eig, C, f, ef, eband, ts, rho = self._density(h, S)
             qorb = mulliken(self.s, rho, self.geo.species, self.sp, self.off)
             qout = np.add.reduceat(qorb, self.off[:-1])
             dqout = qout - self.q0
@@ -965,10 +1114,7 @@ class Calculator:
     def gradients(self, res: Result, analytic: bool = False) -> np.ndarray:
         """Total energy gradient: 2 Sum(rho dH0 - W dS) + shift dS term (forces.F90:548-672),
         gamma derivative (shortgamma.F90:349-411, coulomb.F90:1140-1187/1380-1494), repulsive."""
-        C, eig, f = res._C, res._eig, res._f
-        occ = levels(f)
-        W = (C[:, occ] * (f[occ] * eig[occ])) @ C[:, occ].T
-        erho = pack(W, self.params, self.geo.species, self.sp, self.off)
+        erho = self._energy_density(res._C, res._eig, res._f)
         res.erho = erho
         sp = self.sp
         spc = self.geo.species

@@ -1,0 +1,53 @@
+"""k-point SCC (complex Hermitian problems through their real 2n embedding) against the CPU
+oracle's complex path: energies, charges and gradients at the FP64 tolerances."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MESHES = {
+    # Monkhorst-Pack 2x2x2 (k = +-1/4: complex phases)
+    "mp222": np.array([[a, b, c] for a in (-0.25, 0.25) for b in (-0.25, 0.25)
+                       for c in (-0.25, 0.25)]),
+    # Gamma-centred 3x3x3 (k = 0, 1/3, 2/3)
+    "gc333": np.array([[a, b, c] for a in (0, 1 / 3, 2 / 3) for b in (0, 1 / 3, 2 / 3)
+                       for c in (0, 1 / 3, 2 / 3)]),
+}
+
+
+@pytest.mark.parametrize("mesh", sorted(MESHES))
+def test_kpoint_scc_matches_oracle(mesh):
+    from dftbplus_amd import api, synth
+    from oracle import dftb_ref as R
+    geo, params, opts = synth.workload("diamond1")
+    o = dataclasses.replace(opts, kpoints=MESHES[mesh])
+    calc = R.Calculator(params, geo, o)
+    res = calc.scc()
+    og = calc.gradients(res)
+    dp = api.DftbPlus(params, geo, o)
+    e, q, g = dp.get_energy(), dp.get_gross_charges(), dp.get_gradients()
+    nat = geo.natom
+    print(f"{mesh}: it {dp.n_iter}/{res.n_iter} dE/atom {abs(e - res.energy_mermin) / nat:.2e} "
+          f"dq {np.max(np.abs(q - res.charges)):.2e} dF {np.max(np.abs(g - og)):.2e}")
+    assert dp.converged and res.converged
+    assert abs(e - res.energy_mermin) / nat < 1e-8
+    assert np.max(np.abs(q - res.charges)) < 1e-8
+    assert np.max(np.abs(g - og)) < 1e-7
+    for k, v in (("nonscc", res.e_nonscc), ("scc", res.e_scc), ("ts", res.ts),
+                 ("eband", res.eband)):
+        assert abs(dp.energies[k] - v) / nat < 1e-8, k
+    dp.close()
+
+
+def test_single_gamma_kpoint_equals_real_path():
+    from dftbplus_amd import api, synth
+    geo, params, opts = synth.workload("diamond1")
+    d0 = api.DftbPlus(params, geo, opts)
+    d1 = api.DftbPlus(params, geo, dataclasses.replace(opts, kpoints=np.zeros((1, 3))))
+    assert abs(d0.get_energy() - d1.get_energy()) < 1e-10
+    assert np.max(np.abs(d0.get_gross_charges() - d1.get_gross_charges())) < 1e-10
+    assert np.max(np.abs(d0.get_gradients() - d1.get_gradients())) < 1e-9
+    d0.close()
+    d1.close()

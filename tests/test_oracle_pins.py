@@ -252,3 +252,55 @@ def test_skf_round_trip(tmp_path):
         assert np.array_equal(back.ham, files[(i, j)].ham)
         assert np.array_equal(back.over, files[(i, j)].over)
         assert back.rep.value(2.1) == files[(i, j)].rep.value(2.1)
+
+
+def _cell_and_supercell(rep=3):
+    from dftbplus_amd import synth
+    from dftbplus_amd.model import Geometry
+    geo, params, opts = synth.workload("diamond1")
+    L = geo.latvecs
+    shifts = np.array([[i, j, k] for i in range(rep) for j in range(rep) for k in range(rep)])
+    coords = np.concatenate([geo.coords + s @ L for s in shifts])
+    species = np.tile(geo.species, len(shifts))
+    sup = Geometry(coords, species, L * rep)
+    return geo, sup, params, opts, shifts
+
+
+def test_kpoint_oracle_gamma_equals_real_path():
+    """One k-point at Gamma through the complex path reproduces the real Gamma path."""
+    import dataclasses
+    from dftbplus_amd import synth
+    from oracle import dftb_ref as R
+    geo, params, opts = synth.workload("diamond1")
+    r0 = R.Calculator(params, geo, opts).scc()
+    ok = dataclasses.replace(opts, kpoints=np.zeros((1, 3)))
+    r1 = R.Calculator(params, geo, ok).scc()
+    assert abs(r0.energy_mermin - r1.energy_mermin) < 1e-11
+    assert np.max(np.abs(r0.charges - r1.charges)) < 1e-11
+
+
+def test_kpoint_oracle_matches_supercell():
+    """Bloch theorem: a cell with the Gamma-centred 3x3x3 k-mesh (complex phases at k = 1/3)
+    equals its 3x3x3 supercell at Gamma: energy per cell, charges per atom and the force on each
+    atom (equal to that on every replica)."""
+    import dataclasses
+    from oracle import dftb_ref as R
+    geo, sup, params, opts, shifts = _cell_and_supercell(3)
+    # tight SCC convergence and Ewald sums (whose splitting depends on the cell) so that the
+    # comparison is not limited by either tolerance
+    opts = dataclasses.replace(opts, scc_tol=1e-11, max_scc_iter=200, ewald_tol=1e-12)
+    mesh = np.array([[a, b, c] for a in (0, 1 / 3, 2 / 3) for b in (0, 1 / 3, 2 / 3)
+                     for c in (0, 1 / 3, 2 / 3)])
+    ok = dataclasses.replace(opts, kpoints=mesh)
+    ck = R.Calculator(params, geo, ok)
+    rk = ck.scc()
+    gk = ck.gradients(rk)
+    cs = R.Calculator(params, sup, opts)
+    rs = cs.scc()
+    gs = cs.gradients(rs)
+    nrep = len(shifts)
+    nat = geo.natom
+    assert rk.converged and rs.converged
+    assert abs(rs.energy_mermin / nrep - rk.energy_mermin) / nat < 1e-12
+    assert np.max(np.abs(rs.charges.reshape(nrep, nat) - rk.charges[None, :])) < 1e-9
+    assert np.max(np.abs(gs.reshape(nrep, nat, 3) - gk[None, :, :])) < 1e-8
