@@ -61,7 +61,7 @@ constexpr int kCH = 8;       // column tiles per SYMV workgroup
 template <typename T>
 struct SytrdWs {
   T* W;            // n x kNB
-  T* Y;            // n x nSlot   partial sums of A22 v, one row of slots per matrix row
+  T* Y;            // nSlot x n   partial sums of A22 v, slot-major: Y(row, slot) at slot n + row
   T* vbuf;         // n
   double* nrm2;    // n   ||x||^2 per column
   double* dot;     // n   w.v per column (before the rank-one correction of w)
@@ -276,8 +276,8 @@ __global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, 
     if (cg == 0) {
       const int k = I * kTB + ii;
       if (k < m)
-        Y[(long)k * nSlot + J] = red[I & 1][0][ii] + red[I & 1][1][ii] + red[I & 1][2][ii] +
-                                 red[I & 1][3][ii];
+        Y[(long)J * n + k] = red[I & 1][0][ii] + red[I & 1][1][ii] + red[I & 1][2][ii] +
+                             red[I & 1][3][ii];
     }
   };
   // two tiles in flight ahead of the one being reduced
@@ -297,56 +297,51 @@ __global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, 
   for (int e = 0; e < 16; ++e) {
     const T z = wave_sum(zacc[e]);
     const int k = J * kTB + cg * 16 + e;
-    if (ii == 0 && k < m) Y[(long)k * nSlot + nRB + ch] = z;
+    if (ii == 0 && k < m) Y[(long)(nRB + ch) * n + k] = z;
   }
 }
 
-// y(r) from its slots (one wave per row), w = tau * (y - V t1 - W t2); store v into column c
-// and accumulate w.v. 16 rows per 256-thread block.
+// y(r) from its slots, w = tau * (y - V t1 - W t2); store v into column c and accumulate w.v.
+// Slots are stored slot-major (Y(row, slot) at slot * n + row): a block handles 64 rows (the
+// lanes) and its 4 waves split the slots and the panel correction between them.
 template <typename T>
 __global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, int c, int j0,
                                                          T* W, long ldw, const T* Y, int nSlot,
                                                          const T* vbuf, const double* t12,
                                                          const T* tauA, double* dot) {
-  __shared__ double sh[16];
+  __shared__ double red[4][64];
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
   const int i = c - j0;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const double tau = tauA[c];
-  for (int q = 0; q < 4; ++q) {
-    const int k = blockIdx.x * 16 + wv * 4 + q;
+  const int k = blockIdx.x * 64 + lane;
+  const int r = c + 1 + k;
+  double acc = 0.0;
+  if (k < m) {
+    const int Ir = k / kTB;
+    const int nch = (nRB - Ir + kCH - 1) / kCH;
+    for (int sl = wv; sl <= Ir; sl += 4) acc += Y[(long)sl * n + k];
+    for (int sl = wv; sl < nch; sl += 4) acc += Y[(long)(nRB + sl) * n + k];
+    for (int p = wv; p < i; p += 4) {
+      const double vr = a[r + (long)(j0 + p) * lda];
+      const double wr = (double)W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dot) * vr;
+      acc -= vr * t12[(long)c * 2 * kNB + p] + wr * t12[(long)c * 2 * kNB + i + p];
+    }
+  }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0) {
     double prod = 0.0;
     if (k < m) {
-      const int r = c + 1 + k;
-      const int Ir = k / kTB;
-      const int nch = (nRB - Ir + kCH - 1) / kCH;
-      const T* yr = Y + (long)k * nSlot;
-      double acc = 0.0;
-      for (int sl = lane; sl <= Ir; sl += 64) acc += yr[sl];
-      for (int sl = lane; sl < nch; sl += 64) acc += yr[nRB + sl];
-      if (lane < i) {
-        const int p = lane;
-        const double vr = a[r + (long)(j0 + p) * lda];
-        const double wr = (double)W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dot) * vr;
-        acc -= vr * t12[(long)c * 2 * kNB + p] + wr * t12[(long)c * 2 * kNB + i + p];
-      }
-      acc = wave_sum(acc);
-      if (lane == 0) {
-        const T w = (T)(tau * acc);
-        W[r + (long)i * ldw] = w;
-        const T v = vbuf[r];
-        a[r + (long)c * lda] = v;
-        prod = (double)w * (double)v;
-      }
+      const double y = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+      const T w = (T)((double)tauA[c] * y);
+      W[r + (long)i * ldw] = w;
+      const T v = vbuf[r];
+      a[r + (long)c * lda] = v;
+      prod = (double)w * (double)v;
     }
-    if (lane == 0) sh[wv * 4 + q] = prod;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int q = 0; q < 16; ++q) t += sh[q];
-    if (t != 0.0) atomicAdd(&dot[c], t);
+    prod = wave_sum(prod);
+    if (lane == 0 && prod != 0.0) atomicAdd(&dot[c], prod);
   }
 }
 
@@ -419,7 +414,7 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
         pf.bytes[pf.used] = sizeof(T) * ((double)m * (m + 1) / 2 + 2.0 * m);
         ++pf.used;
       }
-      hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 16)), dim3(256), 0, s, a, lda, n, c, j0,
+      hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 64)), dim3(256), 0, s, a, lda, n, c, j0,
                          ws.W, ldw, ws.Y, ws.nSlot, ws.vbuf, ws.t12, tau, ws.dot);
       DH_LAUNCH_CHECK();
     }
