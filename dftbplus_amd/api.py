@@ -120,6 +120,8 @@ EXPORTS = {
     "dftbhip_ctx_nr_orbitals": [C.c_void_p],
     "dftbhip_ctx_timings": [C.c_void_p, _dp, C.c_int],
     "dftbhip_ctx_set_kpoints": [C.c_void_p, C.c_int, _dp, _dp],
+    "dftbhip_dsytrd": [C.c_char, C.c_int, _dp, C.c_int, _dp, _dp, _dp],
+    "dftbhip_set_tuning": [C.c_int, C.c_int],
     "dftbhip_batch_create": [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p],
     "dftbhip_batch_destroy": [C.c_void_p],
     "dftbhip_batch_set_geometries": [C.c_void_p, C.c_int, _ip, _dp, _ip],
@@ -236,6 +238,22 @@ def sygvd(h: np.ndarray, s: np.ndarray, jobz: str = "V"):
         raise DftbHipError(f"sygvd info={rc}")
     _check(rc, "dftbhip_sygvd")
     return w, a, b
+
+
+def dsytrd(a: np.ndarray):
+    """LAPACK dsytrd (lower) on the GPU: returns (reflectors in the lower part of a copy of a, d,
+    e, tau)."""
+    a = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    n = a.shape[0]
+    d = np.zeros(n)
+    e = np.zeros(max(n - 1, 1))
+    tau = np.zeros(max(n - 1, 1))
+    _check(lib().dftbhip_dsytrd(b"L", n, _d(a), n, _d(d), _d(e), _d(tau)), "dftbhip_dsytrd")
+    return a, d, e[: max(n - 1, 0)], tau[: max(n - 1, 0)]
+
+
+def set_tuning(param: int, value: int) -> None:
+    _check(lib().dftbhip_set_tuning(param, value), "dftbhip_set_tuning")
 
 
 def dgemm(a: np.ndarray, b: np.ndarray, transa: str = "N", transb: str = "N", alpha: float = 1.0,

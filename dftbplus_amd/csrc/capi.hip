@@ -2205,6 +2205,45 @@ extern "C" int dftbhip_dgemm(char transa, char transb, int m, int n, int k, doub
   return 0;
 }
 
+extern "C" int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e,
+                              double* tau) {
+  DH_CHECK(uplo == 'L' || uplo == 'l', "dsytrd: only uplo = 'L'");
+  DH_CHECK(n >= 0 && a && d && (n < 2 || (e && tau)) && lda >= std::max(n, 1),
+           "dsytrd: bad arguments");
+  if (n == 0) return 0;
+  DH_TRY(dftbhip::ensure_runtime());
+  hipStream_t s = dftbhip::g_rt.stream;
+  dftbhip::DevBuf<double> A, D, E, TAU, W;
+  DH_TRY(A.alloc((size_t)n * n));
+  DH_TRY(D.alloc(n));
+  DH_TRY(E.alloc(std::max(n, 2)));
+  DH_TRY(TAU.alloc(std::max(n, 1)));
+  const size_t ws = dftbhip::sytrd_work_size(n);
+  DH_TRY(W.alloc(ws));
+  DH_HIP(hipMemcpy2DAsync(A.p, n * sizeof(double), a, (size_t)lda * sizeof(double),
+                          n * sizeof(double), n, hipMemcpyHostToDevice, s));
+  DH_TRY(dftbhip::sytrd_lower(s, A.p, n, n, D.p, E.p, TAU.p, W.p, ws));
+  DH_HIP(hipMemcpy2DAsync(a, (size_t)lda * sizeof(double), A.p, n * sizeof(double),
+                          n * sizeof(double), n, hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(d, D.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (n > 1) {
+    DH_HIP(hipMemcpyAsync(e, E.p, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipMemcpyAsync(tau, TAU.p, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_set_tuning(int param, int value) {
+  switch (param) {
+    case 0:
+      return dftbhip::set_symv_tiles(value);
+    default:
+      dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));
+      return dftbhip::kErrArg;
+  }
+}
+
 extern "C" int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz) {
   DH_CHECK(n >= 0 && d && (n <= 1 || e) && z && ldz >= std::max(1, n), "dstedc: bad arguments");
   if (n == 0) return 0;

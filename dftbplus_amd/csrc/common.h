@@ -175,6 +175,7 @@ int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e
 int ormtr_lower_f32(hipStream_t s, const float* a, long lda, int n, const float* tau, float* z,
                     long ldz, int ncols, void* work, size_t workBytes);
 size_t sytrd_work_bytes(int n, int elemSize);
+int set_symv_tiles(int tiles);   // row tiles per SYMV workgroup: 4, 8 or 16
 // two-stage tridiagonalisation (band.hip): full symmetric input, workspace in bytes
 size_t band_work_bytes(int n, int elemSize);
 int sytrd_2stage(hipStream_t s, double* a, long lda, int n, double* d, double* e, void* work,

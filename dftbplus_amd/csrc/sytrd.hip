@@ -56,7 +56,15 @@ static int symv_prof_collect(hipStream_t s) {
   return 0;
 }
 constexpr int kTB = 64;      // SYMV tile
-constexpr int kCH = 8;       // column tiles per SYMV workgroup
+constexpr int kCHMin = 4;    // fewest row tiles per SYMV workgroup (sizes the slot array)
+// row tiles per SYMV workgroup (4, 8 or 16), set through set_symv_tiles
+static int g_symvTiles = 8;
+
+int set_symv_tiles(int tiles) {
+  DH_CHECK(tiles == 4 || tiles == 8 || tiles == 16, "symv tiles must be 4, 8 or 16");
+  g_symvTiles = tiles;
+  return 0;
+}
 
 template <typename T>
 struct SytrdWs {
@@ -74,7 +82,7 @@ struct SytrdWs {
 
 static int sytrd_slots(int n) {
   const int nRB = (n + kTB - 1) / kTB;
-  return nRB + (nRB + kCH - 1) / kCH;
+  return nRB + (nRB + kCHMin - 1) / kCHMin;
 }
 
 // byte layout of the workspace (each array 256-byte aligned)
@@ -181,13 +189,13 @@ __global__ __launch_bounds__(256) void col_update_kernel(T* a, long lda, int n, 
 }
 
 // Partial sums of y = A22 v (A22 = a(c+1:n, c+1:n), lower triangle read once).
-// Workgroup (J, chunk) owns column block J and row blocks J + chunk*kCH ... (kCH of them):
+// Workgroup (J, chunk) owns column block J and row blocks J + chunk*CH ... (CH of them):
 //   y partial of row block I  -> Y(row, J)            (one per tile, reduced over 4 waves)
 //   transposed sums (A^T v)_J -> Y(col, nRB + chunk)  (accumulated in registers over the tiles)
 // Thread (ii = tid % 64, cg = tid / 64) holds rows ii of 16 columns cg*16 .. cg*16+15.
 // The chunk-0 workgroup of each column block publishes v and the panel products t1, t2.
-template <typename T>
-__global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, int c, int j0,
+template <typename T, int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void symv_kernel(const T* a, long lda, int n, int c, int j0,
                                                    const T* W, long ldw, const double* nrm2,
                                                    const T* tauA, const double* dotA, T* Y,
                                                    int nSlot, T* vbuf, double* t12,
@@ -197,9 +205,9 @@ __global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, 
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
   const int J = blockIdx.x, ch = blockIdx.y;
-  const int Ib = J + ch * kCH;
+  const int Ib = J + ch * CH;
   if (J >= nRB || Ib >= nRB) return;
-  const int Ie = min(Ib + kCH, nRB);
+  const int Ie = min(Ib + CH, nRB);
   const int tid = threadIdx.x;
   const int ii = tid & 63, cg = tid >> 6;
   const double alpha = a[(c + 1) + (long)c * lda];
@@ -237,13 +245,13 @@ __global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, 
       if (ii == 0 && prod != 0.0) atomicAdd(&t12[(long)c * 2 * kNB + p], prod);
     }
   }
-  T vj[16];
-#pragma unroll
-  for (int e = 0; e < 16; ++e) vj[e] = vJs[cg * 16 + e];
+  const T* vj = vJs + cg * 16;   // v of this thread's 16 columns (LDS broadcast reads)
   T b0[16], b1[16], b2[16], zacc[16];
+  T v0 = T(0), v1 = T(0), v2 = T(0);
 #pragma unroll
   for (int e = 0; e < 16; ++e) zacc[e] = T(0);
-  auto load = [&](int I, T* dst) {
+  // tile I rows ii of this thread's 16 columns, and v of row ii (prefetched with the tile)
+  auto load = [&](int I, T* dst, T& vdst) {
     const int gr = I * kTB + ii;
     const bool rowOk = gr < m;
     const T* src = a + (c + 1 + gr) + (long)cols0 * lda;
@@ -253,10 +261,21 @@ __global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, 
       const int gc = J * kTB + jj;
       dst[e] = (rowOk && gc < m && (I > J || ii >= jj)) ? src[(long)jj * lda] : T(0);
     }
+    // raw column-c element of the row; scaled at use so that the prefetch is not waited on
+    vdst = (rowOk && gr > 0) ? a[(c + 1 + gr) + (long)c * lda] : T(0);
+  };
+  // barrier for the LDS row sums only: __syncthreads would also wait for the prefetched global
+  // loads of the next tiles (its workgroup fence waits on every outstanding memory operation)
+  auto lds_barrier = [&]() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt untouched (gfx9 encoding)
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
   };
   // one tile: row partials (reduced over the 4 waves through LDS) and transposed sums
-  auto tile = [&](int I, const T* t) {
-    const T vi = vval(I * kTB + ii);
+  auto tile = [&](int I, const T* t, T vraw) {
+    const int gr = I * kTB + ii;
+    const T vi = (gr >= m) ? T(0) : (gr == 0 ? T(1) : (T)((double)vraw * scal));
     T yp = T(0);
     if (I > J) {
 #pragma unroll
@@ -272,7 +291,7 @@ __global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, 
       }
     }
     red[I & 1][cg][ii] = yp;
-    __syncthreads();
+    lds_barrier();
     if (cg == 0) {
       const int k = I * kTB + ii;
       if (k < m)
@@ -281,17 +300,17 @@ __global__ __launch_bounds__(256) void symv_kernel(const T* a, long lda, int n, 
     }
   };
   // two tiles in flight ahead of the one being reduced
-  load(Ib, b0);
-  if (Ib + 1 < Ie) load(Ib + 1, b1);
+  load(Ib, b0, v0);
+  if (Ib + 1 < Ie) load(Ib + 1, b1, v1);
   for (int I = Ib; I < Ie; I += 3) {
-    if (I + 2 < Ie) load(I + 2, b2);
-    tile(I, b0);
+    if (I + 2 < Ie) load(I + 2, b2, v2);
+    tile(I, b0, v0);
     if (I + 1 >= Ie) break;
-    if (I + 3 < Ie) load(I + 3, b0);
-    tile(I + 1, b1);
+    if (I + 3 < Ie) load(I + 3, b0, v0);
+    tile(I + 1, b1, v1);
     if (I + 2 >= Ie) break;
-    if (I + 4 < Ie) load(I + 4, b1);
-    tile(I + 2, b2);
+    if (I + 4 < Ie) load(I + 4, b1, v1);
+    tile(I + 2, b2, v2);
   }
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
@@ -308,7 +327,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, int c, int j0,
                                                          T* W, long ldw, const T* Y, int nSlot,
                                                          const T* vbuf, const double* t12,
-                                                         const T* tauA, double* dot) {
+                                                         const T* tauA, double* dot, int tiles) {
   __shared__ double red[4][64];
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
@@ -319,7 +338,7 @@ __global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, 
   double acc = 0.0;
   if (k < m) {
     const int Ir = k / kTB;
-    const int nch = (nRB - Ir + kCH - 1) / kCH;
+    const int nch = (nRB - Ir + tiles - 1) / tiles;
     for (int sl = wv; sl <= Ir; sl += 4) acc += Y[(long)sl * n + k];
     for (int sl = wv; sl < nch; sl += 4) acc += Y[(long)(nRB + sl) * n + k];
     for (int p = wv; p < i; p += 4) {
@@ -393,7 +412,8 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
       const int m = n - c - 1;
       if (m <= 0) continue;
       const int nRB = (m + kTB - 1) / kTB;
-      const int nCh = (nRB + kCH - 1) / kCH;
+      const int tiles = g_symvTiles;
+      const int nCh = (nRB + tiles - 1) / tiles;
       SymvProfile& pf = g_symvProf;
       if (pf.on) {
         if (pf.ev.size() < 2 * (pf.used + 1)) {
@@ -404,9 +424,14 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
         }
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used], s));
       }
-      hipLaunchKernelGGL(symv_kernel<T>, dim3(nRB, nCh), dim3(256), 0, s, a, lda, n, c, j0, ws.W,
-                         ldw, ws.nrm2, tau, ws.dot, ws.Y, ws.nSlot, ws.vbuf, ws.t12, ws.scal, e,
-                         tau);
+#define DH_SYMV(CH_)                                                                       \
+  hipLaunchKernelGGL((symv_kernel<T, CH_>), dim3(nRB, nCh), dim3(256), 0, s, a, lda, n, c, j0,  \
+                     ws.W, ldw, ws.nrm2, tau, ws.dot, ws.Y, ws.nSlot, ws.vbuf, ws.t12, ws.scal, \
+                     e, tau)
+      if (tiles == 4) DH_SYMV(4);
+      else if (tiles == 16) DH_SYMV(16);
+      else DH_SYMV(8);
+#undef DH_SYMV
       DH_LAUNCH_CHECK();
       if (pf.on) {
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used + 1], s));
@@ -415,7 +440,7 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
         ++pf.used;
       }
       hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 64)), dim3(256), 0, s, a, lda, n, c, j0,
-                         ws.W, ldw, ws.Y, ws.nSlot, ws.vbuf, ws.t12, tau, ws.dot);
+                         ws.W, ldw, ws.Y, ws.nSlot, ws.vbuf, ws.t12, tau, ws.dot, tiles);
       DH_LAUNCH_CHECK();
     }
     const int r0 = j0 + pw;

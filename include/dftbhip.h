@@ -35,6 +35,14 @@ int dftbhip_sygvd(int itype, char jobz, char uplo, int n, double* a, int lda, do
 int dftbhip_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a,
                   int lda, const double* b, int ldb, double beta, double* c, int ldc);
 int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz);
+/* LAPACK dsytrd (uplo = 'L'): Householder tridiagonalisation of the symmetric a (lower
+ * triangle read); on exit d, e, tau and the reflectors below the subdiagonal of a, as LAPACK
+ * (the reduction DFTB+ reaches through the dsygvd / dsyevd routes of
+ * src/dftbp/math/eigensolver.F90:561-660). */
+int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e, double* tau);
+/* Kernel tuning. param 0: row tiles per workgroup of the tridiagonalisation's SYMV (4, 8, 16;
+ * default 8). No counterpart in the reference. */
+int dftbhip_set_tuning(int param, int value);
 
 /* ---- density matrices: replace makeDensityMtxRealGPU / makeEnergyDensityMtxRealGPU
  *      (src/dftbp/dftb/densitymatrix.F90:1105-1395). Lower triangle of dm/edm is set:

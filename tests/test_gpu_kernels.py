@@ -141,3 +141,38 @@ def test_sparse_kernels(system, request):
     q = api.mulliken(ra, calc.s, res.rho)
     qref = R.mulliken(calc.s, res.rho, geo.species, calc.sp, calc.off)
     assert np.max(np.abs(q - qref)) < 1e-13
+
+
+@pytest.mark.parametrize("tiles", [4, 8, 16])
+@pytest.mark.parametrize("n", [1, 2, 77, 700])
+def test_dsytrd_lapack_conventions(n, tiles):
+    """Tridiagonalisation against LAPACK's own dsytrd ('L'): d, e, tau and the reflectors, and
+    A = Q T Q^T with Q = H(0) ... H(n-2), H(i) = I - tau_i v v^T, v(i+1) = 1, v(i+2:) stored
+    below the subdiagonal of column i."""
+    from scipy.linalg import lapack
+    api = _api()
+    rng = np.random.default_rng(n + tiles)
+    a = rng.standard_normal((n, n))
+    a = a + a.T
+    api.set_tuning(0, tiles)
+    try:
+        r, d, e, tau = api.dsytrd(a)
+    finally:
+        api.set_tuning(0, 8)
+    scale = max(1.0, np.abs(a).max())
+    tol = 1e-12 * max(n, 1) * scale
+    ref, dr, er, taur, info = lapack.dsytrd(a, lower=1)
+    assert info == 0
+    assert np.max(np.abs(d - dr)) < tol
+    if n > 1:
+        assert np.max(np.abs(e - er)) < tol
+        assert np.max(np.abs(tau - taur[: n - 1])) < tol
+        assert np.max(np.abs(np.tril(r, -2) - np.tril(ref, -2))) < tol
+        q = np.eye(n)
+        for i in range(n - 2, -1, -1):
+            v = np.zeros(n)
+            v[i + 1] = 1.0
+            v[i + 2:] = r[i + 2:, i]
+            q -= tau[i] * np.outer(v, v @ q)
+        t = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        assert np.max(np.abs(q @ t @ q.T - a)) < tol
