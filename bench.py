@@ -110,6 +110,117 @@ def run_leg(api, geo, params, opts, leg, steps, warmup):
     }
 
 
+def mode_sytrd(args):
+    """Copy kernel (counter calibration: 8 n^2 bytes read and written) and the one-stage
+    tridiagonalisation of a random symmetric n x n matrix."""
+    import numpy as np
+    from dftbplus_amd import api
+    L = api.lib()
+    n = args.n
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((n, n))
+    a = a + a.T
+    api.dlacpy(a)
+    L.dftbhip_kernel_profile(1)
+    t0 = time.perf_counter()
+    api.dsytrd(a)
+    dt = time.perf_counter() - t0
+    st = np.zeros(3)
+    L.dftbhip_kernel_stats(api._d(st))
+    L.dftbhip_kernel_profile(0)
+    launches, ksec, kbytes = st
+    print(json.dumps({"mode": "sytrd", "n": n, "seconds_incl_transfers": dt,
+                      "symv_launches": int(launches), "symv_avg_us": 1e6 * ksec / launches,
+                      "symv_alg_bytes_per_launch": kbytes / launches,
+                      "symv_achieved_GBs": kbytes / ksec / 1e9,
+                      "copy_kernel_bytes_each_way": 8 * n * n}), flush=True)
+
+
+def mode_batch(args):
+    """The molecule batch (SCC + forces of nmol molecules in one set of launches) against a loop
+    of single-molecule runs of the plain path over every 128th molecule, scaled to nmol."""
+    import numpy as np
+    from dftbplus_amd import api, synth
+    mols, params, opts = synth.workload(f"molecules{args.nmol}")
+    t0 = time.perf_counter()
+    b = api.MoleculeBatch(params, mols, opts)
+    t1 = time.perf_counter()
+    print(f"[bench:batch] {len(mols)} molecules set up in {t1 - t0:.2f}s", file=sys.stderr,
+          flush=True)
+    b.scc_begin()
+    it = 0
+    while True:
+        ts = time.perf_counter()
+        nactive = b.scc_step()
+        it += 1
+        if it % 10 == 1 or nactive == 0:
+            print(f"[bench:batch] iteration {it}: {time.perf_counter() - ts:.3f}s, "
+                  f"{nactive} molecules not converged", file=sys.stderr, flush=True)
+        if nactive == 0 or it >= opts.max_scc_iter:
+            break
+    e = b.scc_end()
+    t2 = time.perf_counter()
+    g = b.gradients()
+    t3 = time.perf_counter()
+    print(f"[bench:batch] {len(mols)} molecules: setup {t1 - t0:.2f}s scc {t2 - t1:.2f}s "
+          f"gradients {t3 - t2:.2f}s", file=sys.stderr, flush=True)
+    sample = list(range(0, len(mols), 128))
+    dev = 0.0
+    t4 = time.perf_counter()
+    for m in sample:
+        dp = api.DftbPlus(params, mols[m], opts)
+        em = dp.get_energy()
+        gm = dp.get_gradients()
+        dev = max(dev, abs(em - e[m]) / mols[m].natom, float(np.max(np.abs(gm - g[m]))))
+        dp.close()
+    loop_s = (time.perf_counter() - t4) * len(mols) / len(sample)
+    batch_s = t3 - t0
+    print(json.dumps({"mode": "batch", "molecules": len(mols),
+                      "orbitals": int(sum(int(np.sum([params.species[x].norb for x in mm.species]))
+                                          for mm in mols)),
+                      "batch_setup_s": t1 - t0, "batch_scc_s": t2 - t1,
+                      "batch_gradients_s": t3 - t2, "batch_total_s": batch_s,
+                      "converged": int(np.sum(b.converged)),
+                      "mean_scc_iterations": float(np.mean(b.n_iter)),
+                      "loop_sample": len(sample), "loop_estimate_s": loop_s,
+                      "speedup_vs_loop": loop_s / batch_s,
+                      "sample_max_deviation": dev}), flush=True)
+    b.close()
+
+
+def mode_kpoints(args):
+    """SCC iterations and one force evaluation of a periodic workload with the 2x2x2
+    Monkhorst-Pack mesh (8 complex k-points)."""
+    import dataclasses
+    import numpy as np
+    from dftbplus_amd import api, synth
+    geo, params, opts = synth.workload(args.workload)
+    mesh = np.array([[a, b, c] for a in (-0.25, 0.25) for b in (-0.25, 0.25)
+                     for c in (-0.25, 0.25)])
+    opts = dataclasses.replace(opts, kpoints=mesh)
+    t0 = time.perf_counter()
+    dp = api.DftbPlus(params, geo, opts)
+    setup = time.perf_counter() - t0
+    n = dp.get_nr_orbitals()
+    dp.scc_begin()
+    times = []
+    for i in range(args.steps):
+        t1 = time.perf_counter()
+        conv, err = dp.scc_step()
+        times.append(time.perf_counter() - t1)
+        print(f"[bench:kpoints] step {i + 1}: {times[-1]:.2f}s charge error {err:.3e}",
+              file=sys.stderr, flush=True)
+    dp.scc_end()
+    t2 = time.perf_counter()
+    dp.get_gradients()
+    force_s = time.perf_counter() - t2
+    print(json.dumps({"mode": "kpoints", "workload": args.workload, "atoms": geo.natom,
+                      "orbitals": n, "kpoints": len(mesh), "setup_s": setup,
+                      "scc_iteration_s": times, "force_eval_s": force_s,
+                      "phase_s": dp.timings()}), flush=True)
+    dp.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +230,13 @@ def main():
     ap.add_argument("--legs", default="fp64,mixed_fp32,mixed_refined",
                     help="comma-separated subset of " + ",".join(LEGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", default="scc", choices=["scc", "sytrd", "batch", "kpoints"],
+                    help="scc: the headline SCC benchmark (default); sytrd: the tridiagonalisation "
+                         "alone on a random matrix (SYMV counter runs); batch: the molecule batch "
+                         "against a sampled loop of single-molecule runs; kpoints: SCC iterations "
+                         "and a force evaluation with a k-point mesh")
+    ap.add_argument("--n", type=int, default=8192, help="matrix size of --mode sytrd")
+    ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCC path of a single Gamma-point system runs on one "
@@ -127,6 +245,9 @@ def main():
     import numpy as np
     from dftbplus_amd import api, synth
     # torch is not loaded here: its bundled HIP runtime would be a second runtime in the process
+    if args.mode != "scc":
+        {"sytrd": mode_sytrd, "batch": mode_batch, "kpoints": mode_kpoints}[args.mode](args)
+        return
     geo, params, opts = synth.workload(args.workload)
     legs = [x for x in args.legs.split(",") if x]
     if "fp64" not in legs:

@@ -122,11 +122,15 @@ EXPORTS = {
     "dftbhip_ctx_set_kpoints": [C.c_void_p, C.c_int, _dp, _dp],
     "dftbhip_dsytrd": [C.c_char, C.c_int, _dp, C.c_int, _dp, _dp, _dp],
     "dftbhip_set_tuning": [C.c_int, C.c_int],
+    "dftbhip_dlacpy": [C.c_char, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int],
     "dftbhip_batch_create": [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p],
     "dftbhip_batch_destroy": [C.c_void_p],
     "dftbhip_batch_set_geometries": [C.c_void_p, C.c_int, _ip, _dp, _ip],
     "dftbhip_batch_scc": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_batch_gradients": [C.c_void_p, _dp],
+    "dftbhip_batch_scc_begin": [C.c_void_p],
+    "dftbhip_batch_scc_step": [C.c_void_p, _ip],
+    "dftbhip_batch_scc_end": [C.c_void_p, _dp, _dp, _ip, _ip],
 }
 
 
@@ -250,6 +254,15 @@ def dsytrd(a: np.ndarray):
     tau = np.zeros(max(n - 1, 1))
     _check(lib().dftbhip_dsytrd(b"L", n, _d(a), n, _d(d), _d(e), _d(tau)), "dftbhip_dsytrd")
     return a, d, e[: max(n - 1, 0)], tau[: max(n - 1, 0)]
+
+
+def dlacpy(a: np.ndarray) -> np.ndarray:
+    """b = a through the device copy kernel."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    b = np.zeros_like(a, order="F")
+    m, n = a.shape
+    _check(lib().dftbhip_dlacpy(b"A", m, n, _d(a), m, _d(b), m), "dftbhip_dlacpy")
+    return b
 
 
 def set_tuning(param: int, value: int) -> None:
@@ -530,13 +543,27 @@ class MoleculeBatch:
                                                   _i(species)), "dftbhip_batch_set_geometries")
         self._done = False
 
+    def scc_begin(self):
+        _check(lib().dftbhip_batch_scc_begin(self.h), "dftbhip_batch_scc_begin")
+
+    def scc_step(self) -> int:
+        """One SCC iteration of every molecule not yet converged; returns how many remain."""
+        na = C.c_int()
+        _check(lib().dftbhip_batch_scc_step(self.h, C.byref(na)), "dftbhip_batch_scc_step")
+        return na.value
+
+    def scc_end(self):
+        return self._finish("dftbhip_batch_scc_end")
+
     def scc(self):
+        return self._finish("dftbhip_batch_scc")
+
+    def _finish(self, fn):
         e = np.zeros((self.nmol, 8))
         q = np.zeros(int(self.offsets[-1]))
         nit = np.zeros(self.nmol, dtype=np.int32)
         conv = np.zeros(self.nmol, dtype=np.int32)
-        _check(lib().dftbhip_batch_scc(self.h, _d(e), _d(q), _i(nit), _i(conv)),
-               "dftbhip_batch_scc")
+        _check(getattr(lib(), fn)(self.h, _d(e), _d(q), _i(nit), _i(conv)), fn)
         self.energies = {k: e[:, i].copy() for i, k in enumerate(ENERGY_KEYS)}
         self.charges = [q[self.offsets[m]:self.offsets[m + 1]] for m in range(self.nmol)]
         self.n_iter = nit

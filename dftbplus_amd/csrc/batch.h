@@ -22,11 +22,14 @@ int launch_batch_unpack(hipStream_t s, const PairsDev& pr, const int* norb,
 int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                       const BatchLayout& L, const double* dense, double* x);
 // eigenvalues w (ascending per molecule, at orb0), eigenvectors C (packed), info per molecule
-// (0, or j+1 when the leading minor j+1 of S is not positive definite); work 3 x packed size
+// (0, or j+1 when the leading minor j+1 of S is not positive definite); work 3 x packed size.
+// Molecules with active[m] == 0 are skipped (active may be null: all). warm: start the Jacobi
+// sweeps from the eigenvectors of the previous call on the same work array.
 int launch_batch_sygv(hipStream_t s, const BatchLayout& L, const double* S, const double* H,
-                      double* work, double* w, double* C, int* info);
+                      double* work, double* w, double* C, int* info, const int* active,
+                      bool warm);
 int launch_batch_density(hipStream_t s, const BatchLayout& L, const double* C, const double* f,
-                         const int* nocc, double* P);
+                         const int* nocc, double* P, const int* active);
 int launch_batch_gamma(hipStream_t s, const BatchLayout& L, const double* coords,
                        const int* species, const double* hubbu, const double* gcut, int nsp,
                        double* g);

@@ -62,15 +62,35 @@ __global__ void batch_pack_kernel(PairsDev pr, const int* norbAtom, const long* 
 // Work: 3 n^2 doubles per molecule at 3 dOff[m] (L, A, V).
 // ------------------------------------------------------------------------------------------
 constexpr int kJacobiMaxSweeps = 30;
+constexpr int kSygvThreads = 1024;
 
 __device__ __forceinline__ void jacobi_pair(int N2, int r, int k, int& p, int& q) {
   p = (k == 0) ? 0 : 1 + (r + k - 1) % (N2 - 1);
   q = 1 + (r + N2 - 2 - k) % (N2 - 1);
 }
 
-__global__ __launch_bounds__(256) void batch_sygv_kernel(BatchLayout L, const double* S,
-                                                         const double* H, double* work,
-                                                         double* w, double* C, int* info) {
+// sum_{k < len} x[k * sx] * y[k * sy] with four independent partial sums
+__device__ __forceinline__ double dot4(const double* __restrict__ x, long sx,
+                                       const double* __restrict__ y, long sy, int len) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    s0 += x[k * sx] * y[k * sy];
+    s1 += x[(k + 1) * sx] * y[(k + 1) * sy];
+    s2 += x[(k + 2) * sx] * y[(k + 2) * sy];
+    s3 += x[(k + 3) * sx] * y[(k + 3) * sy];
+  }
+  for (; k < len; ++k) s0 += x[k * sx] * y[k * sy];
+  return (s0 + s1) + (s2 + s3);
+}
+
+__global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L, const double* S,
+                                                                  const double* H, double* work,
+                                                                  double* w, double* C,
+                                                                  int* info, const int* active,
+                                                                  int warm) {
+  constexpr int NT = kSygvThreads;
+  if (active && !active[blockIdx.x]) return;   // converged: eigenpairs stay from the last pass
   __shared__ double rc[kBatchMaxOrb / 2], rs[kBatchMaxOrb / 2];
   __shared__ int rp[kBatchMaxOrb / 2], rq[kBatchMaxOrb / 2];
   __shared__ int perm[kBatchMaxOrb];
@@ -81,14 +101,14 @@ __global__ __launch_bounds__(256) void batch_sygv_kernel(BatchLayout L, const do
   const int n = (int)(L.orb0[m + 1] - L.orb0[m]);
   const double* Sm = S + base;
   const double* Hm = H + base;
-  double* Lm = work + 3 * base;
+  double* Lm = work + 3 * base;    // L in the lower triangle, L^T mirrored into the upper one
   double* Am = Lm + (long)n * n;
   double* Vm = Am + (long)n * n;
   double* Cm = C + base;
   double* wm = w + L.orb0[m];
   const long nn = (long)n * n;
   if (tid == 0) flag = 0;
-  for (long e = tid; e < nn; e += 256) Lm[e] = Sm[e];
+  for (long e = tid; e < nn; e += NT) Lm[e] = Sm[e];
   __syncthreads();
   // Cholesky S = L L^T (lower, right-looking)
   for (int j = 0; j < n; ++j) {
@@ -106,31 +126,50 @@ __global__ __launch_bounds__(256) void batch_sygv_kernel(BatchLayout L, const do
       return;
     }
     const double djj = Lm[j + (long)j * n];
-    for (int i = j + 1 + tid; i < n; i += 256) Lm[i + (long)j * n] /= djj;
+    for (int i = j + 1 + tid; i < n; i += NT) Lm[i + (long)j * n] /= djj;
     __syncthreads();
     const int len = n - j - 1;
-    for (long e = tid; e < (long)len * len; e += 256) {
+    for (long e = tid; e < (long)len * len; e += NT) {
       const int i = j + 1 + (int)(e % len), k = j + 1 + (int)(e / len);
       if (i >= k) Lm[i + (long)k * n] -= Lm[i + (long)j * n] * Lm[k + (long)j * n];
     }
     __syncthreads();
   }
-  // Y = L^-1 H (into V), A = L^-1 Y^T
-  for (int c = tid; c < n; c += 256)
+  // L^T into the upper triangle: row i of L is then contiguous (column i of the upper part)
+  for (long e = tid; e < nn; e += NT) {
+    const int i = (int)(e % n), k = (int)(e / n);
+    if (i > k) Lm[k + (long)i * n] = Lm[i + (long)k * n];
+  }
+  __syncthreads();
+  // Y = L^-1 H (into C), one column per thread, row i from L^T's column i
+  for (int c = tid; c < n; c += NT)
     for (int i = 0; i < n; ++i) {
-      double acc = Hm[i + (long)c * n];
-      for (int k = 0; k < i; ++k) acc -= Lm[i + (long)k * n] * Vm[k + (long)c * n];
-      Vm[i + (long)c * n] = acc / Lm[i + (long)i * n];
+      const double acc = Hm[i + (long)c * n] - dot4(Lm + (long)i * n, 1, Cm + (long)c * n, 1, i);
+      Cm[i + (long)c * n] = acc / Lm[i + (long)i * n];
     }
   __syncthreads();
-  for (int c = tid; c < n; c += 256)
+  // A = L^-1 Y^T
+  for (int c = tid; c < n; c += NT)
     for (int i = 0; i < n; ++i) {
-      double acc = Vm[c + (long)i * n];
-      for (int k = 0; k < i; ++k) acc -= Lm[i + (long)k * n] * Am[k + (long)c * n];
+      const double acc = Cm[c + (long)i * n] - dot4(Lm + (long)i * n, 1, Am + (long)c * n, 1, i);
       Am[i + (long)c * n] = acc / Lm[i + (long)i * n];
     }
   __syncthreads();
-  for (long e = tid; e < nn; e += 256) {
+  if (warm) {
+    // warm start: V holds the eigenvectors of the previous pass; A <- V^T A V is nearly
+    // diagonal when H changed little. T = A V into C, then A = V^T T.
+    for (long e = tid; e < nn; e += NT) {
+      const int i = (int)(e % n), j = (int)(e / n);
+      Cm[e] = dot4(Am + i, n, Vm + (long)j * n, 1, n);
+    }
+    __syncthreads();
+    for (long e = tid; e < nn; e += NT) {
+      const int i = (int)(e % n), j = (int)(e / n);
+      Am[e] = dot4(Vm + (long)i * n, 1, Cm + (long)j * n, 1, n);
+    }
+    __syncthreads();
+  }
+  for (long e = tid; e < nn; e += NT) {
     const int i = (int)(e % n), j = (int)(e / n);
     if (i > j) {
       const double v = 0.5 * (Am[i + (long)j * n] + Am[j + (long)i * n]);
@@ -138,17 +177,21 @@ __global__ __launch_bounds__(256) void batch_sygv_kernel(BatchLayout L, const do
       Am[j + (long)i * n] = v;
     }
   }
-  for (long e = tid; e < nn; e += 256) Vm[e] = ((e % n) == (e / n)) ? 1.0 : 0.0;
+  if (!warm)
+    for (long e = tid; e < nn; e += NT) Vm[e] = ((e % n) == (e / n)) ? 1.0 : 0.0;
   __syncthreads();
-  // parallel cyclic Jacobi, round-robin pairing (n odd: one dummy index)
+  // parallel cyclic Jacobi, round-robin pairing (n odd: one dummy index). The rotation of a
+  // round touch disjoint column (row) pairs; each thread updates four elements per pass with
+  // all loads ahead of the stores.
   const int N2 = n + (n & 1);
   const int np = N2 / 2;
+  const long work4 = ((long)np * n + 3) / 4;
   const double eps = 2.220446049250313e-16;
   for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
     if (tid == 0) nrot = 0;
     __syncthreads();
     for (int r = 0; r < N2 - 1; ++r) {
-      for (int k = tid; k < np; k += 256) {
+      for (int k = tid; k < np; k += NT) {
         int p, q;
         jacobi_pair(N2, r, k, p, q);
         double c = 1.0, s = 0.0;
@@ -170,30 +213,65 @@ __global__ __launch_bounds__(256) void batch_sygv_kernel(BatchLayout L, const do
       }
       __syncthreads();
       // columns p, q of A and V
-      for (long e = tid; e < (long)np * n; e += 256) {
-        const int k = (int)(e / n), i = (int)(e % n);
-        const double s = rs[k];
-        if (s == 0.0) continue;
-        const int p = rp[k], q = rq[k];
-        const double c = rc[k];
-        const double ap = Am[i + (long)p * n], aq = Am[i + (long)q * n];
-        Am[i + (long)p * n] = c * ap - s * aq;
-        Am[i + (long)q * n] = s * ap + c * aq;
-        const double vp = Vm[i + (long)p * n], vq = Vm[i + (long)q * n];
-        Vm[i + (long)p * n] = c * vp - s * vq;
-        Vm[i + (long)q * n] = s * vp + c * vq;
+      for (long g = tid; g < work4; g += NT) {
+        double ap[4], aq[4], vp[4], vq[4];
+        long ip[4], iq[4];
+        double cc[4], ss[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const long e = g * 4 + u;
+          ss[u] = 0.0;
+          if (e < (long)np * n) {
+            const int k = (int)(e / n), i = (int)(e % n);
+            ss[u] = rs[k];
+            cc[u] = rc[k];
+            ip[u] = i + (long)rp[k] * n;
+            iq[u] = i + (long)rq[k] * n;
+            if (ss[u] != 0.0) {
+              ap[u] = Am[ip[u]];
+              aq[u] = Am[iq[u]];
+              vp[u] = Vm[ip[u]];
+              vq[u] = Vm[iq[u]];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (ss[u] == 0.0) continue;
+          Am[ip[u]] = cc[u] * ap[u] - ss[u] * aq[u];
+          Am[iq[u]] = ss[u] * ap[u] + cc[u] * aq[u];
+          Vm[ip[u]] = cc[u] * vp[u] - ss[u] * vq[u];
+          Vm[iq[u]] = ss[u] * vp[u] + cc[u] * vq[u];
+        }
       }
       __syncthreads();
       // rows p, q of A
-      for (long e = tid; e < (long)np * n; e += 256) {
-        const int k = (int)(e / n), j = (int)(e % n);
-        const double s = rs[k];
-        if (s == 0.0) continue;
-        const int p = rp[k], q = rq[k];
-        const double c = rc[k];
-        const double ap = Am[p + (long)j * n], aq = Am[q + (long)j * n];
-        Am[p + (long)j * n] = c * ap - s * aq;
-        Am[q + (long)j * n] = s * ap + c * aq;
+      for (long g = tid; g < work4; g += NT) {
+        double ap[4], aq[4];
+        long ip[4], iq[4];
+        double cc[4], ss[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const long e = g * 4 + u;
+          ss[u] = 0.0;
+          if (e < (long)np * n) {
+            const int k = (int)(e / n), j = (int)(e % n);
+            ss[u] = rs[k];
+            cc[u] = rc[k];
+            ip[u] = rp[k] + (long)j * n;
+            iq[u] = rq[k] + (long)j * n;
+            if (ss[u] != 0.0) {
+              ap[u] = Am[ip[u]];
+              aq[u] = Am[iq[u]];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (ss[u] == 0.0) continue;
+          Am[ip[u]] = cc[u] * ap[u] - ss[u] * aq[u];
+          Am[iq[u]] = ss[u] * ap[u] + cc[u] * aq[u];
+        }
       }
       __syncthreads();
     }
@@ -201,7 +279,7 @@ __global__ __launch_bounds__(256) void batch_sygv_kernel(BatchLayout L, const do
     __syncthreads();
   }
   // ascending eigenvalues (stable ranks), C = L^-T V(:, perm)
-  for (int i = tid; i < n; i += 256) {
+  for (int i = tid; i < n; i += NT) {
     const double di = Am[i + (long)i * n];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
@@ -212,11 +290,12 @@ __global__ __launch_bounds__(256) void batch_sygv_kernel(BatchLayout L, const do
     wm[rank] = di;
   }
   __syncthreads();
-  for (int c = tid; c < n; c += 256) {
+  for (int c = tid; c < n; c += NT) {
     const int pc = perm[c];
     for (int i = n - 1; i >= 0; --i) {
-      double acc = Vm[i + (long)pc * n];
-      for (int k = i + 1; k < n; ++k) acc -= Lm[k + (long)i * n] * Cm[k + (long)c * n];
+      const int len = n - 1 - i;
+      const double acc = Vm[i + (long)pc * n] -
+                         dot4(Lm + (i + 1) + (long)i * n, 1, Cm + (i + 1) + (long)c * n, 1, len);
       Cm[i + (long)c * n] = acc / Lm[i + (long)i * n];
     }
   }
@@ -226,8 +305,9 @@ __global__ __launch_bounds__(256) void batch_sygv_kernel(BatchLayout L, const do
 // P_m = C_m diag(f) C_m^T over the first nocc[m] columns (lower and upper)
 __global__ __launch_bounds__(256) void batch_density_kernel(BatchLayout L, const double* C,
                                                             const double* f, const int* nocc,
-                                                            double* P) {
+                                                            double* P, const int* active) {
   const int m = blockIdx.x;
+  if (active && !active[m]) return;
   const long base = L.dOff[m];
   const int n = (int)(L.orb0[m + 1] - L.orb0[m]);
   const double* Cm = C + base;
@@ -325,17 +405,20 @@ int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const 
 }
 
 int launch_batch_sygv(hipStream_t s, const BatchLayout& L, const double* S, const double* H,
-                      double* work, double* w, double* C, int* info) {
+                      double* work, double* w, double* C, int* info, const int* active,
+                      bool warm) {
   if (L.nmol == 0) return 0;
-  hipLaunchKernelGGL(batch_sygv_kernel, dim3(L.nmol), dim3(256), 0, s, L, S, H, work, w, C, info);
+  hipLaunchKernelGGL(batch_sygv_kernel, dim3(L.nmol), dim3(kSygvThreads), 0, s, L, S, H, work, w,
+                     C, info, active, warm ? 1 : 0);
   DH_LAUNCH_CHECK();
   return 0;
 }
 
 int launch_batch_density(hipStream_t s, const BatchLayout& L, const double* C, const double* f,
-                         const int* nocc, double* P) {
+                         const int* nocc, double* P, const int* active) {
   if (L.nmol == 0) return 0;
-  hipLaunchKernelGGL(batch_density_kernel, dim3(L.nmol), dim3(256), 0, s, L, C, f, nocc, P);
+  hipLaunchKernelGGL(batch_density_kernel, dim3(L.nmol), dim3(256), 0, s, L, C, f, nocc, P,
+                     active);
   DH_LAUNCH_CHECK();
   return 0;
 }

@@ -1738,9 +1738,10 @@ struct BatchContext {
   long nnz = 0, denseSize = 0, gammaSize = 0;
   DevBuf<double> coordsD, h0, ov, hs, rho, erho, vorb, vatom, dq, qorb, eorb, grad, eatom, gamma;
   DevBuf<double> Sd, Hd, Cv, P, work, w, f;
-  DevBuf<int> atom0D, atomMolD, infoD, noccD;
+  DevBuf<int> atom0D, atomMolD, infoD, noccD, activeD;
   DevBuf<long> orb0D, dOffD, gOffD;
   BatchLayout L{};
+  int it = 0, nactive = 0;
   bool sccDone = false;
 };
 
@@ -1846,6 +1847,7 @@ static int batch_geometry(BatchContext& b, int nmol, const int* natMol, const do
   DH_TRY(b.gamma.alloc(b.gammaSize));
   DH_TRY(b.infoD.alloc(nmol));
   DH_TRY(b.noccD.alloc(nmol));
+  DH_TRY(b.activeD.alloc(nmol));
   DH_TRY(launch_build_h0s(s, b.sk.dev, b.dblk.pr, b.coordsD.p, b.h0.p, b.ov.p));
   DH_TRY(launch_batch_unpack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.ov.p,
                              b.Sd.p, b.denseSize));
@@ -1876,9 +1878,9 @@ static int batch_potential(BatchContext& b, const std::vector<double>& dqa,
 
 // The SCC cycle of every molecule (main.F90:1429-1563 per molecule); molecules that have
 // converged keep their input charges, so their state stays that of their last iteration.
-static int batch_scc(BatchContext& b, double* energies, double* charges, int* niter,
-                     int* converged) {
-  hipStream_t s = b.s;
+// The SCC cycle of every molecule (main.F90:1429-1563 per molecule); molecules that have
+// converged keep their input charges, so their state stays that of their last iteration.
+static int batch_scc_begin(BatchContext& b) {
   const dftbhip_options& o = b.opt;
   const int nmol = b.nmol, nat = b.nat, n = b.norbTot;
   b.qin = b.q0atom;
@@ -1895,76 +1897,99 @@ static int batch_scc(BatchContext& b, double* energies, double* charges, int* ni
   b.fill.assign(n, 0.0);
   b.qout.assign(nat, 0.0);
   b.dqout.assign(nat, 0.0);
+  b.it = 0;
+  b.nactive = nmol;
+  b.sccDone = false;
+  return 0;
+}
+
+static int batch_scc_step(BatchContext& b) {
+  hipStream_t s = b.s;
+  const dftbhip_options& o = b.opt;
+  const int nmol = b.nmol, nat = b.nat, n = b.norbTot;
+  DH_CHECK(b.nactive > 0 && b.it < o.max_scc_iter, "batch scc_step: nothing left to iterate");
+  const int it = ++b.it;
+  int& nactive = b.nactive;
   std::vector<double> dqa(nat), qorbH(n), eorbH(n), vout;
   std::vector<int> info(nmol);
-  int nactive = nmol;
-  for (int it = 1; it <= o.max_scc_iter && nactive > 0; ++it) {
-    for (int a = 0; a < nat; ++a) dqa[a] = b.qin[a] - b.q0atom[a];
-    DH_TRY(batch_potential(b, dqa, nullptr));
-    DH_TRY(launch_shift(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.h0.p, b.ov.p,
-                        b.vorb.p, b.hs.p));
-    DH_TRY(launch_batch_unpack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.hs.p,
-                               b.Hd.p, b.denseSize));
-    DH_TRY(launch_batch_sygv(s, b.L, b.Sd.p, b.Hd.p, b.work.p, b.w.p, b.Cv.p, b.infoD.p));
-    b.eigv.resize(n);
-    DH_HIP(hipMemcpyAsync(b.eigv.data(), b.w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    DH_HIP(hipMemcpyAsync(info.data(), b.infoD.p, nmol * sizeof(int), hipMemcpyDeviceToHost, s));
-    DH_HIP(hipStreamSynchronize(s));
-    for (int m = 0; m < nmol; ++m)
-      if (info[m] != 0) {
-        set_error("batch: overlap of molecule " + std::to_string(m) +
-                  " not positive definite (minor " + std::to_string(info[m]) + ")");
-        return kErrState;
-      }
-    for (int m = 0; m < nmol; ++m) {
-      const long o0 = b.orb0[m], nm = b.orb0[m + 1] - o0;
-      std::vector<double> e(b.eigv.begin() + o0, b.eigv.begin() + o0 + nm), fm;
-      DH_TRY(fermi_filling(e, b.nel[m], o.kT, fm, b.fr[m]));
-      std::copy(fm.begin(), fm.end(), b.fill.begin() + o0);
-      b.nocc[m] = occupied_levels(fm);
+  for (int a = 0; a < nat; ++a) dqa[a] = b.qin[a] - b.q0atom[a];
+  DH_TRY(batch_potential(b, dqa, nullptr));
+  DH_TRY(launch_shift(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.h0.p, b.ov.p,
+                      b.vorb.p, b.hs.p));
+  DH_TRY(launch_batch_unpack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.hs.p,
+                             b.Hd.p, b.denseSize));
+  std::vector<int> active(nmol);
+  for (int m = 0; m < nmol; ++m) active[m] = b.conv[m] ? 0 : 1;
+  DH_HIP(hipMemcpyAsync(b.activeD.p, active.data(), nmol * sizeof(int), hipMemcpyHostToDevice, s));
+  DH_TRY(launch_batch_sygv(s, b.L, b.Sd.p, b.Hd.p, b.work.p, b.w.p, b.Cv.p, b.infoD.p,
+                           b.activeD.p, it > 1));
+  b.eigv.resize(n);
+  DH_HIP(hipMemcpyAsync(b.eigv.data(), b.w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(info.data(), b.infoD.p, nmol * sizeof(int), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  for (int m = 0; m < nmol; ++m)
+    if (info[m] != 0) {
+      set_error("batch: overlap of molecule " + std::to_string(m) +
+                " not positive definite (minor " + std::to_string(info[m]) + ")");
+      return kErrState;
     }
-    DH_HIP(hipMemcpyAsync(b.f.p, b.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-    DH_HIP(hipMemcpyAsync(b.noccD.p, b.nocc.data(), nmol * sizeof(int), hipMemcpyHostToDevice, s));
-    DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p));
-    DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p,
-                             b.rho.p));
-    DH_TRY(launch_mulliken(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.ov.p, b.h0.p,
-                           b.rho.p, b.qorb.p, b.eorb.p, n));
-    DH_HIP(hipMemcpyAsync(qorbH.data(), b.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    DH_HIP(hipMemcpyAsync(eorbH.data(), b.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    DH_HIP(hipStreamSynchronize(s));
-    std::vector<double> qo(nat, 0.0);
-    for (int a = 0; a < nat; ++a)
-      for (long oi = b.atomStart[a]; oi < b.atomStart[a + 1]; ++oi) qo[a] += qorbH[oi];
-    std::vector<double> dqo(nat);
-    for (int a = 0; a < nat; ++a) dqo[a] = qo[a] - b.q0atom[a];
-    DH_TRY(batch_potential(b, dqo, &vout));
-    for (int m = 0; m < nmol; ++m) {
-      if (b.conv[m]) continue;
-      b.iters[m] = it;
-      double en = 0.0, es = 0.0, er = 0.0;
-      for (long oi = b.orb0[m]; oi < b.orb0[m + 1]; ++oi) en += eorbH[oi];
-      std::vector<double> q(b.qin.begin() + b.atom0[m], b.qin.begin() + b.atom0[m + 1]);
-      std::vector<double> diff(q.size());
-      for (int a = b.atom0[m]; a < b.atom0[m + 1]; ++a) {
-        es += 0.5 * vout[a] * dqo[a];
-        diff[a - b.atom0[m]] = qo[a] - b.qin[a];
-        er = std::max(er, std::fabs(diff[a - b.atom0[m]]));
-        b.qout[a] = qo[a];
-        b.dqout[a] = dqo[a];
-      }
-      b.eNonScc[m] = en;
-      b.eScc[m] = es;
-      b.err[m] = er;
-      if (er < o.scc_tol && it >= o.min_scc_iter) {
-        b.conv[m] = 1;
-        --nactive;
-      } else if (it != o.max_scc_iter) {
-        b.mixers[m].mix(q, diff);
-        std::copy(q.begin(), q.end(), b.qin.begin() + b.atom0[m]);
-      }
+  for (int m = 0; m < nmol; ++m) {
+    if (b.conv[m]) continue;   // fillings of converged molecules stay from their last pass
+    const long o0 = b.orb0[m], nm = b.orb0[m + 1] - o0;
+    std::vector<double> e(b.eigv.begin() + o0, b.eigv.begin() + o0 + nm), fm;
+    DH_TRY(fermi_filling(e, b.nel[m], o.kT, fm, b.fr[m]));
+    std::copy(fm.begin(), fm.end(), b.fill.begin() + o0);
+    b.nocc[m] = occupied_levels(fm);
+  }
+  DH_HIP(hipMemcpyAsync(b.f.p, b.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  DH_HIP(hipMemcpyAsync(b.noccD.p, b.nocc.data(), nmol * sizeof(int), hipMemcpyHostToDevice, s));
+  DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p, b.activeD.p));
+  DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p,
+                           b.rho.p));
+  DH_TRY(launch_mulliken(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.ov.p, b.h0.p,
+                         b.rho.p, b.qorb.p, b.eorb.p, n));
+  DH_HIP(hipMemcpyAsync(qorbH.data(), b.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(eorbH.data(), b.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  std::vector<double> qo(nat, 0.0);
+  for (int a = 0; a < nat; ++a)
+    for (long oi = b.atomStart[a]; oi < b.atomStart[a + 1]; ++oi) qo[a] += qorbH[oi];
+  std::vector<double> dqo(nat);
+  for (int a = 0; a < nat; ++a) dqo[a] = qo[a] - b.q0atom[a];
+  DH_TRY(batch_potential(b, dqo, &vout));
+  for (int m = 0; m < nmol; ++m) {
+    if (b.conv[m]) continue;
+    b.iters[m] = it;
+    double en = 0.0, es = 0.0, er = 0.0;
+    for (long oi = b.orb0[m]; oi < b.orb0[m + 1]; ++oi) en += eorbH[oi];
+    std::vector<double> q(b.qin.begin() + b.atom0[m], b.qin.begin() + b.atom0[m + 1]);
+    std::vector<double> diff(q.size());
+    for (int a = b.atom0[m]; a < b.atom0[m + 1]; ++a) {
+      es += 0.5 * vout[a] * dqo[a];
+      diff[a - b.atom0[m]] = qo[a] - b.qin[a];
+      er = std::max(er, std::fabs(diff[a - b.atom0[m]]));
+      b.qout[a] = qo[a];
+      b.dqout[a] = dqo[a];
+    }
+    b.eNonScc[m] = en;
+    b.eScc[m] = es;
+    b.err[m] = er;
+    if (er < o.scc_tol && it >= o.min_scc_iter) {
+      b.conv[m] = 1;
+      --nactive;
+    } else if (it != o.max_scc_iter) {
+      b.mixers[m].mix(q, diff);
+      std::copy(q.begin(), q.end(), b.qin.begin() + b.atom0[m]);
     }
   }
+  return 0;
+}
+
+static int batch_scc_end(BatchContext& b, double* energies, double* charges, int* niter,
+                         int* converged) {
+  DH_CHECK(b.it > 0, "batch scc_end: no SCC iteration was run");
+  hipStream_t s = b.s;
+  const int nmol = b.nmol, nat = b.nat;
   // repulsive energies and the final energies per molecule
   DH_TRY(fill<double>(s, b.eatom.p, nat, 0.0));
   DH_TRY(fill<double>(s, b.grad.p, 3 * (size_t)nat, 0.0));
@@ -1996,6 +2021,14 @@ static int batch_scc(BatchContext& b, double* energies, double* charges, int* ni
   return 0;
 }
 
+
+static int batch_scc(BatchContext& b, double* energies, double* charges, int* niter,
+                     int* converged) {
+  DH_TRY(batch_scc_begin(b));
+  while (b.nactive > 0 && b.it < b.opt.max_scc_iter) DH_TRY(batch_scc_step(b));
+  return batch_scc_end(b, energies, charges, niter, converged);
+}
+
 static int batch_gradients(BatchContext& b, double* grad) {
   DH_CHECK(b.sccDone, "batch gradients: run the SCC cycle first");
   hipStream_t s = b.s;
@@ -2004,7 +2037,7 @@ static int batch_gradients(BatchContext& b, double* grad) {
   for (int m = 0; m < b.nmol; ++m)
     for (long k = 0; k < b.nocc[m]; ++k) fe[b.orb0[m] + k] = b.fill[b.orb0[m] + k] * b.eigv[b.orb0[m] + k];
   DH_HIP(hipMemcpyAsync(b.f.p, fe.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-  DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p));
+  DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p, nullptr));
   DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p, b.erho.p));
   DH_TRY(batch_potential(b, b.dqout, nullptr));
   DH_TRY(fill<double>(s, b.eatom.p, nat, 0.0));
@@ -2061,6 +2094,24 @@ extern "C" int dftbhip_batch_scc(dftbhip_batch* b, double* energies, double* cha
                                  int* converged) {
   DH_CHECK(b && b->b.nmol > 0, "batch_scc: no geometries");
   return dftbhip::batch_scc(b->b, energies, charges, niter, converged);
+}
+
+extern "C" int dftbhip_batch_scc_begin(dftbhip_batch* b) {
+  DH_CHECK(b && b->b.nmol > 0, "batch_scc_begin: no geometries");
+  return dftbhip::batch_scc_begin(b->b);
+}
+
+extern "C" int dftbhip_batch_scc_step(dftbhip_batch* b, int* nactive) {
+  DH_CHECK(b, "batch_scc_step: null pointer");
+  DH_TRY(dftbhip::batch_scc_step(b->b));
+  if (nactive) *nactive = b->b.nactive;
+  return 0;
+}
+
+extern "C" int dftbhip_batch_scc_end(dftbhip_batch* b, double* energies, double* charges,
+                                     int* niter, int* converged) {
+  DH_CHECK(b, "batch_scc_end: null pointer");
+  return dftbhip::batch_scc_end(b->b, energies, charges, niter, converged);
 }
 
 extern "C" int dftbhip_batch_gradients(dftbhip_batch* b, double* grad) {
@@ -2230,6 +2281,26 @@ extern "C" int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, d
     DH_HIP(hipMemcpyAsync(e, E.p, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
     DH_HIP(hipMemcpyAsync(tau, TAU.p, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
   }
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_dlacpy(char uplo, int m, int n, const double* a, int lda, double* b,
+                              int ldb) {
+  DH_CHECK(uplo == 'A' || uplo == 'a', "dlacpy: only uplo = 'A'");
+  DH_CHECK(m >= 0 && n >= 0 && a && b && lda >= std::max(m, 1) && ldb >= std::max(m, 1),
+           "dlacpy: bad arguments");
+  if (m == 0 || n == 0) return 0;
+  DH_TRY(dftbhip::ensure_runtime());
+  hipStream_t s = dftbhip::g_rt.stream;
+  dftbhip::DevBuf<double> A, B;
+  DH_TRY(A.alloc((size_t)m * n));
+  DH_TRY(B.alloc((size_t)m * n));
+  DH_HIP(hipMemcpy2DAsync(A.p, m * sizeof(double), a, (size_t)lda * sizeof(double),
+                          m * sizeof(double), n, hipMemcpyHostToDevice, s));
+  DH_TRY(dftbhip::copy2d<double>(s, A.p, m, B.p, m, m, n));
+  DH_HIP(hipMemcpy2DAsync(b, (size_t)ldb * sizeof(double), B.p, m * sizeof(double),
+                          m * sizeof(double), n, hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
   return 0;
 }

@@ -43,6 +43,9 @@ int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e, d
 /* Kernel tuning. param 0: row tiles per workgroup of the tridiagonalisation's SYMV (4, 8, 16;
  * default 8). No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
+/* LAPACK dlacpy (uplo = 'A'): b = a through the device copy kernel (the copy used for HBM
+ * counter calibration; DFTB+ copies square matrices with plain array assignment). */
+int dftbhip_dlacpy(char uplo, int m, int n, const double* a, int lda, double* b, int ldb);
 
 /* ---- density matrices: replace makeDensityMtxRealGPU / makeEnergyDensityMtxRealGPU
  *      (src/dftbp/dftb/densitymatrix.F90:1105-1395). Lower triangle of dm/edm is set:
@@ -170,6 +173,12 @@ int dftbhip_batch_set_geometries(dftbhip_batch* batch, int nMol, const int* nAto
                                  const double* coords, const int* species);
 int dftbhip_batch_scc(dftbhip_batch* batch, double* energies, double* charges, int* niter,
                       int* converged);
+/* one SCC iteration at a time (as dftbhip_ctx_scc_begin/step/end); nactive = molecules not yet
+ * converged after the iteration */
+int dftbhip_batch_scc_begin(dftbhip_batch* batch);
+int dftbhip_batch_scc_step(dftbhip_batch* batch, int* nactive);
+int dftbhip_batch_scc_end(dftbhip_batch* batch, double* energies, double* charges, int* niter,
+                          int* converged);
 int dftbhip_batch_gradients(dftbhip_batch* batch, double* grad);
 
 /* ---- measurement: per-launch HIP-event timing of the tridiagonalisation's symmetric
