@@ -121,6 +121,7 @@ def mode_sytrd(args):
     a = rng.standard_normal((n, n))
     a = a + a.T
     api.dlacpy(a)
+    api.set_tuning(0, args.tiles)
     L.dftbhip_kernel_profile(1)
     t0 = time.perf_counter()
     api.dsytrd(a)
@@ -129,7 +130,8 @@ def mode_sytrd(args):
     L.dftbhip_kernel_stats(api._d(st))
     L.dftbhip_kernel_profile(0)
     launches, ksec, kbytes = st
-    print(json.dumps({"mode": "sytrd", "n": n, "seconds_incl_transfers": dt,
+    print(json.dumps({"mode": "sytrd", "n": n, "symv_tiles": args.tiles or "by size",
+                      "seconds_incl_transfers": dt,
                       "symv_launches": int(launches), "symv_avg_us": 1e6 * ksec / launches,
                       "symv_alg_bytes_per_launch": kbytes / launches,
                       "symv_achieved_GBs": kbytes / ksec / 1e9,
@@ -237,6 +239,8 @@ def main():
                          "and a force evaluation with a k-point mesh")
     ap.add_argument("--n", type=int, default=8192, help="matrix size of --mode sytrd")
     ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
+    ap.add_argument("--tiles", type=int, default=0,
+                    help="SYMV row tiles per workgroup for --mode sytrd (0: by trailing size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCC path of a single Gamma-point system runs on one "

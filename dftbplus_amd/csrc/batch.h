@@ -14,6 +14,7 @@ struct BatchLayout {
   const long* dOff;     // nmol + 1: offsets of the packed n_m x n_m dense matrices
   const long* gOff;     // nmol + 1: offsets of the packed nat_m x nat_m gamma matrices
   const int* atomMol;   // molecule of each atom
+  const int* order;     // nmol: molecules by decreasing orbital count (launch order)
 };
 
 int launch_batch_unpack(hipStream_t s, const PairsDev& pr, const int* norb,
@@ -22,9 +23,10 @@ int launch_batch_unpack(hipStream_t s, const PairsDev& pr, const int* norb,
 int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                       const BatchLayout& L, const double* dense, double* x);
 // eigenvalues w (ascending per molecule, at orb0), eigenvectors C (packed), info per molecule
-// (0, or j+1 when the leading minor j+1 of S is not positive definite); work 3 x packed size.
-// Molecules with active[m] == 0 are skipped (active may be null: all). warm: start the Jacobi
-// sweeps from the eigenvectors of the previous call on the same work array.
+// (0, or j+1 when the leading minor j+1 of S is not positive definite); work 3 x packed size,
+// kept between calls. Molecules with active[m] == 0 are skipped (active may be null: all).
+// warm: S is the one of the previous call on the same work array and the Jacobi sweeps start
+// from that call's eigenvectors (a cold call factors S).
 int launch_batch_sygv(hipStream_t s, const BatchLayout& L, const double* S, const double* H,
                       double* work, double* w, double* C, int* info, const int* active,
                       bool warm);

@@ -4,9 +4,10 @@
 // consecutively and the neighbour list holds only intra-molecule pairs, so the sparse kernels
 // (H0/S assembly, shift, pack, Mulliken, pair forces, repulsive) run unchanged over the union.
 // What is per molecule lives here: the dense matrices (packed one after the other, n_m x n_m
-// column-major), the generalized eigensolver (one workgroup per molecule: Cholesky, two
-// triangular substitutions, parallel cyclic Jacobi, back-substitution), the density matrices and
-// the cluster gamma matrix with its products and gradient.
+// column-major), the generalized eigensolver (one workgroup per molecule: Cholesky and L^-T once
+// per geometry, then per call a congruence with workgroup GEMMs and parallel cyclic Jacobi), the
+// density matrices and the cluster gamma matrix with its products and gradient. The eigensolver
+// and density kernels visit the molecules largest first (BatchLayout::order).
 #include "batch.h"
 #include "gamma_dev.h"
 
@@ -58,11 +59,21 @@ __global__ void batch_pack_kernel(PairsDev pr, const int* norbAtom, const long* 
 }
 
 // ------------------------------------------------------------------------------------------
-// Generalized eigensolver H C = S C E per molecule, one 256-thread workgroup each.
-// Work: 3 n^2 doubles per molecule at 3 dOff[m] (L, A, V).
+// Generalized eigensolver H C = S C E per molecule, one 1024-thread workgroup each.
+//
+// S is fixed for a geometry, so the reduction to standard form is done once: a cold call
+// factors S = L L^T and starts from C = L^-T. Every call forms A = C^T H C with two workgroup
+// GEMMs and diagonalizes A by parallel cyclic Jacobi, applying each round's rotations to A from
+// both sides (one pass over its 2 x 2 blocks) and to the columns of C. Rotations keep
+// C^T S C = I, so the rotated C holds the generalized eigenvectors. A warm call starts from the
+// C of the previous call, for which A is already nearly diagonal when H changed little.
+// Work: 3 n^2 doubles per molecule at 3 dOff[m]: B0 (L, then A), B1 (H C) and Cw (the C above,
+// columns in Jacobi order, kept between calls).
 // ------------------------------------------------------------------------------------------
 constexpr int kJacobiMaxSweeps = 30;
 constexpr int kSygvThreads = 1024;
+constexpr int kGT = 64;   // workgroup GEMM output tile
+constexpr int kGK = 16;   // workgroup GEMM k chunk
 
 __device__ __forceinline__ void jacobi_pair(int N2, int r, int k, int& p, int& q) {
   p = (k == 0) ? 0 : 1 + (r + k - 1) % (N2 - 1);
@@ -84,107 +95,152 @@ __device__ __forceinline__ double dot4(const double* __restrict__ x, long sx,
   return (s0 + s1) + (s2 + s3);
 }
 
+// Workgroup GEMM on n x n column-major matrices (leading dimension n):
+//   Z = op(X)(:, 0:kd) diag(wk) op(Y)(0:kd, :),  op(X) = X or X^T, op(Y) = Y or Y^T,
+// wk optional. 1024 threads; 64 x 64 output tiles, each thread rows tx, tx + 32 by columns
+// ty, ty + 32; k in chunks of 16 staged in LDS, the next chunk's loads issued before the
+// current chunk's products. Z must not overlap X or Y.
+template <bool TX, bool TY>
+__device__ void wg_gemm(int n, int kd, const double* __restrict__ X, const double* __restrict__ Y,
+                        const double* __restrict__ wk, double* __restrict__ Z,
+                        double (*Xs)[kGT + 1], double (*Ys)[kGT + 1]) {
+  const int tid = threadIdx.x;
+  const int tx = tid % 32, ty = tid / 32;
+  // the element each thread stages per chunk: op(X)(xr, xk) and op(Y)(yk, yc)
+  const int xr = TX ? tid / kGK : tid % kGT, xk = TX ? tid % kGK : tid / kGT;
+  const int yk = TY ? tid / kGT : tid % kGK, yc = TY ? tid % kGT : tid / kGK;
+  const int nt = (n + kGT - 1) / kGT;
+  for (int tile = 0; tile < nt * nt; ++tile) {
+    const int r0 = (tile % nt) * kGT, c0 = (tile / nt) * kGT;
+    auto ldx = [&](int k0) -> double {
+      const int r = r0 + xr, k = k0 + xk;
+      if (r >= n || k >= kd) return 0.0;
+      const double v = TX ? X[k + (long)r * n] : X[r + (long)k * n];
+      return wk ? v * wk[k] : v;
+    };
+    auto ldy = [&](int k0) -> double {
+      const int k = k0 + yk, c = c0 + yc;
+      if (c >= n || k >= kd) return 0.0;
+      return TY ? Y[c + (long)k * n] : Y[k + (long)c * n];
+    };
+    double a00 = 0.0, a10 = 0.0, a01 = 0.0, a11 = 0.0;
+    double xv = ldx(0), yv = ldy(0);
+    for (int k0 = 0; k0 < kd; k0 += kGK) {
+      __syncthreads();
+      Xs[xk][xr] = xv;
+      Ys[yk][yc] = yv;
+      __syncthreads();
+      if (k0 + kGK < kd) {
+        xv = ldx(k0 + kGK);
+        yv = ldy(k0 + kGK);
+      }
+#pragma unroll
+      for (int k = 0; k < kGK; ++k) {
+        const double x0 = Xs[k][tx], x1 = Xs[k][tx + 32];
+        const double y0 = Ys[k][ty], y1 = Ys[k][ty + 32];
+        a00 += x0 * y0;
+        a10 += x1 * y0;
+        a01 += x0 * y1;
+        a11 += x1 * y1;
+      }
+    }
+    const int ra = r0 + tx, rb = ra + 32, ca = c0 + ty, cb = ca + 32;
+    if (ra < n && ca < n) Z[ra + (long)ca * n] = a00;
+    if (rb < n && ca < n) Z[rb + (long)ca * n] = a10;
+    if (ra < n && cb < n) Z[ra + (long)cb * n] = a01;
+    if (rb < n && cb < n) Z[rb + (long)cb * n] = a11;
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L, const double* S,
                                                                   const double* H, double* work,
                                                                   double* w, double* C,
                                                                   int* info, const int* active,
                                                                   int warm) {
   constexpr int NT = kSygvThreads;
-  if (active && !active[blockIdx.x]) return;   // converged: eigenpairs stay from the last pass
+  const int m = L.order[blockIdx.x];
+  if (active && !active[m]) return;   // converged: eigenpairs stay from the last pass
+  __shared__ double Xs[kGK][kGT + 1], Ys[kGK][kGT + 1];
   __shared__ double rc[kBatchMaxOrb / 2], rs[kBatchMaxOrb / 2];
   __shared__ int rp[kBatchMaxOrb / 2], rq[kBatchMaxOrb / 2];
   __shared__ int perm[kBatchMaxOrb];
   __shared__ int flag, nrot;
-  const int m = blockIdx.x;
   const int tid = threadIdx.x;
   const long base = L.dOff[m];
   const int n = (int)(L.orb0[m + 1] - L.orb0[m]);
-  const double* Sm = S + base;
+  const long nn = (long)n * n;
   const double* Hm = H + base;
-  double* Lm = work + 3 * base;    // L in the lower triangle, L^T mirrored into the upper one
-  double* Am = Lm + (long)n * n;
-  double* Vm = Am + (long)n * n;
+  double* B0 = work + 3 * base;
+  double* B1 = B0 + nn;
+  double* Cw = B1 + nn;
   double* Cm = C + base;
   double* wm = w + L.orb0[m];
-  const long nn = (long)n * n;
-  if (tid == 0) flag = 0;
-  for (long e = tid; e < nn; e += NT) Lm[e] = Sm[e];
-  __syncthreads();
-  // Cholesky S = L L^T (lower, right-looking)
-  for (int j = 0; j < n; ++j) {
-    if (tid == 0) {
-      const double d = Lm[j + (long)j * n];
-      if (!(d > 0.0)) {
-        flag = j + 1;
-      } else {
-        Lm[j + (long)j * n] = sqrt(d);
+  if (!warm) {
+    const double* Sm = S + base;
+    if (tid == 0) flag = 0;
+    for (long e = tid; e < nn; e += NT) B0[e] = Sm[e];
+    __syncthreads();
+    // Cholesky S = L L^T (lower, right-looking)
+    for (int j = 0; j < n; ++j) {
+      if (tid == 0) {
+        const double d = B0[j + (long)j * n];
+        if (!(d > 0.0)) {
+          flag = j + 1;
+        } else {
+          B0[j + (long)j * n] = sqrt(d);
+        }
       }
+      __syncthreads();
+      if (flag != 0) {
+        if (tid == 0) info[m] = flag;
+        return;
+      }
+      const double djj = B0[j + (long)j * n];
+      for (int i = j + 1 + tid; i < n; i += NT) B0[i + (long)j * n] /= djj;
+      __syncthreads();
+      const int len = n - j - 1;
+      for (long e = tid; e < (long)len * len; e += NT) {
+        const int i = j + 1 + (int)(e % len), k = j + 1 + (int)(e / len);
+        if (i >= k) B0[i + (long)k * n] -= B0[i + (long)j * n] * B0[k + (long)j * n];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (flag != 0) {
-      if (tid == 0) info[m] = flag;
-      return;
-    }
-    const double djj = Lm[j + (long)j * n];
-    for (int i = j + 1 + tid; i < n; i += NT) Lm[i + (long)j * n] /= djj;
-    __syncthreads();
-    const int len = n - j - 1;
-    for (long e = tid; e < (long)len * len; e += NT) {
-      const int i = j + 1 + (int)(e % len), k = j + 1 + (int)(e / len);
-      if (i >= k) Lm[i + (long)k * n] -= Lm[i + (long)j * n] * Lm[k + (long)j * n];
-    }
-    __syncthreads();
-  }
-  // L^T into the upper triangle: row i of L is then contiguous (column i of the upper part)
-  for (long e = tid; e < nn; e += NT) {
-    const int i = (int)(e % n), k = (int)(e / n);
-    if (i > k) Lm[k + (long)i * n] = Lm[i + (long)k * n];
-  }
-  __syncthreads();
-  // Y = L^-1 H (into C), one column per thread, row i from L^T's column i
-  for (int c = tid; c < n; c += NT)
-    for (int i = 0; i < n; ++i) {
-      const double acc = Hm[i + (long)c * n] - dot4(Lm + (long)i * n, 1, Cm + (long)c * n, 1, i);
-      Cm[i + (long)c * n] = acc / Lm[i + (long)i * n];
-    }
-  __syncthreads();
-  // A = L^-1 Y^T
-  for (int c = tid; c < n; c += NT)
-    for (int i = 0; i < n; ++i) {
-      const double acc = Cm[c + (long)i * n] - dot4(Lm + (long)i * n, 1, Am + (long)c * n, 1, i);
-      Am[i + (long)c * n] = acc / Lm[i + (long)i * n];
-    }
-  __syncthreads();
-  if (warm) {
-    // warm start: V holds the eigenvectors of the previous pass; A <- V^T A V is nearly
-    // diagonal when H changed little. T = A V into C, then A = V^T T.
+    // L^T into the upper triangle: row i of L is then contiguous (column i of the upper part)
     for (long e = tid; e < nn; e += NT) {
-      const int i = (int)(e % n), j = (int)(e / n);
-      Cm[e] = dot4(Am + i, n, Vm + (long)j * n, 1, n);
+      const int i = (int)(e % n), k = (int)(e / n);
+      if (i > k) B0[k + (long)i * n] = B0[i + (long)k * n];
     }
     __syncthreads();
-    for (long e = tid; e < nn; e += NT) {
-      const int i = (int)(e % n), j = (int)(e / n);
-      Am[e] = dot4(Vm + (long)i * n, 1, Cm + (long)j * n, 1, n);
-    }
+    // Cw = L^-T: thread c forms column c of L^-1 by forward substitution and stores it as row c
+    // of Cw, so that the lanes of a wave read and write consecutive addresses
+    for (int c = tid; c < n; c += NT)
+      for (int i = 0; i < n; ++i) {
+        double v = 0.0;
+        if (i >= c)
+          v = ((i == c ? 1.0 : 0.0) - dot4(B0 + (long)i * n, 1, Cw + c, n, i)) /
+              B0[i + (long)i * n];
+        Cw[c + (long)i * n] = v;
+      }
     __syncthreads();
   }
+  // A = Cw^T H Cw (symmetrized)
+  wg_gemm<false, false>(n, n, Hm, Cw, nullptr, B1, Xs, Ys);
+  wg_gemm<true, false>(n, n, Cw, B1, nullptr, B0, Xs, Ys);
+  double* A = B0;
   for (long e = tid; e < nn; e += NT) {
     const int i = (int)(e % n), j = (int)(e / n);
     if (i > j) {
-      const double v = 0.5 * (Am[i + (long)j * n] + Am[j + (long)i * n]);
-      Am[i + (long)j * n] = v;
-      Am[j + (long)i * n] = v;
+      const double v = 0.5 * (A[i + (long)j * n] + A[j + (long)i * n]);
+      A[i + (long)j * n] = v;
+      A[j + (long)i * n] = v;
     }
   }
-  if (!warm)
-    for (long e = tid; e < nn; e += NT) Vm[e] = ((e % n) == (e / n)) ? 1.0 : 0.0;
   __syncthreads();
-  // parallel cyclic Jacobi, round-robin pairing (n odd: one dummy index). The rotation of a
-  // round touch disjoint column (row) pairs; each thread updates four elements per pass with
-  // all loads ahead of the stores.
+  // parallel cyclic Jacobi, round-robin pairing (n odd: index n is a dummy that never rotates)
   const int N2 = n + (n & 1);
   const int np = N2 / 2;
+  const long nblk = (long)np * np;
   const long work4 = ((long)np * n + 3) / 4;
   const double eps = 2.220446049250313e-16;
   for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
@@ -196,8 +252,8 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
         jacobi_pair(N2, r, k, p, q);
         double c = 1.0, s = 0.0;
         if (p < n && q < n) {
-          const double apq = Am[p + (long)q * n];
-          const double app = Am[p + (long)p * n], aqq = Am[q + (long)q * n];
+          const double apq = A[p + (long)q * n];
+          const double app = A[p + (long)p * n], aqq = A[q + (long)q * n];
           if (fabs(apq) > 1e-300 && 100.0 * fabs(apq) > eps * fmin(fabs(app), fabs(aqq))) {
             const double th = (aqq - app) / (2.0 * apq);
             const double t = copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
@@ -212,9 +268,54 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
         rs[k] = s;
       }
       __syncthreads();
-      // columns p, q of A and V
+      // A <- J^T A J over the 2 x 2 blocks (rows of pair k1, columns of pair k2); four blocks
+      // per thread with all loads ahead of the stores
+      for (long b0 = tid; b0 < nblk; b0 += 4L * NT) {
+        double x[4][4];
+        long o[4][4];
+        bool v[4][4];
+        double c1[4], s1[4], c2[4], s2[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const long b = b0 + (long)u * NT;
+          s1[u] = 0.0;
+          s2[u] = 0.0;
+          if (b < nblk) {
+            const int k1 = (int)(b % np), k2 = (int)(b / np);
+            c1[u] = rc[k1];
+            s1[u] = rs[k1];
+            c2[u] = rc[k2];
+            s2[u] = rs[k2];
+            const int row[2] = {rp[k1], rq[k1]}, col[2] = {rp[k2], rq[k2]};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int ri = row[e & 1], ci = col[e >> 1];
+              v[u][e] = ri < n && ci < n;
+              o[u][e] = ri + (long)ci * n;
+            }
+            if (s1[u] != 0.0 || s2[u] != 0.0)
+#pragma unroll
+              for (int e = 0; e < 4; ++e) x[u][e] = v[u][e] ? A[o[u][e]] : 0.0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (s1[u] == 0.0 && s2[u] == 0.0) continue;
+          // x[e]: e = 0 (p1, p2), 1 (q1, p2), 2 (p1, q2), 3 (q1, q2); columns first
+          const double y0 = c2[u] * x[u][0] - s2[u] * x[u][2];
+          const double y1 = c2[u] * x[u][1] - s2[u] * x[u][3];
+          const double y2 = s2[u] * x[u][0] + c2[u] * x[u][2];
+          const double y3 = s2[u] * x[u][1] + c2[u] * x[u][3];
+          const double z[4] = {c1[u] * y0 - s1[u] * y1, s1[u] * y0 + c1[u] * y1,
+                               c1[u] * y2 - s1[u] * y3, s1[u] * y2 + c1[u] * y3};
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (v[u][e]) A[o[u][e]] = z[e];
+        }
+      }
+      // columns p, q of Cw
       for (long g = tid; g < work4; g += NT) {
-        double ap[4], aq[4], vp[4], vq[4];
+        double cp[4], cq[4];
         long ip[4], iq[4];
         double cc[4], ss[4];
 #pragma unroll
@@ -228,49 +329,16 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
             ip[u] = i + (long)rp[k] * n;
             iq[u] = i + (long)rq[k] * n;
             if (ss[u] != 0.0) {
-              ap[u] = Am[ip[u]];
-              aq[u] = Am[iq[u]];
-              vp[u] = Vm[ip[u]];
-              vq[u] = Vm[iq[u]];
+              cp[u] = Cw[ip[u]];
+              cq[u] = Cw[iq[u]];
             }
           }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           if (ss[u] == 0.0) continue;
-          Am[ip[u]] = cc[u] * ap[u] - ss[u] * aq[u];
-          Am[iq[u]] = ss[u] * ap[u] + cc[u] * aq[u];
-          Vm[ip[u]] = cc[u] * vp[u] - ss[u] * vq[u];
-          Vm[iq[u]] = ss[u] * vp[u] + cc[u] * vq[u];
-        }
-      }
-      __syncthreads();
-      // rows p, q of A
-      for (long g = tid; g < work4; g += NT) {
-        double ap[4], aq[4];
-        long ip[4], iq[4];
-        double cc[4], ss[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const long e = g * 4 + u;
-          ss[u] = 0.0;
-          if (e < (long)np * n) {
-            const int k = (int)(e / n), j = (int)(e % n);
-            ss[u] = rs[k];
-            cc[u] = rc[k];
-            ip[u] = rp[k] + (long)j * n;
-            iq[u] = rq[k] + (long)j * n;
-            if (ss[u] != 0.0) {
-              ap[u] = Am[ip[u]];
-              aq[u] = Am[iq[u]];
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (ss[u] == 0.0) continue;
-          Am[ip[u]] = cc[u] * ap[u] - ss[u] * aq[u];
-          Am[iq[u]] = ss[u] * ap[u] + cc[u] * aq[u];
+          Cw[ip[u]] = cc[u] * cp[u] - ss[u] * cq[u];
+          Cw[iq[u]] = ss[u] * cp[u] + cc[u] * cq[u];
         }
       }
       __syncthreads();
@@ -278,48 +346,37 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
     if (nrot == 0) break;
     __syncthreads();
   }
-  // ascending eigenvalues (stable ranks), C = L^-T V(:, perm)
+  // ascending eigenvalues (stable ranks), C(:, rank) = Cw(:, i)
   for (int i = tid; i < n; i += NT) {
-    const double di = Am[i + (long)i * n];
+    const double di = A[i + (long)i * n];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-      const double dj = Am[j + (long)j * n];
+      const double dj = A[j + (long)j * n];
       rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
     }
     perm[rank] = i;
     wm[rank] = di;
   }
   __syncthreads();
-  for (int c = tid; c < n; c += NT) {
-    const int pc = perm[c];
-    for (int i = n - 1; i >= 0; --i) {
-      const int len = n - 1 - i;
-      const double acc = Vm[i + (long)pc * n] -
-                         dot4(Lm + (i + 1) + (long)i * n, 1, Cm + (i + 1) + (long)c * n, 1, len);
-      Cm[i + (long)c * n] = acc / Lm[i + (long)i * n];
-    }
+  for (long e = tid; e < nn; e += NT) {
+    const int i = (int)(e % n), c = (int)(e / n);
+    Cm[e] = Cw[i + (long)perm[c] * n];
   }
   if (tid == 0) info[m] = 0;
 }
 
 // P_m = C_m diag(f) C_m^T over the first nocc[m] columns (lower and upper)
-__global__ __launch_bounds__(256) void batch_density_kernel(BatchLayout L, const double* C,
-                                                            const double* f, const int* nocc,
-                                                            double* P, const int* active) {
-  const int m = blockIdx.x;
+__global__ __launch_bounds__(kSygvThreads) void batch_density_kernel(BatchLayout L,
+                                                                     const double* C,
+                                                                     const double* f,
+                                                                     const int* nocc, double* P,
+                                                                     const int* active) {
+  const int m = L.order[blockIdx.x];
   if (active && !active[m]) return;
+  __shared__ double Xs[kGK][kGT + 1], Ys[kGK][kGT + 1];
   const long base = L.dOff[m];
   const int n = (int)(L.orb0[m + 1] - L.orb0[m]);
-  const double* Cm = C + base;
-  const double* fm = f + L.orb0[m];
-  double* Pm = P + base;
-  const int k = nocc[m];
-  for (long e = threadIdx.x; e < (long)n * n; e += 256) {
-    const int i = (int)(e % n), j = (int)(e / n);
-    double acc = 0.0;
-    for (int l = 0; l < k; ++l) acc += Cm[i + (long)l * n] * fm[l] * Cm[j + (long)l * n];
-    Pm[e] = acc;
-  }
+  wg_gemm<false, true>(n, nocc[m], C + base, C + base, f + L.orb0[m], P + base, Xs, Ys);
 }
 
 // gamma_m: 1/r off the diagonal minus the short-range expGamma part within its cutoff (on the
@@ -417,8 +474,8 @@ int launch_batch_sygv(hipStream_t s, const BatchLayout& L, const double* S, cons
 int launch_batch_density(hipStream_t s, const BatchLayout& L, const double* C, const double* f,
                          const int* nocc, double* P, const int* active) {
   if (L.nmol == 0) return 0;
-  hipLaunchKernelGGL(batch_density_kernel, dim3(L.nmol), dim3(256), 0, s, L, C, f, nocc, P,
-                     active);
+  hipLaunchKernelGGL(batch_density_kernel, dim3(L.nmol), dim3(kSygvThreads), 0, s, L, C, f, nocc,
+                     P, active);
   DH_LAUNCH_CHECK();
   return 0;
 }

@@ -1738,7 +1738,7 @@ struct BatchContext {
   long nnz = 0, denseSize = 0, gammaSize = 0;
   DevBuf<double> coordsD, h0, ov, hs, rho, erho, vorb, vatom, dq, qorb, eorb, grad, eatom, gamma;
   DevBuf<double> Sd, Hd, Cv, P, work, w, f;
-  DevBuf<int> atom0D, atomMolD, infoD, noccD, activeD;
+  DevBuf<int> atom0D, atomMolD, infoD, noccD, activeD, orderD;
   DevBuf<long> orb0D, dOffD, gOffD;
   BatchLayout L{};
   int it = 0, nactive = 0;
@@ -1831,7 +1831,14 @@ static int batch_geometry(BatchContext& b, int nmol, const int* natMol, const do
   DH_TRY(b.orb0D.upload(b.orb0.data(), b.orb0.size(), s));
   DH_TRY(b.dOffD.upload(b.dOff.data(), b.dOff.size(), s));
   DH_TRY(b.gOffD.upload(b.gOff.data(), b.gOff.size(), s));
+  std::vector<int> order(nmol);
+  for (int m = 0; m < nmol; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return b.orb0[x + 1] - b.orb0[x] > b.orb0[y + 1] - b.orb0[y];
+  });
+  DH_TRY(b.orderD.upload(order.data(), order.size(), s));
   b.L.nmol = nmol;
+  b.L.order = b.orderD.p;
   b.L.atom0 = b.atom0D.p;
   b.L.orb0 = b.orb0D.p;
   b.L.dOff = b.dOffD.p;

@@ -18,6 +18,14 @@
 namespace dftbhip {
 
 constexpr int kNB = 64;      // panel width
+// Panel products t1 = W^T v, t2 = V^T v of the column being reduced: one 128-byte line per entry
+// (the SYMV's workgroups add into them atomically; entries sharing a line would serialize those
+// atomics) and two columns in a ring, the next column's entries cleared by col_update.
+constexpr int kT12Pad = 16;
+__host__ __device__ inline long t12_at(int c, int p) {
+  return ((long)(c & 1) * 2 * kNB + p) * kT12Pad;
+}
+constexpr long kT12Size = 2L * 2 * kNB * kT12Pad;
 
 // Optional per-launch timing of the symmetric product kernel (HIP events on the launch stream):
 // launches, summed duration and the algorithmic bytes (lower triangle of the trailing matrix).
@@ -56,14 +64,25 @@ static int symv_prof_collect(hipStream_t s) {
   return 0;
 }
 constexpr int kTB = 64;      // SYMV tile
-constexpr int kCHMin = 4;    // fewest row tiles per SYMV workgroup (sizes the slot array)
-// row tiles per SYMV workgroup (4, 8 or 16), set through set_symv_tiles
-static int g_symvTiles = 8;
+constexpr int kSymvTiles8From = 7000;   // trailing sizes from which 8 tiles per workgroup
+constexpr int kSymvTiles4From = 2500;   // ... and from which 4
+constexpr int kCHMin = 2;    // fewest row tiles per SYMV workgroup (sizes the slot array)
+// row tiles per SYMV workgroup (2, 4, 8 or 16; 0: chosen from the trailing size), set through
+// set_symv_tiles
+static int g_symvTiles = 0;
 
 int set_symv_tiles(int tiles) {
-  DH_CHECK(tiles == 4 || tiles == 8 || tiles == 16, "symv tiles must be 4, 8 or 16");
+  DH_CHECK(tiles == 0 || tiles == 2 || tiles == 4 || tiles == 8 || tiles == 16,
+           "symv tiles must be 0 (by size), 2, 4, 8 or 16");
   g_symvTiles = tiles;
   return 0;
+}
+
+// Long tile chains keep more loads in flight per workgroup on a large trailing matrix; a small
+// one has too few column blocks to fill the chip, and short chains there give more workgroups.
+static int symv_tiles(int m) {
+  if (g_symvTiles != 0) return g_symvTiles;
+  return m >= kSymvTiles8From ? 8 : (m >= kSymvTiles4From ? 4 : 2);
 }
 
 template <typename T>
@@ -73,7 +92,7 @@ struct SytrdWs {
   T* vbuf;         // n
   double* nrm2;    // n   ||x||^2 per column
   double* dot;     // n   w.v per column (before the rank-one correction of w)
-  double* t12;     // n x 2kNB
+  double* t12;     // panel products, see t12_at
   double* scal;    // n
   T* P;            // n x 2kNB
   T* Q;            // n x 2kNB
@@ -101,7 +120,7 @@ static size_t sytrd_layout(int n, SytrdWs<T>* ws, char* base) {
   take(&w->vbuf, sizeof(T) * n);
   take(&w->nrm2, sizeof(double) * n);
   take(&w->dot, sizeof(double) * n);
-  take(&w->t12, sizeof(double) * n * 2 * kNB);
+  take(&w->t12, sizeof(double) * kT12Size);
   take(&w->scal, sizeof(double) * n);
   take(&w->P, sizeof(T) * n * 2 * kNB);
   take(&w->Q, sizeof(T) * n * 2 * kNB);
@@ -161,7 +180,8 @@ __device__ __forceinline__ double wcorr(int cp, const T* tau, const double* dot)
 template <typename T>
 __global__ __launch_bounds__(256) void col_update_kernel(T* a, long lda, int n, int c, int j0,
                                                          const T* W, long ldw, const T* tau,
-                                                         const double* dot, double* nrm2) {
+                                                         const double* dot, double* nrm2,
+                                                         double* t12) {
   __shared__ double vc[kNB], wc[kNB], al[kNB], sh[4];
   const int i = c - j0;
   if (threadIdx.x < i) {
@@ -170,15 +190,32 @@ __global__ __launch_bounds__(256) void col_update_kernel(T* a, long lda, int n, 
     vc[p] = a[c + (long)(j0 + p) * lda];
     wc[p] = W[c + (long)p * ldw] + al[p] * vc[p];
   }
+  // the panel products of column c + 1 start from zero (their ring entry was column c - 1's,
+  // whose readers have finished)
+  if (blockIdx.x == 0)
+    for (int e = threadIdx.x; e < 2 * kNB; e += blockDim.x) t12[t12_at(c + 1, e)] = 0.0;
   __syncthreads();
   const int r = c + blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0;
   if (r < n) {
     double v = a[r + (long)c * lda];
-    for (int p = 0; p < i; ++p) {
-      const double vr = a[r + (long)(j0 + p) * lda];
-      const double wr = W[r + (long)p * ldw] + al[p] * vr;
-      v -= vr * wc[p] + wr * vc[p];
+    // sixteen reflectors at a time with their loads ahead of the products
+    for (int p0 = 0; p0 < i; p0 += 16) {
+      double vr[16], wr[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        vr[q] = 0.0;
+        wr[q] = 0.0;
+        if (p0 + q < i) {
+          vr[q] = a[r + (long)(j0 + p0 + q) * lda];
+          wr[q] = W[r + (long)(p0 + q) * ldw];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int p = p0 + q;
+        if (p < i) v -= vr[q] * wc[p] + (wr[q] + al[p] * vr[q]) * vc[p];
+      }
     }
     a[r + (long)c * lda] = (T)v;
     v = (double)(T)v;
@@ -227,22 +264,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
   const int cols0 = c + 1 + J * kTB;
   if (ch == 0) {
     if (tid < kTB && J * kTB + tid < m) vbuf[cols0 + tid] = vJs[tid];
+    // panel products over this block's rows: t12[p] += W(:, p)^T v, t12[i + p] += V(:, p)^T v
+    // for p < i (wave cg takes p = cg + 4q). All loads are issued before the reductions and
+    // atomics, which would otherwise order every load behind the previous atomic.
     const int i = c - j0;
-    for (int p = cg; p < 2 * i; p += 4) {
-      double prod = 0.0;
-      if (J * kTB + ii < m) {
-        const int r = cols0 + ii;
-        double x;
-        if (p < i) {
-          const double vr = a[r + (long)(j0 + p) * lda];
-          x = (double)W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dotA) * vr;
-        } else {
-          x = a[r + (long)(j0 + p - i) * lda];
-        }
-        prod = x * (double)vJs[ii];
+    const bool rowOk = J * kTB + ii < m;
+    const int r = cols0 + ii;
+    const double vv = (double)vJs[ii];
+    double pw[kNB / 4], pv[kNB / 4];
+#pragma unroll
+    for (int q = 0; q < kNB / 4; ++q) {
+      const int p = cg + 4 * q;
+      pw[q] = 0.0;
+      pv[q] = 0.0;
+      if (p < i && rowOk) {
+        const double vr = a[r + (long)(j0 + p) * lda];
+        pv[q] = vr * vv;
+        pw[q] = ((double)W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dotA) * vr) * vv;
       }
-      prod = wave_sum(prod);
-      if (ii == 0 && prod != 0.0) atomicAdd(&t12[(long)c * 2 * kNB + p], prod);
+    }
+#pragma unroll
+    for (int q = 0; q < kNB / 4; ++q) {
+      const int p = cg + 4 * q;
+      if (p < i) {
+        const double sw = wave_sum(pw[q]), sv = wave_sum(pv[q]);
+        if (ii == 0 && sw != 0.0) atomicAdd(&t12[t12_at(c, p)], sw);
+        if (ii == 0 && sv != 0.0) atomicAdd(&t12[t12_at(c, i + p)], sv);
+      }
     }
   }
   const T* vj = vJs + cg * 16;   // v of this thread's 16 columns (LDS broadcast reads)
@@ -339,12 +387,45 @@ __global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, 
   if (k < m) {
     const int Ir = k / kTB;
     const int nch = (nRB - Ir + tiles - 1) / tiles;
-    for (int sl = wv; sl <= Ir; sl += 4) acc += Y[(long)sl * n + k];
-    for (int sl = wv; sl < nch; sl += 4) acc += Y[(long)(nRB + sl) * n + k];
-    for (int p = wv; p < i; p += 4) {
-      const double vr = a[r + (long)(j0 + p) * lda];
-      const double wr = (double)W[r + (long)p * ldw] + wcorr(j0 + p, tauA, dot) * vr;
-      acc -= vr * t12[(long)c * 2 * kNB + p] + wr * t12[(long)c * 2 * kNB + i + p];
+    // slots with four independent sums so that the loads overlap
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int sl = wv;
+    for (; sl + 12 <= Ir; sl += 16) {
+      s0 += Y[(long)sl * n + k];
+      s1 += Y[(long)(sl + 4) * n + k];
+      s2 += Y[(long)(sl + 8) * n + k];
+      s3 += Y[(long)(sl + 12) * n + k];
+    }
+    for (; sl <= Ir; sl += 4) s0 += Y[(long)sl * n + k];
+    const T* Yc = Y + (long)nRB * n;
+    sl = wv;
+    for (; sl + 12 < nch; sl += 16) {
+      s1 += Yc[(long)sl * n + k];
+      s2 += Yc[(long)(sl + 4) * n + k];
+      s3 += Yc[(long)(sl + 8) * n + k];
+      s0 += Yc[(long)(sl + 12) * n + k];
+    }
+    for (; sl < nch; sl += 4) s1 += Yc[(long)sl * n + k];
+    acc = (s0 + s1) + (s2 + s3);
+    // panel correction of this wave's p = wv + 4q, loads ahead of the products
+    double vr[kNB / 4], wr[kNB / 4];
+#pragma unroll
+    for (int q = 0; q < kNB / 4; ++q) {
+      const int p = wv + 4 * q;
+      vr[q] = 0.0;
+      wr[q] = 0.0;
+      if (p < i) {
+        vr[q] = a[r + (long)(j0 + p) * lda];
+        wr[q] = W[r + (long)p * ldw];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kNB / 4; ++q) {
+      const int p = wv + 4 * q;
+      if (p < i) {
+        const double w = wr[q] + wcorr(j0 + p, tauA, dot) * vr[q];
+        acc -= vr[q] * t12[t12_at(c, p)] + w * t12[t12_at(c, i + p)];
+      }
     }
   }
   red[wv][lane] = acc;
@@ -398,7 +479,7 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
   sytrd_layout<T>(n, &ws, static_cast<char*>(work));
   DH_TRY(fill<double>(s, ws.nrm2, n, 0.0));
   DH_TRY(fill<double>(s, ws.dot, n, 0.0));
-  DH_TRY(fill<double>(s, ws.t12, (size_t)n * 2 * kNB, 0.0));
+  DH_TRY(fill<double>(s, ws.t12, (size_t)kT12Size, 0.0));
   DH_TRY(fill<T>(s, e, std::max(n - 1, 1), T(0)));
   DH_TRY(fill<T>(s, tau, std::max(n, 1), T(0)));
   const long ldw = n;
@@ -407,12 +488,12 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
     for (int i = 0; i < pw; ++i) {
       const int c = j0 + i;
       hipLaunchKernelGGL(col_update_kernel<T>, dim3(cdiv(n - c, 256)), dim3(256), 0, s, a, lda, n,
-                         c, j0, ws.W, ldw, tau, ws.dot, ws.nrm2);
+                         c, j0, ws.W, ldw, tau, ws.dot, ws.nrm2, ws.t12);
       DH_LAUNCH_CHECK();
       const int m = n - c - 1;
       if (m <= 0) continue;
       const int nRB = (m + kTB - 1) / kTB;
-      const int tiles = g_symvTiles;
+      const int tiles = symv_tiles(m);
       const int nCh = (nRB + tiles - 1) / tiles;
       SymvProfile& pf = g_symvProf;
       if (pf.on) {
@@ -428,7 +509,8 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
   hipLaunchKernelGGL((symv_kernel<T, CH_>), dim3(nRB, nCh), dim3(256), 0, s, a, lda, n, c, j0,  \
                      ws.W, ldw, ws.nrm2, tau, ws.dot, ws.Y, ws.nSlot, ws.vbuf, ws.t12, ws.scal, \
                      e, tau)
-      if (tiles == 4) DH_SYMV(4);
+      if (tiles == 2) DH_SYMV(2);
+      else if (tiles == 4) DH_SYMV(4);
       else if (tiles == 16) DH_SYMV(16);
       else DH_SYMV(8);
 #undef DH_SYMV

@@ -40,8 +40,9 @@ int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz);
  * (the reduction DFTB+ reaches through the dsygvd / dsyevd routes of
  * src/dftbp/math/eigensolver.F90:561-660). */
 int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e, double* tau);
-/* Kernel tuning. param 0: row tiles per workgroup of the tridiagonalisation's SYMV (4, 8, 16;
- * default 8). No counterpart in the reference. */
+/* Kernel tuning. param 0: row tiles per workgroup of the tridiagonalisation's SYMV (2, 4, 8,
+ * 16, or 0 = chosen from the trailing size per column, the default). No counterpart in the
+ * reference. */
 int dftbhip_set_tuning(int param, int value);
 /* LAPACK dlacpy (uplo = 'A'): b = a through the device copy kernel (the copy used for HBM
  * counter calibration; DFTB+ copies square matrices with plain array assignment). */

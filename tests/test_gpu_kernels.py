@@ -143,7 +143,7 @@ def test_sparse_kernels(system, request):
     assert np.max(np.abs(q - qref)) < 1e-13
 
 
-@pytest.mark.parametrize("tiles", [4, 8, 16])
+@pytest.mark.parametrize("tiles", [2, 4, 8, 16, 0])
 @pytest.mark.parametrize("n", [1, 2, 77, 700])
 def test_dsytrd_lapack_conventions(n, tiles):
     """Tridiagonalisation against LAPACK's own dsytrd ('L'): d, e, tau and the reflectors, and
@@ -158,7 +158,7 @@ def test_dsytrd_lapack_conventions(n, tiles):
     try:
         r, d, e, tau = api.dsytrd(a)
     finally:
-        api.set_tuning(0, 8)
+        api.set_tuning(0, 0)
     scale = max(1.0, np.abs(a).max())
     tol = 1e-12 * max(n, 1) * scale
     ref, dr, er, taur, info = lapack.dsytrd(a, lower=1)
