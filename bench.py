@@ -35,6 +35,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SCC iter time + force evals/s @16k orbs: mixed vs FP64 GPU at ΔE<1e-6 Ha/atom"
 HBM_PEAK_GBS = 8000.0
+# FP64 symv_kernel HBM traffic per launch from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE)
+# of `bench.py --mode sytrd --n 2048`, each scaled by the counter's reading for the copy kernel
+# of the same run (FETCH_SIZE reads 0.5005x the copied bytes, WRITE_SIZE 1.000x):
+# profiles/symv_hbm_counters_n2048.md
+SYMV_TRAFFIC_N2048 = {"n": 2048, "alg_bytes_per_launch": 5611520, "hbm_read_bytes": 7212843,
+                      "hbm_write_bytes": 183350, "traffic": 7396194, "traffic_over_alg": 1.318,
+                      "fetch_scale": 1.998, "write_scale": 1.000}
 LEGS = {"fp64": dict(precision=0),
         "mixed_fp32": dict(precision=1, refine_below=0.0),
         "mixed_refined": dict(precision=1, refine_below=float("inf"), refine_steps=2)}
@@ -293,7 +300,12 @@ def main():
         "roofline": {"kernel": "symv_kernel (tridiagonalisation, lower-triangle symmetric product, "
                                + ("FP32" if fp32 else "FP64") + ")",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     # the counter runs could not be taken at this size (the profiler's counter
+                     # mode ends in a host-side segfault from n = 4096 up); their n = 2048
+                     # figures follow
+                     "traffic": None,
+                     "traffic_n2048": SYMV_TRAFFIC_N2048 if not fp32 else None,
                      "launches": sv["launches"],
                      "avg_launch_us": 1e6 * sv["seconds"] / sv["launches"] if sv["launches"] else None,
                      "alg_bytes_per_launch": sv["bytes"] / sv["launches"] if sv["launches"] else None},
