@@ -198,14 +198,15 @@ def mode_batch(args):
 
 
 def mode_kpoints(args):
-    """SCC iterations and one force evaluation of a periodic workload with the 2x2x2
-    Monkhorst-Pack mesh (8 complex k-points)."""
+    """SCC iterations and one force evaluation of a periodic workload with a 2x2x2 mesh:
+    Gamma-centred (k in {0, 1/2}^3, the eight time-reversal-invariant points, whose H(k) is
+    real) or shifted Monkhorst-Pack (k in {-1/4, 1/4}^3, complex)."""
     import dataclasses
     import numpy as np
     from dftbplus_amd import api, synth
     geo, params, opts = synth.workload(args.workload)
-    mesh = np.array([[a, b, c] for a in (-0.25, 0.25) for b in (-0.25, 0.25)
-                     for c in (-0.25, 0.25)])
+    pts = (0.0, 0.5) if args.kmesh == "gc" else (-0.25, 0.25)
+    mesh = np.array([[a, b, c] for a in pts for b in pts for c in pts])
     opts = dataclasses.replace(opts, kpoints=mesh)
     t0 = time.perf_counter()
     dp = api.DftbPlus(params, geo, opts)
@@ -224,7 +225,7 @@ def mode_kpoints(args):
     dp.get_gradients()
     force_s = time.perf_counter() - t2
     print(json.dumps({"mode": "kpoints", "workload": args.workload, "atoms": geo.natom,
-                      "orbitals": n, "kpoints": len(mesh), "setup_s": setup,
+                      "orbitals": n, "kmesh": args.kmesh, "kpoints": len(mesh), "setup_s": setup,
                       "scc_iteration_s": times, "force_eval_s": force_s,
                       "phase_s": dp.timings()}), flush=True)
     dp.close()
@@ -246,6 +247,8 @@ def main():
                          "and a force evaluation with a k-point mesh")
     ap.add_argument("--n", type=int, default=8192, help="matrix size of --mode sytrd")
     ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
+    ap.add_argument("--kmesh", default="gc", choices=["gc", "mp"],
+                    help="2x2x2 mesh of --mode kpoints: Gamma-centred (gc) or shifted (mp)")
     ap.add_argument("--tiles", type=int, default=0,
                     help="SYMV row tiles per workgroup for --mode sytrd (0: by trailing size)")
     args = ap.parse_args()

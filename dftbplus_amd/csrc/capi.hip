@@ -1275,8 +1275,10 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
   DH_TRY(c.feps.alloc(n));
   DH_TRY(c.grad.alloc(3 * (size_t)nat));
   DH_TRY(c.eatom.alloc(nat));
-  // k-points: complex problems as their real embedding of size 2n
-  const int nd = c.nk > 0 ? 2 * n : n;
+  // k-points: complex problems as their real embedding of size 2n; time-reversal-invariant
+  // points (2k integral) as real n x n problems
+  int nd = n;
+  for (int k = 0; k < c.nk; ++k) nd = std::max(nd, kpoint_dim(&c.kpts[3 * k], n));
   DH_CHECK(c.nk == 0 || c.periodic, "set_geometry: k-points need lattice vectors");
   DH_TRY(c.Hd.alloc((size_t)nd * nd));
   DH_TRY(c.Sd.alloc((size_t)nd * nd));
@@ -1355,13 +1357,15 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
     c.fillK.assign(c.nk, std::vector<double>());
     c.noccK.assign(c.nk, 0);
     for (int k = 0; k < c.nk && info == 0; ++k) {
+      const int ndk = kpoint_dim(&c.kpts[3 * k], n);
       DH_TRY(launch_unpack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
-                               &c.kpts[3 * k], c.ov.p, c.Sd.p, nd, n));
-      DH_TRY(c.eig.factor_overlap(s, c.Sd.p, nd, &info));
+                               &c.kpts[3 * k], c.ov.p, c.Sd.p, ndk, n));
+      DH_TRY(c.eig.ensure(ndk));
+      DH_TRY(c.eig.factor_overlap(s, c.Sd.p, ndk, &info));
       if (info != 0) break;
-      DH_TRY(c.LinvK[k].alloc((size_t)nd * nd));
-      DH_TRY(c.XK[k].alloc((size_t)nd * nd));
-      DH_TRY(copy2d<double>(s, c.eig.Linv.p, nd, c.LinvK[k].p, nd, nd, nd));
+      DH_TRY(c.LinvK[k].alloc((size_t)ndk * ndk));
+      DH_TRY(c.XK[k].alloc((size_t)ndk * ndk));
+      DH_TRY(copy2d<double>(s, c.eig.Linv.p, ndk, c.LinvK[k].p, ndk, ndk, ndk));
     }
   }
   if (info != 0) {
@@ -1449,23 +1453,26 @@ static int ctx_scc_step(Context& c) {
   const bool mixed = o.precision == 1;
   const bool refined = mixed && st.prevErr <= o.refine_below;
   if (c.nk > 0) {
-    // k-points: per k the embedded H(k), eigenpairs and all eigenvectors; then the k-weighted
-    // Fermi level and the density matrix k by k (packHS_cmplx_kpts)
-    const int nd = 2 * n;
+    // k-points: per k H(k) (real n x n at time-reversal-invariant points, else the 2n
+    // embedding), eigenpairs and all eigenvectors; then the k-weighted Fermi level and the
+    // density matrix k by k (packHS_cmplx_kpts)
     c.xf64 = true;
     DH_HIP(hipEventRecord(ev[1], s));
     std::vector<std::vector<double>> ec(c.nk);
     for (int k = 0; k < c.nk; ++k) {
+      const int ndk = kpoint_dim(&c.kpts[3 * k], n);
       DH_TRY(launch_unpack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
-                               &c.kpts[3 * k], c.hs.p, c.Hd.p, nd, n));
-      DH_TRY(copy2d<double>(s, c.LinvK[k].p, nd, c.eig.Linv.p, nd, nd, nd));
+                               &c.kpts[3 * k], c.hs.p, c.Hd.p, ndk, n));
+      DH_TRY(c.eig.ensure(ndk));
+      DH_TRY(copy2d<double>(s, c.LinvK[k].p, ndk, c.eig.Linv.p, ndk, ndk, ndk));
       int info = 0;
-      DH_TRY(c.eig.solve(s, c.Hd.p, nd, c.eigK[k], &info, evt));
+      DH_TRY(c.eig.solve(s, c.Hd.p, ndk, c.eigK[k], &info, evt));
       if (info != 0) return info;
-      DH_TRY(c.eig.vectors(s, nd, c.XK[k].p, nd));
+      DH_TRY(c.eig.vectors(s, ndk, c.XK[k].p, ndk));
       // each complex level appears twice in the embedding
+      const int stride = ndk / n;
       ec[k].resize(n);
-      for (int j = 0; j < n; ++j) ec[k][j] = c.eigK[k][2 * j];
+      for (int j = 0; j < n; ++j) ec[k][j] = c.eigK[k][stride * j];
     }
     DH_HIP(hipEventRecord(ev[2], s));
     const double th = now();
@@ -1473,15 +1480,17 @@ static int ctx_scc_step(Context& c) {
     c.times[kTHost] += now() - th;
     DH_HIP(hipEventRecord(ev[3], s));
     DH_TRY(fill<double>(s, c.rho.p, c.nnz, 0.0));
-    std::vector<double> fe(nd);
+    std::vector<double> fe(2 * (size_t)n);
     for (int k = 0; k < c.nk; ++k) {
+      const int ndk = kpoint_dim(&c.kpts[3 * k], n), stride = ndk / n;
       c.noccK[k] = occupied_levels(c.fillK[k]);
-      for (int j = 0; j < n; ++j) fe[2 * j] = fe[2 * j + 1] = c.fillK[k][j];
-      DH_HIP(hipMemcpyAsync(c.f.p, fe.data(), nd * sizeof(double), hipMemcpyHostToDevice, s));
-      DH_TRY(gemm<double>(s, false, true, nd, nd, 2 * c.noccK[k], 1.0, c.XK[k].p, nd, c.XK[k].p,
-                          nd, 0.0, c.P.p, nd, true, kKFull, c.f.p));
+      for (int j = 0; j < n; ++j)
+        for (int t = 0; t < stride; ++t) fe[stride * j + t] = c.fillK[k][j];
+      DH_HIP(hipMemcpyAsync(c.f.p, fe.data(), ndk * sizeof(double), hipMemcpyHostToDevice, s));
+      DH_TRY(gemm<double>(s, false, true, ndk, ndk, stride * c.noccK[k], 1.0, c.XK[k].p, ndk,
+                          c.XK[k].p, ndk, 0.0, c.P.p, ndk, true, kKFull, c.f.p));
       DH_TRY(launch_pack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
-                             &c.kpts[3 * k], c.kw[k], c.P.p, nd, n, c.rho.p));
+                             &c.kpts[3 * k], c.kw[k], c.P.p, ndk, n, c.rho.p));
       DH_HIP(hipStreamSynchronize(s));   // fe is reused for the next k-point
     }
     c.eigv.clear();
@@ -1663,16 +1672,18 @@ static int ctx_gradients(Context& c, double* grad) {
   std::vector<double> fe(n, 0.0);
   for (int i = 0; i < c.nocc && i < (int)c.eigv.size(); ++i) fe[i] = c.fill[i] * c.eigv[i];
   if (c.nk > 0) {
-    const int nd = 2 * n;
-    std::vector<double> fk(nd);
+    std::vector<double> fk(2 * (size_t)n);
     DH_TRY(fill<double>(s, c.erho.p, c.nnz, 0.0));
     for (int k = 0; k < c.nk; ++k) {
-      for (int j = 0; j < n; ++j) fk[2 * j] = fk[2 * j + 1] = c.fillK[k][j] * c.eigK[k][2 * j];
-      DH_HIP(hipMemcpyAsync(c.f.p, fk.data(), nd * sizeof(double), hipMemcpyHostToDevice, s));
-      DH_TRY(gemm<double>(s, false, true, nd, nd, 2 * c.noccK[k], 1.0, c.XK[k].p, nd, c.XK[k].p,
-                          nd, 0.0, c.P.p, nd, true, kKFull, c.f.p));
+      const int ndk = kpoint_dim(&c.kpts[3 * k], n), stride = ndk / n;
+      for (int j = 0; j < n; ++j)
+        for (int t = 0; t < stride; ++t)
+          fk[stride * j + t] = c.fillK[k][j] * c.eigK[k][stride * j];
+      DH_HIP(hipMemcpyAsync(c.f.p, fk.data(), ndk * sizeof(double), hipMemcpyHostToDevice, s));
+      DH_TRY(gemm<double>(s, false, true, ndk, ndk, stride * c.noccK[k], 1.0, c.XK[k].p, ndk,
+                          c.XK[k].p, ndk, 0.0, c.P.p, ndk, true, kKFull, c.f.p));
       DH_TRY(launch_pack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
-                             &c.kpts[3 * k], c.kw[k], c.P.p, nd, n, c.erho.p));
+                             &c.kpts[3 * k], c.kw[k], c.P.p, ndk, n, c.erho.p));
       DH_HIP(hipStreamSynchronize(s));
     }
   } else if (!c.xf64) {

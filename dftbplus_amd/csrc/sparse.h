@@ -62,8 +62,12 @@ int launch_unpack(hipStream_t s, const PairsDev& pr, const int* norb, const long
 template <typename T>
 int launch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                 const T* dense, long ld, double* x);
-// k-point matrices as their real 2n x 2n embedding (blkT: fractional cell translation of the
-// image of each block, kf: fractional k-point); pack_kpt accumulates into x
+// k-point matrices (blkT: fractional cell translation of the image of each block, kf:
+// fractional k-point): at a time-reversal-invariant point (2 kf integral, kpoint_is_real) the
+// phases are +-1 and the matrix is the real n x n one; otherwise its real 2n x 2n embedding.
+// pack_kpt accumulates into x.
+bool kpoint_is_real(const double* kf);
+int kpoint_dim(const double* kf, int n);   // n or 2n
 int launch_unpack_kpt(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                       const double* blkT, const double* kf, const double* x, double* dense,
                       long ld, int n);

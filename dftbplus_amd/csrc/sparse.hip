@@ -12,8 +12,10 @@
 // Orbital order within an atom: s, p_y, p_z, p_x. Species carry at most two shells with l <= 1,
 // i.e. at most 4 orbitals per atom.
 #include "common.h"
-#include "sparse.h"
 #include "gamma_dev.h"
+#include "sparse.h"
+
+#include <cmath>
 
 namespace dftbhip {
 
@@ -245,14 +247,30 @@ __global__ void pack_kernel(PairsDev pr, const int* norbAtom, const long* iAtomS
     }
 }
 
+// phase factor exp(i 2 pi kf.T) of block k; at a time-reversal-invariant point (m = 2 kf
+// integral) it is (-1)^(m.T), taken exactly from the parity
+__device__ inline void block_phase(const double* blkT, int k, double kx, double ky, double kz,
+                                   int real2k, int m0, int m1, int m2, double& sn, double& cs) {
+  if (real2k) {
+    const long par = llround(m0 * blkT[3 * k] + m1 * blkT[3 * k + 1] + m2 * blkT[3 * k + 2]);
+    cs = (par & 1) ? -1.0 : 1.0;
+    sn = 0.0;
+  } else {
+    sincos(2.0 * M_PI * (kx * blkT[3 * k] + ky * blkT[3 * k + 1] + kz * blkT[3 * k + 2]), &sn,
+           &cs);
+  }
+}
+
 // Complex matrix at the k-point kf (fractional) into its real 2n x 2n embedding
 // [[Re, -Im], [Im, Re]]: blocks times exp(i 2 pi kf.T) at (atom2 rows, atom1 columns) and the
 // Hermitian counterpart; on-site blocks contribute their lower triangle with a real diagonal
-// ('L' storage of unpackHS_cmplx_kpts, sparse2dense.F90:185-262). The dense array is zeroed
+// ('L' storage of unpackHS_cmplx_kpts, sparse2dense.F90:185-262). At a time-reversal-invariant
+// point (real2k) the matrix is real: n x n, both triangles. The dense array is zeroed
 // beforehand.
 __global__ void unpack_kpt_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
                                   const double* blkT, double kx, double ky, double kz,
-                                  const double* x, double* dense, long ld, long n) {
+                                  int real2k, int m0, int m1, int m2, const double* x,
+                                  double* dense, long ld, long n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= pr.nblk) return;
   const int a1 = pr.at1[k], a2 = pr.at2[k];
@@ -260,13 +278,18 @@ __global__ void unpack_kpt_kernel(PairsDev pr, const int* norbAtom, const long* 
   const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
   const long st = pr.start[k];
   double sn, cs;
-  sincos(2.0 * M_PI * (kx * blkT[3 * k] + ky * blkT[3 * k + 1] + kz * blkT[3 * k + 2]), &sn, &cs);
+  block_phase(blkT, k, kx, ky, kz, real2k, m0, m1, m2, sn, cs);
   for (int c = 0; c < n1; ++c)
     for (int r = 0; r < n2; ++r) {
       if (a1 == a2 && r < c) continue;
       const double v = x[st + r + c * n2];
       const double re = v * cs, im = v * sn;
       const long i = o2 + r, j = o1 + c;
+      if (real2k) {
+        atomicAdd(&dense[i + j * ld], re);
+        if (i != j) atomicAdd(&dense[j + i * ld], re);
+        continue;
+      }
       atomicAdd(&dense[i + j * ld], re);
       atomicAdd(&dense[(n + i) + (n + j) * ld], re);
       if (i == j) continue;
@@ -283,8 +306,9 @@ __global__ void unpack_kpt_kernel(PairsDev pr, const int* norbAtom, const long* 
 // embedded 2n x 2n matrix; on-site blocks completed from their lower triangle by conjugation
 // (packHS_cmplx_kpts, sparse2dense.F90:820-918)
 __global__ void pack_kpt_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
-                                const double* blkT, double kx, double ky, double kz, double w,
-                                const double* dense, long ld, long n, double* x) {
+                                const double* blkT, double kx, double ky, double kz, int real2k,
+                                int m0, int m1, int m2, double w, const double* dense, long ld,
+                                long n, double* x) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= pr.nblk) return;
   const int a1 = pr.at1[k], a2 = pr.at2[k];
@@ -292,18 +316,18 @@ __global__ void pack_kpt_kernel(PairsDev pr, const int* norbAtom, const long* iA
   const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
   const long st = pr.start[k];
   double sn, cs;
-  sincos(2.0 * M_PI * (kx * blkT[3 * k] + ky * blkT[3 * k + 1] + kz * blkT[3 * k + 2]), &sn, &cs);
+  block_phase(blkT, k, kx, ky, kz, real2k, m0, m1, m2, sn, cs);
   for (int c = 0; c < n1; ++c)
     for (int r = 0; r < n2; ++r) {
-      double re, im;
+      double re, im = 0.0;
       if (a1 == a2 && r < c) {
         const long i = o2 + c, j = o1 + r;
         re = dense[i + j * ld];
-        im = -dense[(n + i) + j * ld];
+        if (!real2k) im = -dense[(n + i) + j * ld];
       } else {
         const long i = o2 + r, j = o1 + c;
         re = dense[i + j * ld];
-        im = dense[(n + i) + j * ld];
+        if (!real2k) im = dense[(n + i) + j * ld];
       }
       x[st + r + c * n2] += w * (re * cs + im * sn);
     }
@@ -663,13 +687,26 @@ int launch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* 
   return 0;
 }
 
+bool kpoint_is_real(const double* kf) {
+  for (int q = 0; q < 3; ++q)
+    if (std::fabs(2.0 * kf[q] - std::nearbyint(2.0 * kf[q])) > 1e-12) return false;
+  return true;
+}
+
+int kpoint_dim(const double* kf, int n) { return kpoint_is_real(kf) ? n : 2 * n; }
+
 int launch_unpack_kpt(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                       const double* blkT, const double* kf, const double* x, double* dense,
                       long ld, int n) {
-  DH_TRY(fill2d<double>(s, dense, ld, 2 * n, 2 * n, 0.0));
+  const int nd = kpoint_dim(kf, n);
+  DH_CHECK(ld >= nd, "unpack_kpt: leading dimension too small");
+  DH_TRY(fill2d<double>(s, dense, ld, nd, nd, 0.0));
   if (pr.nblk == 0) return 0;
+  const int re = kpoint_is_real(kf) ? 1 : 0;
   hipLaunchKernelGGL(unpack_kpt_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
-                     iAtomStart, blkT, kf[0], kf[1], kf[2], x, dense, ld, (long)n);
+                     iAtomStart, blkT, kf[0], kf[1], kf[2], re, (int)std::nearbyint(2.0 * kf[0]),
+                     (int)std::nearbyint(2.0 * kf[1]), (int)std::nearbyint(2.0 * kf[2]), x,
+                     dense, ld, (long)n);
   DH_LAUNCH_CHECK();
   return 0;
 }
@@ -678,8 +715,12 @@ int launch_pack_kpt(hipStream_t s, const PairsDev& pr, const int* norb, const lo
                     const double* blkT, const double* kf, double w, const double* dense, long ld,
                     int n, double* x) {
   if (pr.nblk == 0) return 0;
+  DH_CHECK(ld >= kpoint_dim(kf, n), "pack_kpt: leading dimension too small");
+  const int re = kpoint_is_real(kf) ? 1 : 0;
   hipLaunchKernelGGL(pack_kpt_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
-                     iAtomStart, blkT, kf[0], kf[1], kf[2], w, dense, ld, (long)n, x);
+                     iAtomStart, blkT, kf[0], kf[1], kf[2], re, (int)std::nearbyint(2.0 * kf[0]),
+                     (int)std::nearbyint(2.0 * kf[1]), (int)std::nearbyint(2.0 * kf[2]), w,
+                     dense, ld, (long)n, x);
   DH_LAUNCH_CHECK();
   return 0;
 }

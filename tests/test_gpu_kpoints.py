@@ -1,5 +1,6 @@
-"""k-point SCC (complex Hermitian problems through their real 2n embedding) against the CPU
-oracle's complex path: energies, charges and gradients at the FP64 tolerances."""
+"""k-point SCC against the CPU oracle's complex path, at the FP64 tolerances: complex
+Hermitian problems through their real 2n embedding, time-reversal-invariant points (2k
+integral) as real n x n problems."""
 import dataclasses
 
 import numpy as np
@@ -11,7 +12,9 @@ MESHES = {
     # Monkhorst-Pack 2x2x2 (k = +-1/4: complex phases)
     "mp222": np.array([[a, b, c] for a in (-0.25, 0.25) for b in (-0.25, 0.25)
                        for c in (-0.25, 0.25)]),
-    # Gamma-centred 3x3x3 (k = 0, 1/3, 2/3)
+    # Gamma-centred 2x2x2 (k = 0, 1/2: the eight time-reversal-invariant points, real H(k))
+    "gc222": np.array([[a, b, c] for a in (0, 0.5) for b in (0, 0.5) for c in (0, 0.5)]),
+    # Gamma-centred 3x3x3 (k = 0, 1/3, 2/3: Gamma real, the others complex)
     "gc333": np.array([[a, b, c] for a in (0, 1 / 3, 2 / 3) for b in (0, 1 / 3, 2 / 3)
                        for c in (0, 1 / 3, 2 / 3)]),
 }
