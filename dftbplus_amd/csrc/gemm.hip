@@ -120,12 +120,28 @@ __device__ __forceinline__ void gemm_tile(int tm, int tn, int M, int N, int K, T
   }
   // Output layout of the 16x16 accumulator, column l%16 for lane l; element r sits in row
   // l/16 + 4r for FP64 and in row 4(l/16) + r for FP32 (pinned by the layout probes below).
-  // Each wave stages 16 columns x 64 rows of its sub-tile in LDS and writes them back one
-  // column per store instruction (64 consecutive rows).
+  // Each wave stages 16 columns x 64 rows of its sub-tile in its own LDS area and writes them
+  // back one column per store instruction (64 consecutive rows). The area is private to the
+  // wave, so only the first staging waits for the other waves (their K loop reads the same
+  // LDS); within the wave an LDS-count wait orders the staging before the reads. The 16 C
+  // values a lane updates are loaded before any of its stores.
   __syncthreads();
   T* Cs = smem + wave * 16 * 65;
+  auto wave_lds_sync = [&]() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt untouched (gfx9 encoding)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  };
+  const int gi = i0 + wm * 64 + lane;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
+    const int gj0 = j0 + wn * 64 + n * 16;
+    T cold[16];
+    if (beta != T(0) && nkz == 1) {
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc)
+        cold[cc] = (gi < M && gj0 + cc < N) ? C[gi + (long)(gj0 + cc) * ldc] : T(0);
+    }
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -133,25 +149,24 @@ __device__ __forceinline__ void gemm_tile(int tm, int tn, int M, int N, int K, T
         const int rr = Mfma<T>::kRowMajorQuad ? (lane >> 4) * 4 + r : (lane >> 4) + 4 * r;
         Cs[(lane & 15) * 65 + m * 16 + rr] = acc[m][n][r];
       }
-    __syncthreads();
-    const int gi = i0 + wm * 64 + lane;
+    wave_lds_sync();
     if (gi < M) {
-#pragma unroll 4
+#pragma unroll
       for (int cc = 0; cc < 16; ++cc) {
-        const int gj = j0 + wn * 64 + n * 16 + cc;
+        const int gj = gj0 + cc;
         if (gj < N) {
           T v = alpha * Cs[cc * 65 + lane];
           T* dst = C + gi + (long)gj * ldc;
           if (nkz > 1) {
             atomicAdd(dst, v);
           } else {
-            if (beta != T(0)) v += beta * *dst;
+            if (beta != T(0)) v += beta * cold[cc];
             *dst = v;
           }
         }
       }
     }
-    __syncthreads();
+    wave_lds_sync();   // this group's reads before the next group's staging
   }
 }
 
@@ -164,7 +179,7 @@ __device__ __forceinline__ void lower_tile_index(int x, int& tm, int& tn) {
 }
 
 template <typename T, bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T* A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(int M, int N, int K, T alpha, const T* A,
                                                    long lda, const T* B, long ldb, T beta, T* C,
                                                    long ldc, int lowerOut, int klim, const T* w) {
   int tm, tn;
@@ -179,7 +194,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
 }
 
 template <typename T, bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_batched_kernel(const GemmDesc<T>* desc, T alpha,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_batched_kernel(const GemmDesc<T>* desc, T alpha,
                                                            T beta) {
   const GemmDesc<T> d = desc[blockIdx.z];
   const int tm = blockIdx.x, tn = blockIdx.y;
