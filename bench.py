@@ -46,6 +46,10 @@ LEGS = {"fp64": dict(precision=0),
         "mixed_fp32": dict(precision=1, refine_below=0.0),
         "mixed_refined": dict(precision=1, refine_below=float("inf"), refine_steps=2)}
 E_BOUND, F_BOUND = 1e-6, 1e-5
+# full SCC cycles of --cycle: FP64 throughout, and FP32 iterations until the charge error of
+# the previous iteration is at most 1e-3, plain FP64 iterations after
+CYCLE_LEGS = {"fp64": dict(precision=0),
+              "mixed_then_fp64": dict(precision=1, refine_below=1e-3, refine_steps=0)}
 
 
 def cpu_baseline(sample_cells=5, iters=2):
@@ -115,6 +119,32 @@ def run_leg(api, geo, params, opts, leg, steps, warmup):
         "symv": {"launches": int(launches), "seconds": ksec, "bytes": kbytes,
                  "achieved": achieved},
     }
+
+
+def run_cycle(api, geo, params, opts, leg):
+    """A full SCC cycle from neutral charges and one force evaluation."""
+    tag = f"[bench:cycle:{leg}]"
+    dp = api.DftbPlus(params, geo, opts, **CYCLE_LEGS[leg])
+    dp.scc_begin()
+    t0 = time.perf_counter()
+    it, err = 0, float("inf")
+    while it < opts.max_scc_iter:
+        conv, err = dp.scc_step()
+        it += 1
+        print(f"{tag} iteration {it}: charge error {err:.3e}", file=sys.stderr, flush=True)
+        if conv:
+            break
+    dp.scc_end()
+    scc_s = time.perf_counter() - t0
+    energy, charges = dp.get_energy(), dp.get_gross_charges()
+    t1 = time.perf_counter()
+    grad = dp.get_gradients()
+    force_s = time.perf_counter() - t1
+    ph = dp.timings()
+    dp.close()
+    return {"iterations": it, "converged": bool(conv), "scc_s": scc_s, "force_s": force_s,
+            "energy": energy, "charges": charges, "grad": grad,
+            "refine_s": ph.get("refine", 0.0)}
 
 
 def mode_sytrd(args):
@@ -240,6 +270,8 @@ def main():
     ap.add_argument("--legs", default="fp64,mixed_fp32,mixed_refined",
                     help="comma-separated subset of " + ",".join(LEGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cycle", action="store_true",
+                    help="also run full SCC cycles: FP64 against FP32 iterations then FP64 ones")
     ap.add_argument("--mode", default="scc", choices=["scc", "sytrd", "batch", "kpoints"],
                     help="scc: the headline SCC benchmark (default); sytrd: the tridiagonalisation "
                          "alone on a random matrix (SYMV counter runs); batch: the molecule batch "
@@ -328,6 +360,18 @@ def main():
                                "sample": f"numpy/LAPACK oracle, 2 SCC iterations of the 1000-atom "
                                          f"cell ({ns} orbitals, {dt:.2f} s/iteration) scaled by "
                                          f"(N/{ns})^3 to N={n}"}
+    if args.cycle:
+        cyc = {p: run_cycle(api, geo, params, opts, p) for p in CYCLE_LEGS}
+        c0 = cyc["fp64"]
+        rec["scc_cycle"] = {
+            p: {"iterations": x["iterations"], "converged": x["converged"],
+                "scc_seconds": x["scc_s"], "force_eval_s": x["force_s"],
+                "speedup_vs_fp64": c0["scc_s"] / x["scc_s"],
+                "fp64_iterations_per_s": c0["iterations"] / x["scc_s"],
+                "error_vs_fp64": {"dE_per_atom": abs(x["energy"] - c0["energy"]) / nat,
+                                  "dq_max": float(np.max(np.abs(x["charges"] - c0["charges"]))),
+                                  "dF_max": float(np.max(np.abs(x["grad"] - c0["grad"])))}}
+            for p, x in cyc.items()}
     print(json.dumps(rec), flush=True)
 
 

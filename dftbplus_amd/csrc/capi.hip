@@ -1450,8 +1450,11 @@ static int ctx_scc_step(Context& c) {
   DH_TRY(launch_shift(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.h0.p, c.ov.p, c.vorb.p,
                       c.hs.p));
   double evt[3] = {0, 0, 0};
-  const bool mixed = o.precision == 1;
-  const bool refined = mixed && st.prevErr <= o.refine_below;
+  // mixed precision: below the switch point (charge error of the previous iteration at most
+  // refine_below) the iteration is refined in FP64, or with refine_steps = 0 runs in FP64
+  const bool below = o.precision == 1 && st.prevErr <= o.refine_below;
+  const bool mixed = o.precision == 1 && !(below && o.refine_steps == 0);
+  const bool refined = mixed && below;
   if (c.nk > 0) {
     // k-points: per k H(k) (real n x n at time-reversal-invariant points, else the 2n
     // embedding), eigenpairs and all eigenvectors; then the k-weighted Fermi level and the

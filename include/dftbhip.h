@@ -124,7 +124,8 @@ typedef struct {
   int precision;        /* 0: FP64, 1: mixed (FP32 MFMA bulk + FP64 refinement) */
   /* mixed precision only: an SCC iteration refines its eigenpairs in FP64 (Ogita-Aishima,
    * refine_steps steps, then FP64 density matrices) when the charge error of the previous
-   * iteration is at most refine_below (0: never, +inf: every iteration) */
+   * iteration is at most refine_below (0: never, +inf: every iteration); with refine_steps = 0
+   * such iterations run the plain FP64 path instead */
   double refine_below;
   int refine_steps;
   int tridiag;          /* 0: two-stage (dense -> band -> tridiagonal), 1: one-stage Householder */

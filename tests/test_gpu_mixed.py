@@ -62,3 +62,21 @@ def test_mixed_switch_converges_to_fp64(cluster216):
     assert np.max(np.abs(g - res.gradients)) < 1e-7
     assert t["refine"] > 0.0
     dp.close()
+
+
+def test_mixed_then_fp64_converges_to_fp64(cluster216):
+    """refine_steps = 0: FP32 iterations until the charge error drops below 1e-3, then plain
+    FP64 iterations; the converged result meets the FP64 tolerances."""
+    from dftbplus_amd import api
+    geo, params, opts, calc, res = cluster216
+    dp = api.DftbPlus(params, geo, opts, precision=1, refine_below=1e-3, refine_steps=0)
+    e, q, g = dp.get_energy(), dp.get_gross_charges(), dp.get_gradients()
+    t = dp.timings()
+    nat = geo.natom
+    print(f"FP32 then FP64: it {dp.n_iter} (oracle {res.n_iter})")
+    assert dp.converged
+    assert abs(e - res.energy_mermin) / nat < 1e-8
+    assert np.max(np.abs(q - res.charges)) < 1e-8
+    assert np.max(np.abs(g - res.gradients)) < 1e-7
+    assert t["refine"] == 0.0
+    dp.close()
