@@ -597,25 +597,41 @@ __global__ void form_v_kernel(const T* a, long lda, int n, int j0, int pw, int s
   V[k + (long)p * ldv] = v;
 }
 
-// T(i,i) = tau_i, T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), G = V^T V   (LAPACK dlarft 'F','C')
+// T(i,i) = tau_i, T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), G = V^T V   (LAPACK dlarft 'F','C').
+// The strict upper triangle of G is read once into LDS; T is built in LDS, both packed by
+// columns (entry (r, c), r <= c, at r + c(c+1)/2), so the recurrence makes no global accesses.
+constexpr int kOBPacked = kOB * (kOB + 1) / 2;
 template <typename T>
 __global__ __launch_bounds__(128) void larft_kernel(const T* G, const T* tau, int j0, int pw,
                                                     T* Tout) {
-  __shared__ T Ts[kOB][kOB + 1];
+  __shared__ T Tp[kOBPacked];
+  __shared__ T Gp[kOBPacked];
   const int t = threadIdx.x;
-  for (int e = t; e < kOB * kOB; e += blockDim.x) Ts[e % kOB][e / kOB] = 0.0;
+  for (int e = t; e < pw * pw; e += blockDim.x) {
+    const int r = e % pw, c = e / pw;
+    if (r < c) Gp[r + c * (c + 1) / 2] = G[r + (long)c * kOB];
+  }
   __syncthreads();
   for (int i = 0; i < pw; ++i) {
     const T ti = tau[j0 + i];
+    const int ci = i * (i + 1) / 2;
     if (t < i) {
       T acc = T(0);
-      for (int k = t; k < i; ++k) acc += Ts[t][k] * G[k + i * kOB];
-      Ts[t][i] = -ti * acc;
+      int ck = t * (t + 1) / 2;
+      for (int k = t; k < i; ++k) {
+        acc += Tp[t + ck] * Gp[k + ci];
+        ck += k + 1;
+      }
+      Tp[t + ci] = -ti * acc;
+    } else if (t == i) {
+      Tp[i + ci] = ti;
     }
-    if (t == 0) Ts[i][i] = ti;
     __syncthreads();
   }
-  for (int e = t; e < kOB * kOB; e += blockDim.x) Tout[e] = Ts[e % kOB][e / kOB];
+  for (int e = t; e < kOB * kOB; e += blockDim.x) {
+    const int r = e % kOB, c = e / kOB;
+    Tout[e] = (r <= c && c < pw) ? Tp[r + c * (c + 1) / 2] : T(0);
+  }
 }
 
 template <typename T>
