@@ -169,22 +169,34 @@ __global__ void rotate_kernel(const MergeDev* md, const int* rp, const int* rn, 
   }
 }
 
-__device__ __forceinline__ double secular_f(const double* DL, const double* W, int k, double rho,
-                                            double dorg, double tau) {
+// Sums over the k terms of a merge are split over a group of kSecG lanes (lane sub takes
+// i = sub, sub + kSecG, ...) and combined by an xor butterfly, which leaves the same total in
+// every lane of the group (IEEE addition is commutative), so the group's decisions agree.
+constexpr int kSecG = 16;
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int s = kSecG / 2; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
+  return v;
+}
+
+__device__ __forceinline__ double secular_f_group(const double* DL, const double* W, int k,
+                                                  double rho, double dorg, double tau, int sub) {
   double acc = 0.0;
-  for (int i = 0; i < k; ++i) {
+  for (int i = sub; i < k; i += kSecG) {
     const double den = (DL[i] - dorg) - tau;
     acc += W[i] * W[i] / den;
   }
-  return 1.0 + rho * acc;
+  return 1.0 + rho * group_sum(acc);
 }
 
-// Roots of 1 + rho sum_i W_i^2 / (DL_i - lambda) = 0, one thread per root.
+// Roots of 1 + rho sum_i W_i^2 / (DL_i - lambda) = 0, one lane group per root.
 __global__ void secular_kernel(const MergeDev* md, const double* DLall, const double* Wall,
                                int* orgOut, double* tauOut, double* lamOut) {
   const MergeDev m = md[blockIdx.y];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m.k) return;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = gid / kSecG, sub = gid % kSecG;
+  if (j >= m.k) return;   // the whole group: j is the same in its lanes
   const double* DL = DLall + m.dlOff;
   const double* W = Wall + m.dlOff;
   const int k = m.k;
@@ -194,7 +206,7 @@ __global__ void secular_kernel(const MergeDev* md, const double* DLall, const do
   bool positive;    // tau > 0
   if (j < k - 1) {
     const double gap = DL[j + 1] - DL[j];
-    const double fm = secular_f(DL, W, k, rho, DL[j], 0.5 * gap);
+    const double fm = secular_f_group(DL, W, k, rho, DL[j], 0.5 * gap, sub);
     if (fm >= 0.0) {
       org = j;
       positive = true;
@@ -205,7 +217,8 @@ __global__ void secular_kernel(const MergeDev* md, const double* DLall, const do
     hi = 0.5 * gap;
   } else {
     double nz = 0.0;
-    for (int i = 0; i < k; ++i) nz += W[i] * W[i];
+    for (int i = sub; i < k; i += kSecG) nz += W[i] * W[i];
+    nz = group_sum(nz);
     org = k - 1;
     positive = true;
     hi = rho * nz;
@@ -216,7 +229,7 @@ __global__ void secular_kernel(const MergeDev* md, const double* DLall, const do
   while (hi_b - lo_b > 1ull) {
     const unsigned long long mid = lo_b + (hi_b - lo_b) / 2;
     const double t = __longlong_as_double((long long)mid);
-    const double f = secular_f(DL, W, k, rho, dorg, positive ? t : -t);
+    const double f = secular_f_group(DL, W, k, rho, dorg, positive ? t : -t, sub);
     // positive: f increases with t, root where f >= 0 first; negative: f(-t) decreases with t
     const bool right = positive ? (f >= 0.0) : (f < 0.0);
     if (right) hi_b = mid;
@@ -225,14 +238,16 @@ __global__ void secular_kernel(const MergeDev* md, const double* DLall, const do
   double t = __longlong_as_double((long long)hi_b);
   if (lo_b > 0ull) {
     const double tl = __longlong_as_double((long long)lo_b);
-    const double fh = fabs(secular_f(DL, W, k, rho, dorg, positive ? t : -t));
-    const double fl = fabs(secular_f(DL, W, k, rho, dorg, positive ? tl : -tl));
+    const double fh = fabs(secular_f_group(DL, W, k, rho, dorg, positive ? t : -t, sub));
+    const double fl = fabs(secular_f_group(DL, W, k, rho, dorg, positive ? tl : -tl, sub));
     if (fl < fh) t = tl;
   }
   const double tau = positive ? t : -t;
-  orgOut[m.dlOff + j] = org;
-  tauOut[m.dlOff + j] = tau;
-  lamOut[m.dlOff + j] = dorg + tau;
+  if (sub == 0) {
+    orgOut[m.dlOff + j] = org;
+    tauOut[m.dlOff + j] = tau;
+    lamOut[m.dlOff + j] = dorg + tau;
+  }
 }
 
 // zhat_i = sign(W_i) sqrt( -(DL_i - lam_i)/rho * prod_{j != i} (DL_i - lam_j)/(DL_i - DL_j) )
@@ -263,21 +278,23 @@ __global__ void zhat_kernel(const MergeDev* md, const double* DLall, const doubl
   zhat[m.dlOff + i] = w >= 0.0 ? a : -a;
 }
 
-// 1 / || zhat / (DL - lam_j) || for each root j
+// 1 / || zhat / (DL - lam_j) || for each root j, one lane group per root
 __global__ void colnorm_kernel(const MergeDev* md, const double* DLall, const int* orgAll,
                                const double* tauAll, const double* zhat, double* inv) {
   const MergeDev m = md[blockIdx.y];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = gid / kSecG, sub = gid % kSecG;
   if (j >= m.k) return;
   const double* DL = DLall + m.dlOff;
   const double dorg = DL[orgAll[m.dlOff + j]];
   const double tj = tauAll[m.dlOff + j];
   double acc = 0.0;
-  for (int i = 0; i < m.k; ++i) {
+  for (int i = sub; i < m.k; i += kSecG) {
     const double v = zhat[m.dlOff + i] / ((DL[i] - dorg) - tj);
     acc += v * v;
   }
-  inv[m.dlOff + j] = 1.0 / sqrt(acc);
+  acc = group_sum(acc);
+  if (sub == 0) inv[m.dlOff + j] = 1.0 / sqrt(acc);
 }
 
 // U(row(i), j) = zhat_i / (DL_i - lam_j) * inv_j, rows in column-group order
@@ -579,14 +596,14 @@ int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double
       DH_LAUNCH_CHECK();
     }
     if (maxk > 0) {
-      const dim3 g1(cdiv(maxk, 64), nm);
-      hipLaunchKernelGGL(secular_kernel, g1, dim3(64), 0, s, dmd.p, dDL.p, dW.p, dorg.p, dtau.p,
+      const dim3 g1(cdiv(maxk, 64), nm), gg(cdiv((long)maxk * kSecG, 64), nm);
+      hipLaunchKernelGGL(secular_kernel, gg, dim3(64), 0, s, dmd.p, dDL.p, dW.p, dorg.p, dtau.p,
                          dlam.p);
       DH_LAUNCH_CHECK();
       hipLaunchKernelGGL(zhat_kernel, g1, dim3(64), 0, s, dmd.p, dDL.p, dW.p, dorg.p, dtau.p,
                          dzhat.p);
       DH_LAUNCH_CHECK();
-      hipLaunchKernelGGL(colnorm_kernel, g1, dim3(64), 0, s, dmd.p, dDL.p, dorg.p, dtau.p,
+      hipLaunchKernelGGL(colnorm_kernel, gg, dim3(64), 0, s, dmd.p, dDL.p, dorg.p, dtau.p,
                          dzhat.p, dinv.p);
       DH_LAUNCH_CHECK();
       hipLaunchKernelGGL(buildu_kernel, dim3(cdiv(maxk, 256), maxk, nm), dim3(256), 0, s, dmd.p,

@@ -616,13 +616,18 @@ __global__ __launch_bounds__(128) void larft_kernel(const T* G, const T* tau, in
     const T ti = tau[j0 + i];
     const int ci = i * (i + 1) / 2;
     if (t < i) {
-      T acc = T(0);
-      int ck = t * (t + 1) / 2;
-      for (int k = t; k < i; ++k) {
-        acc += Tp[t + ck] * Gp[k + ci];
-        ck += k + 1;
+      // eight independent partial sums, so that the LDS reads of a group overlap
+      T acc[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] = T(0);
+      int k = t;
+      for (; k + 8 <= i; k += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += Tp[t + (k + u) * (k + u + 1) / 2] * Gp[k + u + ci];
       }
-      Tp[t + ci] = -ti * acc;
+      for (; k < i; ++k) acc[0] += Tp[t + k * (k + 1) / 2] * Gp[k + ci];
+      Tp[t + ci] = -ti * (((acc[0] + acc[1]) + (acc[2] + acc[3])) +
+                          ((acc[4] + acc[5]) + (acc[6] + acc[7])));
     } else if (t == i) {
       Tp[i + ci] = ti;
     }
