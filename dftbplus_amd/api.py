@@ -117,6 +117,8 @@ EXPORTS = {
     "dftbhip_ctx_scc_end": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_ctx_gradients": [C.c_void_p, _dp],
     "dftbhip_ctx_eigenvalues": [C.c_void_p, _dp, _dp, C.c_int],
+    "dftbhip_ctx_eigenvalues_k": [C.c_void_p, C.c_int, _dp, _dp, C.c_int],
+    "dftbhip_ctx_orbital_populations": [C.c_void_p, _dp, C.c_int],
     "dftbhip_ctx_nr_orbitals": [C.c_void_p],
     "dftbhip_ctx_timings": [C.c_void_p, _dp, C.c_int],
     "dftbhip_ctx_set_kpoints": [C.c_void_p, C.c_int, _dp, _dp],
@@ -505,6 +507,25 @@ class DftbPlus:
         f = np.zeros(n)
         _check(lib().dftbhip_ctx_eigenvalues(self.h, _d(e), _d(f), n), "dftbhip_ctx_eigenvalues")
         return e, f
+
+    def get_eigenvalues_k(self, k: int):
+        """Eigenvalues and fillings (electrons per level) of k-point k (0 without k-points)."""
+        self._ensure_scc()
+        n = self.get_nr_orbitals()
+        e = np.zeros(n)
+        f = np.zeros(n)
+        _check(lib().dftbhip_ctx_eigenvalues_k(self.h, k, _d(e), _d(f), n),
+               "dftbhip_ctx_eigenvalues_k")
+        return e, f
+
+    def get_orbital_populations(self) -> np.ndarray:
+        """Output Mulliken populations per orbital (electrons), atom by atom."""
+        self._ensure_scc()
+        n = self.get_nr_orbitals()
+        q = np.zeros(n)
+        _check(lib().dftbhip_ctx_orbital_populations(self.h, _d(q), n),
+               "dftbhip_ctx_orbital_populations")
+        return q
 
     def timings(self) -> dict:
         t = np.zeros(len(TIMER_KEYS))

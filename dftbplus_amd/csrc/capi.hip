@@ -1191,6 +1191,7 @@ struct Context {
   std::vector<int> noccK;
   EigenSolver eig;
   std::vector<double> q0atom, eigv, fill, qout, dqout;
+  std::vector<double> qorbLast;   // output orbital populations of the last SCC iteration
   double energies[8]{};
   int nocc = 0;
   bool xf64 = true;     // the eigenvectors in X are FP64 (else FP32, mixed path)
@@ -1599,6 +1600,7 @@ static int ctx_scc_step(Context& c) {
   c.times[kTDC] += evt[2];
   for (auto& x : ev) (void)hipEventDestroy(x);
   const double th2 = now();
+  c.qorbLast = qorbH;
   std::vector<double> qout(nat, 0.0);
   st.eNonScc = 0.0;
   for (int a = 0; a < nat; ++a)
@@ -2231,6 +2233,34 @@ extern "C" int dftbhip_ctx_eigenvalues(dftbhip_ctx* ctx, double* eig, double* fi
   DH_CHECK(n == ctx->c.norbTot, "eigenvalues: size mismatch");
   if (eig) std::copy(ctx->c.eigv.begin(), ctx->c.eigv.end(), eig);
   if (fill) std::copy(ctx->c.fill.begin(), ctx->c.fill.end(), fill);
+  return 0;
+}
+
+extern "C" int dftbhip_ctx_eigenvalues_k(dftbhip_ctx* ctx, int k, double* eig, double* fill,
+                                         int n) {
+  DH_CHECK(ctx && ctx->c.sccDone, "eigenvalues_k: run the SCC cycle first");
+  const dftbhip::Context& c = ctx->c;
+  DH_CHECK(n == c.norbTot, "eigenvalues_k: size mismatch");
+  if (c.nk == 0) {
+    DH_CHECK(k == 0, "eigenvalues_k: k out of range");
+    if (eig) std::copy(c.eigv.begin(), c.eigv.end(), eig);
+    if (fill) std::copy(c.fill.begin(), c.fill.end(), fill);
+    return 0;
+  }
+  DH_CHECK(k >= 0 && k < c.nk, "eigenvalues_k: k out of range");
+  const int stride = dftbhip::kpoint_dim(&c.kpts[3 * k], n) / n;
+  for (int j = 0; j < n; ++j) {
+    if (eig) eig[j] = c.eigK[k][(size_t)stride * j];
+    if (fill) fill[j] = c.fillK[k][j];
+  }
+  return 0;
+}
+
+extern "C" int dftbhip_ctx_orbital_populations(dftbhip_ctx* ctx, double* q, int n) {
+  DH_CHECK(ctx && ctx->c.sccDone && q, "orbital_populations: run the SCC cycle first");
+  DH_CHECK(n == ctx->c.norbTot && (int)ctx->c.qorbLast.size() == n,
+           "orbital_populations: size mismatch");
+  std::copy(ctx->c.qorbLast.begin(), ctx->c.qorbLast.end(), q);
   return 0;
 }
 

@@ -98,7 +98,7 @@ def write_skf(path: str, skf: SKFile) -> None:
     lines = [f"{skf.dist:.12f} {skf.nrows + 1}"]
     if skf.onsite is not None:
         vals = list(skf.onsite) + [0.0] + list(skf.hubbu) + list(skf.occ)
-        lines.append(" ".join(f"{v:.15e}" for v in vals))
+        lines.append(" ".join(f"{v:.17e}" for v in vals))
         lines.append(f"{skf.mass:.8f} " + " ".join(["0.0"] * 19))
     else:
         lines.append(" ".join(["0.0"] * 20))
@@ -117,6 +117,19 @@ def write_skf(path: str, skf: SKFile) -> None:
         fh.write("\n".join(lines) + "\n")
 
 
+def _floats(line: str) -> List[float]:
+    """Numbers of an SK file line; commas separate like blanks and ``n*v`` repeats v n times
+    (Fortran list-directed input)."""
+    out: List[float] = []
+    for tok in line.replace(",", " ").split():
+        if "*" in tok:
+            n, v = tok.split("*", 1)
+            out += [float(v)] * int(n)
+        else:
+            out.append(float(tok))
+    return out
+
+
 def read_skf(path: str, homo: bool) -> SKFile:
     """Read an old-format SK file (oldskdata.F90:100-200, spline block :210-260)."""
     with open(path) as fh:
@@ -130,14 +143,14 @@ def read_skf(path: str, homo: bool) -> SKFile:
     onsite = hubbu = occ = None
     mass = 0.0
     if homo:
-        v = [float(x) for x in raw[pos].replace(",", " ").split()[:10]]
+        v = _floats(raw[pos])[:10]
         onsite, hubbu, occ = np.array(v[0:3]), np.array(v[4:7]), np.array(v[7:10])
         pos += 1
-        mass = float(raw[pos].split()[0])
+        mass = _floats(raw[pos])[0]
         pos += 1
     else:
         pos += 1
-    data = np.array([[float(x) for x in raw[pos + i].replace(",", " ").split()[:20]] for i in range(nrows)])
+    data = np.array([_floats(raw[pos + i])[:20] for i in range(nrows)])
     pos += nrows
     while raw[pos] != "Spline":
         pos += 1

@@ -157,6 +157,14 @@ int dftbhip_ctx_scc_end(dftbhip_ctx* ctx, double* energies, double* charges, int
 /* gradient dE/dR (3, nAtom); needs a converged dftbhip_ctx_scc */
 int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad);
 int dftbhip_ctx_eigenvalues(dftbhip_ctx* ctx, double* eig, double* fill, int n);
+/* eigenvalues and fillings of k-point k (0 without k-points) of the last SCC iteration, the
+ * eigen(:, iK, 1) / filling(:, iK, 1) arrays of getFillingsAndBandEnergies
+ * (src/dftbp/dftbplus/main.F90:4544-4651); fillings are electrons per level (0..2) */
+int dftbhip_ctx_eigenvalues_k(dftbhip_ctx* ctx, int k, double* eig, double* fill, int n);
+/* output Mulliken populations per orbital of the last SCC iteration, qOutput of
+ * getMullikenPopulation (src/dftbp/dftbplus/main.F90:4653-4753), orbitals atom by atom in the
+ * order s, p_y, p_z, p_x */
+int dftbhip_ctx_orbital_populations(dftbhip_ctx* ctx, double* q, int n);
 int dftbhip_ctx_nr_orbitals(dftbhip_ctx* ctx);
 /* per-phase wall times (s) of the last SCC/gradient call and the number of calls */
 int dftbhip_ctx_timings(dftbhip_ctx* ctx, double* t, int n);
