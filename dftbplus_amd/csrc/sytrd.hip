@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
                                                    int nSlot, T* vbuf, double* t12,
                                                    double* scalOut, T* eOut, T* tauOut) {
   __shared__ T vJs[kTB];
-  __shared__ T red[2][4][kTB];
+  __shared__ T red[4][4][kTB];
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
   const int J = blockIdx.x, ch = blockIdx.y;
@@ -338,13 +338,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
         if (ii > cg * 16 + e) zacc[e] += t[e] * vi;
       }
     }
-    red[I & 1][cg][ii] = yp;
+    red[I & 3][cg][ii] = yp;
+    // every second tile (and the last): one barrier, then the rows of the last two tiles are
+    // summed over the waves (wave 0 and wave 1 take one tile each). A buffer is written again
+    // four tiles later, after at least one more barrier.
+    if (((I - Ib) & 1) == 0 && I + 1 < Ie) return;
     lds_barrier();
-    if (cg == 0) {
-      const int k = I * kTB + ii;
+    const int Ir = (cg == 0) ? I : I - 1;
+    if (cg < 2 && Ir >= Ib && (cg == 0 || ((I - Ib) & 1) == 1)) {
+      const int k = Ir * kTB + ii;
       if (k < m)
-        Y[(long)J * n + k] = red[I & 1][0][ii] + red[I & 1][1][ii] + red[I & 1][2][ii] +
-                             red[I & 1][3][ii];
+        Y[(long)J * n + k] = red[Ir & 3][0][ii] + red[Ir & 3][1][ii] + red[Ir & 3][2][ii] +
+                             red[Ir & 3][3][ii];
     }
   };
   // two tiles in flight ahead of the one being reduced
