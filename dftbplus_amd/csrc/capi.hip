@@ -1562,10 +1562,13 @@ static int ctx_scc_step(Context& c) {
     }
     c.nocc = nocc;
     DH_HIP(hipEventRecord(ev[3], s));
-    if (c.xf64)
-      DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n, true,
-                          kKFull, c.f.p));
-    else
+    if (c.xf64) {
+      // P = Y Y^T with Y = X diag(sqrt f) (fillings are >= 0) staged in the eigensolver's
+      // scratch matrix, which is free once the eigenvectors are formed
+      double* Y = c.eig.T.p;
+      DH_TRY(scale_cols_sqrt<double>(s, c.X.p, n, c.f.p, Y, n, n, nocc));
+      DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, Y, n, Y, n, 0.0, c.P.p, n, true));
+    } else
       DH_TRY(gemm<float>(s, false, true, n, n, nocc, 1.0f, X32, n, X32, n, 0.0f, P32, n, true,
                          kKFull, c.f32.p));
     DH_HIP(hipEventRecord(ev[4], s));

@@ -150,6 +150,10 @@ template <typename T>
 int fill2d(hipStream_t s, T* p, long ld, int m, int n, T v);
 template <typename T>
 int copy2d(hipStream_t s, const T* src, long lds, T* dst, long ldd, int m, int n);
+// dst(:, j) = src(:, j) * sqrt(w(j)) for w >= 0 (the sqrt(f)-scaled columns of a SYRK)
+template <typename T>
+int scale_cols_sqrt(hipStream_t s, const T* src, long lds, const T* w, T* dst, long ldd, int m,
+                    int n);
 template <typename Tin, typename Tout>
 int convert2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n);
 int symmetrize_lower(hipStream_t s, double* a, long lda, int n);
