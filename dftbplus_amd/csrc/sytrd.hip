@@ -560,7 +560,8 @@ struct OrmWs {
   T* W2;        // kOB x ncols
 };
 
-constexpr int kOB = 128;   // reflectors per block of the back-transformation
+constexpr int kOB = 256;    // reflectors per block of the back-transformation
+constexpr int kOBH = 128;   // half block: the compact-WY T of a half is formed in LDS
 
 template <typename T>
 static size_t ormtr_layout(int n, int ncols, OrmWs<T>* ws, char* base) {
@@ -605,16 +606,18 @@ __global__ void form_v_kernel(const T* a, long lda, int n, int j0, int pw, int s
 // T(i,i) = tau_i, T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), G = V^T V   (LAPACK dlarft 'F','C').
 // The strict upper triangle of G is read once into LDS; T is built in LDS, both packed by
 // columns (entry (r, c), r <= c, at r + c(c+1)/2), so the recurrence makes no global accesses.
-constexpr int kOBPacked = kOB * (kOB + 1) / 2;
+// pw <= kOBH reflectors; G and Tout with leading dimension ld (only the pw x pw block of Tout
+// is written)
+constexpr int kOBPacked = kOBH * (kOBH + 1) / 2;
 template <typename T>
-__global__ __launch_bounds__(128) void larft_kernel(const T* G, const T* tau, int j0, int pw,
-                                                    T* Tout) {
+__global__ __launch_bounds__(128) void larft_kernel(const T* G, long ld, const T* tau, int j0,
+                                                    int pw, T* Tout) {
   __shared__ T Tp[kOBPacked];
   __shared__ T Gp[kOBPacked];
   const int t = threadIdx.x;
   for (int e = t; e < pw * pw; e += blockDim.x) {
     const int r = e % pw, c = e / pw;
-    if (r < c) Gp[r + c * (c + 1) / 2] = G[r + (long)c * kOB];
+    if (r < c) Gp[r + c * (c + 1) / 2] = G[r + (long)c * ld];
   }
   __syncthreads();
   for (int i = 0; i < pw; ++i) {
@@ -638,9 +641,9 @@ __global__ __launch_bounds__(128) void larft_kernel(const T* G, const T* tau, in
     }
     __syncthreads();
   }
-  for (int e = t; e < kOB * kOB; e += blockDim.x) {
-    const int r = e % kOB, c = e / kOB;
-    Tout[e] = (r <= c && c < pw) ? Tp[r + c * (c + 1) / 2] : T(0);
+  for (int e = t; e < pw * pw; e += blockDim.x) {
+    const int r = e % pw, c = e / pw;
+    Tout[r + (long)c * ld] = (r <= c) ? Tp[r + c * (c + 1) / 2] : T(0);
   }
 }
 
@@ -662,8 +665,24 @@ int ormtr_lower_t(hipStream_t s, const T* a, long lda, int n, const T* tau, T* z
     const int sk = std::max(1, std::min(64, m / 512));
     DH_TRY(gemm<T>(s, true, false, pw, pw, m, T(1), ws.V, m, ws.V, m, T(0), ws.G, kOB, false,
                    kKFull, nullptr, sk));
-    hipLaunchKernelGGL(larft_kernel<T>, dim3(1), dim3(128), 0, s, ws.G, tau, j0, pw, ws.Tm);
+    // T of each half in LDS, then the coupling block of H1 H2 = I - [V1 V2] T [V1 V2]^T:
+    // T12 = -T11 (V1^T V2) T22 (the product goes through G's unused lower-left block)
+    const int p1 = std::min(pw, kOBH), p2 = pw - p1;
+    DH_TRY(fill<T>(s, ws.Tm, (size_t)kOB * kOB, T(0)));
+    hipLaunchKernelGGL(larft_kernel<T>, dim3(1), dim3(128), 0, s, ws.G, (long)kOB, tau, j0, p1,
+                       ws.Tm);
     DH_LAUNCH_CHECK();
+    if (p2 > 0) {
+      T* T22 = ws.Tm + p1 + (long)p1 * kOB;
+      hipLaunchKernelGGL(larft_kernel<T>, dim3(1), dim3(128), 0, s, ws.G + p1 + (long)p1 * kOB,
+                         (long)kOB, tau, j0 + p1, p2, T22);
+      DH_LAUNCH_CHECK();
+      T* X = ws.G + p1;
+      DH_TRY(gemm<T>(s, false, false, p1, p2, p2, T(1), ws.G + (long)p1 * kOB, kOB, T22, kOB,
+                     T(0), X, kOB));
+      DH_TRY(gemm<T>(s, false, false, p1, p2, p1, T(-1), ws.Tm, kOB, X, kOB, T(0),
+                     ws.Tm + (long)p1 * kOB, kOB));
+    }
     T* zr = z + (j0 + shift);
     const int skz = std::max(1, std::min(16, m / 1024));
     DH_TRY(gemm<T>(s, true, false, pw, ncols, m, T(1), ws.V, m, zr, ldz, T(0), ws.W1, kOB, false,
