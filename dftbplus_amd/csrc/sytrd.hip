@@ -140,18 +140,6 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-__device__ __forceinline__ double block_reduce_sum(double v, double* sh) {
-  // 256 threads
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) sh[wv] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) r = sh[0] + sh[1] + sh[2] + sh[3];
-  __syncthreads();
-  return r;
-}
-
 // Reflector constants of column c from alpha = A(c+1,c) and ||x||^2 (LAPACK dlarfg without the
 // rescaling loop): beta = -sign(alpha) sqrt(alpha^2 + ||x||^2), tau = (beta - alpha)/beta,
 // x scale = 1/(alpha - beta).
@@ -177,13 +165,16 @@ __device__ __forceinline__ double wcorr(int cp, const T* tau, const double* dot)
 }
 
 // A(r,c) -= sum_p V(r,p) W(c,p) + W(r,p) V(c,p), rows r >= c; accumulate ||A(c+2:n,c)||^2.
+// A block handles 64 rows (the lanes); wave g takes the reflectors p = 16 g .. 16 g + 15 with
+// all their loads in one batch, and the four partial sums meet in LDS.
 template <typename T>
 __global__ __launch_bounds__(256) void col_update_kernel(T* a, long lda, int n, int c, int j0,
                                                          const T* W, long ldw, const T* tau,
                                                          const double* dot, double* nrm2,
                                                          double* t12) {
-  __shared__ double vc[kNB], wc[kNB], al[kNB], sh[4];
+  __shared__ double vc[kNB], wc[kNB], al[kNB], part[4][64];
   const int i = c - j0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (threadIdx.x < i) {
     const int p = threadIdx.x;
     al[p] = wcorr(j0 + p, tau, dot);
@@ -195,34 +186,40 @@ __global__ __launch_bounds__(256) void col_update_kernel(T* a, long lda, int n, 
   if (blockIdx.x == 0)
     for (int e = threadIdx.x; e < 2 * kNB; e += blockDim.x) t12[t12_at(c + 1, e)] = 0.0;
   __syncthreads();
-  const int r = c + blockIdx.x * blockDim.x + threadIdx.x;
-  double sq = 0.0;
-  if (r < n) {
-    double v = a[r + (long)c * lda];
-    // sixteen reflectors at a time with their loads ahead of the products
-    for (int p0 = 0; p0 < i; p0 += 16) {
-      double vr[16], wr[16];
+  const int r = c + blockIdx.x * 64 + lane;
+  double d = 0.0;
+  const int p0 = 16 * wv;
+  if (r < n && p0 < i) {
+    double vr[16], wr[16];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        vr[q] = 0.0;
-        wr[q] = 0.0;
-        if (p0 + q < i) {
-          vr[q] = a[r + (long)(j0 + p0 + q) * lda];
-          wr[q] = W[r + (long)(p0 + q) * ldw];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int p = p0 + q;
-        if (p < i) v -= vr[q] * wc[p] + (wr[q] + al[p] * vr[q]) * vc[p];
+    for (int q = 0; q < 16; ++q) {
+      vr[q] = 0.0;
+      wr[q] = 0.0;
+      if (p0 + q < i) {
+        vr[q] = a[r + (long)(j0 + p0 + q) * lda];
+        wr[q] = W[r + (long)(p0 + q) * ldw];
       }
     }
-    a[r + (long)c * lda] = (T)v;
-    v = (double)(T)v;
-    if (r >= c + 2) sq = v * v;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int p = p0 + q;
+      if (p < i) d += vr[q] * wc[p] + (wr[q] + al[p] * vr[q]) * vc[p];
+    }
   }
-  const double tot = block_reduce_sum(sq, sh);
-  if (threadIdx.x == 0 && tot != 0.0) atomicAdd(&nrm2[c], tot);
+  part[wv][lane] = d;
+  __syncthreads();
+  if (wv == 0) {
+    double sq = 0.0;
+    if (r < n) {
+      double v = a[r + (long)c * lda] - ((part[0][lane] + part[1][lane]) +
+                                         (part[2][lane] + part[3][lane]));
+      a[r + (long)c * lda] = (T)v;
+      v = (double)(T)v;
+      if (r >= c + 2) sq = v * v;
+    }
+    const double tot = wave_sum(sq);
+    if (lane == 0 && tot != 0.0) atomicAdd(&nrm2[c], tot);
+  }
 }
 
 // Partial sums of y = A22 v (A22 = a(c+1:n, c+1:n), lower triangle read once).
@@ -375,13 +372,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
 
 // y(r) from its slots, w = tau * (y - V t1 - W t2); store v into column c and accumulate w.v.
 // Slots are stored slot-major (Y(row, slot) at slot * n + row): a block handles 64 rows (the
-// lanes) and its 4 waves split the slots and the panel correction between them.
+// lanes) and its kFinW waves split the slots and the panel correction between them (many
+// waves per row block keep enough loads in flight: there are only m / 64 blocks).
+constexpr int kFinW = 16;
 template <typename T>
-__global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, int c, int j0,
-                                                         T* W, long ldw, const T* Y, int nSlot,
-                                                         const T* vbuf, const double* t12,
-                                                         const T* tauA, double* dot, int tiles) {
-  __shared__ double red[4][64];
+__global__ __launch_bounds__(64 * kFinW) void finalize_w_kernel(T* a, long lda, int n, int c,
+                                                                int j0, T* W, long ldw, const T* Y,
+                                                                int nSlot, const T* vbuf,
+                                                                const double* t12, const T* tauA,
+                                                                double* dot, int tiles) {
+  __shared__ double red[kFinW][64];
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
   const int i = c - j0;
@@ -392,31 +392,31 @@ __global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, 
   if (k < m) {
     const int Ir = k / kTB;
     const int nch = (nRB - Ir + tiles - 1) / tiles;
-    // slots with four independent sums so that the loads overlap
+    // this wave's slots (sl = wv, wv + kFinW, ...) with four independent sums
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int sl = wv;
-    for (; sl + 12 <= Ir; sl += 16) {
+    for (; sl + 3 * kFinW <= Ir; sl += 4 * kFinW) {
       s0 += Y[(long)sl * n + k];
-      s1 += Y[(long)(sl + 4) * n + k];
-      s2 += Y[(long)(sl + 8) * n + k];
-      s3 += Y[(long)(sl + 12) * n + k];
+      s1 += Y[(long)(sl + kFinW) * n + k];
+      s2 += Y[(long)(sl + 2 * kFinW) * n + k];
+      s3 += Y[(long)(sl + 3 * kFinW) * n + k];
     }
-    for (; sl <= Ir; sl += 4) s0 += Y[(long)sl * n + k];
+    for (; sl <= Ir; sl += kFinW) s0 += Y[(long)sl * n + k];
     const T* Yc = Y + (long)nRB * n;
     sl = wv;
-    for (; sl + 12 < nch; sl += 16) {
+    for (; sl + 3 * kFinW < nch; sl += 4 * kFinW) {
       s1 += Yc[(long)sl * n + k];
-      s2 += Yc[(long)(sl + 4) * n + k];
-      s3 += Yc[(long)(sl + 8) * n + k];
-      s0 += Yc[(long)(sl + 12) * n + k];
+      s2 += Yc[(long)(sl + kFinW) * n + k];
+      s3 += Yc[(long)(sl + 2 * kFinW) * n + k];
+      s0 += Yc[(long)(sl + 3 * kFinW) * n + k];
     }
-    for (; sl < nch; sl += 4) s1 += Yc[(long)sl * n + k];
+    for (; sl < nch; sl += kFinW) s1 += Yc[(long)sl * n + k];
     acc = (s0 + s1) + (s2 + s3);
-    // panel correction of this wave's p = wv + 4q, loads ahead of the products
-    double vr[kNB / 4], wr[kNB / 4];
+    // panel correction of this wave's p = wv + kFinW q, loads ahead of the products
+    double vr[kNB / kFinW], wr[kNB / kFinW];
 #pragma unroll
-    for (int q = 0; q < kNB / 4; ++q) {
-      const int p = wv + 4 * q;
+    for (int q = 0; q < kNB / kFinW; ++q) {
+      const int p = wv + kFinW * q;
       vr[q] = 0.0;
       wr[q] = 0.0;
       if (p < i) {
@@ -425,8 +425,8 @@ __global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, 
       }
     }
 #pragma unroll
-    for (int q = 0; q < kNB / 4; ++q) {
-      const int p = wv + 4 * q;
+    for (int q = 0; q < kNB / kFinW; ++q) {
+      const int p = wv + kFinW * q;
       if (p < i) {
         const double w = wr[q] + wcorr(j0 + p, tauA, dot) * vr[q];
         acc -= vr[q] * t12[t12_at(c, p)] + w * t12[t12_at(c, i + p)];
@@ -438,7 +438,9 @@ __global__ __launch_bounds__(256) void finalize_w_kernel(T* a, long lda, int n, 
   if (wv == 0) {
     double prod = 0.0;
     if (k < m) {
-      const double y = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+      double y = 0.0;
+#pragma unroll
+      for (int q = 0; q < kFinW; ++q) y += red[q][lane];
       const T w = (T)((double)tauA[c] * y);
       W[r + (long)i * ldw] = w;
       const T v = vbuf[r];
@@ -492,7 +494,7 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
     const int pw = std::min(kNB, n - j0);
     for (int i = 0; i < pw; ++i) {
       const int c = j0 + i;
-      hipLaunchKernelGGL(col_update_kernel<T>, dim3(cdiv(n - c, 256)), dim3(256), 0, s, a, lda, n,
+      hipLaunchKernelGGL(col_update_kernel<T>, dim3(cdiv(n - c, 64)), dim3(256), 0, s, a, lda, n,
                          c, j0, ws.W, ldw, tau, ws.dot, ws.nrm2, ws.t12);
       DH_LAUNCH_CHECK();
       const int m = n - c - 1;
@@ -526,7 +528,7 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
         pf.bytes[pf.used] = sizeof(T) * ((double)m * (m + 1) / 2 + 2.0 * m);
         ++pf.used;
       }
-      hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 64)), dim3(256), 0, s, a, lda, n, c, j0,
+      hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 64)), dim3(64 * kFinW), 0, s, a, lda, n, c, j0,
                          ws.W, ldw, ws.Y, ws.nSlot, ws.vbuf, ws.t12, tau, ws.dot, tiles);
       DH_LAUNCH_CHECK();
     }
