@@ -286,7 +286,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCC path of a single Gamma-point system runs on one "
-              "device (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "device and no multi-GPU sharding is built; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     from dftbplus_amd import api, synth
