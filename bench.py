@@ -261,6 +261,66 @@ def mode_kpoints(args):
     dp.close()
 
 
+def mode_autograd(args):
+    """The differentiation form at the water-box size: energy and forces through the
+    torch.autograd.Function (analytic adjoint backward, library in a worker process) in FP64
+    and mixed precision with refinement; central finite differences (h = 1e-4 bohr) of the FP64
+    energy on a few coordinates; the mixed gradients against the FP64 ones."""
+    import numpy as np
+    import torch
+    from dftbplus_amd import synth, torch_api
+    geo, params, opts = synth.workload(args.workload)
+    out = {"mode": "autograd", "workload": args.workload, "atoms": geo.natom}
+    grads = {}
+    # mixed: FP32 bulk with two refinement steps every iteration (this box's charges move too
+    # slowly under its mixing for the FP32-only iterations to reach a switch point)
+    for name, kw in (("fp64", {}), ("mixed_refined", dict(precision=1, refine_below=float("inf"),
+                                                          refine_steps=2))):
+        calc = torch_api.Calculator(params, geo, opts, force_mode=1, **kw)
+        x = torch.tensor(geo.coords, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        try:
+            e = torch_api.energy(x, calc)
+        except RuntimeError as err:   # an SCC cycle that does not converge is reported as such
+            out[name] = {"error": str(err), "seconds": time.perf_counter() - t0}
+            print(f"[bench:autograd] {name}: {err}", file=sys.stderr, flush=True)
+            calc.close()
+            continue
+        t1 = time.perf_counter()
+        e.backward()
+        t2 = time.perf_counter()
+        grads[name] = x.grad.numpy().copy()
+        rec = {"energy": float(e.item()), "scc_s": t1 - t0, "backward_s": t2 - t1,
+               "force_evals_per_s_incl_scc": 1.0 / (t2 - t0), "backward_per_s": 1.0 / (t2 - t1)}
+        print(f"[bench:autograd] {name}: SCC {t1 - t0:.2f}s backward {t2 - t1:.3f}s",
+              file=sys.stderr, flush=True)
+        if name == "fp64":
+            h = 1e-4
+            rng = np.random.default_rng(48)
+            idx = rng.choice(3 * geo.natom, size=args.fd_coords, replace=False)
+            c0 = geo.coords.reshape(-1).copy()
+            err = 0.0
+            for k in idx:
+                c = c0.copy()
+                c[k] += h
+                ep = calc.energy(c)
+                c[k] -= 2 * h
+                em = calc.energy(c)
+                dk = abs((ep - em) / (2 * h) - grads[name].reshape(-1)[k])
+                err = max(err, dk)
+                print(f"[bench:autograd] FD coordinate {k}: |FD - adjoint| {dk:.2e}",
+                      file=sys.stderr, flush=True)
+            rec["fd_coords"] = int(args.fd_coords)
+            rec["fd_max_err"] = err
+        calc.close()
+        out[name] = rec
+    if "fp64" in grads and "mixed_refined" in grads:
+        out["mixed_vs_fp64"] = {
+            "dE_per_atom": abs(out["mixed_refined"]["energy"] - out["fp64"]["energy"]) / geo.natom,
+            "dF_max": float(np.max(np.abs(grads["mixed_refined"] - grads["fp64"])))}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,13 +332,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cycle", action="store_true",
                     help="also run full SCC cycles: FP64 against FP32 iterations then FP64 ones")
-    ap.add_argument("--mode", default="scc", choices=["scc", "sytrd", "batch", "kpoints"],
+    ap.add_argument("--mode", default="scc",
+                    choices=["scc", "sytrd", "batch", "kpoints", "autograd"],
                     help="scc: the headline SCC benchmark (default); sytrd: the tridiagonalisation "
                          "alone on a random matrix (SYMV counter runs); batch: the molecule batch "
                          "against a sampled loop of single-molecule runs; kpoints: SCC iterations "
-                         "and a force evaluation with a k-point mesh")
+                         "and a force evaluation with a k-point mesh; autograd: energy and forces "
+                         "through torch.autograd with finite-difference checks")
     ap.add_argument("--n", type=int, default=8192, help="matrix size of --mode sytrd")
     ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
+    ap.add_argument("--fd-coords", type=int, default=4,
+                    help="coordinates checked by finite differences in --mode autograd")
     ap.add_argument("--kmesh", default="gc", choices=["gc", "mp"],
                     help="2x2x2 mesh of --mode kpoints: Gamma-centred (gc) or shifted (mp)")
     ap.add_argument("--tiles", type=int, default=0,
@@ -292,7 +356,8 @@ def main():
     from dftbplus_amd import api, synth
     # torch is not loaded here: its bundled HIP runtime would be a second runtime in the process
     if args.mode != "scc":
-        {"sytrd": mode_sytrd, "batch": mode_batch, "kpoints": mode_kpoints}[args.mode](args)
+        {"sytrd": mode_sytrd, "batch": mode_batch, "kpoints": mode_kpoints,
+         "autograd": mode_autograd}[args.mode](args)
         return
     geo, params, opts = synth.workload(args.workload)
     legs = [x for x in args.legs.split(",") if x]

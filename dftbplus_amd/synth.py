@@ -281,10 +281,12 @@ def workload(name: str):
         return geo, params, Options(kT=0.01, scc_tol=1.0e-8)
     if name.startswith("water"):
         # kT = 0.003 Ha: with this synthetic parameter set the frontier levels of the water box
-        # are near-degenerate and the SCC cycle at kT = 1e-8 does not converge in 200 iterations
+        # are near-degenerate and the SCC cycle at kT = 1e-8 does not converge in 200 iterations.
+        # Broyden mixing parameter 0.05: the first iteration moves about 1.8 e between the
+        # atoms, and at 0.2 the 1000-molecule box oscillates without converging (300 iterations)
         geo, el = water_box(int(name[len("water"):] or 1000), 2)
         params, _ = make_parameters(el, 102)
-        return geo, params, Options(kT=3.0e-3, scc_tol=1.0e-8)
+        return geo, params, Options(kT=3.0e-3, scc_tol=1.0e-8, mix_param=0.05, max_scc_iter=300)
     if name.startswith("molecules"):
         # the first N molecules of the 4096-molecule batch recipe (seed 3), as a list of
         # geometries. kT = 0.01 Ha: with this synthetic parameter set most of these small
