@@ -6,6 +6,10 @@
 namespace dftbhip {
 
 constexpr int kBatchMaxOrb = 512;   // orbitals per molecule in a batch
+// size classes of the batched eigensolver: molecules up to kBatchLdsSmall / kBatchLdsMid
+// orbitals keep their reduced matrix in LDS (39 / 148 KB per workgroup), larger ones in HBM
+constexpr int kBatchLdsSmall = 96;
+constexpr int kBatchLdsMid = 190;
 
 struct BatchLayout {
   int nmol;
@@ -15,6 +19,7 @@ struct BatchLayout {
   const long* gOff;     // nmol + 1: offsets of the packed nat_m x nat_m gamma matrices
   const int* atomMol;   // molecule of each atom
   const int* order;     // nmol: molecules by decreasing orbital count (launch order)
+  int nBig, nMid;       // leading entries of order above kBatchLdsMid, then above kBatchLdsSmall
 };
 
 int launch_batch_unpack(hipStream_t s, const PairsDev& pr, const int* norb,
@@ -27,9 +32,10 @@ int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const 
 // kept between calls. Molecules with active[m] == 0 are skipped (active may be null: all).
 // warm: S is the one of the previous call on the same work array and the Jacobi sweeps start
 // from that call's eigenvectors (a cold call factors S).
-int launch_batch_sygv(hipStream_t s, const BatchLayout& L, const double* S, const double* H,
-                      double* work, double* w, double* C, int* info, const int* active,
-                      bool warm);
+// st[0..2]: the streams of the three size classes (largest first; may be the same stream)
+int launch_batch_sygv(const hipStream_t* st, const BatchLayout& L, const double* S,
+                      const double* H, double* work, double* w, double* C, int* info,
+                      const int* active, bool warm);
 int launch_batch_density(hipStream_t s, const BatchLayout& L, const double* C, const double* f,
                          const int* nocc, double* P, const int* active);
 int launch_batch_gamma(hipStream_t s, const BatchLayout& L, const double* coords,

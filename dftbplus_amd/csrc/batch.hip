@@ -153,13 +153,14 @@ __device__ void wg_gemm(int n, int kd, const double* __restrict__ X, const doubl
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L, const double* S,
+__global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L, const int* list,
+                                                                  const double* S,
                                                                   const double* H, double* work,
                                                                   double* w, double* C,
                                                                   int* info, const int* active,
                                                                   int warm) {
   constexpr int NT = kSygvThreads;
-  const int m = L.order[blockIdx.x];
+  const int m = list[blockIdx.x];
   if (active && !active[m]) return;   // converged: eigenpairs stay from the last pass
   __shared__ double Xs[kGK][kGT + 1], Ys[kGK][kGT + 1];
   __shared__ double rc[kBatchMaxOrb / 2], rs[kBatchMaxOrb / 2];
@@ -365,6 +366,214 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
   if (tid == 0) info[m] = 0;
 }
 
+// The same eigensolver for molecules of at most kMaxN orbitals with A resident in LDS (upper
+// triangle packed by columns: (i, j), i <= j, at i + j(j+1)/2) and each row of C owned by one
+// thread: a Jacobi round then touches global memory only for the owner's own row of C, so its
+// barriers wait on LDS alone. The LDS block doubles as the GEMM staging before the sweeps.
+__device__ __forceinline__ int sym_idx(int i, int j) {
+  return i <= j ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2;
+}
+
+template <int kMaxN>
+__global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
+    BatchLayout L, const int* list, const double* S, const double* H, double* work, double* w,
+    double* C, int* info, const int* active, int warm) {
+  constexpr int NT = kSygvThreads;
+  constexpr int kPacked = kMaxN * (kMaxN + 1) / 2;
+  constexpr int kStage = 2 * kGK * (kGT + 1);
+  __shared__ double As[kPacked > kStage ? kPacked : kStage];
+  __shared__ double rc[kMaxN / 2 + 1], rs[kMaxN / 2 + 1];
+  __shared__ int rp[kMaxN / 2 + 1], rq[kMaxN / 2 + 1];
+  __shared__ int perm[kMaxN];
+  __shared__ int flag, nrot;
+  const int m = list[blockIdx.x];
+  if (active && !active[m]) return;   // converged: eigenpairs stay from the last pass
+  const int tid = threadIdx.x;
+  const long base = L.dOff[m];
+  const int n = (int)(L.orb0[m + 1] - L.orb0[m]);
+  const long nn = (long)n * n;
+  double* B0 = work + 3 * base;
+  double* B1 = B0 + nn;
+  double* Cw = B1 + nn;
+  if (!warm) {
+    if (tid == 0) flag = 0;
+    for (long e = tid; e < nn; e += NT) B0[e] = S[base + e];
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+      if (tid == 0) {
+        const double d = B0[j + (long)j * n];
+        if (!(d > 0.0)) {
+          flag = j + 1;
+        } else {
+          B0[j + (long)j * n] = sqrt(d);
+        }
+      }
+      __syncthreads();
+      if (flag != 0) {
+        if (tid == 0) info[m] = flag;
+        return;
+      }
+      const double djj = B0[j + (long)j * n];
+      for (int i = j + 1 + tid; i < n; i += NT) B0[i + (long)j * n] /= djj;
+      __syncthreads();
+      const int len = n - j - 1;
+      for (long e = tid; e < (long)len * len; e += NT) {
+        const int i = j + 1 + (int)(e % len), k = j + 1 + (int)(e / len);
+        if (i >= k) B0[i + (long)k * n] -= B0[i + (long)j * n] * B0[k + (long)j * n];
+      }
+      __syncthreads();
+    }
+    for (long e = tid; e < nn; e += NT) {
+      const int i = (int)(e % n), k = (int)(e / n);
+      if (i > k) B0[k + (long)i * n] = B0[i + (long)k * n];
+    }
+    __syncthreads();
+    for (int c = tid; c < n; c += NT)
+      for (int i = 0; i < n; ++i) {
+        double v = 0.0;
+        if (i >= c)
+          v = ((i == c ? 1.0 : 0.0) - dot4(B0 + (long)i * n, 1, Cw + c, n, i)) /
+              B0[i + (long)i * n];
+        Cw[c + (long)i * n] = v;
+      }
+    __syncthreads();
+  }
+  double (*Xs)[kGT + 1] = reinterpret_cast<double (*)[kGT + 1]>(As);
+  double (*Ys)[kGT + 1] = reinterpret_cast<double (*)[kGT + 1]>(As + kGK * (kGT + 1));
+  wg_gemm<false, false>(n, n, H + base, Cw, nullptr, B1, Xs, Ys);
+  wg_gemm<true, false>(n, n, Cw, B1, nullptr, B0, Xs, Ys);
+  // A (symmetrized) into LDS
+  for (long e = tid; e < nn; e += NT) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (i <= j) As[i + j * (j + 1) / 2] = 0.5 * (B0[i + (long)j * n] + B0[j + (long)i * n]);
+  }
+  __syncthreads();
+  auto lds_barrier = [&]() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt untouched (gfx9 encoding)
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  };
+  const int N2 = n + (n & 1);
+  const int np = N2 / 2;
+  const int nunit = np * (np + 1) / 2;
+  const double eps = 2.220446049250313e-16;
+  for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
+    if (tid == 0) nrot = 0;
+    lds_barrier();
+    for (int r = 0; r < N2 - 1; ++r) {
+      for (int k = tid; k < np; k += NT) {
+        int p, q;
+        jacobi_pair(N2, r, k, p, q);
+        double c = 1.0, s = 0.0;
+        if (p < n && q < n) {
+          const double apq = As[sym_idx(p, q)];
+          const double app = As[sym_idx(p, p)], aqq = As[sym_idx(q, q)];
+          if (fabs(apq) > 1e-300 && 100.0 * fabs(apq) > eps * fmin(fabs(app), fabs(aqq))) {
+            const double th = (aqq - app) / (2.0 * apq);
+            const double t = copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+            atomicAdd(&nrot, 1);
+          }
+        }
+        rp[k] = p;
+        rq[k] = q;
+        rc[k] = c;
+        rs[k] = s;
+      }
+      lds_barrier();
+      // A <- J^T A J over the unordered 2 x 2 blocks (pairs k1 <= k2)
+      for (int u = tid; u < nunit; u += NT) {
+        int k2 = (int)((sqrt(8.0 * u + 1.0) - 1.0) * 0.5);
+        while ((k2 + 1) * (k2 + 2) / 2 <= u) ++k2;
+        while (k2 * (k2 + 1) / 2 > u) --k2;
+        const int k1 = u - k2 * (k2 + 1) / 2;
+        const double s1 = rs[k1], s2 = rs[k2];
+        if (s1 == 0.0 && s2 == 0.0) continue;
+        const double c1 = rc[k1], c2 = rc[k2];
+        const int row[2] = {rp[k1], rq[k1]}, col[2] = {rp[k2], rq[k2]};
+        double x[4];
+        int o[4];
+        bool v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int ri = row[e & 1], ci = col[e >> 1];
+          v[e] = ri < n && ci < n;
+          o[e] = v[e] ? sym_idx(ri, ci) : 0;
+          x[e] = v[e] ? As[o[e]] : 0.0;
+        }
+        // e = 0 (p1, p2), 1 (q1, p2), 2 (p1, q2), 3 (q1, q2); columns first
+        const double y0 = c2 * x[0] - s2 * x[2];
+        const double y1 = c2 * x[1] - s2 * x[3];
+        const double y2 = s2 * x[0] + c2 * x[2];
+        const double y3 = s2 * x[1] + c2 * x[3];
+        const double z[4] = {c1 * y0 - s1 * y1, s1 * y0 + c1 * y1, c1 * y2 - s1 * y3,
+                             s1 * y2 + c1 * y3};
+        if (k1 == k2) {
+          // (p, p), (q, q) and the shared off-diagonal element
+          if (v[0]) As[o[0]] = z[0];
+          if (v[3]) As[o[3]] = z[3];
+          if (v[2]) As[o[2]] = z[2];
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (v[e]) As[o[e]] = z[e];
+        }
+      }
+      // the owner of each row of Cw applies the round's rotations to it
+      if (tid < n) {
+        double* crow = Cw + tid;
+        for (int k0 = 0; k0 < np; k0 += 8) {
+          double cp[8], cq[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int k = k0 + u;
+            cp[u] = 0.0;
+            cq[u] = 0.0;
+            if (k < np && rs[k] != 0.0) {
+              cp[u] = crow[(long)rp[k] * n];
+              cq[u] = crow[(long)rq[k] * n];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int k = k0 + u;
+            if (k < np && rs[k] != 0.0) {
+              const double c = rc[k], s = rs[k];
+              crow[(long)rp[k] * n] = c * cp[u] - s * cq[u];
+              crow[(long)rq[k] * n] = s * cp[u] + c * cq[u];
+            }
+          }
+        }
+      }
+      lds_barrier();
+    }
+    const int nr = nrot;
+    lds_barrier();   // every thread has read nrot before the next sweep clears it
+    if (nr == 0) break;
+  }
+  __syncthreads();   // every row of Cw written before other threads read it
+  double* wm = w + L.orb0[m];
+  for (int i = tid; i < n; i += NT) {
+    const double di = As[sym_idx(i, i)];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = As[sym_idx(j, j)];
+      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+    }
+    perm[rank] = i;
+    wm[rank] = di;
+  }
+  __syncthreads();
+  double* Cm = C + base;
+  for (long e = tid; e < nn; e += NT) {
+    const int i = (int)(e % n), c = (int)(e / n);
+    Cm[e] = Cw[i + (long)perm[c] * n];
+  }
+  if (tid == 0) info[m] = 0;
+}
+
 // P_m = C_m diag(f) C_m^T over the first nocc[m] columns (lower and upper)
 __global__ __launch_bounds__(kSygvThreads) void batch_density_kernel(BatchLayout L,
                                                                      const double* C,
@@ -461,13 +670,26 @@ int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const 
   return 0;
 }
 
-int launch_batch_sygv(hipStream_t s, const BatchLayout& L, const double* S, const double* H,
-                      double* work, double* w, double* C, int* info, const int* active,
-                      bool warm) {
-  if (L.nmol == 0) return 0;
-  hipLaunchKernelGGL(batch_sygv_kernel, dim3(L.nmol), dim3(kSygvThreads), 0, s, L, S, H, work, w,
-                     C, info, active, warm ? 1 : 0);
-  DH_LAUNCH_CHECK();
+int launch_batch_sygv(const hipStream_t* st, const BatchLayout& L, const double* S,
+                      const double* H, double* work, double* w, double* C, int* info,
+                      const int* active, bool warm) {
+  const int nSmall = L.nmol - L.nBig - L.nMid;
+  const int wm = warm ? 1 : 0;
+  if (L.nBig > 0) {
+    hipLaunchKernelGGL(batch_sygv_kernel, dim3(L.nBig), dim3(kSygvThreads), 0, st[0], L, L.order,
+                       S, H, work, w, C, info, active, wm);
+    DH_LAUNCH_CHECK();
+  }
+  if (L.nMid > 0) {
+    hipLaunchKernelGGL((batch_sygv_lds_kernel<kBatchLdsMid>), dim3(L.nMid), dim3(kSygvThreads), 0,
+                       st[1], L, L.order + L.nBig, S, H, work, w, C, info, active, wm);
+    DH_LAUNCH_CHECK();
+  }
+  if (nSmall > 0) {
+    hipLaunchKernelGGL((batch_sygv_lds_kernel<kBatchLdsSmall>), dim3(nSmall), dim3(kSygvThreads),
+                       0, st[2], L, L.order + L.nBig + L.nMid, S, H, work, w, C, info, active, wm);
+    DH_LAUNCH_CHECK();
+  }
   return 0;
 }
 

@@ -1762,6 +1762,16 @@ struct BatchContext {
   BatchLayout L{};
   int it = 0, nactive = 0;
   bool sccDone = false;
+  // streams of the eigensolver's two smaller size classes (default priority) and the events
+  // that fork them from s and join them back
+  hipStream_t sx[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+  ~BatchContext() {
+    for (hipStream_t x : sx)
+      if (x) (void)hipStreamDestroy(x);
+    for (hipEvent_t e : {fork, join[0], join[1]})
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 static int batch_geometry(BatchContext& b, int nmol, const int* natMol, const double* coords,
@@ -1858,6 +1868,12 @@ static int batch_geometry(BatchContext& b, int nmol, const int* natMol, const do
   DH_TRY(b.orderD.upload(order.data(), order.size(), s));
   b.L.nmol = nmol;
   b.L.order = b.orderD.p;
+  b.L.nBig = b.L.nMid = 0;
+  for (int m : order) {
+    const long nm = b.orb0[m + 1] - b.orb0[m];
+    if (nm > kBatchLdsMid) ++b.L.nBig;
+    else if (nm > kBatchLdsSmall) ++b.L.nMid;
+  }
   b.L.atom0 = b.atom0D.p;
   b.L.orb0 = b.orb0D.p;
   b.L.dOff = b.dOffD.p;
@@ -1947,8 +1963,16 @@ static int batch_scc_step(BatchContext& b) {
   std::vector<int> active(nmol);
   for (int m = 0; m < nmol; ++m) active[m] = b.conv[m] ? 0 : 1;
   DH_HIP(hipMemcpyAsync(b.activeD.p, active.data(), nmol * sizeof(int), hipMemcpyHostToDevice, s));
-  DH_TRY(launch_batch_sygv(s, b.L, b.Sd.p, b.Hd.p, b.work.p, b.w.p, b.Cv.p, b.infoD.p,
+  // the three size classes of the eigensolver run side by side
+  DH_HIP(hipEventRecord(b.fork, s));
+  for (int q = 0; q < 2; ++q) DH_HIP(hipStreamWaitEvent(b.sx[q], b.fork, 0));
+  const hipStream_t st[3] = {s, b.sx[0], b.sx[1]};
+  DH_TRY(launch_batch_sygv(st, b.L, b.Sd.p, b.Hd.p, b.work.p, b.w.p, b.Cv.p, b.infoD.p,
                            b.activeD.p, it > 1));
+  for (int q = 0; q < 2; ++q) {
+    DH_HIP(hipEventRecord(b.join[q], b.sx[q]));
+    DH_HIP(hipStreamWaitEvent(s, b.join[q], 0));
+  }
   b.eigv.resize(n);
   DH_HIP(hipMemcpyAsync(b.eigv.data(), b.w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipMemcpyAsync(info.data(), b.infoD.p, nmol * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2096,6 +2120,18 @@ extern "C" int dftbhip_batch_create(dftbhip_batch** bp, const dftbhip_params* p,
   auto* h = new dftbhip_batch();
   h->b.opt = *opt;
   h->b.s = dftbhip::g_rt.stream;
+  for (int q = 0; q < 2; ++q) {
+    if (hipStreamCreate(&h->b.sx[q]) != hipSuccess || hipEventCreate(&h->b.join[q]) != hipSuccess) {
+      delete h;
+      dftbhip::set_error("batch_create: stream or event creation failed");
+      return dftbhip::kErrDevice;
+    }
+  }
+  if (hipEventCreate(&h->b.fork) != hipSuccess) {
+    delete h;
+    dftbhip::set_error("batch_create: event creation failed");
+    return dftbhip::kErrDevice;
+  }
   const int rc = dftbhip::sk_setup(h->b.sk, p, h->b.s);
   if (rc != 0) {
     delete h;
