@@ -6,10 +6,13 @@
 namespace dftbhip {
 
 constexpr int kBatchMaxOrb = 512;   // orbitals per molecule in a batch
-// size classes of the batched eigensolver: molecules up to kBatchLdsSmall / kBatchLdsMid
-// orbitals keep their reduced matrix in LDS (39 / 148 KB per workgroup), larger ones in HBM
-constexpr int kBatchLdsSmall = 96;
+// size classes of the batched eigensolver, by orbital count: up to kBatchLdsSmall and
+// kBatchLdsC the reduced matrix A and the eigenvectors C stay in LDS (49 / 153 KB per
+// workgroup), up to kBatchLdsMid A alone (148 KB, C in HBM), larger molecules keep both in HBM
+constexpr int kBatchLdsSmall = 64;
+constexpr int kBatchLdsC = 112;
 constexpr int kBatchLdsMid = 190;
+constexpr int kBatchClasses = 4;
 
 struct BatchLayout {
   int nmol;
@@ -19,7 +22,7 @@ struct BatchLayout {
   const long* gOff;     // nmol + 1: offsets of the packed nat_m x nat_m gamma matrices
   const int* atomMol;   // molecule of each atom
   const int* order;     // nmol: molecules by decreasing orbital count (launch order)
-  int nBig, nMid;       // leading entries of order above kBatchLdsMid, then above kBatchLdsSmall
+  int ncls[kBatchClasses];   // molecules per size class, largest class first, in order
 };
 
 int launch_batch_unpack(hipStream_t s, const PairsDev& pr, const int* norb,
@@ -32,7 +35,7 @@ int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const 
 // kept between calls. Molecules with active[m] == 0 are skipped (active may be null: all).
 // warm: S is the one of the previous call on the same work array and the Jacobi sweeps start
 // from that call's eigenvectors (a cold call factors S).
-// st[0..2]: the streams of the three size classes (largest first; may be the same stream)
+// st[0 .. kBatchClasses-1]: the streams of the size classes (largest first; may coincide)
 int launch_batch_sygv(const hipStream_t* st, const BatchLayout& L, const double* S,
                       const double* H, double* work, double* w, double* C, int* info,
                       const int* active, bool warm);

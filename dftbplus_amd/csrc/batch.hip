@@ -80,6 +80,15 @@ __device__ __forceinline__ void jacobi_pair(int N2, int r, int k, int& p, int& q
   q = 1 + (r + N2 - 2 - k) % (N2 - 1);
 }
 
+// Unit u of the unordered pairs (k1, k2), k1 <= k2, numbered k2 (k2 + 1) / 2 + k1.
+__device__ __forceinline__ void unit_pairs(int u, int& k1, int& k2) {
+  int b = (int)((sqrtf(8.0f * (float)u + 1.0f) - 1.0f) * 0.5f);
+  while ((b + 1) * (b + 2) / 2 <= u) ++b;
+  while (b * (b + 1) / 2 > u) --b;
+  k2 = b;
+  k1 = u - b * (b + 1) / 2;
+}
+
 // sum_{k < len} x[k * sx] * y[k * sy] with four independent partial sums
 __device__ __forceinline__ double dot4(const double* __restrict__ x, long sx,
                                        const double* __restrict__ y, long sy, int len) {
@@ -94,6 +103,44 @@ __device__ __forceinline__ double dot4(const double* __restrict__ x, long sx,
   for (; k < len; ++k) s0 += x[k * sx] * y[k * sy];
   return (s0 + s1) + (s2 + s3);
 }
+
+// Applies one Jacobi round's rotations (pairs rp/rq with cosines rc and sines rs, s = 0: none)
+// to the columns of the n x n column-major C (leading dimension n, global memory or LDS): thread
+// t takes row i = t mod n and the pairs g, g + G, ... with g = t / n, G = threads / n, all of
+// its kB loads issued before the stores. Needs n <= the workgroup size.
+template <int kB>
+__device__ __forceinline__ void rotate_columns(double* C, int n, int np, const int* rp,
+                                               const int* rq, const double* rc,
+                                               const double* rs) {
+  const int G = kSygvThreads / n;
+  const int g = threadIdx.x / n, i = threadIdx.x - g * n;
+  if (g >= G) return;
+  for (int k0 = g; k0 < np; k0 += kB * G) {
+    double cp[kB], cq[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int k = k0 + u * G;
+      cp[u] = 0.0;
+      cq[u] = 0.0;
+      if (k < np && rs[k] != 0.0) {
+        cp[u] = C[i + rp[k] * n];
+        cq[u] = C[i + rq[k] * n];
+      }
+    }
+    // the pair data are read again from LDS below rather than held in registers
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int k = k0 + u * G;
+      if (k < np && rs[k] != 0.0) {
+        const double c = rc[k], sn = rs[k];
+        C[i + rp[k] * n] = c * cp[u] - sn * cq[u];
+        C[i + rq[k] * n] = sn * cp[u] + c * cq[u];
+      }
+    }
+  }
+}
+
 
 // Workgroup GEMM on n x n column-major matrices (leading dimension n):
 //   Z = op(X)(:, 0:kd) diag(wk) op(Y)(0:kd, :),  op(X) = X or X^T, op(Y) = Y or Y^T,
@@ -241,8 +288,8 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
   // parallel cyclic Jacobi, round-robin pairing (n odd: index n is a dummy that never rotates)
   const int N2 = n + (n & 1);
   const int np = N2 / 2;
-  const long nblk = (long)np * np;
-  const long work4 = ((long)np * n + 3) / 4;
+  const int nunit = np * (np + 1) / 2;
+  constexpr int kAU = 4;   // A-update units per thread and pass
   const double eps = 2.220446049250313e-16;
   for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
     if (tid == 0) nrot = 0;
@@ -269,79 +316,54 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
         rs[k] = s;
       }
       __syncthreads();
-      // A <- J^T A J over the 2 x 2 blocks (rows of pair k1, columns of pair k2); four blocks
-      // per thread with all loads ahead of the stores
-      for (long b0 = tid; b0 < nblk; b0 += 4L * NT) {
-        double x[4][4];
-        long o[4][4];
-        bool v[4][4];
-        double c1[4], s1[4], c2[4], s2[4];
+      // A <- J^T A J over the unordered 2 x 2 blocks (rows of pair k1, columns of pair k2,
+      // k1 <= k2; the block and its transpose take the same values), kAU per thread with all
+      // loads ahead of the stores; only the loaded values stay in registers in between
+      for (int u0 = tid; u0 < nunit; u0 += kAU * NT) {
+        double x[kAU][4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const long b = b0 + (long)u * NT;
-          s1[u] = 0.0;
-          s2[u] = 0.0;
-          if (b < nblk) {
-            const int k1 = (int)(b % np), k2 = (int)(b / np);
-            c1[u] = rc[k1];
-            s1[u] = rs[k1];
-            c2[u] = rc[k2];
-            s2[u] = rs[k2];
-            const int row[2] = {rp[k1], rq[k1]}, col[2] = {rp[k2], rq[k2]};
+        for (int a = 0; a < kAU; ++a) {
+          const int u = u0 + a * NT;
+          if (u >= nunit) continue;
+          int k1, k2;
+          unit_pairs(u, k1, k2);
+          if (rs[k1] == 0.0 && rs[k2] == 0.0) continue;
+          const int row[2] = {rp[k1], rq[k1]}, col[2] = {rp[k2], rq[k2]};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int ri = row[e & 1], ci = col[e >> 1];
-              v[u][e] = ri < n && ci < n;
-              o[u][e] = ri + (long)ci * n;
-            }
-            if (s1[u] != 0.0 || s2[u] != 0.0)
-#pragma unroll
-              for (int e = 0; e < 4; ++e) x[u][e] = v[u][e] ? A[o[u][e]] : 0.0;
+          for (int e = 0; e < 4; ++e) {
+            const int ri = row[e & 1], ci = col[e >> 1];
+            x[a][e] = ri < n && ci < n ? A[ri + ci * n] : 0.0;   // n <= kBatchMaxOrb
           }
         }
+        __asm__ volatile("" ::: "memory");
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (s1[u] == 0.0 && s2[u] == 0.0) continue;
+        for (int a = 0; a < kAU; ++a) {
+          const int u = u0 + a * NT;
+          if (u >= nunit) continue;
+          int k1, k2;
+          unit_pairs(u, k1, k2);
+          const double c1 = rc[k1], s1 = rs[k1], c2 = rc[k2], s2 = rs[k2];
+          if (s1 == 0.0 && s2 == 0.0) continue;
           // x[e]: e = 0 (p1, p2), 1 (q1, p2), 2 (p1, q2), 3 (q1, q2); columns first
-          const double y0 = c2[u] * x[u][0] - s2[u] * x[u][2];
-          const double y1 = c2[u] * x[u][1] - s2[u] * x[u][3];
-          const double y2 = s2[u] * x[u][0] + c2[u] * x[u][2];
-          const double y3 = s2[u] * x[u][1] + c2[u] * x[u][3];
-          const double z[4] = {c1[u] * y0 - s1[u] * y1, s1[u] * y0 + c1[u] * y1,
-                               c1[u] * y2 - s1[u] * y3, s1[u] * y2 + c1[u] * y3};
+          const double y0 = c2 * x[a][0] - s2 * x[a][2];
+          const double y1 = c2 * x[a][1] - s2 * x[a][3];
+          const double y2 = s2 * x[a][0] + c2 * x[a][2];
+          const double y3 = s2 * x[a][1] + c2 * x[a][3];
+          double z[4] = {c1 * y0 - s1 * y1, s1 * y0 + c1 * y1, c1 * y2 - s1 * y3,
+                         s1 * y2 + c1 * y3};
+          if (k1 == k2) z[1] = z[2];   // (q, p) and (p, q) alike
+          const int row[2] = {rp[k1], rq[k1]}, col[2] = {rp[k2], rq[k2]};
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (v[u][e]) A[o[u][e]] = z[e];
-        }
-      }
-      // columns p, q of Cw
-      for (long g = tid; g < work4; g += NT) {
-        double cp[4], cq[4];
-        long ip[4], iq[4];
-        double cc[4], ss[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const long e = g * 4 + u;
-          ss[u] = 0.0;
-          if (e < (long)np * n) {
-            const int k = (int)(e / n), i = (int)(e % n);
-            ss[u] = rs[k];
-            cc[u] = rc[k];
-            ip[u] = i + (long)rp[k] * n;
-            iq[u] = i + (long)rq[k] * n;
-            if (ss[u] != 0.0) {
-              cp[u] = Cw[ip[u]];
-              cq[u] = Cw[iq[u]];
+          for (int e = 0; e < 4; ++e) {
+            const int ri = row[e & 1], ci = col[e >> 1];
+            if (ri < n && ci < n) {
+              A[ri + ci * n] = z[e];
+              if (k1 != k2) A[ci + ri * n] = z[e];
             }
           }
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (ss[u] == 0.0) continue;
-          Cw[ip[u]] = cc[u] * cp[u] - ss[u] * cq[u];
-          Cw[iq[u]] = ss[u] * cp[u] + cc[u] * cq[u];
-        }
       }
+      rotate_columns<8>(Cw, n, np, rp, rq, rc, rs);
       __syncthreads();
     }
     if (nrot == 0) break;
@@ -367,14 +389,14 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
 }
 
 // The same eigensolver for molecules of at most kMaxN orbitals with A resident in LDS (upper
-// triangle packed by columns: (i, j), i <= j, at i + j(j+1)/2) and each row of C owned by one
-// thread: a Jacobi round then touches global memory only for the owner's own row of C, so its
-// barriers wait on LDS alone. The LDS block doubles as the GEMM staging before the sweeps.
+// triangle packed by columns: (i, j), i <= j, at i + j(j+1)/2) and, for kCLds, C as well: a
+// Jacobi round then waits on LDS alone. Otherwise C stays in HBM and each round ends with a
+// full barrier. The LDS block of A doubles as the GEMM staging before the sweeps.
 __device__ __forceinline__ int sym_idx(int i, int j) {
   return i <= j ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2;
 }
 
-template <int kMaxN>
+template <int kMaxN, bool kCLds>
 __global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
     BatchLayout L, const int* list, const double* S, const double* H, double* work, double* w,
     double* C, int* info, const int* active, int warm) {
@@ -382,6 +404,7 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
   constexpr int kPacked = kMaxN * (kMaxN + 1) / 2;
   constexpr int kStage = 2 * kGK * (kGT + 1);
   __shared__ double As[kPacked > kStage ? kPacked : kStage];
+  __shared__ double Cs[kCLds ? kMaxN * kMaxN : 1];
   __shared__ double rc[kMaxN / 2 + 1], rs[kMaxN / 2 + 1];
   __shared__ int rp[kMaxN / 2 + 1], rq[kMaxN / 2 + 1];
   __shared__ int perm[kMaxN];
@@ -391,6 +414,10 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
   const int tid = threadIdx.x;
   const long base = L.dOff[m];
   const int n = (int)(L.orb0[m + 1] - L.orb0[m]);
+  if (n > kMaxN) {   // a molecule in the wrong size class: reported, nothing touched
+    if (tid == 0) info[m] = -n;
+    return;
+  }
   const long nn = (long)n * n;
   double* B0 = work + 3 * base;
   double* B1 = B0 + nn;
@@ -446,8 +473,10 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
   for (long e = tid; e < nn; e += NT) {
     const int i = (int)(e % n), j = (int)(e / n);
     if (i <= j) As[i + j * (j + 1) / 2] = 0.5 * (B0[i + (long)j * n] + B0[j + (long)i * n]);
+    if (kCLds) Cs[e] = Cw[e];
   }
   __syncthreads();
+  double* Cr = kCLds ? Cs : Cw;   // the C the sweeps rotate
   auto lds_barrier = [&]() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt untouched (gfx9 encoding)
@@ -485,10 +514,8 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
       lds_barrier();
       // A <- J^T A J over the unordered 2 x 2 blocks (pairs k1 <= k2)
       for (int u = tid; u < nunit; u += NT) {
-        int k2 = (int)((sqrt(8.0 * u + 1.0) - 1.0) * 0.5);
-        while ((k2 + 1) * (k2 + 2) / 2 <= u) ++k2;
-        while (k2 * (k2 + 1) / 2 > u) --k2;
-        const int k1 = u - k2 * (k2 + 1) / 2;
+        int k1, k2;
+        unit_pairs(u, k1, k2);
         const double s1 = rs[k1], s2 = rs[k2];
         if (s1 == 0.0 && s2 == 0.0) continue;
         const double c1 = rc[k1], c2 = rc[k2];
@@ -521,31 +548,13 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
             if (v[e]) As[o[e]] = z[e];
         }
       }
-      // the owner of each row of Cw applies the round's rotations to it
-      if (tid < n) {
-        double* crow = Cw + tid;
-        for (int k0 = 0; k0 < np; k0 += 8) {
-          double cp[8], cq[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int k = k0 + u;
-            cp[u] = 0.0;
-            cq[u] = 0.0;
-            if (k < np && rs[k] != 0.0) {
-              cp[u] = crow[(long)rp[k] * n];
-              cq[u] = crow[(long)rq[k] * n];
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int k = k0 + u;
-            if (k < np && rs[k] != 0.0) {
-              const double c = rc[k], s = rs[k];
-              crow[(long)rp[k] * n] = c * cp[u] - s * cq[u];
-              crow[(long)rq[k] * n] = s * cp[u] + c * cq[u];
-            }
-          }
-        }
+      // columns of C: LDS-resident C needs only the LDS barrier below; C in HBM is shared
+      // between the threads of a row across rounds, so its stores complete first
+      if (kCLds) {
+        rotate_columns<8>(Cr, n, np, rp, rq, rc, rs);
+      } else {
+        rotate_columns<10>(Cr, n, np, rp, rq, rc, rs);
+        __syncthreads();
       }
       lds_barrier();
     }
@@ -553,7 +562,7 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
     lds_barrier();   // every thread has read nrot before the next sweep clears it
     if (nr == 0) break;
   }
-  __syncthreads();   // every row of Cw written before other threads read it
+  __syncthreads();   // every column of C written before other threads read it
   double* wm = w + L.orb0[m];
   for (int i = tid; i < n; i += NT) {
     const double di = As[sym_idx(i, i)];
@@ -569,7 +578,8 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
   double* Cm = C + base;
   for (long e = tid; e < nn; e += NT) {
     const int i = (int)(e % n), c = (int)(e / n);
-    Cm[e] = Cw[i + (long)perm[c] * n];
+    Cm[e] = Cr[i + (long)perm[c] * n];
+    if (kCLds) Cw[e] = Cs[e];   // the next warm call starts from the rotated C
   }
   if (tid == 0) info[m] = 0;
 }
@@ -673,23 +683,26 @@ int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const 
 int launch_batch_sygv(const hipStream_t* st, const BatchLayout& L, const double* S,
                       const double* H, double* work, double* w, double* C, int* info,
                       const int* active, bool warm) {
-  const int nSmall = L.nmol - L.nBig - L.nMid;
+  DH_CHECK(L.ncls[0] + L.ncls[1] + L.ncls[2] + L.ncls[3] == L.nmol,
+           "batch_sygv: size classes do not cover the batch");
   const int wm = warm ? 1 : 0;
-  if (L.nBig > 0) {
-    hipLaunchKernelGGL(batch_sygv_kernel, dim3(L.nBig), dim3(kSygvThreads), 0, st[0], L, L.order,
+  const int* list = L.order;   // molecules by decreasing size: the classes in turn
+  if (L.ncls[0] > 0)
+    hipLaunchKernelGGL(batch_sygv_kernel, dim3(L.ncls[0]), dim3(kSygvThreads), 0, st[0], L, list,
                        S, H, work, w, C, info, active, wm);
-    DH_LAUNCH_CHECK();
-  }
-  if (L.nMid > 0) {
-    hipLaunchKernelGGL((batch_sygv_lds_kernel<kBatchLdsMid>), dim3(L.nMid), dim3(kSygvThreads), 0,
-                       st[1], L, L.order + L.nBig, S, H, work, w, C, info, active, wm);
-    DH_LAUNCH_CHECK();
-  }
-  if (nSmall > 0) {
-    hipLaunchKernelGGL((batch_sygv_lds_kernel<kBatchLdsSmall>), dim3(nSmall), dim3(kSygvThreads),
-                       0, st[2], L, L.order + L.nBig + L.nMid, S, H, work, w, C, info, active, wm);
-    DH_LAUNCH_CHECK();
-  }
+  list += L.ncls[0];
+  if (L.ncls[1] > 0)
+    hipLaunchKernelGGL((batch_sygv_lds_kernel<kBatchLdsMid, false>), dim3(L.ncls[1]),
+                       dim3(kSygvThreads), 0, st[1], L, list, S, H, work, w, C, info, active, wm);
+  list += L.ncls[1];
+  if (L.ncls[2] > 0)
+    hipLaunchKernelGGL((batch_sygv_lds_kernel<kBatchLdsC, true>), dim3(L.ncls[2]),
+                       dim3(kSygvThreads), 0, st[2], L, list, S, H, work, w, C, info, active, wm);
+  list += L.ncls[2];
+  if (L.ncls[3] > 0)
+    hipLaunchKernelGGL((batch_sygv_lds_kernel<kBatchLdsSmall, true>), dim3(L.ncls[3]),
+                       dim3(kSygvThreads), 0, st[3], L, list, S, H, work, w, C, info, active, wm);
+  DH_LAUNCH_CHECK();
   return 0;
 }
 

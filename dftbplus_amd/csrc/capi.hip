@@ -1762,15 +1762,17 @@ struct BatchContext {
   BatchLayout L{};
   int it = 0, nactive = 0;
   bool sccDone = false;
-  // streams of the eigensolver's two smaller size classes (default priority) and the events
-  // that fork them from s and join them back
-  hipStream_t sx[2] = {nullptr, nullptr};
-  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+  // streams of the eigensolver's smaller size classes (default priority) and the events that
+  // fork them from s and join them back
+  static constexpr int kSx = kBatchClasses - 1;
+  hipStream_t sx[kSx] = {};
+  hipEvent_t fork = nullptr, join[kSx] = {};
   ~BatchContext() {
     for (hipStream_t x : sx)
       if (x) (void)hipStreamDestroy(x);
-    for (hipEvent_t e : {fork, join[0], join[1]})
+    for (hipEvent_t e : join)
       if (e) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
   }
 };
 
@@ -1868,11 +1870,10 @@ static int batch_geometry(BatchContext& b, int nmol, const int* natMol, const do
   DH_TRY(b.orderD.upload(order.data(), order.size(), s));
   b.L.nmol = nmol;
   b.L.order = b.orderD.p;
-  b.L.nBig = b.L.nMid = 0;
+  for (int& c : b.L.ncls) c = 0;
   for (int m : order) {
     const long nm = b.orb0[m + 1] - b.orb0[m];
-    if (nm > kBatchLdsMid) ++b.L.nBig;
-    else if (nm > kBatchLdsSmall) ++b.L.nMid;
+    ++b.L.ncls[nm > kBatchLdsMid ? 0 : nm > kBatchLdsC ? 1 : nm > kBatchLdsSmall ? 2 : 3];
   }
   b.L.atom0 = b.atom0D.p;
   b.L.orb0 = b.orb0D.p;
@@ -1963,13 +1964,14 @@ static int batch_scc_step(BatchContext& b) {
   std::vector<int> active(nmol);
   for (int m = 0; m < nmol; ++m) active[m] = b.conv[m] ? 0 : 1;
   DH_HIP(hipMemcpyAsync(b.activeD.p, active.data(), nmol * sizeof(int), hipMemcpyHostToDevice, s));
-  // the three size classes of the eigensolver run side by side
+  // the size classes of the eigensolver run side by side
   DH_HIP(hipEventRecord(b.fork, s));
-  for (int q = 0; q < 2; ++q) DH_HIP(hipStreamWaitEvent(b.sx[q], b.fork, 0));
-  const hipStream_t st[3] = {s, b.sx[0], b.sx[1]};
+  for (int q = 0; q < BatchContext::kSx; ++q) DH_HIP(hipStreamWaitEvent(b.sx[q], b.fork, 0));
+  hipStream_t st[kBatchClasses] = {s};
+  for (int q = 0; q < BatchContext::kSx; ++q) st[q + 1] = b.sx[q];
   DH_TRY(launch_batch_sygv(st, b.L, b.Sd.p, b.Hd.p, b.work.p, b.w.p, b.Cv.p, b.infoD.p,
                            b.activeD.p, it > 1));
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < BatchContext::kSx; ++q) {
     DH_HIP(hipEventRecord(b.join[q], b.sx[q]));
     DH_HIP(hipStreamWaitEvent(s, b.join[q], 0));
   }
@@ -1979,8 +1981,11 @@ static int batch_scc_step(BatchContext& b) {
   DH_HIP(hipStreamSynchronize(s));
   for (int m = 0; m < nmol; ++m)
     if (info[m] != 0) {
-      set_error("batch: overlap of molecule " + std::to_string(m) +
-                " not positive definite (minor " + std::to_string(info[m]) + ")");
+      set_error(info[m] < 0 ? "batch: molecule " + std::to_string(m) + " of " +
+                                  std::to_string(-info[m]) + " orbitals in a smaller size class"
+                            : "batch: overlap of molecule " + std::to_string(m) +
+                                  " not positive definite (minor " + std::to_string(info[m]) +
+                                  ")");
       return kErrState;
     }
   for (int m = 0; m < nmol; ++m) {
@@ -2120,7 +2125,7 @@ extern "C" int dftbhip_batch_create(dftbhip_batch** bp, const dftbhip_params* p,
   auto* h = new dftbhip_batch();
   h->b.opt = *opt;
   h->b.s = dftbhip::g_rt.stream;
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < dftbhip::BatchContext::kSx; ++q) {
     if (hipStreamCreate(&h->b.sx[q]) != hipSuccess || hipEventCreate(&h->b.join[q]) != hipSuccess) {
       delete h;
       dftbhip::set_error("batch_create: stream or event creation failed");
