@@ -14,6 +14,8 @@ Three legs run on the same input:
                 matrices; FP64 Cholesky, divide and conquer, Mulliken and energy reductions;
   mixed_refined the FP32 eigenpairs of every iteration refined in FP64 (two Ogita-Aishima
                 steps) and FP64 density matrices.
+Before the legs, the SYMV tile count is autotuned at the workload's size (FP64 and FP32; the
+tables and the tuning time are reported, `--no-autotune` keeps the fixed thresholds).
 Each leg first runs one probe iteration from neutral charges followed by a force evaluation
 (timed); its energy, charges and gradients are compared with the fp64 leg's probe. Then the
 timed iterations run. `value` is the SCC iterations/s of the fastest leg whose probe meets
@@ -321,6 +323,33 @@ def mode_autograd(args):
     print(json.dumps(out), flush=True)
 
 
+def _fixed_tiles(m):
+    """The tile count of the fixed thresholds (the library's default without a tuned table)."""
+    return 8 if m >= 7000 else (4 if m >= 2500 else 2)
+
+
+def autotune_symv(api, geo, params, legs):
+    """Tunes the SYMV tile count at the workload's size for FP64 and, when a mixed leg runs, FP32
+    (installed in the library for the legs that follow); returns the tables, the tuning time and
+    the SYMV time per tridiagonalisation that the table saves over the fixed thresholds on the
+    tuning run."""
+    n = int(sum(params.species[s].norb for s in geo.species))
+    out = {"bucket_rows": 256}
+    t0 = time.perf_counter()
+    for name, prec in (("fp64", 0), ("fp32", 1)):
+        if prec == 1 and all(p == "fp64" for p in legs):
+            continue
+        rows, secs = api.autotune(n, prec)
+        opts = (2, 4, 8, 16)
+        fixed = sum(secs[opts.index(_fixed_tiles(256 * b)), b] for b in range(secs.shape[1]))
+        best = float(secs.min(axis=0).sum())
+        out[name] = {"table": rows, "symv_s_tuned": best, "symv_s_fixed_thresholds": float(fixed)}
+        print(f"[bench] autotune {name}: {rows} (SYMV {best:.3f} s vs {fixed:.3f} s with the "
+              f"fixed thresholds)", file=sys.stderr, flush=True)
+    out["seconds"] = time.perf_counter() - t0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,6 +359,8 @@ def main():
     ap.add_argument("--legs", default="fp64,mixed_fp32,mixed_refined",
                     help="comma-separated subset of " + ",".join(LEGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the fixed SYMV tile thresholds instead of autotuning them")
     ap.add_argument("--cycle", action="store_true",
                     help="also run full SCC cycles: FP64 against FP32 iterations then FP64 ones")
     ap.add_argument("--mode", default="scc",
@@ -363,6 +394,7 @@ def main():
     legs = [x for x in args.legs.split(",") if x]
     if "fp64" not in legs:
         legs.insert(0, "fp64")   # the reference leg of the error comparison
+    tuned = None if args.no_autotune else autotune_symv(api, geo, params, legs)
     res = {p: run_leg(api, geo, params, opts, p, args.steps, args.warmup) for p in legs}
     nat = geo.natom
     ref = res["fp64"]
@@ -417,6 +449,7 @@ def main():
                      "phase_s_per_step": x["phases"],
                      "symv_achieved_GBs": x["symv"]["achieved"]} for p, x in res.items()},
     }
+    rec["autotune"] = tuned
     if not args.no_cpu_baseline:
         ns, dt, threads = cpu_baseline()
         scale = (n / ns) ** 3

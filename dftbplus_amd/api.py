@@ -124,6 +124,7 @@ EXPORTS = {
     "dftbhip_ctx_set_kpoints": [C.c_void_p, C.c_int, _dp, _dp],
     "dftbhip_dsytrd": [C.c_char, C.c_int, _dp, C.c_int, _dp, _dp, _dp],
     "dftbhip_set_tuning": [C.c_int, C.c_int],
+    "dftbhip_autotune": [C.c_int, C.c_int, _ip, C.c_int, _ip, _dp, C.c_int],
     "dftbhip_dlacpy": [C.c_char, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int],
     "dftbhip_batch_create": [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p],
     "dftbhip_batch_destroy": [C.c_void_p],
@@ -269,6 +270,22 @@ def dlacpy(a: np.ndarray) -> np.ndarray:
 
 def set_tuning(param: int, value: int) -> None:
     _check(lib().dftbhip_set_tuning(param, value), "dftbhip_set_tuning")
+
+
+def autotune(n: int, precision: int = 0):
+    """Autotunes the tridiagonalisation's SYMV tile count on an n x n synthetic matrix for FP64
+    (precision 0) or FP32 (1) and installs the table. Returns (rows, seconds): rows as
+    [(first trailing size, tiles), ...] ascending, seconds[o, b] the summed SYMV time of option
+    (2, 4, 8, 16)[o] in the 256-row bucket b of trailing sizes."""
+    nb = (n + 255) // 256
+    table = np.zeros(2 * 256, dtype=np.int32)
+    nrows = np.zeros(1, dtype=np.int32)
+    detail = np.zeros(4 * nb)
+    _check(lib().dftbhip_autotune(n, precision, _i(table), 256, _i(nrows), _d(detail), detail.size),
+           "dftbhip_autotune")
+    k = min(int(nrows[0]), 256)
+    rows = [(int(table[2 * r]), int(table[2 * r + 1])) for r in range(k)]
+    return rows, detail.reshape(4, nb)
 
 
 def dgemm(a: np.ndarray, b: np.ndarray, transa: str = "N", transb: str = "N", alpha: float = 1.0,

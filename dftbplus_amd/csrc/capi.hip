@@ -2404,10 +2404,30 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
   switch (param) {
     case 0:
       return dftbhip::set_symv_tiles(value);
+    case 1:
+      DH_CHECK(value == 0, "set_tuning: parameter 1 takes 0 (drop the autotuned tables)");
+      return dftbhip::clear_symv_tables();
     default:
       dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));
       return dftbhip::kErrArg;
   }
+}
+
+extern "C" int dftbhip_autotune(int n, int precision, int* table, int maxRows, int* nRows,
+                                double* detail, int detailLen) {
+  DH_CHECK(maxRows >= 0 && (maxRows == 0 || table) && nRows, "autotune: bad arguments");
+  DH_TRY(ensure_runtime());
+  std::vector<std::pair<int, int>> rows;
+  std::vector<double> det;
+  DH_TRY(symv_autotune(g_rt.stream, n, precision, rows, detail ? &det : nullptr));
+  *nRows = (int)rows.size();
+  for (int r = 0; r < std::min(maxRows, (int)rows.size()); ++r) {
+    table[2 * r] = rows[r].first;
+    table[2 * r + 1] = rows[r].second;
+  }
+  if (detail)
+    for (int k = 0; k < std::min(detailLen, (int)det.size()); ++k) detail[k] = det[k];
+  return 0;
 }
 
 extern "C" int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz) {

@@ -179,7 +179,12 @@ int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e
 int ormtr_lower_f32(hipStream_t s, const float* a, long lda, int n, const float* tau, float* z,
                     long ldz, int ncols, void* work, size_t workBytes);
 size_t sytrd_work_bytes(int n, int elemSize);
-int set_symv_tiles(int tiles);   // row tiles per SYMV workgroup: 4, 8 or 16
+int set_symv_tiles(int tiles);   // row tiles per SYMV workgroup: 2, 4, 8, 16 or 0 (by size)
+// autotuned tile tables of the SYMV (FP64: precision 0, FP32: 1), rows (first size, tiles);
+// detail: summed seconds per option (2, 4, 8, 16) and 256-row bucket of trailing sizes
+int symv_autotune(hipStream_t s, int n, int precision, std::vector<std::pair<int, int>>& rows,
+                  std::vector<double>* detail);
+int clear_symv_tables();
 // two-stage tridiagonalisation (band.hip): full symmetric input, workspace in bytes
 size_t band_work_bytes(int n, int elemSize);
 int sytrd_2stage(hipStream_t s, double* a, long lda, int n, double* d, double* e, void* work,

@@ -33,8 +33,10 @@ struct SymvProfile {
   bool on = false;
   std::vector<hipEvent_t> ev;
   std::vector<double> bytes;
+  std::vector<int> size;   // trailing size m of each recorded launch
   size_t used = 0;
   double launches = 0, seconds = 0, algBytes = 0;
+  std::vector<std::pair<int, float>>* sink = nullptr;   // (m, ms) per launch, when set
 };
 static SymvProfile g_symvProf;
 
@@ -58,6 +60,7 @@ static int symv_prof_collect(hipStream_t s) {
     DH_HIP(hipEventElapsedTime(&ms, p.ev[2 * i], p.ev[2 * i + 1]));
     p.seconds += ms * 1e-3;
     p.algBytes += p.bytes[i];
+    if (p.sink) p.sink->push_back({p.size[i], ms});
     p.launches += 1;
   }
   p.used = 0;
@@ -78,10 +81,29 @@ int set_symv_tiles(int tiles) {
   return 0;
 }
 
+// Tile choice by trailing size from the autotuner (symv_autotune), per arithmetic type
+// ([0] FP64, [1] FP32): rows (first trailing size, tiles) in ascending size; empty: the fixed
+// thresholds above.
+static std::vector<std::pair<int, int>> g_tileTable[2];
+
+int clear_symv_tables() {
+  g_tileTable[0].clear();
+  g_tileTable[1].clear();
+  return 0;
+}
+
 // Long tile chains keep more loads in flight per workgroup on a large trailing matrix; a small
 // one has too few column blocks to fill the chip, and short chains there give more workgroups.
+template <typename T>
 static int symv_tiles(int m) {
   if (g_symvTiles != 0) return g_symvTiles;
+  const auto& t = g_tileTable[std::is_same<T, float>::value ? 1 : 0];
+  if (!t.empty()) {
+    int v = t.front().second;
+    for (const auto& r : t)
+      if (m >= r.first) v = r.second;
+    return v;
+  }
   return m >= kSymvTiles8From ? 8 : (m >= kSymvTiles4From ? 4 : 2);
 }
 
@@ -500,7 +522,7 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
       const int m = n - c - 1;
       if (m <= 0) continue;
       const int nRB = (m + kTB - 1) / kTB;
-      const int tiles = symv_tiles(m);
+      const int tiles = symv_tiles<T>(m);
       const int nCh = (nRB + tiles - 1) / tiles;
       SymvProfile& pf = g_symvProf;
       if (pf.on) {
@@ -509,6 +531,7 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
           pf.ev.resize(2 * (pf.used + 1) + 2 * 1024);
           for (size_t q = old; q < pf.ev.size(); ++q) DH_HIP(hipEventCreate(&pf.ev[q]));
           pf.bytes.resize(pf.ev.size() / 2);
+          pf.size.resize(pf.ev.size() / 2);
         }
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used], s));
       }
@@ -526,6 +549,7 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used + 1], s));
         // lower triangle of the m x m trailing matrix plus v in and the partial sums out
         pf.bytes[pf.used] = sizeof(T) * ((double)m * (m + 1) / 2 + 2.0 * m);
+        pf.size[pf.used] = m;
         ++pf.used;
       }
       hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 64)), dim3(64 * kFinW), 0, s, a, lda, n, c, j0,
@@ -704,6 +728,101 @@ int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e
                     void* work, size_t workBytes) {
   return sytrd_lower_t<float>(s, a, lda, n, d, e, tau, work, workBytes);
 }
+// ---------------------------------------------------------------------------------------------
+// Autotuner of the SYMV tile count: the tridiagonalisation of a synthetic symmetric n x n matrix
+// is run once per tile option with every SYMV launch timed by HIP events; launches are grouped by
+// trailing size in buckets of kTuneBucket rows and each bucket takes the option of least summed
+// time. The result is installed as the tile table of the arithmetic type.
+// ---------------------------------------------------------------------------------------------
+constexpr int kTuneBucket = 256;
+constexpr int kTuneOptions[4] = {2, 4, 8, 16};
+
+// lower triangle of a symmetric matrix from a hash of (i, j): values in [-1, 1), diagonal
+// shifted by i / n
+template <typename T>
+__global__ void synth_sym_kernel(T* a, long lda, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n || i < j) return;
+  uint32_t h = (uint32_t)i * 0x9E3779B1u ^ ((uint32_t)j * 0x85EBCA77u + 0x165667B1u);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  double v = (double)(h >> 8) * (1.0 / 8388608.0) - 1.0;
+  if (i == j) v += (double)i / n;
+  a[i + (long)j * lda] = (T)v;
+}
+
+template <typename T>
+static int symv_autotune_t(hipStream_t s, int n, std::vector<std::pair<int, int>>& rows,
+                           std::vector<double>* detail) {
+  DH_CHECK(n >= kTuneBucket && n <= 65535, "autotune: n must be in [256, 65535]");
+  DevBuf<T> a, d, e, tau;
+  DevBuf<char> work;
+  DH_TRY(a.alloc((size_t)n * n));
+  DH_TRY(d.alloc(n));
+  DH_TRY(e.alloc(n));
+  DH_TRY(tau.alloc(n));
+  const size_t wb = sytrd_layout<T>(n, nullptr, nullptr);
+  DH_TRY(work.alloc(wb));
+  const int nb = (n + kTuneBucket - 1) / kTuneBucket;
+  std::vector<double> t(4 * (size_t)nb, 0.0);
+  std::vector<std::pair<int, float>> rec;
+  SymvProfile& pf = g_symvProf;
+  const bool on0 = pf.on;
+  const double l0 = pf.launches, s0 = pf.seconds, b0 = pf.algBytes;
+  const int tiles0 = g_symvTiles;
+  int rc = 0;
+  for (int o = 0; o < 4 && rc == 0; ++o) {
+    g_symvTiles = kTuneOptions[o];
+    // untimed pass on a small matrix first: the option's kernels are loaded before timing
+    for (int nn : {kTuneBucket, n}) {
+      hipLaunchKernelGGL(synth_sym_kernel<T>, dim3(cdiv(nn, 256), nn), dim3(256), 0, s, a.p,
+                         (long)nn, nn);
+      rc = hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+      if (rc) {
+        set_error("autotune: launch of the test matrix kernel failed");
+        break;
+      }
+      pf.on = nn == n;
+      pf.sink = nn == n ? &rec : nullptr;
+      rec.clear();
+      rc = sytrd_lower_t<T>(s, a.p, nn, nn, d.p, e.p, tau.p, work.p, wb);
+      pf.sink = nullptr;
+      if (rc) break;
+    }
+    for (const auto& r : rec) t[(size_t)o * nb + r.first / kTuneBucket] += 1e-3 * r.second;
+  }
+  g_symvTiles = tiles0;
+  pf.on = on0;
+  pf.launches = l0;
+  pf.seconds = s0;
+  pf.algBytes = b0;
+  pf.sink = nullptr;
+  if (rc) return rc;
+  DH_HIP(hipStreamSynchronize(s));
+  rows.clear();
+  for (int b = 0; b < nb; ++b) {
+    int best = 0;
+    for (int o = 1; o < 4; ++o)
+      if (t[(size_t)o * nb + b] < t[(size_t)best * nb + b]) best = o;
+    if (rows.empty() || rows.back().second != kTuneOptions[best])
+      rows.push_back({b * kTuneBucket, kTuneOptions[best]});
+  }
+  rows.front().first = 0;
+  if (detail) *detail = t;
+  g_tileTable[std::is_same<T, float>::value ? 1 : 0] = rows;
+  return 0;
+}
+
+int symv_autotune(hipStream_t s, int n, int precision, std::vector<std::pair<int, int>>& rows,
+                  std::vector<double>* detail) {
+  DH_CHECK(precision == 0 || precision == 1, "autotune: precision must be 0 (FP64) or 1 (FP32)");
+  return precision == 0 ? symv_autotune_t<double>(s, n, rows, detail)
+                        : symv_autotune_t<float>(s, n, rows, detail);
+}
+
 int ormtr_lower(hipStream_t s, const double* a, long lda, int n, const double* tau, double* z,
                 long ldz, int ncols, double* work, size_t workSize) {
   return ormtr_lower_t<double>(s, a, lda, n, tau, z, ldz, ncols, work, workSize * sizeof(double),

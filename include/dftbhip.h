@@ -41,9 +41,19 @@ int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz);
  * src/dftbp/math/eigensolver.F90:561-660). */
 int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e, double* tau);
 /* Kernel tuning. param 0: row tiles per workgroup of the tridiagonalisation's SYMV (2, 4, 8,
- * 16, or 0 = chosen from the trailing size per column, the default). No counterpart in the
- * reference. */
+ * 16, or 0 = chosen from the trailing size per column, the default: the autotuned table of the
+ * arithmetic type when there is one, else fixed thresholds); param 1, value 0: drop the
+ * autotuned tables. No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
+/* Autotuner of the SYMV tile count (no counterpart in the reference): tridiagonalises a
+ * synthetic symmetric n x n matrix (256 <= n <= 65535) once per option 2, 4, 8, 16 with every
+ * SYMV launch timed, picks per 256-row bucket of trailing sizes the option of least time and
+ * installs the result for precision 0 (FP64) or 1 (FP32). table: up to maxRows rows
+ * (first trailing size, tiles), ascending; *nRows: rows in the installed table. detail
+ * (optional, detailLen doubles): summed seconds, option-major: detail[o * nb + b] for option
+ * o and bucket b, nb = ceil(n / 256). Allocates device memory for the test matrix. */
+int dftbhip_autotune(int n, int precision, int* table, int maxRows, int* nRows, double* detail,
+                     int detailLen);
 /* LAPACK dlacpy (uplo = 'A'): b = a through the device copy kernel (the copy used for HBM
  * counter calibration; DFTB+ copies square matrices with plain array assignment). */
 int dftbhip_dlacpy(char uplo, int m, int n, const double* a, int lda, double* b, int ldb);
