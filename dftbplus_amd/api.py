@@ -276,7 +276,8 @@ def autotune(n: int, precision: int = 0):
     """Autotunes the tridiagonalisation's SYMV tile count on an n x n synthetic matrix for FP64
     (precision 0) or FP32 (1) and installs the table. Returns (rows, seconds): rows as
     [(first trailing size, tiles), ...] ascending, seconds[o, b] the summed SYMV time of option
-    (2, 4, 8, 16)[o] in the 256-row bucket b of trailing sizes."""
+    (2, 4, 8, 16)[o] in the 256-row bucket b of trailing sizes. A bucket keeps the option of the
+    bucket below unless another is at least 2 % faster there."""
     nb = (n + 255) // 256
     table = np.zeros(2 * 256, dtype=np.int32)
     nrows = np.zeros(1, dtype=np.int32)

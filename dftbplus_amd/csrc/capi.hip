@@ -1160,6 +1160,10 @@ struct SccState {
   FillResult fr{};
 };
 
+// k-points of one real size tridiagonalised together (1, the default) or one by one (0); set
+// through dftbhip_set_tuning parameter 2
+static int g_kBatch = 1;
+
 struct Context {
   dftbhip_options opt{};
   SccState st;
@@ -1187,6 +1191,10 @@ struct Context {
   std::vector<double> kpts, kw;
   DevBuf<double> blkT;
   std::vector<DevBuf<double>> LinvK, XK;
+  // k-points tridiagonalised together (all real, one size): reduced matrices and reflectors,
+  // d, e, tau and the workspace blocks of every point
+  DevBuf<double> kA, kD, kE, kTau;
+  DevBuf<char> kWs;
   std::vector<std::vector<double>> eigK, fillK;
   std::vector<int> noccK;
   EigenSolver eig;
@@ -1463,7 +1471,52 @@ static int ctx_scc_step(Context& c) {
     c.xf64 = true;
     DH_HIP(hipEventRecord(ev[1], s));
     std::vector<std::vector<double>> ec(c.nk);
-    for (int k = 0; k < c.nk; ++k) {
+    bool together = g_kBatch && c.nk > 1 && !c.eig.twoStage;
+    for (int k = 0; k < c.nk && together; ++k) together = kpoint_dim(&c.kpts[3 * k], n) == n;
+    if (together) {
+      // every point real and of size n: the reductions per point, one tridiagonalisation pass
+      // over all points (one launch per column kernel for all of them), then per point the
+      // divide and conquer and the back-transformation
+      const int K = c.nk;
+      const size_t nn = (size_t)n * n, wsStride = sytrd_work_bytes(n, 8);
+      DH_TRY(c.eig.ensure(n));
+      DH_TRY(c.kA.alloc(K * nn));
+      DH_TRY(c.kD.alloc((size_t)K * n));
+      DH_TRY(c.kE.alloc((size_t)K * n));
+      DH_TRY(c.kTau.alloc((size_t)K * n));
+      DH_TRY(c.kWs.alloc(K * wsStride));
+      hipEvent_t kev[3];
+      for (auto& x : kev) DH_HIP(hipEventCreate(&x));
+      DH_HIP(hipEventRecord(kev[0], s));
+      for (int k = 0; k < K; ++k) {
+        DH_TRY(launch_unpack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
+                                 &c.kpts[3 * k], c.hs.p, c.Hd.p, n, n));
+        DH_TRY(c.eig.reduce(s, c.LinvK[k].p, c.Hd.p, n, c.kA.p + k * nn));
+      }
+      DH_HIP(hipEventRecord(kev[1], s));
+      DH_TRY(sytrd_lower_batched(s, c.kA.p, n, (long)nn, n, c.kD.p, c.kE.p, c.kTau.p, n,
+                                 c.kWs.p, wsStride, K));
+      DH_HIP(hipEventRecord(kev[2], s));
+      DH_HIP(hipEventSynchronize(kev[2]));
+      float ms;
+      DH_HIP(hipEventElapsedTime(&ms, kev[0], kev[1]));
+      evt[0] += ms * 1e-3;
+      DH_HIP(hipEventElapsedTime(&ms, kev[1], kev[2]));
+      evt[1] += ms * 1e-3;
+      for (auto& x : kev) (void)hipEventDestroy(x);
+      for (int k = 0; k < K; ++k) {
+        const double t0 = now();
+        int info = 0;
+        DH_TRY(c.eig.tridiag_eigen(s, c.kD.p + (size_t)k * n, c.kE.p + (size_t)k * n, c.eigK[k],
+                                   &info));
+        if (info != 0) return info;
+        evt[2] += now() - t0;
+        DH_TRY(c.eig.vectors_from(s, c.LinvK[k].p, c.kA.p + k * nn, c.kTau.p + (size_t)k * n, n,
+                                  c.XK[k].p, n));
+        ec[k] = c.eigK[k];
+      }
+    }
+    for (int k = 0; k < c.nk && !together; ++k) {
       const int ndk = kpoint_dim(&c.kpts[3 * k], n);
       DH_TRY(launch_unpack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
                                &c.kpts[3 * k], c.hs.p, c.Hd.p, ndk, n));
@@ -2407,6 +2460,10 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
     case 1:
       DH_CHECK(value == 0, "set_tuning: parameter 1 takes 0 (drop the autotuned tables)");
       return dftbhip::clear_symv_tables();
+    case 2:
+      DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 2 takes 0 or 1");
+      dftbhip::g_kBatch = value;
+      return 0;
     default:
       dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));
       return dftbhip::kErrArg;

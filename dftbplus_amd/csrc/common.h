@@ -167,6 +167,10 @@ int zero_strict_upper(hipStream_t s, double* a, long lda, int n);
 int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e, double* tau,
                 double* work, size_t workSize);
 size_t sytrd_work_size(int n);
+// K problems of size n in one pass (matrices at a + k sA, vectors at + k sV, workspace blocks at
+// work + k wsStride bytes, wsStride >= sytrd_work_bytes(n, 8) and a multiple of 256)
+int sytrd_lower_batched(hipStream_t s, double* a, long lda, long sA, int n, double* d, double* e,
+                        double* tau, long sV, void* work, size_t wsStride, int K);
 // per-launch timing of the tridiagonalisation's symmetric product: launches, seconds, bytes
 void symv_profiling(bool on);
 void symv_profile_stats(double* out);

@@ -35,6 +35,17 @@ struct EigenSolver {
   // The first nvec generalized eigenvectors into X (n x nvec).
   int vectors(hipStream_t s, int nvec, double* X, long ldx);
 
+  // Pieces of solve/vectors for problems that keep their own inverse factor and reflectors
+  // (k-points tridiagonalised together): A = Linv H Linv^T (lower triangle, n x n, T as the
+  // intermediate); the eigenvalues (to wHost) and tridiagonal eigenvectors (kept for
+  // vectors_from) of the device tridiagonal (dDev, eDev); X = Linv^T Q Z for the reflectors
+  // (A, tauDev) of the one-stage tridiagonalisation.
+  int reduce(hipStream_t s, const double* Li, const double* H, long ldh, double* A);
+  int tridiag_eigen(hipStream_t s, const double* dDev, const double* eDev,
+                    std::vector<double>& wHost, int* info);
+  int vectors_from(hipStream_t s, const double* Li, const double* A, const double* tauDev,
+                   int nvec, double* X, long ldx);
+
   // Mixed precision: the reduction, tridiagonalisation and back-transformation run in FP32
   // (FP32 MFMA, FP32 accumulation); the Cholesky factor comes from the FP64 factor_overlap and
   // the tridiagonal eigenproblem is solved in FP64. prepare_mixed rounds Linv once per geometry.

@@ -50,6 +50,39 @@ int EigenSolver::factor_overlap(hipStream_t s, const double* S, long lds, int* i
   return 0;
 }
 
+int EigenSolver::reduce(hipStream_t s, const double* Li, const double* H, long ldh, double* A) {
+  DH_CHECK(factored, "eigensolver: overlap not factorised");
+  DH_TRY(gemm<double>(s, false, false, n, n, n, 1.0, Li, n, H, ldh, 0.0, T.p, n, false,
+                      kKALower));
+  DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, T.p, n, Li, n, 0.0, A, n, true, kKBUpper));
+  return 0;
+}
+
+int EigenSolver::tridiag_eigen(hipStream_t s, const double* dDev, const double* eDev,
+                               std::vector<double>& wHost, int* info) {
+  *info = 0;
+  std::vector<double> dh(n), eh(std::max(n - 1, 1));
+  DH_HIP(hipMemcpyAsync(dh.data(), dDev, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (n > 1)
+    DH_HIP(hipMemcpyAsync(eh.data(), eDev, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  DH_TRY(stedc(s, n, dh.data(), eh.data(), w.p, Zd.p, n, work.p, workSize, info));
+  if (*info != 0) return 0;
+  wHost.resize(n);
+  DH_HIP(hipMemcpyAsync(wHost.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int EigenSolver::vectors_from(hipStream_t s, const double* Li, const double* A,
+                              const double* tauDev, int nvec, double* X, long ldx) {
+  if (nvec <= 0) return 0;
+  DH_TRY(ormtr_lower(s, A, n, n, tauDev, Zd.p, n, nvec, work.p, workSize));
+  DH_TRY(gemm<double>(s, true, false, n, nvec, n, 1.0, Li, n, Zd.p, n, 0.0, X, ldx, false,
+                      kKAUpper));
+  return 0;
+}
+
 int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<double>& wHost,
                        int* info, double* times) {
   DH_CHECK(factored, "eigensolver: overlap not factorised");
@@ -57,11 +90,7 @@ int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<dou
   hipEvent_t ev[5];
   for (auto& x : ev) DH_HIP(hipEventCreate(&x));
   DH_HIP(hipEventRecord(ev[0], s));
-  // A = Linv H Linv^T
-  DH_TRY(gemm<double>(s, false, false, n, n, n, 1.0, Linv.p, n, H, ldh, 0.0, T.p, n, false,
-                      kKALower));
-  DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, T.p, n, Linv.p, n, 0.0, C.p, n, true,
-                      kKBUpper));
+  DH_TRY(reduce(s, Linv.p, H, ldh, C.p));
   DH_HIP(hipEventRecord(ev[1], s));
   if (twoStage) {
     DH_TRY(symmetrize_lower(s, C.p, n, n));
@@ -70,16 +99,13 @@ int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<dou
     DH_TRY(sytrd_lower(s, C.p, n, n, d.p, e.p, tau.p, work.p, workSize));
   }
   DH_HIP(hipEventRecord(ev[2], s));
-  std::vector<double> dh(n), eh(std::max(n - 1, 1));
-  DH_HIP(hipMemcpyAsync(dh.data(), d.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (n > 1)
-    DH_HIP(hipMemcpyAsync(eh.data(), e.p, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
-  DH_HIP(hipStreamSynchronize(s));
-  DH_TRY(stedc(s, n, dh.data(), eh.data(), w.p, Zd.p, n, work.p, workSize, info));
+  DH_TRY(tridiag_eigen(s, d.p, e.p, wHost, info));
+  if (*info != 0) {
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    return 0;
+  }
   DH_HIP(hipEventRecord(ev[3], s));
-  wHost.resize(n);
-  DH_HIP(hipMemcpyAsync(wHost.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-  DH_HIP(hipStreamSynchronize(s));
+  DH_HIP(hipEventSynchronize(ev[3]));
   if (times) {
     float ms;
     DH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -95,14 +121,14 @@ int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<dou
 
 int EigenSolver::vectors(hipStream_t s, int nvec, double* X, long ldx) {
   if (nvec <= 0) return 0;
-  if (twoStage)
+  if (twoStage) {
     DH_TRY(ormtr_2stage(s, C.p, n, n, Zd.p, n, nvec, band.p, bandBytes, work.p,
                         workSize * sizeof(double)));
-  else
-    DH_TRY(ormtr_lower(s, C.p, n, n, tau.p, Zd.p, n, nvec, work.p, workSize));
-  DH_TRY(gemm<double>(s, true, false, n, nvec, n, 1.0, Linv.p, n, Zd.p, n, 0.0, X, ldx, false,
-                      kKAUpper));
-  return 0;
+    DH_TRY(gemm<double>(s, true, false, n, nvec, n, 1.0, Linv.p, n, Zd.p, n, 0.0, X, ldx, false,
+                        kKAUpper));
+    return 0;
+  }
+  return vectors_from(s, Linv.p, C.p, tau.p, nvec, X, ldx);
 }
 
 // Ogita-Aishima correction E (n x nv) from G = X^T S X and M = X^T H X (both full):

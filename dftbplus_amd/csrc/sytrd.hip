@@ -156,6 +156,19 @@ size_t sytrd_work_bytes(int n, int elemSize) {
 }
 size_t sytrd_work_size(int n) { return (sytrd_work_bytes(n, 8) + 7) / 8; }
 
+// Several problems of one size in one launch: problem blockIdx.z finds its matrix, workspace
+// block and vectors at these strides from those of problem 0 (all zero for a single problem).
+struct ProblemStrides {
+  long a;    // matrix elements
+  long ws;   // workspace bytes
+  long v;    // elements of d, e, tau
+};
+
+template <typename P>
+__device__ __forceinline__ P* ws_of(P* p, long bytes) {
+  return (P*)((const char*)p + (long)blockIdx.z * bytes);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -193,8 +206,14 @@ template <typename T>
 __global__ __launch_bounds__(256) void col_update_kernel(T* a, long lda, int n, int c, int j0,
                                                          const T* W, long ldw, const T* tau,
                                                          const double* dot, double* nrm2,
-                                                         double* t12) {
+                                                         double* t12, ProblemStrides ps) {
   __shared__ double vc[kNB], wc[kNB], al[kNB], part[4][64];
+  a += (long)blockIdx.z * ps.a;
+  tau += (long)blockIdx.z * ps.v;
+  W = ws_of(W, ps.ws);
+  dot = ws_of(dot, ps.ws);
+  nrm2 = ws_of(nrm2, ps.ws);
+  t12 = ws_of(t12, ps.ws);
   const int i = c - j0;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (threadIdx.x < i) {
@@ -255,9 +274,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
                                                    const T* W, long ldw, const double* nrm2,
                                                    const T* tauA, const double* dotA, T* Y,
                                                    int nSlot, T* vbuf, double* t12,
-                                                   double* scalOut, T* eOut, T* tauOut) {
+                                                   double* scalOut, T* eOut, T* tauOut,
+                                                   ProblemStrides ps) {
   __shared__ T vJs[kTB];
   __shared__ T red[4][4][kTB];
+  a += (long)blockIdx.z * ps.a;
+  tauA += (long)blockIdx.z * ps.v;
+  eOut += (long)blockIdx.z * ps.v;
+  tauOut += (long)blockIdx.z * ps.v;
+  W = ws_of(W, ps.ws);
+  nrm2 = ws_of(nrm2, ps.ws);
+  dotA = ws_of(dotA, ps.ws);
+  Y = ws_of(Y, ps.ws);
+  vbuf = ws_of(vbuf, ps.ws);
+  t12 = ws_of(t12, ps.ws);
+  scalOut = ws_of(scalOut, ps.ws);
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
   const int J = blockIdx.x, ch = blockIdx.y;
@@ -402,8 +433,16 @@ __global__ __launch_bounds__(64 * kFinW) void finalize_w_kernel(T* a, long lda, 
                                                                 int j0, T* W, long ldw, const T* Y,
                                                                 int nSlot, const T* vbuf,
                                                                 const double* t12, const T* tauA,
-                                                                double* dot, int tiles) {
+                                                                double* dot, int tiles,
+                                                                ProblemStrides ps) {
   __shared__ double red[kFinW][64];
+  a += (long)blockIdx.z * ps.a;
+  tauA += (long)blockIdx.z * ps.v;
+  W = ws_of(W, ps.ws);
+  Y = ws_of(Y, ps.ws);
+  vbuf = ws_of(vbuf, ps.ws);
+  t12 = ws_of(t12, ps.ws);
+  dot = ws_of(dot, ps.ws);
   const int m = n - c - 1;
   const int nRB = (m + kTB - 1) / kTB;
   const int i = c - j0;
@@ -500,24 +539,34 @@ __global__ void panel_restore_kernel(T* a, long lda, int n, int j0, int pw, T* d
   if (c + 1 < n) a[(c + 1) + (long)c * lda] = e[c];
 }
 
+// K problems of size n at once: problem k's matrix at a + k sA, d, e, tau at + k sV and its
+// workspace block at work + k wsStride bytes (wsStride >= sytrd_layout(n)). The column kernels
+// run all problems in one launch each; the panel products and trailing updates run per problem.
 template <typename T>
-int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void* work,
-                  size_t workBytes) {
-  DH_CHECK(workBytes >= sytrd_layout<T>(n, nullptr, nullptr), "sytrd: workspace too small");
-  SytrdWs<T> ws;
-  sytrd_layout<T>(n, &ws, static_cast<char*>(work));
-  DH_TRY(fill<double>(s, ws.nrm2, n, 0.0));
-  DH_TRY(fill<double>(s, ws.dot, n, 0.0));
-  DH_TRY(fill<double>(s, ws.t12, (size_t)kT12Size, 0.0));
-  DH_TRY(fill<T>(s, e, std::max(n - 1, 1), T(0)));
-  DH_TRY(fill<T>(s, tau, std::max(n, 1), T(0)));
+int sytrd_lower_batched_t(hipStream_t s, T* a, long lda, long sA, int n, T* d, T* e, T* tau,
+                          long sV, void* work, size_t wsStride, int K) {
+  DH_CHECK(K >= 1, "sytrd: no problems");
+  DH_CHECK(wsStride >= sytrd_layout<T>(n, nullptr, nullptr), "sytrd: workspace too small");
+  DH_CHECK(K == 1 || (sA >= lda * (long)n && sV >= n && wsStride % 256 == 0),
+           "sytrd: problems overlap or are misaligned");
+  std::vector<SytrdWs<T>> wk(K);
+  for (int k = 0; k < K; ++k) {
+    sytrd_layout<T>(n, &wk[k], static_cast<char*>(work) + (size_t)k * wsStride);
+    DH_TRY(fill<double>(s, wk[k].nrm2, n, 0.0));
+    DH_TRY(fill<double>(s, wk[k].dot, n, 0.0));
+    DH_TRY(fill<double>(s, wk[k].t12, (size_t)kT12Size, 0.0));
+    DH_TRY(fill<T>(s, e + k * sV, std::max(n - 1, 1), T(0)));
+    DH_TRY(fill<T>(s, tau + k * sV, std::max(n, 1), T(0)));
+  }
+  const SytrdWs<T>& ws = wk[0];
+  const ProblemStrides ps{K > 1 ? sA : 0, K > 1 ? (long)wsStride : 0, K > 1 ? sV : 0};
   const long ldw = n;
   for (int j0 = 0; j0 < n; j0 += kNB) {
     const int pw = std::min(kNB, n - j0);
     for (int i = 0; i < pw; ++i) {
       const int c = j0 + i;
-      hipLaunchKernelGGL(col_update_kernel<T>, dim3(cdiv(n - c, 64)), dim3(256), 0, s, a, lda, n,
-                         c, j0, ws.W, ldw, tau, ws.dot, ws.nrm2, ws.t12);
+      hipLaunchKernelGGL(col_update_kernel<T>, dim3(cdiv(n - c, 64), 1, K), dim3(256), 0, s, a,
+                         lda, n, c, j0, ws.W, ldw, tau, ws.dot, ws.nrm2, ws.t12, ps);
       DH_LAUNCH_CHECK();
       const int m = n - c - 1;
       if (m <= 0) continue;
@@ -535,10 +584,10 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
         }
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used], s));
       }
-#define DH_SYMV(CH_)                                                                       \
-  hipLaunchKernelGGL((symv_kernel<T, CH_>), dim3(nRB, nCh), dim3(256), 0, s, a, lda, n, c, j0,  \
-                     ws.W, ldw, ws.nrm2, tau, ws.dot, ws.Y, ws.nSlot, ws.vbuf, ws.t12, ws.scal, \
-                     e, tau)
+#define DH_SYMV(CH_)                                                                        \
+  hipLaunchKernelGGL((symv_kernel<T, CH_>), dim3(nRB, nCh, K), dim3(256), 0, s, a, lda, n, c, \
+                     j0, ws.W, ldw, ws.nrm2, tau, ws.dot, ws.Y, ws.nSlot, ws.vbuf, ws.t12,      \
+                     ws.scal, e, tau, ps)
       if (tiles == 2) DH_SYMV(2);
       else if (tiles == 4) DH_SYMV(4);
       else if (tiles == 16) DH_SYMV(16);
@@ -547,30 +596,44 @@ int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void
       DH_LAUNCH_CHECK();
       if (pf.on) {
         DH_HIP(hipEventRecord(pf.ev[2 * pf.used + 1], s));
-        // lower triangle of the m x m trailing matrix plus v in and the partial sums out
-        pf.bytes[pf.used] = sizeof(T) * ((double)m * (m + 1) / 2 + 2.0 * m);
+        // lower triangle of the m x m trailing matrix plus v in and the partial sums out, for
+        // each problem
+        pf.bytes[pf.used] = (double)K * sizeof(T) * ((double)m * (m + 1) / 2 + 2.0 * m);
         pf.size[pf.used] = m;
         ++pf.used;
       }
-      hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 64)), dim3(64 * kFinW), 0, s, a, lda, n, c, j0,
-                         ws.W, ldw, ws.Y, ws.nSlot, ws.vbuf, ws.t12, tau, ws.dot, tiles);
+      hipLaunchKernelGGL(finalize_w_kernel<T>, dim3(cdiv(m, 64), 1, K), dim3(64 * kFinW), 0, s,
+                         a, lda, n, c, j0, ws.W, ldw, ws.Y, ws.nSlot, ws.vbuf, ws.t12, tau,
+                         ws.dot, tiles, ps);
       DH_LAUNCH_CHECK();
     }
     const int r0 = j0 + pw;
-    if (r0 < n) {
-      const int mt = n - r0;
-      hipLaunchKernelGGL(panel_pack_kernel<T>, dim3(cdiv(mt, 256), pw), dim3(256), 0, s, a, lda, n,
-                         r0, j0, pw, ws.W, ldw, tau, ws.dot, ws.P, ws.Q, (long)mt);
+    for (int k = 0; k < K; ++k) {
+      T* ak = a + k * sA;
+      const T* tk = tau + k * sV;
+      if (r0 < n) {
+        const int mt = n - r0;
+        hipLaunchKernelGGL(panel_pack_kernel<T>, dim3(cdiv(mt, 256), pw), dim3(256), 0, s, ak,
+                           lda, n, r0, j0, pw, wk[k].W, ldw, tk, wk[k].dot, wk[k].P, wk[k].Q,
+                           (long)mt);
+        DH_LAUNCH_CHECK();
+        T* a22 = ak + r0 + (long)r0 * lda;
+        DH_TRY(gemm<T>(s, false, true, mt, mt, 2 * pw, T(-1), wk[k].P, mt, wk[k].Q, mt, T(1),
+                       a22, lda, true));
+      }
+      hipLaunchKernelGGL(panel_restore_kernel<T>, dim3(1), dim3(64), 0, s, ak, lda, n, j0, pw,
+                         d + k * sV, e + k * sV);
       DH_LAUNCH_CHECK();
-      T* a22 = a + r0 + (long)r0 * lda;
-      DH_TRY(gemm<T>(s, false, true, mt, mt, 2 * pw, T(-1), ws.P, mt, ws.Q, mt, T(1), a22, lda,
-                     true));
     }
-    hipLaunchKernelGGL(panel_restore_kernel<T>, dim3(1), dim3(64), 0, s, a, lda, n, j0, pw, d, e);
-    DH_LAUNCH_CHECK();
   }
   DH_TRY(symv_prof_collect(s));
   return 0;
+}
+
+template <typename T>
+int sytrd_lower_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, T* tau, void* work,
+                  size_t workBytes) {
+  return sytrd_lower_batched_t<T>(s, a, lda, 0, n, d, e, tau, 0, work, workBytes, 1);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -724,6 +787,10 @@ int sytrd_lower(hipStream_t s, double* a, long lda, int n, double* d, double* e,
                 double* work, size_t workSize) {
   return sytrd_lower_t<double>(s, a, lda, n, d, e, tau, work, workSize * sizeof(double));
 }
+int sytrd_lower_batched(hipStream_t s, double* a, long lda, long sA, int n, double* d, double* e,
+                        double* tau, long sV, void* work, size_t wsStride, int K) {
+  return sytrd_lower_batched_t<double>(s, a, lda, sA, n, d, e, tau, sV, work, wsStride, K);
+}
 int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e, float* tau,
                     void* work, size_t workBytes) {
   return sytrd_lower_t<float>(s, a, lda, n, d, e, tau, work, workBytes);
@@ -732,10 +799,12 @@ int sytrd_lower_f32(hipStream_t s, float* a, long lda, int n, float* d, float* e
 // Autotuner of the SYMV tile count: the tridiagonalisation of a synthetic symmetric n x n matrix
 // is run once per tile option with every SYMV launch timed by HIP events; launches are grouped by
 // trailing size in buckets of kTuneBucket rows and each bucket takes the option of least summed
-// time. The result is installed as the tile table of the arithmetic type.
+// time, switching away from the option of the bucket below only for a gain of kTuneGain. The
+// result is installed as the tile table of the arithmetic type.
 // ---------------------------------------------------------------------------------------------
 constexpr int kTuneBucket = 256;
 constexpr int kTuneOptions[4] = {2, 4, 8, 16};
+constexpr double kTuneGain = 0.02;
 
 // lower triangle of a symmetric matrix from a hash of (i, j): values in [-1, 1), diagonal
 // shifted by i / n
@@ -802,13 +871,18 @@ static int symv_autotune_t(hipStream_t s, int n, std::vector<std::pair<int, int>
   pf.sink = nullptr;
   if (rc) return rc;
   DH_HIP(hipStreamSynchronize(s));
+  // a bucket keeps the option of the bucket below unless another is kTuneGain faster there
+  // (buckets of small trailing sizes are launch bound and differ by noise only)
   rows.clear();
+  int cur = -1;
   for (int b = 0; b < nb; ++b) {
     int best = 0;
     for (int o = 1; o < 4; ++o)
       if (t[(size_t)o * nb + b] < t[(size_t)best * nb + b]) best = o;
-    if (rows.empty() || rows.back().second != kTuneOptions[best])
-      rows.push_back({b * kTuneBucket, kTuneOptions[best]});
+    if (cur < 0 || t[(size_t)best * nb + b] < (1.0 - kTuneGain) * t[(size_t)cur * nb + b]) {
+      if (best != cur) rows.push_back({b * kTuneBucket, kTuneOptions[best]});
+      cur = best;
+    }
   }
   rows.front().first = 0;
   if (detail) *detail = t;

@@ -43,11 +43,14 @@ int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e, d
 /* Kernel tuning. param 0: row tiles per workgroup of the tridiagonalisation's SYMV (2, 4, 8,
  * 16, or 0 = chosen from the trailing size per column, the default: the autotuned table of the
  * arithmetic type when there is one, else fixed thresholds); param 1, value 0: drop the
- * autotuned tables. No counterpart in the reference. */
+ * autotuned tables; param 2: k-points that are all real and of one size are tridiagonalised
+ * together, one launch per column kernel for all points (1, the default) or one by one (0).
+ * No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Autotuner of the SYMV tile count (no counterpart in the reference): tridiagonalises a
  * synthetic symmetric n x n matrix (256 <= n <= 65535) once per option 2, 4, 8, 16 with every
- * SYMV launch timed, picks per 256-row bucket of trailing sizes the option of least time and
+ * SYMV launch timed, picks per 256-row bucket of trailing sizes the option of least time
+ * (keeping the option of the bucket below unless another is at least 2 % faster) and
  * installs the result for precision 0 (FP64) or 1 (FP32). table: up to maxRows rows
  * (first trailing size, tiles), ascending; *nRows: rows in the installed table. detail
  * (optional, detailLen doubles): summed seconds, option-major: detail[o * nb + b] for option

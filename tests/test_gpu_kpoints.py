@@ -54,3 +54,27 @@ def test_single_gamma_kpoint_equals_real_path():
     assert np.max(np.abs(d0.get_gradients() - d1.get_gradients())) < 1e-9
     d0.close()
     d1.close()
+
+
+def test_kpoints_together_equal_one_by_one():
+    """Eight real points of 247 orbitals tridiagonalised in one pass (one launch per column
+    kernel for all points) give the results of the point-by-point eigensolves."""
+    from dftbplus_amd import api, synth
+    geo, params, opts = synth.workload("diamond2")
+    o = dataclasses.replace(opts, kpoints=MESHES["gc222"])
+    out = {}
+    try:
+        for mode in (1, 0):
+            api.set_tuning(2, mode)
+            dp = api.DftbPlus(params, geo, o)
+            out[mode] = (dp.get_energy(), dp.get_gross_charges(), dp.get_gradients(),
+                         dp.n_iter, dp.converged)
+            dp.close()
+    finally:
+        api.set_tuning(2, 1)
+    (e1, q1, g1, n1, c1), (e0, q0, g0, n0, c0) = out[1], out[0]
+    print(f"together {e1:.14f} ({n1} it), one by one {e0:.14f} ({n0} it)")
+    assert c1 and c0 and n1 == n0
+    assert abs(e1 - e0) / geo.natom < 1e-13
+    assert np.max(np.abs(q1 - q0)) < 1e-12
+    assert np.max(np.abs(g1 - g0)) < 1e-11
