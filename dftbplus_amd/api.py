@@ -125,6 +125,7 @@ EXPORTS = {
     "dftbhip_dsytrd": [C.c_char, C.c_int, _dp, C.c_int, _dp, _dp, _dp],
     "dftbhip_set_tuning": [C.c_int, C.c_int],
     "dftbhip_autotune": [C.c_int, C.c_int, _ip, C.c_int, _ip, _dp, C.c_int],
+    "dftbhip_neighbour_list": [C.c_int, _dp, _dp, C.c_double, _ip, _ip, _dp, _ip, _ip, _ip, _dp],
     "dftbhip_dlacpy": [C.c_char, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int],
     "dftbhip_batch_create": [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p],
     "dftbhip_batch_destroy": [C.c_void_p],
@@ -270,6 +271,32 @@ def dlacpy(a: np.ndarray) -> np.ndarray:
 
 def set_tuning(param: int, value: int) -> None:
     _check(lib().dftbhip_set_tuning(param, value), "dftbhip_set_tuning")
+
+
+def neighbour_list(coords: np.ndarray, latvecs: Optional[np.ndarray], cutoff: float):
+    """Half neighbour list built on the device (updateNeighbourList): returns (coords_all
+    (nAll, 3), img2cent (nAll,), neigh, dist2) with 0-based indices; neigh[a] and dist2[a] start
+    with the atom itself."""
+    x = np.ascontiguousarray(coords, dtype=np.float64)
+    nat = x.shape[0]
+    lat = None if latvecs is None else np.ascontiguousarray(latvecs, dtype=np.float64)
+    nall = np.zeros(1, dtype=np.int32)
+    mnb = np.zeros(1, dtype=np.int32)
+    latp = None if lat is None else _d(lat)
+    _check(lib().dftbhip_neighbour_list(nat, _d(x), latp, cutoff, _i(nall), _i(mnb), None, None,
+                                        None, None, None), "dftbhip_neighbour_list")
+    na, mn = int(nall[0]), int(mnb[0])
+    call = np.zeros((na, 3))
+    i2c = np.zeros(na, dtype=np.int32)
+    inb = np.zeros((nat, mn + 1), dtype=np.int32)    # row a = column a of iNeighbour
+    nnb = np.zeros(nat, dtype=np.int32)
+    d2 = np.zeros((nat, mn + 1))
+    _check(lib().dftbhip_neighbour_list(nat, _d(x), latp, cutoff, _i(nall), _i(mnb), _d(call),
+                                        _i(i2c), _i(inb), _i(nnb), _d(d2)),
+           "dftbhip_neighbour_list")
+    neigh = [inb[a, : nnb[a] + 1].astype(np.int64) - 1 for a in range(nat)]
+    dist2 = [d2[a, : nnb[a] + 1].copy() for a in range(nat)]
+    return call, i2c.astype(np.int64) - 1, neigh, dist2
 
 
 def autotune(n: int, precision: int = 0):

@@ -19,6 +19,7 @@
 #include "../../include/dftbhip.h"
 #include "common.h"
 #include "batch.h"
+#include "neigh.h"
 #include "eigen.h"
 #include "sparse.h"
 
@@ -413,6 +414,35 @@ struct DevBlocks {
     pr.species = this->species.p;
     return 0;
   }
+  // the same from lists already on the device (start null: zeros)
+  int from_device(int n, const int* dAt1, const int* dImg2, const int* dAt2, const long* dStart,
+                  const std::vector<int>& species, const std::vector<int>& norbAtom,
+                  const std::vector<long>& atomStart, hipStream_t s) {
+    const size_t m = std::max(n, 1);
+    DH_TRY(at1.alloc(m));
+    DH_TRY(img2.alloc(m));
+    DH_TRY(at2.alloc(m));
+    DH_TRY(start.alloc(m));
+    if (n > 0) {
+      DH_HIP(hipMemcpyAsync(at1.p, dAt1, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+      DH_HIP(hipMemcpyAsync(img2.p, dImg2, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+      DH_HIP(hipMemcpyAsync(at2.p, dAt2, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+      if (dStart)
+        DH_HIP(hipMemcpyAsync(start.p, dStart, n * sizeof(long), hipMemcpyDeviceToDevice, s));
+      else
+        DH_TRY(fill<long>(s, start.p, n, 0L));
+    }
+    if (!species.empty()) DH_TRY(this->species.upload(species.data(), species.size(), s));
+    DH_TRY(norb.upload(norbAtom.data(), norbAtom.size(), s));
+    DH_TRY(this->atomStart.upload(atomStart.data(), atomStart.size(), s));
+    pr.nblk = n;
+    pr.at1 = at1.p;
+    pr.img2 = img2.p;
+    pr.at2 = at2.p;
+    pr.start = start.p;
+    pr.species = this->species.p;
+    return 0;
+  }
 };
 
 int atom_layout(int nAtom, const int* iAtomStart, std::vector<int>& norb, std::vector<long>& off) {
@@ -579,36 +609,6 @@ struct NeighbourList {
   std::vector<std::vector<double>> d2;
 };
 
-static void cell_translations(const double* lat, double cutoff, std::vector<double>& tr) {
-  tr.assign(3, 0.0);
-  if (!lat) return;
-  // reciprocal vectors / 2 pi: rows of inv(lat)^T
-  const double* a = lat;
-  const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
-                     a[2] * (a[3] * a[7] - a[4] * a[6]);
-  double inv[9];
-  inv[0] = (a[4] * a[8] - a[5] * a[7]) / det;
-  inv[1] = -(a[1] * a[8] - a[2] * a[7]) / det;
-  inv[2] = (a[1] * a[5] - a[2] * a[4]) / det;
-  inv[3] = -(a[3] * a[8] - a[5] * a[6]) / det;
-  inv[4] = (a[0] * a[8] - a[2] * a[6]) / det;
-  inv[5] = -(a[0] * a[5] - a[2] * a[3]) / det;
-  inv[6] = (a[3] * a[7] - a[4] * a[6]) / det;
-  inv[7] = -(a[0] * a[7] - a[1] * a[6]) / det;
-  inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
-  int nmax[3];
-  for (int i = 0; i < 3; ++i) {
-    // row i of inv^T = column i of inv
-    const double nr = std::sqrt(inv[i] * inv[i] + inv[3 + i] * inv[3 + i] + inv[6 + i] * inv[6 + i]);
-    nmax[i] = (int)std::ceil(cutoff * nr) + 1;
-  }
-  for (int x = -nmax[0]; x <= nmax[0]; ++x)
-    for (int y = -nmax[1]; y <= nmax[1]; ++y)
-      for (int z = -nmax[2]; z <= nmax[2]; ++z) {
-        if (x == 0 && y == 0 && z == 0) continue;
-        for (int c = 0; c < 3; ++c) tr.push_back(x * lat[c] + y * lat[3 + c] + z * lat[6 + c]);
-      }
-}
 
 static int build_neighbours(int nat, const double* coords, const double* lat, double cutoff,
                             NeighbourList& nl) {
@@ -1163,6 +1163,9 @@ struct SccState {
 // k-points of one real size tridiagonalised together (1, the default) or one by one (0); set
 // through dftbhip_set_tuning parameter 2
 static int g_kBatch = 1;
+// neighbour list, pairs and SK blocks of a geometry built on the device (1, the default) or on
+// the host (0); set through dftbhip_set_tuning parameter 3
+static int g_devNeighbours = 1;
 
 struct Context {
   dftbhip_options opt{};
@@ -1191,6 +1194,8 @@ struct Context {
   std::vector<double> kpts, kw;
   DevBuf<double> blkT;
   std::vector<DevBuf<double>> LinvK, XK;
+  DevNeighbourList dnl;      // neighbour list built on the device
+  bool nlOnDevice = false;
   // k-points tridiagonalised together (all real, one size): reduced matrices and reflectors,
   // d, e, tau and the workspace blocks of every point
   DevBuf<double> kA, kD, kE, kTau;
@@ -1235,39 +1240,54 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
   double cutoff = c.sk.skcut;
   for (double r : c.sk.repcut) cutoff = std::max(cutoff, r);
   for (double g : c.sk.gcut) cutoff = std::max(cutoff, g);
-  DH_TRY(build_neighbours(nat, coords, c.periodic ? c.lat : nullptr, cutoff, c.nl));
-  // SK blocks and all neighbour pairs
-  c.blk = HostBlocks();
-  c.nbp = HostBlocks();
-  long st = 0;
-  const double sk2 = c.sk.skcut * c.sk.skcut;
-  for (int a = 0; a < nat; ++a) {
-    const auto& nb = c.nl.nb[a];
-    const auto& d2 = c.nl.d2[a];
-    for (size_t k = 0; k < nb.size(); ++k) {
-      const int img = nb[k], jc = c.nl.img2cent[img];
-      c.nbp.at1.push_back(a);
-      c.nbp.img2.push_back(img);
-      c.nbp.at2.push_back(jc);
-      c.nbp.start.push_back(0);
-      if (d2[k] <= sk2) {
-        c.blk.at1.push_back(a);
-        c.blk.img2.push_back(img);
-        c.blk.at2.push_back(jc);
-        c.blk.start.push_back(st);
-        st += (long)c.norbAtom[a] * c.norbAtom[jc];
+  c.nlOnDevice = g_devNeighbours != 0;
+  if (c.nlOnDevice) {
+    // neighbour list, pairs and SK blocks built on the device
+    DH_TRY(device_neighbours(s, nat, coords, c.periodic ? c.lat : nullptr, cutoff, c.sk.skcut,
+                             c.norbAtom.data(), c.dnl));
+    c.nnz = c.dnl.nnz;
+    DH_TRY(c.dblk.from_device(c.dnl.nBlk, c.dnl.bAt1.p, c.dnl.bImg2.p, c.dnl.bAt2.p,
+                              c.dnl.bStart.p, c.species, c.norbAtom, c.atomStart, s));
+    DH_TRY(c.dnb.from_device(c.dnl.nPairs, c.dnl.pAt1.p, c.dnl.pImg2.p, c.dnl.pAt2.p, nullptr,
+                             c.species, c.norbAtom, c.atomStart, s));
+    DH_TRY(c.coordsAll.alloc(3 * (size_t)c.dnl.nAll));
+    DH_HIP(hipMemcpyAsync(c.coordsAll.p, c.dnl.coordsAll.p, 3 * (size_t)c.dnl.nAll * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+  } else {
+    DH_TRY(build_neighbours(nat, coords, c.periodic ? c.lat : nullptr, cutoff, c.nl));
+    // SK blocks and all neighbour pairs
+    c.blk = HostBlocks();
+    c.nbp = HostBlocks();
+    long st = 0;
+    const double sk2 = c.sk.skcut * c.sk.skcut;
+    for (int a = 0; a < nat; ++a) {
+      const auto& nb = c.nl.nb[a];
+      const auto& d2 = c.nl.d2[a];
+      for (size_t k = 0; k < nb.size(); ++k) {
+        const int img = nb[k], jc = c.nl.img2cent[img];
+        c.nbp.at1.push_back(a);
+        c.nbp.img2.push_back(img);
+        c.nbp.at2.push_back(jc);
+        c.nbp.start.push_back(0);
+        if (d2[k] <= sk2) {
+          c.blk.at1.push_back(a);
+          c.blk.img2.push_back(img);
+          c.blk.at2.push_back(jc);
+          c.blk.start.push_back(st);
+          st += (long)c.norbAtom[a] * c.norbAtom[jc];
+        }
       }
     }
+    c.nnz = st;
+    DH_TRY(c.dblk.upload(c.blk, c.species, c.norbAtom, c.atomStart, s));
+    DH_TRY(c.dnb.upload(c.nbp, c.species, c.norbAtom, c.atomStart, s));
+    DH_TRY(c.coordsAll.upload(c.nl.coordsAll.data(), c.nl.coordsAll.size(), s));
   }
-  c.nnz = st;
-  DH_TRY(c.dblk.upload(c.blk, c.species, c.norbAtom, c.atomStart, s));
-  DH_TRY(c.dnb.upload(c.nbp, c.species, c.norbAtom, c.atomStart, s));
   c.nbdev.n = c.dnb.pr.nblk;
   c.nbdev.at1 = c.dnb.pr.at1;
   c.nbdev.img2 = c.dnb.pr.img2;
   c.nbdev.at2 = c.dnb.pr.at2;
   c.nbdev.species = c.dnb.pr.species;
-  DH_TRY(c.coordsAll.upload(c.nl.coordsAll.data(), c.nl.coordsAll.size(), s));
   DH_TRY(c.coordsC.upload(coords, 3 * (size_t)nat, s));
   const int n = c.norbTot;
   DH_TRY(c.h0.alloc(c.nnz));
@@ -1308,16 +1328,22 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
     inv[6] = (L[3] * L[7] - L[4] * L[6]) / det;
     inv[7] = -(L[0] * L[7] - L[1] * L[6]) / det;
     inv[8] = (L[0] * L[4] - L[1] * L[3]) / det;
-    const size_t nb = c.blk.at1.size();
-    std::vector<double> T(3 * std::max<size_t>(nb, 1), 0.0);
-    for (size_t k = 0; k < nb; ++k) {
-      const int img = c.blk.img2[k], jc = c.blk.at2[k];
-      double d[3];
-      for (int q = 0; q < 3; ++q) d[q] = c.nl.coordsAll[3 * img + q] - coords[3 * jc + q];
-      for (int q = 0; q < 3; ++q)
-        T[3 * k + q] = std::nearbyint(d[0] * inv[q] + d[1] * inv[3 + q] + d[2] * inv[6 + q]);
+    if (c.nlOnDevice) {
+      DH_TRY(c.blkT.alloc(3 * (size_t)std::max(c.dnl.nBlk, 1)));
+      DH_TRY(block_translations(s, c.dnl.nBlk, c.dnl.bImg2.p, c.dnl.bAt2.p, c.coordsAll.p,
+                                c.coordsC.p, inv, c.blkT.p));
+    } else {
+      const size_t nb = c.blk.at1.size();
+      std::vector<double> T(3 * std::max<size_t>(nb, 1), 0.0);
+      for (size_t k = 0; k < nb; ++k) {
+        const int img = c.blk.img2[k], jc = c.blk.at2[k];
+        double d[3];
+        for (int q = 0; q < 3; ++q) d[q] = c.nl.coordsAll[3 * img + q] - coords[3 * jc + q];
+        for (int q = 0; q < 3; ++q)
+          T[3 * k + q] = std::nearbyint(d[0] * inv[q] + d[1] * inv[3 + q] + d[2] * inv[6 + q]);
+      }
+      DH_TRY(c.blkT.upload(T.data(), T.size(), s));
     }
-    DH_TRY(c.blkT.upload(T.data(), T.size(), s));
   }
   DH_TRY(c.gamma.alloc((size_t)nat * nat));
   // H0, S
@@ -2464,10 +2490,63 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
       DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 2 takes 0 or 1");
       dftbhip::g_kBatch = value;
       return 0;
+    case 3:
+      DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 3 takes 0 or 1");
+      dftbhip::g_devNeighbours = value;
+      return 0;
     default:
       dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));
       return dftbhip::kErrArg;
   }
+}
+
+extern "C" int dftbhip_neighbour_list(int nAtom, const double* coords, const double* latVecs,
+                                      double cutoff, int* nAll, int* mNeighbour,
+                                      double* coordsAll, int* img2CentCell, int* iNeighbour,
+                                      int* nNeighbour, double* neighDist2) {
+  DH_CHECK(nAtom >= 1 && coords && cutoff > 0.0 && nAll && mNeighbour,
+           "neighbour_list: bad arguments");
+  const bool fillOut = coordsAll != nullptr;
+  DH_CHECK(!fillOut || (img2CentCell && iNeighbour && nNeighbour),
+           "neighbour_list: output arrays missing");
+  const int capAll = *nAll, capNb = *mNeighbour;
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  DevNeighbourList nl;
+  std::vector<int> ones(nAtom, 1);
+  DH_TRY(device_neighbours(s, nAtom, coords, latVecs, cutoff, cutoff, ones.data(), nl));
+  *nAll = nl.nAll;
+  *mNeighbour = nl.maxNeighbour;
+  if (!fillOut) return 0;
+  DH_CHECK(capAll >= nl.nAll && capNb >= nl.maxNeighbour,
+           "neighbour_list: output arrays smaller than nAll = " + std::to_string(nl.nAll) +
+               ", mNeighbour = " + std::to_string(nl.maxNeighbour));
+  std::vector<int> img(nl.nPairs), cnt(nAtom), i2c(nl.nAll);
+  std::vector<long> start(nAtom + 1);
+  std::vector<double> d2(nl.nPairs);
+  DH_HIP(hipMemcpyAsync(coordsAll, nl.coordsAll.p, 3 * (size_t)nl.nAll * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(i2c.data(), nl.img2cent.p, nl.nAll * sizeof(int), hipMemcpyDeviceToHost,
+                        s));
+  DH_HIP(hipMemcpyAsync(img.data(), nl.pImg2.p, nl.nPairs * sizeof(int), hipMemcpyDeviceToHost,
+                        s));
+  DH_HIP(hipMemcpyAsync(d2.data(), nl.pD2.p, nl.nPairs * sizeof(double), hipMemcpyDeviceToHost,
+                        s));
+  DH_HIP(hipMemcpyAsync(cnt.data(), nl.pCount.p, nAtom * sizeof(int), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(start.data(), nl.pStart.p, (nAtom + 1) * sizeof(long),
+                        hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < nl.nAll; ++i) img2CentCell[i] = i2c[i] + 1;
+  const long ld = (long)capNb + 1;
+  for (int a = 0; a < nAtom; ++a) {
+    nNeighbour[a] = cnt[a];
+    for (long k = 0; k <= capNb; ++k) {
+      const bool in = k <= cnt[a];
+      iNeighbour[k + a * ld] = in ? img[start[a] + k] + 1 : 0;
+      if (neighDist2) neighDist2[k + a * ld] = in ? d2[start[a] + k] : 0.0;
+    }
+  }
+  return 0;
 }
 
 extern "C" int dftbhip_autotune(int n, int precision, int* table, int maxRows, int* nRows,

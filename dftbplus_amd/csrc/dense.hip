@@ -263,6 +263,7 @@ int trtri_from_diag(hipStream_t s, const double* l, long ldl, double* linv, long
 template int fill<double>(hipStream_t, double*, size_t, double);
 template int fill<float>(hipStream_t, float*, size_t, float);
 template int fill<int>(hipStream_t, int*, size_t, int);
+template int fill<long>(hipStream_t, long*, size_t, long);
 template int fill2d<double>(hipStream_t, double*, long, int, int, double);
 template int fill2d<float>(hipStream_t, float*, long, int, int, float);
 template int copy2d<double>(hipStream_t, const double*, long, double*, long, int, int);

@@ -44,9 +44,23 @@ int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e, d
  * 16, or 0 = chosen from the trailing size per column, the default: the autotuned table of the
  * arithmetic type when there is one, else fixed thresholds); param 1, value 0: drop the
  * autotuned tables; param 2: k-points that are all real and of one size are tridiagonalised
- * together, one launch per column kernel for all points (1, the default) or one by one (0).
- * No counterpart in the reference. */
+ * together, one launch per column kernel for all points (1, the default) or one by one (0);
+ * param 3: the neighbour list, pairs and SK blocks of a geometry are built on the device (1,
+ * the default) or on the host (0). No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
+/* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)
+ * builds it: atom a lists itself, then the images of atoms b >= a within cutoff by increasing
+ * distance (ties by cell translation, then atom); image atoms follow the central ones in
+ * (translation, atom) order. latVecs: the three lattice vectors (9 doubles) or NULL for a
+ * cluster. Query: with coordsAll NULL only *nAll and *mNeighbour (largest neighbour count,
+ * self excluded) are set. Fill: *nAll and *mNeighbour give the capacities of coordsAll
+ * (3 x nAll), img2CentCell (nAll, 1-based), iNeighbour and neighDist2 ((mNeighbour + 1) x
+ * nAtom, column k = 0 the atom itself, 1-based image indices, 0 past the last neighbour;
+ * neighDist2 optional), nNeighbour (nAtom, self excluded); on exit the actual sizes. Errors:
+ * atoms closer than 0.01 bohr, capacities too small. */
+int dftbhip_neighbour_list(int nAtom, const double* coords, const double* latVecs, double cutoff,
+                           int* nAll, int* mNeighbour, double* coordsAll, int* img2CentCell,
+                           int* iNeighbour, int* nNeighbour, double* neighDist2);
 /* Autotuner of the SYMV tile count (no counterpart in the reference): tridiagonalises a
  * synthetic symmetric n x n matrix (256 <= n <= 65535) once per option 2, 4, 8, 16 with every
  * SYMV launch timed, picks per 256-row bucket of trailing sizes the option of least time
