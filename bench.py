@@ -229,6 +229,41 @@ def mode_batch(args):
     b.close()
 
 
+def mode_neighbours(args):
+    """The neighbour list of the workload's geometry built on the device (with the download of
+    the reference-layout arrays) and the geometry set-up of a calculator (neighbour list, pair
+    and SK block lists, H0/S, gamma, overlap factorisation) with the list built on the device
+    and on the host."""
+    import numpy as np
+    from dftbplus_amd import api, synth
+    geo, params, opts = synth.workload(args.workload)
+    cut = max(params.sk_cutoff(), params.rep_cutoff())
+    api.neighbour_list(geo.coords, geo.latvecs, cut)   # first call: runtime and kernels loaded
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call, i2c, neigh, _ = api.neighbour_list(geo.coords, geo.latvecs, cut)
+    t_list = (time.perf_counter() - t0) / reps
+    setup = {}
+    try:
+        for mode, name in ((1, "device"), (0, "host")):
+            api.set_tuning(3, mode)
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                dp = api.DftbPlus(params, geo, opts)
+                ts.append(time.perf_counter() - t0)
+                dp.close()
+            setup[name] = min(ts)
+    finally:
+        api.set_tuning(3, 1)
+    print(json.dumps({"mode": "neighbours", "workload": args.workload, "atoms": geo.natom,
+                      "cutoff": cut, "images": int(len(i2c)),
+                      "pairs": int(sum(len(v) for v in neigh)),
+                      "device_list_s": t_list,
+                      "geometry_setup_s": setup}), flush=True)
+
+
 def mode_kpoints(args):
     """SCC iterations and one force evaluation of a periodic workload with a 2x2x2 mesh:
     Gamma-centred (k in {0, 1/2}^3, the eight time-reversal-invariant points, whose H(k) is
@@ -364,12 +399,14 @@ def main():
     ap.add_argument("--cycle", action="store_true",
                     help="also run full SCC cycles: FP64 against FP32 iterations then FP64 ones")
     ap.add_argument("--mode", default="scc",
-                    choices=["scc", "sytrd", "batch", "kpoints", "autograd"],
+                    choices=["scc", "sytrd", "batch", "kpoints", "autograd", "neighbours"],
                     help="scc: the headline SCC benchmark (default); sytrd: the tridiagonalisation "
                          "alone on a random matrix (SYMV counter runs); batch: the molecule batch "
                          "against a sampled loop of single-molecule runs; kpoints: SCC iterations "
                          "and a force evaluation with a k-point mesh; autograd: energy and forces "
-                         "through torch.autograd with finite-difference checks")
+                         "through torch.autograd with finite-difference checks; neighbours: the "
+                         "neighbour list on the device and the geometry set-up with it and "
+                         "with the host-built list")
     ap.add_argument("--n", type=int, default=8192, help="matrix size of --mode sytrd")
     ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
     ap.add_argument("--fd-coords", type=int, default=4,
@@ -388,6 +425,7 @@ def main():
     # torch is not loaded here: its bundled HIP runtime would be a second runtime in the process
     if args.mode != "scc":
         {"sytrd": mode_sytrd, "batch": mode_batch, "kpoints": mode_kpoints,
+         "neighbours": mode_neighbours,
          "autograd": mode_autograd}[args.mode](args)
         return
     geo, params, opts = synth.workload(args.workload)

@@ -5,11 +5,11 @@
 !> replace (magmaHegvd in src/dftbp/math/eigensolver.F90:1143-1245, makeDensityMtxRealGPU /
 !> makeEnergyDensityMtxRealGPU in src/dftbp/dftb/densitymatrix.F90:1105-1395).
 module dftbhip
-  use iso_c_binding, only : c_int, c_double, c_char, c_ptr, c_null_char
+  use iso_c_binding, only : c_int, c_double, c_char, c_ptr, c_null_char, c_null_ptr, c_loc
   implicit none
   private
   public :: dftbhip_gpus_available, dftbhip_hegvd_dreal, dftbhip_dm_dreal, dftbhip_edm_dreal
-  public :: dftbhip_error_message
+  public :: dftbhip_error_message, dftbhip_update_neighbour_list
 
   interface
     function c_gpus(ngpus) result(rc) bind(C, name="dftbhip_get_gpus_available")
@@ -45,6 +45,18 @@ module dftbhip
       real(c_double), intent(inout) :: edm(lddm, *)
       integer(c_int) :: rc
     end function c_edm
+
+    function c_neigh(nAtom, coords, latVecs, cutoff, nAll, mNeighbour, coordsAll, img2CentCell,&
+        & iNeighbour, nNeighbour, neighDist2) result(rc) bind(C, name="dftbhip_neighbour_list")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nAtom
+      real(c_double), intent(in) :: coords(3, *)
+      type(c_ptr), value :: latVecs
+      real(c_double), value :: cutoff
+      integer(c_int), intent(inout) :: nAll, mNeighbour
+      type(c_ptr), value :: coordsAll, img2CentCell, iNeighbour, nNeighbour, neighDist2
+      integer(c_int) :: rc
+    end function c_neigh
 
     function c_last_error(buf, blen) result(rc) bind(C, name="dftbhip_last_error")
       import :: c_int, c_char
@@ -106,6 +118,37 @@ contains
     info = c_edm(int(size(eigenvecs, dim=1), c_int), int(nLevels, c_int), eigenvecs,&
         & int(size(eigenvecs, dim=1), c_int), filling, eigen, edm, int(size(edm, dim=1), c_int))
   end subroutine dftbhip_edm_dreal
+
+  !> Half neighbour list on the device (the arrays updateNeighbourList fills, src/dftbp/dftb/
+  !> periodic.F90:446-696): coordinates of central atoms and images, image -> central atom,
+  !> neighbours iNeighbour(0:, iAt) (0: the atom itself) by increasing distance, their counts
+  !> and squared distances. latVecs (columns: lattice vectors) absent for a cluster.
+  subroutine dftbhip_update_neighbour_list(coord0, cutoff, coord, img2CentCell, iNeighbour,&
+      & nNeighbour, neighDist2, nAllAtom, info, latVecs)
+    real(c_double), intent(in) :: coord0(:,:)
+    real(c_double), intent(in) :: cutoff
+    real(c_double), allocatable, target, intent(out) :: coord(:,:), neighDist2(:,:)
+    integer(c_int), allocatable, target, intent(out) :: img2CentCell(:), iNeighbour(:,:)
+    integer(c_int), allocatable, target, intent(out) :: nNeighbour(:)
+    integer, intent(out) :: nAllAtom, info
+    real(c_double), target, intent(in), optional :: latVecs(3, 3)
+    type(c_ptr) :: lat
+    integer(c_int) :: nAtom, nAll, mNb
+    nAtom = size(coord0, dim=2)
+    lat = c_null_ptr
+    if (present(latVecs)) lat = c_loc(latVecs)
+    nAll = 0
+    mNb = 0
+    info = c_neigh(nAtom, coord0, lat, cutoff, nAll, mNb, c_null_ptr, c_null_ptr, c_null_ptr,&
+        & c_null_ptr, c_null_ptr)
+    nAllAtom = nAll
+    if (info /= 0) return
+    allocate(coord(3, nAll), img2CentCell(nAll), iNeighbour(0:mNb, nAtom), nNeighbour(nAtom))
+    allocate(neighDist2(0:mNb, nAtom))
+    info = c_neigh(nAtom, coord0, lat, cutoff, nAll, mNb, c_loc(coord), c_loc(img2CentCell),&
+        & c_loc(iNeighbour), c_loc(nNeighbour), c_loc(neighDist2))
+    nAllAtom = nAll
+  end subroutine dftbhip_update_neighbour_list
 
   !> Message of the last failing call.
   function dftbhip_error_message() result(msg)

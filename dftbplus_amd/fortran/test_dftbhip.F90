@@ -1,11 +1,15 @@
 !> Fortran caller of the bind(C) module: generalized eigenproblem of a small symmetric pair and
-!> the density matrix, checked by residuals (prints PASS/FAIL; exit status 1 on failure).
+!> the density matrix, checked by residuals, and the neighbour list of a periodic cell checked
+!> by its invariants (prints PASS/FAIL; exit status 1 on failure).
 program test_dftbhip
   use dftbhip
   implicit none
   integer, parameter :: n = 40
   real(8) :: h(n,n), s(n,n), h0(n,n), s0(n,n), w(n), f(n), dm(n,n), r
-  integer :: info, i, j, ngpu
+  integer :: info, i, j, k, ngpu, nAll
+  real(8) :: x0(3,2), lat(3,3), cut
+  real(8), allocatable :: xall(:,:), d2(:,:)
+  integer, allocatable :: i2c(:), inb(:,:), nnb(:)
   call dftbhip_gpus_available(ngpu)
   print '(a,i0)', "gpus: ", ngpu
   if (ngpu < 1) then
@@ -34,5 +38,28 @@ program test_dftbhip
     print '(a)', "FAIL"
     error stop 1
   end if
+  ! two atoms in a cubic cell of 3 bohr, cutoff 4.5: every atom lists itself first, then
+  ! images of atoms with an index not below its own, by non-decreasing distance within the cutoff
+  x0 = reshape([0.0d0, 0.0d0, 0.0d0, 1.5d0, 1.5d0, 1.5d0], [3, 2])
+  lat = 0.0d0
+  lat(1,1) = 3.0d0
+  lat(2,2) = 3.0d0
+  lat(3,3) = 3.0d0
+  cut = 4.5d0
+  call dftbhip_update_neighbour_list(x0, cut, xall, i2c, inb, nnb, d2, nAll, info, lat)
+  if (info /= 0) then
+    print '(a,i0,1x,a)', "FAIL: neighbour list info = ", info, trim(dftbhip_error_message())
+    error stop 1
+  end if
+  print '(a,i0,a,2(1x,i0))', "images ", nAll, ", neighbours", nnb
+  do i = 1, 2
+    if (inb(0, i) /= i .or. d2(0, i) /= 0.0d0 .or. nnb(i) < 1) error stop 1
+    do k = 1, nnb(i)
+      j = inb(k, i)
+      if (j < 1 .or. j > nAll) error stop 1
+      if (i2c(j) < i .or. d2(k, i) > cut**2 .or. d2(k, i) < d2(k - 1, i)) error stop 1
+      if (abs(sum((xall(:, j) - x0(:, i))**2) - d2(k, i)) > 1.0d-12) error stop 1
+    end do
+  end do
   print '(a)', "PASS"
 end program test_dftbhip
