@@ -52,3 +52,41 @@ def test_batch_matches_oracle(batch24):
         assert abs(res.energy_mermin - e[m]) / nat < 1e-8
         assert np.max(np.abs(res.charges - b.charges[m])) < 1e-8
         assert np.max(np.abs(og - g[m])) < 1e-7
+
+
+def _chain(norb, rng):
+    """A molecule of exactly norb orbitals (H: 1, C/N/O: 4): a helical chain, bonds 2.4-2.6
+    bohr, no two atoms closer than 1.9 bohr."""
+    from dftbplus_amd.model import Geometry
+    nheavy = norb // 4
+    spec = [int(rng.integers(1, 4)) for _ in range(nheavy)] + [0] * (norb - 4 * nheavy)
+    rng.shuffle(spec)
+    k = np.arange(len(spec))
+    pos = np.stack([2.3 * k, 0.9 * np.sin(0.9 * k), 0.9 * np.cos(0.9 * k)], axis=1)
+    pos += rng.uniform(-0.05, 0.05, size=pos.shape)
+    return Geometry(pos, np.array(spec, dtype=np.int64), None)
+
+
+def test_batch_size_classes_one_iteration():
+    """Molecules at the edges of the eigensolver's size classes (LDS A and C up to 64 and 112
+    orbitals, LDS A up to 190, HBM above), single atoms and odd sizes included, after one SCC
+    iteration: the batch equals the single-molecule runs of the plain path."""
+    import dataclasses
+    from dftbplus_amd import api, synth
+    _, params, opts = synth.workload("molecules1")
+    o = dataclasses.replace(opts, max_scc_iter=1)
+    rng = np.random.default_rng(21)
+    sizes = [1, 2, 5, 63, 64, 65, 111, 112, 113, 189, 190, 191, 223]
+    mols = [_chain(n, rng) for n in sizes]
+    b = api.MoleculeBatch(params, mols, o)
+    e = b.scc()
+    g = b.gradients()
+    for m, geo in enumerate(mols):
+        dp = api.DftbPlus(params, geo, o)
+        es, qs, gs = dp.get_energy(), dp.get_gross_charges(), dp.get_gradients()
+        dp.close()
+        print(f"{sizes[m]} orbitals: dE {abs(es - e[m]):.2e} dq {np.max(np.abs(qs - b.charges[m])):.2e}")
+        assert abs(es - e[m]) / geo.natom < 1e-10
+        assert np.max(np.abs(qs - b.charges[m])) < 1e-9
+        assert np.max(np.abs(gs - g[m])) < 1e-8
+    b.close()

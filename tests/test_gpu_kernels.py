@@ -75,7 +75,7 @@ def test_dstedc_clustered():
     _tridiag_check(d, e, w, z)
 
 
-@pytest.mark.parametrize("n", [10, 100, 333, 1100])
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 100, 333, 1100])
 def test_sygvd_random(n):
     api = _api()
     rng = np.random.default_rng(7 * n)
