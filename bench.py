@@ -264,6 +264,37 @@ def mode_neighbours(args):
                       "geometry_setup_s": setup}), flush=True)
 
 
+def mode_kernels(args):
+    """One SCC iteration and one force evaluation of the workload, then an FP64 GEMM of size --n
+    (NN and NT) through the C ABI, for kernel traces and counter runs; prints the sizes that fix
+    the algorithmic bytes of the sparse kernels (SK blocks, sparse elements) and the flops of the
+    density SYRK (orbitals, filled levels)."""
+    import numpy as np
+    from dftbplus_amd import api, synth
+    geo, params, opts = synth.workload(args.workload)
+    dp = api.DftbPlus(params, geo, opts)
+    dp.scc_begin()
+    dp.scc_step()
+    dp.scc_end()
+    dp.get_gradients()
+    _, f = dp.get_eigenvalues()
+    n = dp.get_nr_orbitals()
+    dp.close()
+    nocc = int(np.nonzero(f > np.finfo(float).eps)[0].max()) + 1 if np.any(f > 0) else 0
+    norb = np.array([params.species[x].norb for x in geo.species])
+    _, i2c, neigh, _ = api.neighbour_list(geo.coords, geo.latvecs, params.sk_cutoff())
+    nblk = int(sum(len(v) for v in neigh))
+    nnz = int(sum(int(norb[a]) * int(norb[i2c[neigh[a]]].sum()) for a in range(geo.natom)))
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((args.n, args.n))
+    b = rng.standard_normal((args.n, args.n))
+    for tb in ("N", "T"):
+        api.dgemm(a, b, "N", tb)
+    print(json.dumps({"mode": "kernels", "workload": args.workload, "atoms": geo.natom,
+                      "orbitals": n, "sk_blocks": nblk, "sparse_elements": nnz,
+                      "filled_levels": nocc, "gemm_n": args.n}), flush=True)
+
+
 def mode_kpoints(args):
     """SCC iterations and one force evaluation of a periodic workload with a 2x2x2 mesh:
     Gamma-centred (k in {0, 1/2}^3, the eight time-reversal-invariant points, whose H(k) is
@@ -399,15 +430,19 @@ def main():
     ap.add_argument("--cycle", action="store_true",
                     help="also run full SCC cycles: FP64 against FP32 iterations then FP64 ones")
     ap.add_argument("--mode", default="scc",
-                    choices=["scc", "sytrd", "batch", "kpoints", "autograd", "neighbours"],
+                    choices=["scc", "sytrd", "batch", "kpoints", "autograd", "neighbours",
+                             "kernels"],
                     help="scc: the headline SCC benchmark (default); sytrd: the tridiagonalisation "
                          "alone on a random matrix (SYMV counter runs); batch: the molecule batch "
                          "against a sampled loop of single-molecule runs; kpoints: SCC iterations "
                          "and a force evaluation with a k-point mesh; autograd: energy and forces "
                          "through torch.autograd with finite-difference checks; neighbours: the "
                          "neighbour list on the device and the geometry set-up with it and "
-                         "with the host-built list")
-    ap.add_argument("--n", type=int, default=8192, help="matrix size of --mode sytrd")
+                         "with the host-built list; kernels: one SCC iteration, forces and a "
+                         "GEMM of size --n with the sizes that fix their bytes and flops (for "
+                         "kernel traces and counter runs)")
+    ap.add_argument("--n", type=int, default=8192,
+                    help="matrix size of --mode sytrd and of the GEMM of --mode kernels")
     ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
     ap.add_argument("--fd-coords", type=int, default=4,
                     help="coordinates checked by finite differences in --mode autograd")
@@ -425,7 +460,7 @@ def main():
     # torch is not loaded here: its bundled HIP runtime would be a second runtime in the process
     if args.mode != "scc":
         {"sytrd": mode_sytrd, "batch": mode_batch, "kpoints": mode_kpoints,
-         "neighbours": mode_neighbours,
+         "neighbours": mode_neighbours, "kernels": mode_kernels,
          "autograd": mode_autograd}[args.mode](args)
         return
     geo, params, opts = synth.workload(args.workload)

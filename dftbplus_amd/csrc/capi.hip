@@ -1767,8 +1767,11 @@ static int ctx_gradients(Context& c, double* grad) {
         for (int t = 0; t < stride; ++t)
           fk[stride * j + t] = c.fillK[k][j] * c.eigK[k][stride * j];
       DH_HIP(hipMemcpyAsync(c.f.p, fk.data(), ndk * sizeof(double), hipMemcpyHostToDevice, s));
-      DH_TRY(gemm<double>(s, false, true, ndk, ndk, stride * c.noccK[k], 1.0, c.XK[k].p, ndk,
-                          c.XK[k].p, ndk, 0.0, c.P.p, ndk, true, kKFull, c.f.p));
+      // W = (X diag(f e)) X^T: the scaled columns staged in the eigensolver's scratch matrix
+      const int kk = stride * c.noccK[k];
+      DH_TRY(scale_cols<double>(s, c.XK[k].p, ndk, c.f.p, c.eig.T.p, ndk, ndk, kk));
+      DH_TRY(gemm<double>(s, false, true, ndk, ndk, kk, 1.0, c.eig.T.p, ndk, c.XK[k].p, ndk,
+                          0.0, c.P.p, ndk, true));
       DH_TRY(launch_pack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
                              &c.kpts[3 * k], c.kw[k], c.P.p, ndk, n, c.erho.p));
       DH_HIP(hipStreamSynchronize(s));
@@ -1784,8 +1787,11 @@ static int ctx_gradients(Context& c, double* grad) {
     DH_HIP(hipStreamSynchronize(s));   // fe32 leaves scope
   } else {
     DH_HIP(hipMemcpyAsync(c.feps.p, fe.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-    DH_TRY(gemm<double>(s, false, true, n, n, c.nocc, 1.0, c.X.p, n, c.X.p, n, 0.0, c.P.p, n,
-                        true, kKFull, c.feps.p));
+    // W = (X diag(f e)) X^T: the scaled columns staged in the eigensolver's scratch matrix,
+    // free once the eigenvectors are formed (the weights change sign, so no SYRK of square roots)
+    DH_TRY(scale_cols<double>(s, c.X.p, n, c.feps.p, c.eig.T.p, n, n, c.nocc));
+    DH_TRY(gemm<double>(s, false, true, n, n, c.nocc, 1.0, c.eig.T.p, n, c.X.p, n, 0.0, c.P.p, n,
+                        true));
     DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
                                c.erho.p));
   }

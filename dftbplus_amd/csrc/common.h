@@ -154,6 +154,9 @@ int copy2d(hipStream_t s, const T* src, long lds, T* dst, long ldd, int m, int n
 template <typename T>
 int scale_cols_sqrt(hipStream_t s, const T* src, long lds, const T* w, T* dst, long ldd, int m,
                     int n);
+// dst(:, j) = src(:, j) * w(j)
+template <typename T>
+int scale_cols(hipStream_t s, const T* src, long lds, const T* w, T* dst, long ldd, int m, int n);
 template <typename Tin, typename Tout>
 int convert2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n);
 int symmetrize_lower(hipStream_t s, double* a, long lda, int n);

@@ -64,11 +64,11 @@ int copy2d(hipStream_t s, const T* src, long lds, T* dst, long ldd, int m, int n
   return convert2d<T, T>(s, src, lds, dst, ldd, m, n);
 }
 
-template <typename T>
-__global__ void scale_cols_sqrt_kernel(const T* src, long lds, const T* w, T* dst, long ldd, int m,
-                                       int n) {
+template <typename T, bool kSqrt>
+__global__ void scale_cols_kernel(const T* src, long lds, const T* w, T* dst, long ldd, int m,
+                                  int n) {
   for (int j = blockIdx.y; j < n; j += gridDim.y) {
-    const T f = sqrt(w[j]);
+    const T f = kSqrt ? sqrt(w[j]) : w[j];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
       dst[i + (long)j * ldd] = src[i + (long)j * lds] * f;
   }
@@ -80,11 +80,24 @@ int scale_cols_sqrt(hipStream_t s, const T* src, long lds, const T* w, T* dst, l
                     int n) {
   if (m <= 0 || n <= 0) return 0;
   dim3 grid(std::min(cdiv(m, 256), 64u), std::min(n, 65535));
-  hipLaunchKernelGGL((scale_cols_sqrt_kernel<T>), grid, dim3(256), 0, s, src, lds, w, dst, ldd, m,
-                     n);
+  hipLaunchKernelGGL((scale_cols_kernel<T, true>), grid, dim3(256), 0, s, src, lds, w, dst, ldd,
+                     m, n);
   DH_LAUNCH_CHECK();
   return 0;
 }
+
+// dst(:, j) = src(:, j) * w(j)
+template <typename T>
+int scale_cols(hipStream_t s, const T* src, long lds, const T* w, T* dst, long ldd, int m, int n) {
+  if (m <= 0 || n <= 0) return 0;
+  dim3 grid(std::min(cdiv(m, 256), 64u), std::min(n, 65535));
+  hipLaunchKernelGGL((scale_cols_kernel<T, false>), grid, dim3(256), 0, s, src, lds, w, dst, ldd,
+                     m, n);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+template int scale_cols<double>(hipStream_t, const double*, long, const double*, double*, long,
+                                int, int);
 template int scale_cols_sqrt<double>(hipStream_t, const double*, long, const double*, double*,
                                      long, int, int);
 template int scale_cols_sqrt<float>(hipStream_t, const float*, long, const float*, float*, long,

@@ -209,42 +209,45 @@ __global__ void build_h0s_kernel(SkDev sk, PairsDev pr, const double* coords, do
     }
 }
 
+// The per-iteration sparse kernels below give each SK block kBlkLanes lanes, lane e holding
+// element e = r + c n2 of the column-major n2 x n1 block (lanes past n1 n2 idle): the blocks of a
+// wave are consecutive in the sparse arrays, so their loads and stores are contiguous.
+constexpr int kBlkLanes = 16;
+
 // Dense fill of both triangles; blocks of periodic images of the same pair are summed.
 template <typename T>
 __global__ void unpack_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
                               const double* x, T* dense, long ld) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int k = (int)(t / kBlkLanes), e = (int)(t % kBlkLanes);
   if (k >= pr.nblk) return;
   const int a1 = pr.at1[k], a2 = pr.at2[k];
   const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  if (e >= n1 * n2) return;
+  const int r = e % n2, c = e / n2;
   const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
-  const long st = pr.start[k];
-  for (int c = 0; c < n1; ++c)
-    for (int r = 0; r < n2; ++r) {
-      const T v = (T)x[st + r + c * n2];
-      atomicAdd(&dense[(o2 + r) + (o1 + c) * ld], v);
-      if (a1 != a2) atomicAdd(&dense[(o1 + c) + (o2 + r) * ld], v);
-    }
+  const T v = (T)x[pr.start[k] + e];
+  atomicAdd(&dense[(o2 + r) + (o1 + c) * ld], v);
+  if (a1 != a2) atomicAdd(&dense[(o1 + c) + (o2 + r) * ld], v);
 }
 
 template <typename T>
 __global__ void pack_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
                             const T* dense, long ld, double* x) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int k = (int)(t / kBlkLanes), e = (int)(t % kBlkLanes);
   if (k >= pr.nblk) return;
   const int a1 = pr.at1[k], a2 = pr.at2[k];
   const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  if (e >= n1 * n2) return;
+  const int r = e % n2, c = e / n2;
   const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
-  const long st = pr.start[k];
-  for (int c = 0; c < n1; ++c)
-    for (int r = 0; r < n2; ++r) {
-      long rr = r, cc = c;
-      if (a1 == a2 && r < c) {   // on-site block symmetrised from its lower triangle
-        rr = c;
-        cc = r;
-      }
-      x[st + r + c * n2] = (double)dense[(o2 + rr) + (o1 + cc) * ld];
-    }
+  long rr = r, cc = c;
+  if (a1 == a2 && r < c) {   // on-site block symmetrised from its lower triangle
+    rr = c;
+    cc = r;
+  }
+  x[pr.start[k] + e] = (double)dense[(o2 + rr) + (o1 + cc) * ld];
 }
 
 // phase factor exp(i 2 pi kf.T) of block k; at a time-reversal-invariant point (m = 2 kf
@@ -335,50 +338,70 @@ __global__ void pack_kpt_kernel(PairsDev pr, const int* norbAtom, const long* iA
 
 __global__ void shift_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
                              const double* h0, const double* s, const double* vorb, double* h) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int k = (int)(t / kBlkLanes), e = (int)(t % kBlkLanes);
   if (k >= pr.nblk) return;
   const int a1 = pr.at1[k], a2 = pr.at2[k];
   const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  if (e >= n1 * n2) return;
+  const int r = e % n2, c = e / n2;
   const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
-  const long st = pr.start[k];
-  for (int c = 0; c < n1; ++c)
-    for (int r = 0; r < n2; ++r) {
-      const long e = st + r + c * n2;
-      h[e] = h0[e] + 0.5 * s[e] * (vorb[o2 + r] + vorb[o1 + c]);
-    }
+  const long i = pr.start[k] + e;
+  h[i] = h0[i] + 0.5 * s[i] * (vorb[o2 + r] + vorb[o1 + c]);
 }
 
 // q_mu += sum_nu S rho (column sums to atom 1, row sums to atom 2 for off-site blocks);
-// the same contraction with H0 gives the non-SCC energy per orbital.
+// the same contraction with H0 gives the non-SCC energy per orbital. A block's column and row
+// sums are gathered by shuffles inside its lane group (every lane takes part), then one lane
+// per column (r = 0) and per row (c = 0) adds them.
 __global__ void mulliken_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
                                 const double* s, const double* h0, const double* rho,
                                 double* qorb, double* eorb) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= pr.nblk) return;
-  const int a1 = pr.at1[k], a2 = pr.at2[k];
-  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
-  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
-  const long st = pr.start[k];
-  double rq[4] = {0, 0, 0, 0}, re[4] = {0, 0, 0, 0};
-  for (int c = 0; c < n1; ++c) {
-    double cq = 0.0, ce = 0.0;
-    for (int r = 0; r < n2; ++r) {
-      const long e = st + r + c * n2;
-      const double pq = s[e] * rho[e];
-      const double pe = h0 ? h0[e] * rho[e] : 0.0;
-      cq += pq;
-      ce += pe;
-      rq[r] += pq;
-      re[r] += pe;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int k = (int)(t / kBlkLanes), e = (int)(t % kBlkLanes);
+  const int lane = threadIdx.x & 63;
+  int a1 = 0, a2 = 0, n1 = 1, n2 = 1;
+  double pq = 0.0, pe = 0.0;
+  const bool blk = k < pr.nblk;
+  if (blk) {
+    a1 = pr.at1[k];
+    a2 = pr.at2[k];
+    n1 = norbAtom[a1];
+    n2 = norbAtom[a2];
+    if (e < n1 * n2) {
+      const long i = pr.start[k] + e;
+      pq = s[i] * rho[i];
+      pe = h0 ? h0[i] * rho[i] : 0.0;
     }
+  }
+  const bool valid = blk && e < n1 * n2;
+  const int r = e % n2, c = e / n2;
+  double cq = 0.0, ce = 0.0, rq = 0.0, re = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    // column c: lanes e + j (r = 0 lanes), row r: lanes e + j n2 (c = 0 lanes)
+    const double vq = __shfl(pq, min(lane + j, 63), 64);
+    const double ve = __shfl(pe, min(lane + j, 63), 64);
+    const double wq = __shfl(pq, min(lane + j * n2, 63), 64);
+    const double we = __shfl(pe, min(lane + j * n2, 63), 64);
+    if (j < n2) {
+      cq += vq;
+      ce += ve;
+    }
+    if (j < n1) {
+      rq += wq;
+      re += we;
+    }
+  }
+  if (!valid) return;
+  const long o1 = iAtomStart[a1], o2 = iAtomStart[a2];
+  if (r == 0) {
     atomicAdd(&qorb[o1 + c], cq);
     if (eorb) atomicAdd(&eorb[o1 + c], ce);
   }
-  if (a1 != a2) {
-    for (int r = 0; r < n2; ++r) {
-      atomicAdd(&qorb[o2 + r], rq[r]);
-      if (eorb) atomicAdd(&eorb[o2 + r], re[r]);
-    }
+  if (c == 0 && a1 != a2) {
+    atomicAdd(&qorb[o2 + r], rq);
+    if (eorb) atomicAdd(&eorb[o2 + r], re);
   }
 }
 
@@ -671,7 +694,8 @@ int launch_unpack(hipStream_t s, const PairsDev& pr, const int* norb, const long
                   const double* x, T* dense, long ld, int n) {
   DH_TRY(fill2d<T>(s, dense, ld, n, n, T(0)));
   if (pr.nblk == 0) return 0;
-  hipLaunchKernelGGL((unpack_kernel<T>), dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+  hipLaunchKernelGGL((unpack_kernel<T>), dim3(cdiv((long)pr.nblk * kBlkLanes, 256)),
+                     dim3(256), 0, s, pr, norb,
                      iAtomStart, x, dense, ld);
   DH_LAUNCH_CHECK();
   return 0;
@@ -681,7 +705,8 @@ template <typename T>
 int launch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                 const T* dense, long ld, double* x) {
   if (pr.nblk == 0) return 0;
-  hipLaunchKernelGGL((pack_kernel<T>), dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+  hipLaunchKernelGGL((pack_kernel<T>), dim3(cdiv((long)pr.nblk * kBlkLanes, 256)),
+                     dim3(256), 0, s, pr, norb,
                      iAtomStart, dense, ld, x);
   DH_LAUNCH_CHECK();
   return 0;
@@ -728,7 +753,8 @@ int launch_pack_kpt(hipStream_t s, const PairsDev& pr, const int* norb, const lo
 int launch_shift(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                  const double* h0, const double* ov, const double* vorb, double* h) {
   if (pr.nblk == 0) return 0;
-  hipLaunchKernelGGL(shift_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+  hipLaunchKernelGGL(shift_kernel, dim3(cdiv((long)pr.nblk * kBlkLanes, 256)),
+                     dim3(256), 0, s, pr, norb,
                      iAtomStart, h0, ov, vorb, h);
   DH_LAUNCH_CHECK();
   return 0;
@@ -740,7 +766,8 @@ int launch_mulliken(hipStream_t s, const PairsDev& pr, const int* norb, const lo
   DH_TRY(fill<double>(s, qorb, nOrb, 0.0));
   if (eorb) DH_TRY(fill<double>(s, eorb, nOrb, 0.0));
   if (pr.nblk == 0) return 0;
-  hipLaunchKernelGGL(mulliken_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+  hipLaunchKernelGGL(mulliken_kernel, dim3(cdiv((long)pr.nblk * kBlkLanes, 256)),
+                     dim3(256), 0, s, pr, norb,
                      iAtomStart, ov, h0, rho, qorb, eorb);
   DH_LAUNCH_CHECK();
   return 0;
