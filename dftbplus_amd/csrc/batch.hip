@@ -396,8 +396,11 @@ __device__ __forceinline__ int sym_idx(int i, int j) {
   return i <= j ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2;
 }
 
+// The smallest class (49 KB of LDS per workgroup) is held to 64 VGPRs so that two 1024-thread
+// workgroups share a CU; the larger classes fill a CU's LDS with one.
 template <int kMaxN, bool kCLds>
-__global__ __launch_bounds__(kSygvThreads) void batch_sygv_lds_kernel(
+__global__ __launch_bounds__(kSygvThreads)
+__attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_kernel(
     BatchLayout L, const int* list, const double* S, const double* H, double* work, double* w,
     double* C, int* info, const int* active, int warm) {
   constexpr int NT = kSygvThreads;
