@@ -13,7 +13,9 @@
 #include <limits>
 #include <map>
 #include <numeric>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/dftbhip.h"
@@ -1200,6 +1202,20 @@ struct Context {
   // d, e, tau and the workspace blocks of every point
   DevBuf<double> kA, kD, kE, kTau;
   DevBuf<char> kWs;
+  // their divide and conquer side by side: per point a stream (default priority), the D&C
+  // buffers and the eigenvalues; the eigenvectors go straight to XK
+  struct KPointDc {
+    hipStream_t st = nullptr;
+    std::shared_ptr<StedcWs> ws;
+    DevBuf<double> work, w;
+    KPointDc() = default;
+    KPointDc(const KPointDc&) = delete;
+    KPointDc& operator=(const KPointDc&) = delete;
+    ~KPointDc() {
+      if (st) (void)hipStreamDestroy(st);
+    }
+  };
+  std::vector<std::unique_ptr<KPointDc>> kdc;
   std::vector<std::vector<double>> eigK, fillK;
   std::vector<int> noccK;
   EigenSolver eig;
@@ -1530,17 +1546,74 @@ static int ctx_scc_step(Context& c) {
       DH_HIP(hipEventElapsedTime(&ms, kev[1], kev[2]));
       evt[1] += ms * 1e-3;
       for (auto& x : kev) (void)hipEventDestroy(x);
+      // divide and conquer and back-transformation of the points side by side, one host thread
+      // and stream each: a D&C is mostly host deflation between short device phases, which the
+      // other points' device work fills
+      if ((int)c.kdc.size() < K) c.kdc.resize(K);
+      // one workspace per point: the D&C's, then the product L^-T (Q Z) and the reflector
+      // workspace of the back-transformation
+      const size_t dcWork = std::max(stedc_work_size(n), nn + ormtr_work_size(n, n));
       for (int k = 0; k < K; ++k) {
-        const double t0 = now();
-        int info = 0;
-        DH_TRY(c.eig.tridiag_eigen(s, c.kD.p + (size_t)k * n, c.kE.p + (size_t)k * n, c.eigK[k],
-                                   &info));
-        if (info != 0) return info;
-        evt[2] += now() - t0;
-        DH_TRY(c.eig.vectors_from(s, c.LinvK[k].p, c.kA.p + k * nn, c.kTau.p + (size_t)k * n, n,
-                                  c.XK[k].p, n));
-        ec[k] = c.eigK[k];
+        if (!c.kdc[k]) c.kdc[k].reset(new Context::KPointDc());
+        Context::KPointDc& r = *c.kdc[k];
+        if (!r.st) DH_HIP(hipStreamCreate(&r.st));
+        if (!r.ws) r.ws.reset(stedc_ws_new(), stedc_ws_delete);
+        DH_TRY(r.work.alloc(dcWork));
+        DH_TRY(r.w.alloc(n));
       }
+      int dev = 0;
+      DH_HIP(hipGetDevice(&dev));
+      std::vector<int> rcK(K, 0), infoK(K, 0);
+      std::vector<std::string> msgK(K);
+      auto dcPoint = [&](int k) -> int {
+        Context::KPointDc& r = *c.kdc[k];
+        DH_HIP(hipSetDevice(dev));
+        std::vector<double> dh(n), eh(std::max(n - 1, 1));
+        DH_HIP(hipMemcpyAsync(dh.data(), c.kD.p + (size_t)k * n, n * sizeof(double),
+                              hipMemcpyDeviceToHost, r.st));
+        if (n > 1)
+          DH_HIP(hipMemcpyAsync(eh.data(), c.kE.p + (size_t)k * n, (n - 1) * sizeof(double),
+                                hipMemcpyDeviceToHost, r.st));
+        DH_HIP(hipStreamSynchronize(r.st));
+        DH_TRY(stedc(r.st, n, dh.data(), eh.data(), r.w.p, c.XK[k].p, n, r.work.p, dcWork,
+                     &infoK[k], r.ws.get()));
+        if (infoK[k] != 0) return 0;
+        c.eigK[k].resize(n);
+        DH_HIP(hipMemcpyAsync(c.eigK[k].data(), r.w.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                              r.st));
+        // back-transformation on the same stream, the D&C workspace (free now) holding the
+        // product L^-T (Q Z) and the reflector workspace: Q Z in place, product, copy back
+        double* prod = r.work.p;
+        double* owork = r.work.p + nn;
+        DH_TRY(ormtr_lower(r.st, c.kA.p + k * nn, n, n, c.kTau.p + (size_t)k * n, c.XK[k].p, n,
+                           n, owork, dcWork - nn));
+        DH_TRY(gemm<double>(r.st, true, false, n, n, n, 1.0, c.LinvK[k].p, n, c.XK[k].p, n, 0.0,
+                            prod, n, false, kKAUpper));
+        DH_HIP(hipMemcpyAsync(c.XK[k].p, prod, nn * sizeof(double), hipMemcpyDeviceToDevice,
+                              r.st));
+        DH_HIP(hipStreamSynchronize(r.st));
+        return 0;
+      };
+      const double tdc = now();
+      {
+        std::vector<std::thread> th;
+        th.reserve(K);
+        for (int k = 0; k < K; ++k)
+          th.emplace_back([&, k]() {
+            rcK[k] = dcPoint(k);
+            if (rcK[k] != 0) msgK[k] = last_error();
+          });
+        for (auto& t : th) t.join();
+      }
+      evt[2] += now() - tdc;
+      for (int k = 0; k < K; ++k) {
+        if (rcK[k] != 0) {
+          set_error(msgK[k]);
+          return rcK[k];
+        }
+        if (infoK[k] != 0) return infoK[k];
+      }
+      for (int k = 0; k < K; ++k) ec[k] = c.eigK[k];
     }
     for (int k = 0; k < c.nk && !together; ++k) {
       const int ndk = kpoint_dim(&c.kpts[3 * k], n);

@@ -204,8 +204,12 @@ int ormtr_2stage(hipStream_t s, const double* a, long lda, int n, double* z, lon
 int ormtr_2stage_f32(hipStream_t s, const float* a, long lda, int n, float* z, long ldz,
                      int ncols, void* work, size_t workBytes, void* work2, size_t work2Bytes);
 size_t ormtr_work_bytes(int n, int ncols, int elemSize);
+// ws: device buffers kept between calls (null: allocated for this call)
+struct StedcWs;
+StedcWs* stedc_ws_new();
+void stedc_ws_delete(StedcWs* w);
 int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double* w, double* z,
-          long ldz, double* work, size_t workSize, int* info);
+          long ldz, double* work, size_t workSize, int* info, StedcWs* ws = nullptr);
 size_t stedc_work_size(int n);
 
 }  // namespace dftbhip

@@ -1,6 +1,7 @@
 // Dense generalized eigensolver state (eigen.hip).
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "common.h"
@@ -16,6 +17,11 @@ struct EigenSolver {
   DevBuf<double> band;          // two-stage tridiagonalisation workspace (reflectors of stage 2)
   size_t workSize = 0, bandBytes = 0;
   bool twoStage = true;         // two-stage (band) tridiagonalisation, else one-stage
+  std::shared_ptr<StedcWs> sws;   // divide-and-conquer device buffers, kept between solves
+  StedcWs* dc_ws() {
+    if (!sws) sws.reset(stedc_ws_new(), stedc_ws_delete);
+    return sws.get();
+  }
 
   int ensure(int n_);
   void release() {
@@ -23,6 +29,7 @@ struct EigenSolver {
       b->release();
     for (DevBuf<float>* b : {&Linv32, &d32, &e32, &tau32}) b->release();
     dinfo.release();
+    sws.reset();
     n = 0;
     factored = false;
   }

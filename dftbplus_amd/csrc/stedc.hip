@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <memory>
+
 #include "common.h"
 
 namespace dftbhip {
@@ -33,6 +35,19 @@ struct MergeDev {
   long rotOff;   // offset into rotation arrays
   long deflOff;  // offset into deflated column list
 };
+
+// Device buffers of one divide and conquer, kept by the caller between calls: they only grow,
+// so repeated solves of one size allocate (and free, which synchronises the device) nothing.
+struct StedcWs {
+  DevBuf<double> dd, de, dD, dz, dDL, dW, dtau, dlam, dzhat, dinv, drc, drs;
+  DevBuf<int> dls, dlz, dinfo, dorg, drow, dgcol, ddcol, drp, drn, dord;
+  DevBuf<MergeDev> dmd;
+  DevBuf<GemmDesc<double>> dgd;
+};
+
+StedcWs* stedc_ws_new() { return new StedcWs(); }
+void stedc_ws_delete(StedcWs* w) { delete w; }
+
 
 // ------------------------------------------------------------------------------------------
 // Leaves: implicit QL (tql2 scheme) with eigenvectors, eigenvalues sorted ascending.
@@ -371,9 +386,14 @@ size_t stedc_work_size(int n) {
 }
 
 int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double* w, double* z,
-          long ldz, double* work, size_t workSize, int* info) {
+          long ldz, double* work, size_t workSize, int* info, StedcWs* ws) {
   *info = 0;
   if (n <= 0) return 0;
+  std::unique_ptr<StedcWs> own;
+  if (!ws) {
+    own.reset(new StedcWs());
+    ws = own.get();
+  }
   DH_CHECK(workSize >= stedc_work_size(n), "stedc: workspace too small");
   double* Qa = work;
   double* Qb = work + (size_t)n * n;
@@ -396,8 +416,8 @@ int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double
       d[mg.s + mg.n1 - 1] -= b;
       d[mg.s + mg.n1] -= b;
     }
-  DevBuf<double> dd, de, dD, dz;
-  DevBuf<int> dls, dlz, dinfo;
+  DevBuf<double>&dd = ws->dd, &de = ws->de, &dD = ws->dD, &dz = ws->dz;
+  DevBuf<int>&dls = ws->dls, &dlz = ws->dlz, &dinfo = ws->dinfo;
   DH_TRY(dd.upload(d.data(), n, s));
   DH_TRY(de.upload(e.data(), std::max(n - 1, 1), s));
   DH_TRY(dls.upload(ls.data(), ls.size(), s));
@@ -427,9 +447,11 @@ int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double
     for (int i = 0; i < lz[q]; ++i) order[ls[q] + i] = ls[q] + i;
   const double eps = 2.220446049250313e-16;
   std::vector<double> zh(n);
-  DevBuf<MergeDev> dmd;
-  DevBuf<double> dDL, dW, dtau, dlam, dzhat, dinv, drc, drs;
-  DevBuf<int> dorg, drow, dgcol, ddcol, drp, drn;
+  DevBuf<MergeDev>& dmd = ws->dmd;
+  DevBuf<double>&dDL = ws->dDL, &dW = ws->dW, &dtau = ws->dtau, &dlam = ws->dlam,
+         &dzhat = ws->dzhat, &dinv = ws->dinv, &drc = ws->drc, &drs = ws->drs;
+  DevBuf<int>&dorg = ws->dorg, &drow = ws->drow, &dgcol = ws->dgcol, &ddcol = ws->ddcol,
+      &drp = ws->drp, &drn = ws->drn;
   for (int L = maxDepth; L >= 0; --L) {
     const std::vector<Merge>& mv = levels[L];
     if (mv.empty()) continue;
@@ -631,7 +653,7 @@ int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double
         gm = std::max(gm, std::max(m.n1, m.n2));
         gn = std::max(gn, m.k);
       }
-      DevBuf<GemmDesc<double>> dgd;
+      DevBuf<GemmDesc<double>>& dgd = ws->dgd;
       DH_TRY(dgd.upload(gd.data(), gd.size(), s));
       DH_TRY(gemm_batched<double>(s, false, false, dgd.p, (int)gd.size(), gm, gn, 1.0, 0.0));
       DH_HIP(hipStreamSynchronize(s));
@@ -671,7 +693,7 @@ int stedc(hipStream_t s, int n, const double* dHost, const double* eHost, double
   std::vector<double> wv(n);
   for (int i = 0; i < n; ++i) wv[i] = D[order[i]];
   DH_HIP(hipMemcpyAsync(w, wv.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-  DevBuf<int> dord;
+  DevBuf<int>& dord = ws->dord;
   DH_TRY(dord.upload(order.data(), n, s));
   hipLaunchKernelGGL(gather_cols_kernel, dim3(std::min(cdiv(n, 256), 64u), n), dim3(256), 0, s, Qa,
                      ldq, n, dord.p, z, ldz);
