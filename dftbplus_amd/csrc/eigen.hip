@@ -66,7 +66,7 @@ int EigenSolver::tridiag_eigen(hipStream_t s, const double* dDev, const double* 
   if (n > 1)
     DH_HIP(hipMemcpyAsync(eh.data(), eDev, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
-  DH_TRY(stedc(s, n, dh.data(), eh.data(), w.p, Zd.p, n, work.p, workSize, info));
+  DH_TRY(stedc(s, n, dh.data(), eh.data(), w.p, Zd.p, n, work.p, workSize, info, dc_ws()));
   if (*info != 0) return 0;
   wHost.resize(n);
   DH_HIP(hipMemcpyAsync(wHost.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -241,7 +241,7 @@ int EigenSolver::solve_mixed(hipStream_t s, const float* H, long ldh, std::vecto
     DH_HIP(hipMemcpyAsync(ef.data(), e32.p, (n - 1) * sizeof(float), hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
   std::vector<double> dh(df.begin(), df.end()), eh(ef.begin(), ef.end());
-  DH_TRY(stedc(s, n, dh.data(), eh.data(), w.p, Zd.p, n, work.p, workSize, info));
+  DH_TRY(stedc(s, n, dh.data(), eh.data(), w.p, Zd.p, n, work.p, workSize, info, dc_ws()));
   DH_HIP(hipEventRecord(ev[3], s));
   wHost.resize(n);
   DH_HIP(hipMemcpyAsync(wHost.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
