@@ -71,3 +71,23 @@ def test_scc_with_device_list_equals_host_list(kind):
     assert abs(e1 - e0) / geo.natom < 1e-13
     assert np.max(np.abs(q1 - q0)) < 1e-12
     assert np.max(np.abs(g1 - g0)) < 1e-11
+
+
+def test_single_and_isolated_atoms():
+    """One atom, and atoms farther apart than the cutoff: every list holds the atom alone."""
+    from dftbplus_amd import api
+    from oracle import dftb_ref as R
+    for x in (np.zeros((1, 3)), np.array([[0.0, 0.0, 0.0], [20.0, 0.0, 0.0], [0.0, 25.0, 3.0]])):
+        call, i2c, neigh, d2 = api.neighbour_list(x, None, 10.0)
+        ref = R.neighbour_list(x, None, 10.0)
+        assert np.array_equal(call, ref.coords_all) and np.array_equal(i2c, ref.img2cent)
+        for a in range(len(x)):
+            assert np.array_equal(neigh[a], [a]) and np.array_equal(d2[a], [0.0])
+
+
+def test_neighbour_capacity_error():
+    """More neighbours of one atom than the LDS sort holds (4096) is reported, not truncated."""
+    from dftbplus_amd import api, synth
+    geo, _, _ = synth.workload("diamond2")
+    with pytest.raises(api.DftbHipError, match="more than 4096"):
+        api.neighbour_list(geo.coords, geo.latvecs, 45.0)
