@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <numeric>
@@ -2104,6 +2105,38 @@ static int batch_scc_begin(BatchContext& b) {
   return 0;
 }
 
+// f(i) for i < n on up to 16 host threads (serially below minPar items); the first error of a
+// thread is raised again on the calling thread (error messages are per thread)
+static int parallel_for(int n, int minPar, const std::function<int(int)>& f) {
+  const int T = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (n < minPar || T == 1) {
+    for (int i = 0; i < n; ++i) DH_TRY(f(i));
+    return 0;
+  }
+  std::vector<int> rc(T, 0);
+  std::vector<std::string> msg(T);
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&, t]() {
+      for (int i = t; i < n; i += T) {
+        const int r = f(i);
+        if (r != 0) {
+          rc[t] = r;
+          msg[t] = last_error();
+          return;
+        }
+      }
+    });
+  for (auto& x : th) x.join();
+  for (int t = 0; t < T; ++t)
+    if (rc[t] != 0) {
+      set_error(msg[t]);
+      return rc[t];
+    }
+  return 0;
+}
+
 static int batch_scc_step(BatchContext& b) {
   hipStream_t s = b.s;
   const dftbhip_options& o = b.opt;
@@ -2146,14 +2179,16 @@ static int batch_scc_step(BatchContext& b) {
                                   ")");
       return kErrState;
     }
-  for (int m = 0; m < nmol; ++m) {
-    if (b.conv[m]) continue;   // fillings of converged molecules stay from their last pass
+  // Fermi levels per molecule, molecules on host threads
+  DH_TRY(parallel_for(nmol, 64, [&](int m) -> int {
+    if (b.conv[m]) return 0;   // fillings of converged molecules stay from their last pass
     const long o0 = b.orb0[m], nm = b.orb0[m + 1] - o0;
     std::vector<double> e(b.eigv.begin() + o0, b.eigv.begin() + o0 + nm), fm;
     DH_TRY(fermi_filling(e, b.nel[m], o.kT, fm, b.fr[m]));
     std::copy(fm.begin(), fm.end(), b.fill.begin() + o0);
     b.nocc[m] = occupied_levels(fm);
-  }
+    return 0;
+  }));
   DH_HIP(hipMemcpyAsync(b.f.p, b.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
   DH_HIP(hipMemcpyAsync(b.noccD.p, b.nocc.data(), nmol * sizeof(int), hipMemcpyHostToDevice, s));
   DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p, b.activeD.p));
@@ -2170,8 +2205,9 @@ static int batch_scc_step(BatchContext& b) {
   std::vector<double> dqo(nat);
   for (int a = 0; a < nat; ++a) dqo[a] = qo[a] - b.q0atom[a];
   DH_TRY(batch_potential(b, dqo, &vout));
-  for (int m = 0; m < nmol; ++m) {
-    if (b.conv[m]) continue;
+  // energies, convergence and Broyden mixing per molecule, molecules on host threads
+  DH_TRY(parallel_for(nmol, 64, [&](int m) -> int {
+    if (b.conv[m]) return 0;
     b.iters[m] = it;
     double en = 0.0, es = 0.0, er = 0.0;
     for (long oi = b.orb0[m]; oi < b.orb0[m + 1]; ++oi) en += eorbH[oi];
@@ -2189,12 +2225,14 @@ static int batch_scc_step(BatchContext& b) {
     b.err[m] = er;
     if (er < o.scc_tol && it >= o.min_scc_iter) {
       b.conv[m] = 1;
-      --nactive;
     } else if (it != o.max_scc_iter) {
       b.mixers[m].mix(q, diff);
       std::copy(q.begin(), q.end(), b.qin.begin() + b.atom0[m]);
     }
-  }
+    return 0;
+  }));
+  nactive = 0;
+  for (int m = 0; m < nmol; ++m) nactive += b.conv[m] ? 0 : 1;
   return 0;
 }
 
