@@ -46,6 +46,7 @@ SYMV_TRAFFIC_N2048 = {"n": 2048, "alg_bytes_per_launch": 5611520, "hbm_read_byte
                       "fetch_scale": 1.998, "write_scale": 1.000}
 LEGS = {"fp64": dict(precision=0),
         "mixed_fp32": dict(precision=1, refine_below=0.0),
+        "mixed_refined1": dict(precision=1, refine_below=float("inf"), refine_steps=1),
         "mixed_refined": dict(precision=1, refine_below=float("inf"), refine_steps=2)}
 E_BOUND, F_BOUND = 1e-6, 1e-5
 # full SCC cycles of --cycle: FP64 throughout, and FP32 iterations until the charge error of
