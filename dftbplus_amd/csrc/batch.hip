@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
   __shared__ double rc[kBatchMaxOrb / 2], rs[kBatchMaxOrb / 2];
   __shared__ int rp[kBatchMaxOrb / 2], rq[kBatchMaxOrb / 2];
   __shared__ int perm[kBatchMaxOrb];
-  __shared__ int flag, nrot;
+  __shared__ int flag, nrot, need;
   const int tid = threadIdx.x;
   const long base = L.dOff[m];
   const int n = (int)(L.orb0[m + 1] - L.orb0[m]);
@@ -291,9 +291,24 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
   const int nunit = np * (np + 1) / 2;
   constexpr int kAU = 4;   // A-update units per thread and pass
   const double eps = 2.220446049250313e-16;
+  const int npair = n * (n - 1) / 2;
   for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
-    if (tid == 0) nrot = 0;
+    // a sweep in which no pair passes the rotation test changes nothing: test all pairs first
+    if (tid == 0) {
+      nrot = 0;
+      need = 0;
+    }
     __syncthreads();
+    for (int u = tid; u < npair; u += NT) {
+      int k1, k2;
+      unit_pairs(u, k1, k2);
+      const int p = k1, q = k2 + 1;
+      const double apq = A[p + (long)q * n];
+      const double app = A[p + (long)p * n], aqq = A[q + (long)q * n];
+      if (fabs(apq) > 1e-300 && 100.0 * fabs(apq) > eps * fmin(fabs(app), fabs(aqq))) need = 1;
+    }
+    __syncthreads();
+    if (need == 0) break;
     for (int r = 0; r < N2 - 1; ++r) {
       for (int k = tid; k < np; k += NT) {
         int p, q;
@@ -411,7 +426,7 @@ __attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_ke
   __shared__ double rc[kMaxN / 2 + 1], rs[kMaxN / 2 + 1];
   __shared__ int rp[kMaxN / 2 + 1], rq[kMaxN / 2 + 1];
   __shared__ int perm[kMaxN];
-  __shared__ int flag, nrot;
+  __shared__ int flag, nrot, need;
   const int m = list[blockIdx.x];
   if (active && !active[m]) return;   // converged: eigenpairs stay from the last pass
   const int tid = threadIdx.x;
@@ -490,9 +505,24 @@ __attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_ke
   const int np = N2 / 2;
   const int nunit = np * (np + 1) / 2;
   const double eps = 2.220446049250313e-16;
+  const int npair = n * (n - 1) / 2;
   for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
-    if (tid == 0) nrot = 0;
+    // a sweep in which no pair passes the rotation test changes nothing: test all pairs first
+    if (tid == 0) {
+      nrot = 0;
+      need = 0;
+    }
     lds_barrier();
+    for (int u = tid; u < npair; u += NT) {
+      int k1, k2;
+      unit_pairs(u, k1, k2);
+      const int p = k1, q = k2 + 1;
+      const double apq = As[sym_idx(p, q)];
+      const double app = As[sym_idx(p, p)], aqq = As[sym_idx(q, q)];
+      if (fabs(apq) > 1e-300 && 100.0 * fabs(apq) > eps * fmin(fabs(app), fabs(aqq))) need = 1;
+    }
+    lds_barrier();
+    if (need == 0) break;
     for (int r = 0; r < N2 - 1; ++r) {
       for (int k = tid; k < np; k += NT) {
         int p, q;
