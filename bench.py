@@ -307,6 +307,7 @@ def mode_kpoints(args):
     pts = (0.0, 0.5) if args.kmesh == "gc" else (-0.25, 0.25)
     mesh = np.array([[a, b, c] for a in pts for b in pts for c in pts])
     opts = dataclasses.replace(opts, kpoints=mesh)
+    api.set_tuning(0, args.tiles)
     t0 = time.perf_counter()
     dp = api.DftbPlus(params, geo, opts)
     setup = time.perf_counter() - t0
@@ -450,7 +451,8 @@ def main():
     ap.add_argument("--kmesh", default="gc", choices=["gc", "mp"],
                     help="2x2x2 mesh of --mode kpoints: Gamma-centred (gc) or shifted (mp)")
     ap.add_argument("--tiles", type=int, default=0,
-                    help="SYMV row tiles per workgroup for --mode sytrd (0: by trailing size)")
+                    help="SYMV row tiles per workgroup for --mode sytrd and kpoints (0: by "
+                         "trailing size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCC path of a single Gamma-point system runs on one "

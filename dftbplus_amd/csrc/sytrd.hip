@@ -571,7 +571,8 @@ int sytrd_lower_batched_t(hipStream_t s, T* a, long lda, long sA, int n, T* d, T
       const int m = n - c - 1;
       if (m <= 0) continue;
       const int nRB = (m + kTB - 1) / kTB;
-      const int tiles = symv_tiles<T>(m);
+      // K problems in one launch fill the chip like one problem of K times the row blocks
+      const int tiles = symv_tiles<T>((int)std::min<long>((long)m * K, 1L << 30));
       const int nCh = (nRB + tiles - 1) / tiles;
       SymvProfile& pf = g_symvProf;
       if (pf.on) {
