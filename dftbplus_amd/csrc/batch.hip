@@ -13,49 +13,52 @@
 
 namespace dftbhip {
 
+// The two kernels below give each SK block 16 lanes, lane e holding element e = r + c n2 of the
+// column-major n2 x n1 block (lanes past n1 n2 idle): a wave's blocks are consecutive in the
+// sparse arrays, so their loads and stores are contiguous.
+constexpr int kBatchBlkLanes = 16;
+
 // sparse -> per-molecule dense (both triangles); the dense array is zeroed beforehand
 __global__ void batch_unpack_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
                                     BatchLayout L, const double* x, double* dense) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int k = (int)(t / kBatchBlkLanes), e = (int)(t % kBatchBlkLanes);
   if (k >= pr.nblk) return;
   const int a1 = pr.at1[k], a2 = pr.at2[k];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  if (e >= n1 * n2) return;
+  const int r = e % n2, c = e / n2;
   const int m = L.atomMol[a1];
   const long o0 = L.orb0[m];
   const int n = (int)(L.orb0[m + 1] - o0);
   double* d = dense + L.dOff[m];
-  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
   const long o1 = iAtomStart[a1] - o0, o2 = iAtomStart[a2] - o0;
-  const long st = pr.start[k];
-  for (int c = 0; c < n1; ++c)
-    for (int r = 0; r < n2; ++r) {
-      const double v = x[st + r + c * n2];
-      atomicAdd(&d[(o2 + r) + (o1 + c) * n], v);
-      if (a1 != a2) atomicAdd(&d[(o1 + c) + (o2 + r) * n], v);
-    }
+  const double v = x[pr.start[k] + e];
+  atomicAdd(&d[(o2 + r) + (o1 + c) * n], v);
+  if (a1 != a2) atomicAdd(&d[(o1 + c) + (o2 + r) * n], v);
 }
 
 // per-molecule dense (lower triangle read for on-site blocks) -> sparse
 __global__ void batch_pack_kernel(PairsDev pr, const int* norbAtom, const long* iAtomStart,
                                   BatchLayout L, const double* dense, double* x) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int k = (int)(t / kBatchBlkLanes), e = (int)(t % kBatchBlkLanes);
   if (k >= pr.nblk) return;
   const int a1 = pr.at1[k], a2 = pr.at2[k];
+  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
+  if (e >= n1 * n2) return;
+  const int r = e % n2, c = e / n2;
   const int m = L.atomMol[a1];
   const long o0 = L.orb0[m];
   const int n = (int)(L.orb0[m + 1] - o0);
   const double* d = dense + L.dOff[m];
-  const int n1 = norbAtom[a1], n2 = norbAtom[a2];
   const long o1 = iAtomStart[a1] - o0, o2 = iAtomStart[a2] - o0;
-  const long st = pr.start[k];
-  for (int c = 0; c < n1; ++c)
-    for (int r = 0; r < n2; ++r) {
-      long rr = r, cc = c;
-      if (a1 == a2 && r < c) {
-        rr = c;
-        cc = r;
-      }
-      x[st + r + c * n2] = d[(o2 + rr) + (o1 + cc) * n];
-    }
+  long rr = r, cc = c;
+  if (a1 == a2 && r < c) {
+    rr = c;
+    cc = r;
+  }
+  x[pr.start[k] + e] = d[(o2 + rr) + (o1 + cc) * n];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -698,7 +701,8 @@ int launch_batch_unpack(hipStream_t s, const PairsDev& pr, const int* norb,
                         double* dense, long denseSize) {
   DH_TRY(fill<double>(s, dense, denseSize, 0.0));
   if (pr.nblk == 0) return 0;
-  hipLaunchKernelGGL(batch_unpack_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+  hipLaunchKernelGGL(batch_unpack_kernel, dim3(cdiv((long)pr.nblk * kBatchBlkLanes, 256)),
+                     dim3(256), 0, s, pr, norb,
                      iAtomStart, L, x, dense);
   DH_LAUNCH_CHECK();
   return 0;
@@ -707,7 +711,8 @@ int launch_batch_unpack(hipStream_t s, const PairsDev& pr, const int* norb,
 int launch_batch_pack(hipStream_t s, const PairsDev& pr, const int* norb, const long* iAtomStart,
                       const BatchLayout& L, const double* dense, double* x) {
   if (pr.nblk == 0) return 0;
-  hipLaunchKernelGGL(batch_pack_kernel, dim3(cdiv(pr.nblk, 128)), dim3(128), 0, s, pr, norb,
+  hipLaunchKernelGGL(batch_pack_kernel, dim3(cdiv((long)pr.nblk * kBatchBlkLanes, 256)),
+                     dim3(256), 0, s, pr, norb,
                      iAtomStart, L, dense, x);
   DH_LAUNCH_CHECK();
   return 0;
