@@ -1899,6 +1899,30 @@ static int ctx_gradients(Context& c, double* grad) {
 // molecules is one block-diagonal system; every molecule keeps its own electron count, Fermi
 // level, Broyden mixer and convergence state, as in a loop over single-molecule calculations.
 // ------------------------------------------------------------------------------------------
+// Pinned host buffer (grown on demand, freed with the owner) for transfers repeated every
+// iteration; pageable copies are staged by the runtime at a fraction of the bandwidth.
+struct PinnedHost {
+  double* p = nullptr;
+  size_t n = 0;
+  PinnedHost() = default;
+  PinnedHost(const PinnedHost&) = delete;
+  PinnedHost& operator=(const PinnedHost&) = delete;
+  int ensure(size_t count) {
+    if (p && count <= n) return 0;
+    release();
+    DH_HIP(hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(double),
+                         hipHostMallocDefault));
+    n = count;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~PinnedHost() { release(); }
+};
+
 struct BatchContext {
   dftbhip_options opt{};
   SkTables sk;
@@ -1921,6 +1945,8 @@ struct BatchContext {
   BatchLayout L{};
   int it = 0, nactive = 0;
   bool sccDone = false;
+  // per-iteration transfers: eigenvalues and orbital charges and energies down, fillings up
+  PinnedHost pEig, pFill, pQorb, pEorb;
   // streams of the eigensolver's smaller size classes (default priority) and the events that
   // fork them from s and join them back
   static constexpr int kSx = kBatchClasses - 1;
@@ -2144,8 +2170,14 @@ static int batch_scc_step(BatchContext& b) {
   DH_CHECK(b.nactive > 0 && b.it < o.max_scc_iter, "batch scc_step: nothing left to iterate");
   const int it = ++b.it;
   int& nactive = b.nactive;
-  std::vector<double> dqa(nat), qorbH(n), eorbH(n), vout;
+  std::vector<double> dqa(nat), vout;
   std::vector<int> info(nmol);
+  DH_TRY(b.pEig.ensure(n));
+  DH_TRY(b.pFill.ensure(n));
+  DH_TRY(b.pQorb.ensure(n));
+  DH_TRY(b.pEorb.ensure(n));
+  double* qorbH = b.pQorb.p;
+  double* eorbH = b.pEorb.p;
   for (int a = 0; a < nat; ++a) dqa[a] = b.qin[a] - b.q0atom[a];
   DH_TRY(batch_potential(b, dqa, nullptr));
   DH_TRY(launch_shift(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.h0.p, b.ov.p,
@@ -2167,9 +2199,10 @@ static int batch_scc_step(BatchContext& b) {
     DH_HIP(hipStreamWaitEvent(s, b.join[q], 0));
   }
   b.eigv.resize(n);
-  DH_HIP(hipMemcpyAsync(b.eigv.data(), b.w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.pEig.p, b.w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipMemcpyAsync(info.data(), b.infoD.p, nmol * sizeof(int), hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
+  std::copy(b.pEig.p, b.pEig.p + n, b.eigv.begin());
   for (int m = 0; m < nmol; ++m)
     if (info[m] != 0) {
       set_error(info[m] < 0 ? "batch: molecule " + std::to_string(m) + " of " +
@@ -2189,15 +2222,16 @@ static int batch_scc_step(BatchContext& b) {
     b.nocc[m] = occupied_levels(fm);
     return 0;
   }));
-  DH_HIP(hipMemcpyAsync(b.f.p, b.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  std::copy(b.fill.begin(), b.fill.end(), b.pFill.p);
+  DH_HIP(hipMemcpyAsync(b.f.p, b.pFill.p, n * sizeof(double), hipMemcpyHostToDevice, s));
   DH_HIP(hipMemcpyAsync(b.noccD.p, b.nocc.data(), nmol * sizeof(int), hipMemcpyHostToDevice, s));
   DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p, b.activeD.p));
   DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p,
                            b.rho.p));
   DH_TRY(launch_mulliken(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.ov.p, b.h0.p,
                          b.rho.p, b.qorb.p, b.eorb.p, n));
-  DH_HIP(hipMemcpyAsync(qorbH.data(), b.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-  DH_HIP(hipMemcpyAsync(eorbH.data(), b.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(qorbH, b.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(eorbH, b.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
   std::vector<double> qo(nat, 0.0);
   for (int a = 0; a < nat; ++a)
