@@ -35,6 +35,9 @@ def lib():
             raise DftbHipError(f"{LIBPATH} not found: build it with dftbplus_amd.build.build()")
         _lib = C.CDLL(LIBPATH)
         _setup(_lib)
+        # the library's resident buffers are released while the HIP runtime is still up
+        import atexit
+        atexit.register(_lib.dftbhip_finalize)
     return _lib
 
 
@@ -113,6 +116,7 @@ EXPORTS = {
     "dftbhip_ctx_set_geometry": [C.c_void_p, C.c_int, _dp, _ip, _dp],
     "dftbhip_ctx_scc": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_ctx_scc_begin": [C.c_void_p],
+    "dftbhip_ctx_set_charge_restart": [C.c_void_p, C.c_int],
     "dftbhip_ctx_scc_step": [C.c_void_p, _ip, _dp],
     "dftbhip_ctx_scc_end": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_ctx_gradients": [C.c_void_p, _dp],
@@ -135,6 +139,13 @@ EXPORTS = {
     "dftbhip_batch_scc_begin": [C.c_void_p],
     "dftbhip_batch_scc_step": [C.c_void_p, _ip],
     "dftbhip_batch_scc_end": [C.c_void_p, _dp, _dp, _ip, _ip],
+    "dftbhip_efilling": [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp,
+                         _dp],
+    "dftbhip_mixer_create": [C.POINTER(C.c_void_p), C.c_int, C.c_double, C.c_double, C.c_double,
+                             C.c_double, C.c_double],
+    "dftbhip_mixer_reset": [C.c_void_p, C.c_int],
+    "dftbhip_mixer_mix": [C.c_void_p, _dp, _dp],
+    "dftbhip_mixer_destroy": [C.c_void_p],
 }
 
 
@@ -342,6 +353,51 @@ def dstedc(d: np.ndarray, e: np.ndarray):
     return dd, z
 
 
+def efilling(eigenvals: np.ndarray, kweights: np.ndarray, n_electrons: float, kT: float):
+    """Fermi filling of one spin channel (``Efilling``, etemp.F90:59-194) on the host:
+    eigenvals (nKPoints, nLevels), ascending per k-point. Returns (filling, Ef, Ebs, TS, E0)."""
+    e = np.ascontiguousarray(np.atleast_2d(eigenvals), dtype=np.float64)
+    nk, nlev = e.shape
+    w = _f64(kweights)
+    f = np.zeros_like(e)
+    out = np.zeros(4)
+    r = [C.c_double() for _ in range(4)]
+    _check(lib().dftbhip_efilling(nlev, nk, _d(e), _d(w), float(n_electrons), float(kT), _d(f),
+                                  *[C.byref(x) for x in r]), "efilling")
+    out[:] = [x.value for x in r]
+    return f, out[0], out[1], out[2], out[3]
+
+
+class BroydenMixer:
+    """The library's modified Broyden mixer (``TBroydenMixerReal``, broydenmixer.F90:112-342)."""
+
+    def __init__(self, n: int, mix_param: float = 0.2, omega0: float = 0.01,
+                 min_weight: float = 1.0, max_weight: float = 1.0e5, weight_fac: float = 1.0e-2,
+                 max_scc_iter: int = 100):
+        self._h = C.c_void_p()
+        _check(lib().dftbhip_mixer_create(C.byref(self._h), max_scc_iter, mix_param, omega0,
+                                          min_weight, max_weight, weight_fac), "mixer_create")
+        self.n = n
+        _check(lib().dftbhip_mixer_reset(self._h, n), "mixer_reset")
+
+    def mix(self, qin: np.ndarray, qdiff: np.ndarray) -> np.ndarray:
+        q = np.array(qin, dtype=np.float64)
+        d = _f64(qdiff)
+        _check(lib().dftbhip_mixer_mix(self._h, _d(q), _d(d)), "mixer_mix")
+        return q
+
+    def close(self):
+        if self._h:
+            lib().dftbhip_mixer_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def dm_real(evec: np.ndarray, filling: np.ndarray, nlevels: Optional[int] = None) -> np.ndarray:
     n = evec.shape[0]
     nl = evec.shape[1] if nlevels is None else nlevels
@@ -498,6 +554,12 @@ class DftbPlus:
         self.converged = bool(conv.value)
         self._grad = None
         self._dirty = False
+
+    def set_charge_restart(self, warm: bool):
+        """Start every later SCC cycle from the input charges of the previous one (as DFTB+
+        keeps qInput between geometry steps) instead of the neutral atoms."""
+        _check(lib().dftbhip_ctx_set_charge_restart(self.h, int(bool(warm))),
+               "dftbhip_ctx_set_charge_restart")
 
     # one-iteration control of the SCC cycle (used to time single iterations)
     def scc_begin(self):

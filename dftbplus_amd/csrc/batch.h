@@ -13,6 +13,9 @@ constexpr int kBatchLdsSmall = 64;
 constexpr int kBatchLdsC = 112;
 constexpr int kBatchLdsMid = 190;
 constexpr int kBatchClasses = 4;
+// info of a molecule whose Jacobi sweeps did not converge within their limit (distinct from the
+// leading minor of a non-positive-definite overlap and from the negative size-class error)
+constexpr int kBatchJacobiNoConv = 1 << 24;
 
 struct BatchLayout {
   int nmol;

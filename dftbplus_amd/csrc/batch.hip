@@ -74,6 +74,16 @@ __global__ void batch_pack_kernel(PairsDev pr, const int* norbAtom, const long* 
 // columns in Jacobi order, kept between calls).
 // ------------------------------------------------------------------------------------------
 constexpr int kJacobiMaxSweeps = 30;
+
+// Rotation test of a Jacobi pair: the off-diagonal element is above the relative level of the
+// smaller diagonal entry AND above the rounding level of the matrix (floor = 4 eps max|a_ii| of
+// the matrix the sweeps start from). Without the floor, a pair touching a diagonal entry near 0
+// would rotate forever on the rounding noise that other rotations leave in its element.
+__device__ inline bool jacobi_rotates(double apq, double app, double aqq, double floor) {
+  const double eps = 2.220446049250313e-16;
+  const double a = fabs(apq);
+  return a > 1e-300 && a > floor && 100.0 * a > eps * fmin(fabs(app), fabs(aqq));
+}
 constexpr int kSygvThreads = 1024;
 constexpr int kGT = 64;   // workgroup GEMM output tile
 constexpr int kGK = 16;   // workgroup GEMM k chunk
@@ -217,6 +227,7 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
   __shared__ int rp[kBatchMaxOrb / 2], rq[kBatchMaxOrb / 2];
   __shared__ int perm[kBatchMaxOrb];
   __shared__ int flag, nrot, need;
+  __shared__ unsigned long long dmaxBits;
   const int tid = threadIdx.x;
   const long base = L.dOff[m];
   const int n = (int)(L.orb0[m + 1] - L.orb0[m]);
@@ -293,8 +304,19 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
   const int np = N2 / 2;
   const int nunit = np * (np + 1) / 2;
   constexpr int kAU = 4;   // A-update units per thread and pass
-  const double eps = 2.220446049250313e-16;
   const int npair = n * (n - 1) / 2;
+  // rounding level of the matrix: 4 eps max |a_ii| (bits of non-negative doubles order as
+  // unsigned integers, so the maximum is an integer atomic)
+  if (tid == 0) dmaxBits = 0ull;
+  __syncthreads();
+  {
+    double dm = 0.0;
+    for (int i = tid; i < n; i += NT) dm = fmax(dm, fabs(A[i + (long)i * n]));
+    atomicMax(&dmaxBits, (unsigned long long)__double_as_longlong(dm));
+  }
+  __syncthreads();
+  const double rotFloor = 4.0 * 2.220446049250313e-16 * __longlong_as_double((long long)dmaxBits);
+  bool done = n <= 1;
   for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
     // a sweep in which no pair passes the rotation test changes nothing: test all pairs first
     if (tid == 0) {
@@ -308,10 +330,13 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
       const int p = k1, q = k2 + 1;
       const double apq = A[p + (long)q * n];
       const double app = A[p + (long)p * n], aqq = A[q + (long)q * n];
-      if (fabs(apq) > 1e-300 && 100.0 * fabs(apq) > eps * fmin(fabs(app), fabs(aqq))) need = 1;
+      if (jacobi_rotates(apq, app, aqq, rotFloor)) need = 1;
     }
     __syncthreads();
-    if (need == 0) break;
+    if (need == 0) {
+      done = true;
+      break;
+    }
     for (int r = 0; r < N2 - 1; ++r) {
       for (int k = tid; k < np; k += NT) {
         int p, q;
@@ -320,7 +345,7 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
         if (p < n && q < n) {
           const double apq = A[p + (long)q * n];
           const double app = A[p + (long)p * n], aqq = A[q + (long)q * n];
-          if (fabs(apq) > 1e-300 && 100.0 * fabs(apq) > eps * fmin(fabs(app), fabs(aqq))) {
+          if (jacobi_rotates(apq, app, aqq, rotFloor)) {
             const double th = (aqq - app) / (2.0 * apq);
             const double t = copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
             c = 1.0 / sqrt(t * t + 1.0);
@@ -384,7 +409,10 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
       rotate_columns<8>(Cw, n, np, rp, rq, rc, rs);
       __syncthreads();
     }
-    if (nrot == 0) break;
+    if (nrot == 0) {
+      done = true;
+      break;
+    }
     __syncthreads();
   }
   // ascending eigenvalues (stable ranks), C(:, rank) = Cw(:, i)
@@ -403,7 +431,9 @@ __global__ __launch_bounds__(kSygvThreads) void batch_sygv_kernel(BatchLayout L,
     const int i = (int)(e % n), c = (int)(e / n);
     Cm[e] = Cw[i + (long)perm[c] * n];
   }
-  if (tid == 0) info[m] = 0;
+  // a molecule whose last allowed sweep still rotated is reported (info > 0), not taken as
+  // converged
+  if (tid == 0) info[m] = done ? 0 : kBatchJacobiNoConv;
 }
 
 // The same eigensolver for molecules of at most kMaxN orbitals with A resident in LDS (upper
@@ -430,6 +460,7 @@ __attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_ke
   __shared__ int rp[kMaxN / 2 + 1], rq[kMaxN / 2 + 1];
   __shared__ int perm[kMaxN];
   __shared__ int flag, nrot, need;
+  __shared__ unsigned long long dmaxBits;
   const int m = list[blockIdx.x];
   if (active && !active[m]) return;   // converged: eigenpairs stay from the last pass
   const int tid = threadIdx.x;
@@ -507,8 +538,19 @@ __attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_ke
   const int N2 = n + (n & 1);
   const int np = N2 / 2;
   const int nunit = np * (np + 1) / 2;
-  const double eps = 2.220446049250313e-16;
   const int npair = n * (n - 1) / 2;
+  // rounding level of the matrix: 4 eps max |a_ii| (bits of non-negative doubles order as
+  // unsigned integers, so the maximum is an integer atomic)
+  if (tid == 0) dmaxBits = 0ull;
+  __syncthreads();
+  {
+    double dm = 0.0;
+    for (int i = tid; i < n; i += NT) dm = fmax(dm, fabs(As[sym_idx(i, i)]));
+    atomicMax(&dmaxBits, (unsigned long long)__double_as_longlong(dm));
+  }
+  __syncthreads();
+  const double rotFloor = 4.0 * 2.220446049250313e-16 * __longlong_as_double((long long)dmaxBits);
+  bool done = n <= 1;
   for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
     // a sweep in which no pair passes the rotation test changes nothing: test all pairs first
     if (tid == 0) {
@@ -522,10 +564,13 @@ __attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_ke
       const int p = k1, q = k2 + 1;
       const double apq = As[sym_idx(p, q)];
       const double app = As[sym_idx(p, p)], aqq = As[sym_idx(q, q)];
-      if (fabs(apq) > 1e-300 && 100.0 * fabs(apq) > eps * fmin(fabs(app), fabs(aqq))) need = 1;
+      if (jacobi_rotates(apq, app, aqq, rotFloor)) need = 1;
     }
     lds_barrier();
-    if (need == 0) break;
+    if (need == 0) {
+      done = true;
+      break;
+    }
     for (int r = 0; r < N2 - 1; ++r) {
       for (int k = tid; k < np; k += NT) {
         int p, q;
@@ -534,7 +579,7 @@ __attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_ke
         if (p < n && q < n) {
           const double apq = As[sym_idx(p, q)];
           const double app = As[sym_idx(p, p)], aqq = As[sym_idx(q, q)];
-          if (fabs(apq) > 1e-300 && 100.0 * fabs(apq) > eps * fmin(fabs(app), fabs(aqq))) {
+          if (jacobi_rotates(apq, app, aqq, rotFloor)) {
             const double th = (aqq - app) / (2.0 * apq);
             const double t = copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
             c = 1.0 / sqrt(t * t + 1.0);
@@ -596,7 +641,10 @@ __attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_ke
     }
     const int nr = nrot;
     lds_barrier();   // every thread has read nrot before the next sweep clears it
-    if (nr == 0) break;
+    if (nr == 0) {
+      done = true;
+      break;
+    }
   }
   __syncthreads();   // every column of C written before other threads read it
   double* wm = w + L.orb0[m];
@@ -617,7 +665,9 @@ __attribute__((amdgpu_waves_per_eu(kMaxN <= 64 ? 8 : 1))) void batch_sygv_lds_ke
     Cm[e] = Cr[i + (long)perm[c] * n];
     if (kCLds) Cw[e] = Cs[e];   // the next warm call starts from the rotated C
   }
-  if (tid == 0) info[m] = 0;
+  // a molecule whose last allowed sweep still rotated is reported (info > 0), not taken as
+  // converged
+  if (tid == 0) info[m] = done ? 0 : kBatchJacobiNoConv;
 }
 
 // P_m = C_m diag(f) C_m^T over the first nocc[m] columns (lower and upper)

@@ -32,12 +32,23 @@ static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 const std::string& last_error() { return g_err; }
 
+// Library state of the stateless drop-ins: the device, its stream, the eigensolver workspaces
+// and the staging matrices of sygvd and the density matrices, kept between calls (grown on
+// demand) and released by dftbhip_finalize. The object is never destroyed: device memory must
+// not be freed from a static destructor, which runs after the HIP runtime has shut down.
 struct Runtime {
   int device = -1;
   hipStream_t stream = nullptr;
   EigenSolver eig;
+  DevBuf<double> stageA, stageB, stageW;
+  void release() {
+    eig.release();
+    stageA.release();
+    stageB.release();
+    stageW.release();
+  }
 };
-static Runtime g_rt;
+static Runtime& g_rt = *new Runtime();
 
 static int ensure_runtime() {
   if (g_rt.stream) return 0;
@@ -248,7 +259,7 @@ extern "C" int dftbhip_init(int device) {
 }
 
 extern "C" int dftbhip_finalize(void) {
-  g_rt.eig.release();
+  g_rt.release();
   if (g_rt.stream) (void)hipStreamDestroy(g_rt.stream);
   g_rt.stream = nullptr;
   return 0;
@@ -276,7 +287,8 @@ extern "C" int dftbhip_sygvd(int itype, char jobz, char uplo, int n, double* a, 
   hipStream_t s = g_rt.stream;
   EigenSolver& E = g_rt.eig;
   DH_TRY(E.ensure(n));
-  DevBuf<double> H, S;
+  DevBuf<double>& H = g_rt.stageA;
+  DevBuf<double>& S = g_rt.stageB;
   DH_TRY(H.alloc((size_t)n * n));
   DH_TRY(S.alloc((size_t)n * n));
   DH_HIP(hipMemcpy2DAsync(H.p, n * sizeof(double), a, lda * sizeof(double), n * sizeof(double), n,
@@ -313,7 +325,9 @@ static int dm_common(int n, int nlev, const double* evec, int lde, const double*
   if (n == 0) return 0;
   DH_TRY(ensure_runtime());
   hipStream_t s = g_rt.stream;
-  DevBuf<double> X, P, f;
+  DevBuf<double>& X = g_rt.stageA;
+  DevBuf<double>& P = g_rt.stageB;
+  DevBuf<double>& f = g_rt.stageW;
   DH_TRY(X.alloc((size_t)n * std::max(nlev, 1)));
   DH_TRY(P.alloc((size_t)n * n));
   DH_TRY(f.upload(wts, std::max(nlev, 1), s));
@@ -904,6 +918,13 @@ static int fermi_filling(const std::vector<double>& e, double nel, double kT,
     r.ts = 2.0 * ts;
     r.e0 = 2.0 * (eb - 0.5 * ts);
   };
+  if (ne < 2.220446049250313e-16 || n == 0) {
+    // no electrons: empty levels, the Fermi level well below the lowest one
+    f.assign(n, 0.0);
+    r.ef = (n ? e.front() : 0.0) - 1000.0 * (kT + 1.1920928955078125e-07);
+    r.eband = r.ts = r.e0 = 0.0;
+    return 0;
+  }
   if (n <= ne) {
     ef = e.back() + 1000.0 * (kT + 1.1920928955078125e-07);
     finish(1.0);
@@ -1001,6 +1022,13 @@ static int fermi_filling_k(const std::vector<std::vector<double>>& e,
       emin = std::min(emin, x);
       emax = std::max(emax, x);
     }
+  if (ne < 2.220446049250313e-16 || nlev == 0) {
+    // no electrons: empty levels, the Fermi level well below the lowest one
+    f.assign(nk, std::vector<double>(nlev, 0.0));
+    r.ef = (nlev ? emin : 0.0) - 1000.0 * (kT + 1.1920928955078125e-07);
+    r.eband = r.ts = r.e0 = 0.0;
+    return 0;
+  }
   if (nlev <= ne) {
     ef = emax + 1000.0 * (kT + 1.1920928955078125e-07);
     finish(1.0);
@@ -1226,6 +1254,10 @@ struct Context {
   int nocc = 0;
   bool xf64 = true;     // the eigenvectors in X are FP64 (else FP32, mixed path)
   bool sccDone = false;
+  // SCC cycles start from the input charges the previous cycle ended with (the reference's
+  // geometry loop keeps qInput between geometry steps, main.F90:219-357) instead of the
+  // neutral reference charges
+  bool warmStart = false;
   double times[kTCount]{};
   int ncalls = 0;
 };
@@ -1471,7 +1503,8 @@ static int ctx_scc_begin(Context& c) {
   SccState& st = c.st;
   st.nel = 0.0;
   for (double q : c.q0atom) st.nel += q;
-  st.qin = c.q0atom;
+  // warm start: the input charges of the previous cycle, when it was on the same atoms
+  if (!(c.warmStart && st.it > 0 && st.qin.size() == c.q0atom.size())) st.qin = c.q0atom;
   st.mixer.reset(c.nat, c.opt, c.opt.max_scc_iter + 1);
   st.it = 0;
   st.conv = false;
@@ -1495,8 +1528,8 @@ static int ctx_scc_step(Context& c) {
   const double tIt = now();
   std::vector<double> dqa(nat), qorbH(n), eorbH(n), vout;
   for (int a = 0; a < nat; ++a) dqa[a] = st.qin[a] - c.q0atom[a];
-  hipEvent_t ev[8];
-  for (auto& x : ev) DH_HIP(hipEventCreate(&x));
+  Events<8> ev;
+  DH_TRY(ev.create());
   DH_HIP(hipEventRecord(ev[0], s));
   DH_TRY(gamma_times(c, dqa, nullptr));
   DH_TRY(launch_shift(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.h0.p, c.ov.p, c.vorb.p,
@@ -1528,8 +1561,8 @@ static int ctx_scc_step(Context& c) {
       DH_TRY(c.kE.alloc((size_t)K * n));
       DH_TRY(c.kTau.alloc((size_t)K * n));
       DH_TRY(c.kWs.alloc(K * wsStride));
-      hipEvent_t kev[3];
-      for (auto& x : kev) DH_HIP(hipEventCreate(&x));
+      Events<3> kev;
+      DH_TRY(kev.create());
       DH_HIP(hipEventRecord(kev[0], s));
       for (int k = 0; k < K; ++k) {
         DH_TRY(launch_unpack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
@@ -1546,7 +1579,6 @@ static int ctx_scc_step(Context& c) {
       evt[0] += ms * 1e-3;
       DH_HIP(hipEventElapsedTime(&ms, kev[1], kev[2]));
       evt[1] += ms * 1e-3;
-      for (auto& x : kev) (void)hipEventDestroy(x);
       // divide and conquer and back-transformation of the points side by side, one host thread
       // and stream each: a D&C is mostly host deflation between short device phases, which the
       // other points' device work fills
@@ -1566,7 +1598,7 @@ static int ctx_scc_step(Context& c) {
       DH_HIP(hipGetDevice(&dev));
       std::vector<int> rcK(K, 0), infoK(K, 0);
       std::vector<std::string> msgK(K);
-      auto dcPoint = [&](int k) -> int {
+      auto dcPointBody = [&](int k) -> int {
         Context::KPointDc& r = *c.kdc[k];
         DH_HIP(hipSetDevice(dev));
         std::vector<double> dh(n), eh(std::max(n - 1, 1));
@@ -1594,6 +1626,13 @@ static int ctx_scc_step(Context& c) {
                               r.st));
         DH_HIP(hipStreamSynchronize(r.st));
         return 0;
+      };
+      // an error leaves the point's thread only after the work it queued has ended, so that no
+      // kernel still writes the point's buffers when the caller reuses them
+      auto dcPoint = [&](int k) -> int {
+        const int rc = dcPointBody(k);
+        if (rc != 0) (void)hipStreamSynchronize(c.kdc[k]->st);
+        return rc;
       };
       const double tdc = now();
       {
@@ -1754,7 +1793,6 @@ static int ctx_scc_step(Context& c) {
   c.times[kTReduce] += evt[0];
   c.times[kTTridiag] += evt[1];
   c.times[kTDC] += evt[2];
-  for (auto& x : ev) (void)hipEventDestroy(x);
   const double th2 = now();
   c.qorbLast = qorbH;
   std::vector<double> qout(nat, 0.0);
@@ -2205,11 +2243,14 @@ static int batch_scc_step(BatchContext& b) {
   std::copy(b.pEig.p, b.pEig.p + n, b.eigv.begin());
   for (int m = 0; m < nmol; ++m)
     if (info[m] != 0) {
-      set_error(info[m] < 0 ? "batch: molecule " + std::to_string(m) + " of " +
-                                  std::to_string(-info[m]) + " orbitals in a smaller size class"
-                            : "batch: overlap of molecule " + std::to_string(m) +
-                                  " not positive definite (minor " + std::to_string(info[m]) +
-                                  ")");
+      set_error(info[m] == kBatchJacobiNoConv
+                    ? "batch: Jacobi eigensolver of molecule " + std::to_string(m) +
+                          " did not converge within its sweep limit"
+                : info[m] < 0 ? "batch: molecule " + std::to_string(m) + " of " +
+                                    std::to_string(-info[m]) + " orbitals in a smaller size class"
+                              : "batch: overlap of molecule " + std::to_string(m) +
+                                    " not positive definite (minor " + std::to_string(info[m]) +
+                                    ")");
       return kErrState;
     }
   // Fermi levels per molecule, molecules on host threads
@@ -2478,6 +2519,12 @@ extern "C" int dftbhip_ctx_scc(dftbhip_ctx* ctx, double* energies, double* charg
   return ctx_scc(ctx->c, energies, charges, niter, converged);
 }
 
+extern "C" int dftbhip_ctx_set_charge_restart(dftbhip_ctx* ctx, int warm) {
+  DH_CHECK(ctx, "set_charge_restart: null context");
+  ctx->c.warmStart = warm != 0;
+  return 0;
+}
+
 extern "C" int dftbhip_ctx_scc_begin(dftbhip_ctx* ctx) {
   DH_CHECK(ctx && ctx->c.nat > 0, "scc_begin: no geometry");
   return ctx_scc_begin(ctx->c);
@@ -2544,6 +2591,101 @@ extern "C" int dftbhip_ctx_timings(dftbhip_ctx* ctx, double* t, int n) {
   DH_CHECK(ctx && t, "timings: null pointer");
   for (int i = 0; i < n && i < kTCount; ++i) t[i] = ctx->c.times[i];
   return ctx->c.ncalls;
+}
+
+// ------------------------------------------------------------------------------------------
+// Host-side pieces of the SCC cycle with the reference's interfaces: Fermi filling and the
+// Broyden mixer (no device work; callable without a GPU)
+// ------------------------------------------------------------------------------------------
+extern "C" int dftbhip_efilling(int nLevels, int nKPoints, const double* eigenvals,
+                                const double* kWeight, double nElectrons, double kT,
+                                double* filling, double* Ef, double* Ebs, double* TS,
+                                double* E0) {
+  DH_CHECK(nLevels >= 0 && nKPoints >= 1, "efilling: bad dimensions");
+  DH_CHECK(eigenvals && kWeight && filling && Ef && Ebs && TS && E0, "efilling: null pointer");
+  DH_CHECK(nElectrons >= 0.0 && kT > 0.0, "efilling: negative electron count or temperature");
+  DH_CHECK(std::ceil(nElectrons) <= 2.0 * nLevels, "efilling: too many electrons for the levels");
+  double wsum = 0.0;
+  for (int k = 0; k < nKPoints; ++k) {
+    DH_CHECK(kWeight[k] >= 0.0, "efilling: negative k-point weight");
+    wsum += kWeight[k];
+  }
+  DH_CHECK(wsum > 0.0, "efilling: k-point weights sum to zero");
+  for (int k = 0; k < nKPoints; ++k)
+    for (int j = 1; j < nLevels; ++j)
+      DH_CHECK(eigenvals[j + (size_t)k * nLevels] >= eigenvals[j - 1 + (size_t)k * nLevels],
+               "efilling: eigenvalues of a k-point must be ascending");
+  // the SCC cycle's fillings carry the factor 2 of the spin-unpolarised case; the
+  // reference's Efilling fills one spin channel with nElectrons
+  FillResult r{};
+  if (nKPoints == 1 && kWeight[0] == 1.0) {
+    std::vector<double> e(eigenvals, eigenvals + nLevels), f;
+    DH_TRY(fermi_filling(e, 2.0 * nElectrons, kT, f, r));
+    for (int j = 0; j < nLevels; ++j) filling[j] = 0.5 * f[j];
+  } else {
+    std::vector<std::vector<double>> e(nKPoints), f;
+    std::vector<double> kw(nKPoints);
+    for (int k = 0; k < nKPoints; ++k) {
+      e[k].assign(eigenvals + (size_t)k * nLevels, eigenvals + (size_t)(k + 1) * nLevels);
+      kw[k] = kWeight[k];
+    }
+    DH_TRY(fermi_filling_k(e, kw, 2.0 * nElectrons, kT, f, r));
+    for (int k = 0; k < nKPoints; ++k)
+      for (int j = 0; j < nLevels; ++j) filling[j + (size_t)k * nLevels] = 0.5 * f[k][j];
+  }
+  *Ef = r.ef;
+  *Ebs = 0.5 * r.eband;
+  *TS = 0.5 * r.ts;
+  *E0 = 0.5 * r.e0;
+  return 0;
+}
+
+struct dftbhip_mixer {
+  dftbhip::Broyden b;
+  dftbhip_options o{};
+  int maxIter = 0;
+  int nElem = 0;
+};
+
+extern "C" int dftbhip_mixer_create(dftbhip_mixer** m, int maxSccIter, double mixParam,
+                                    double omega0, double minWeight, double maxWeight,
+                                    double weightFac) {
+  DH_CHECK(m, "mixer_create: null pointer");
+  DH_CHECK(maxSccIter > 0 && mixParam > 0.0 && omega0 > 0.0, "mixer_create: bad parameters");
+  auto* x = new dftbhip_mixer();
+  x->maxIter = maxSccIter;
+  x->o.mix_param = mixParam;
+  x->o.omega0 = omega0;
+  x->o.min_weight = minWeight;
+  x->o.max_weight = maxWeight;
+  x->o.weight_fac = weightFac;
+  *m = x;
+  return 0;
+}
+
+extern "C" int dftbhip_mixer_reset(dftbhip_mixer* m, int nElem) {
+  DH_CHECK(m && nElem > 0, "mixer_reset: bad arguments");
+  m->nElem = nElem;
+  m->b.reset(nElem, m->o, m->maxIter);
+  return 0;
+}
+
+extern "C" int dftbhip_mixer_mix(dftbhip_mixer* m, double* qInpResult, const double* qDiff) {
+  DH_CHECK(m && qInpResult && qDiff, "mixer_mix: null pointer");
+  DH_CHECK(m->nElem > 0, "mixer_mix: call mixer_reset first");
+  if (m->b.it + 1 > m->maxIter) {
+    set_error("Broyden mixer: Maximal nr. of steps exceeded");
+    return kErrState;
+  }
+  std::vector<double> q(qInpResult, qInpResult + m->nElem), d(qDiff, qDiff + m->nElem);
+  m->b.mix(q, d);
+  std::copy(q.begin(), q.end(), qInpResult);
+  return 0;
+}
+
+extern "C" int dftbhip_mixer_destroy(dftbhip_mixer* m) {
+  delete m;
+  return 0;
 }
 
 // ------------------------------------------------------------------------------------------

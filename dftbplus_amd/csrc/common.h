@@ -109,6 +109,24 @@ struct DevBuf {
 
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// A fixed set of HIP events, destroyed with the owner (also on early error returns).
+template <int N>
+struct Events {
+  hipEvent_t e[N] = {};
+  int create() {
+    for (auto& x : e) DH_HIP(hipEventCreate(&x));
+    return 0;
+  }
+  Events() = default;
+  Events(const Events&) = delete;
+  Events& operator=(const Events&) = delete;
+  ~Events() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+  hipEvent_t& operator[](int i) { return e[i]; }
+};
+
 // ------------------------------------------------------------------------------------------
 // GEMM core (gemm.hip):  C = alpha * op(A) * diag(w) * op(B) + beta * C, column-major.
 // ------------------------------------------------------------------------------------------

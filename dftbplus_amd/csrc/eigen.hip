@@ -87,8 +87,8 @@ int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<dou
                        int* info, double* times) {
   DH_CHECK(factored, "eigensolver: overlap not factorised");
   *info = 0;
-  hipEvent_t ev[5];
-  for (auto& x : ev) DH_HIP(hipEventCreate(&x));
+  Events<5> ev;
+  DH_TRY(ev.create());
   DH_HIP(hipEventRecord(ev[0], s));
   DH_TRY(reduce(s, Linv.p, H, ldh, C.p));
   DH_HIP(hipEventRecord(ev[1], s));
@@ -100,10 +100,7 @@ int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<dou
   }
   DH_HIP(hipEventRecord(ev[2], s));
   DH_TRY(tridiag_eigen(s, d.p, e.p, wHost, info));
-  if (*info != 0) {
-    for (auto& x : ev) (void)hipEventDestroy(x);
-    return 0;
-  }
+  if (*info != 0) return 0;
   DH_HIP(hipEventRecord(ev[3], s));
   DH_HIP(hipEventSynchronize(ev[3]));
   if (times) {
@@ -115,7 +112,6 @@ int EigenSolver::solve(hipStream_t s, const double* H, long ldh, std::vector<dou
     DH_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
     times[2] += ms * 1e-3;
   }
-  for (auto& x : ev) (void)hipEventDestroy(x);
   return 0;
 }
 
@@ -220,8 +216,8 @@ int EigenSolver::solve_mixed(hipStream_t s, const float* H, long ldh, std::vecto
   *info = 0;
   float* A32 = reinterpret_cast<float*>(C.p);
   float* T32 = reinterpret_cast<float*>(T.p);
-  hipEvent_t ev[4];
-  for (auto& x : ev) DH_HIP(hipEventCreate(&x));
+  Events<4> ev;
+  DH_TRY(ev.create());
   DH_HIP(hipEventRecord(ev[0], s));
   DH_TRY(gemm<float>(s, false, false, n, n, n, 1.0f, Linv32.p, n, H, ldh, 0.0f, T32, n, false,
                      kKALower));
@@ -253,7 +249,6 @@ int EigenSolver::solve_mixed(hipStream_t s, const float* H, long ldh, std::vecto
       times[k] += ms * 1e-3;
     }
   }
-  for (auto& x : ev) (void)hipEventDestroy(x);
   return 0;
 }
 

@@ -410,6 +410,8 @@ def _kpoints(ham: Node, periodic: bool):
         shift = np.array(v[9:])
         pts = np.array([[(i + shift[0]) / n[0], (j + shift[1]) / n[1], (k + shift[2]) / n[2]]
                         for i in range(n[0]) for j in range(n[1]) for k in range(n[2])])
+        # folded into [0, 1) as getSuperSampling does (periodic.F90:1236)
+        pts = np.mod(pts, 1.0)
         return _reduce_by_inversion(pts, np.full(len(pts), 1.0 / len(pts)))
     v = [_f(t) for t in node.data]
     if len(v) % 4 or not v:
@@ -420,19 +422,22 @@ def _kpoints(ham: Node, periodic: bool):
 
 
 def _reduce_by_inversion(k: np.ndarray, w: np.ndarray):
-    """Merge k and -k (modulo reciprocal lattice vectors), summing their weights
-    (time-reversal symmetry, ``reduceByInversion`` of periodic.F90:1239-1262)."""
-    keep_k, keep_w = [], []
-    for kp, wp in zip(k, w):
-        for idx, q in enumerate(keep_k):
-            d = kp + q
-            if np.all(np.abs(d - np.round(d)) < 1e-8):
-                keep_w[idx] += wp
-                break
-        else:
-            keep_k.append(kp.copy())
-            keep_w.append(wp)
-    return np.array(keep_k), np.array(keep_w)
+    """Time-reversal reduction of getSuperSampling (periodic.F90:1239-1269): for each point
+    still irreducible, in order, every later point within 1e-4 (per component) of
+    modulo(-k, 1) is dropped and its weight added to the earlier one."""
+    nk = len(k)
+    irreducible = np.ones(nk, dtype=bool)
+    w = np.array(w, dtype=float)
+    tol = 1e-4
+    for i1 in range(nk):
+        if not irreducible[i1]:
+            continue
+        rr = np.mod(-k[i1], 1.0)
+        for i2 in range(i1 + 1, nk):
+            if irreducible[i2] and np.all(np.abs(k[i2] - rr) < tol):
+                irreducible[i2] = False
+                w[i1] += w[i2]
+    return k[irreducible].copy(), w[irreducible]
 
 
 def _sk_paths(ham: Node, names: List[str], base: str) -> Dict[Tuple[int, int], str]:

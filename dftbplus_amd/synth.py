@@ -294,8 +294,10 @@ def workload(name: str):
         mols, el = molecule_batch(int(name[len("molecules"):] or 4096), 3)
         params, _ = make_parameters(el, 103)
         return mols, params, Options(kT=1.0e-2, scc_tol=1.0e-8, max_scc_iter=200)
-    if name == "amorphous2000":
-        geo, el = amorphous_network(2000, 4)
+    if name.startswith("amorphous"):
+        # the 2000-atom C/Si/H/O network (seed 4, 2.3 g/cm^3) or a smaller cell of the same
+        # recipe (amorphousN: N atoms, same seed, density and parameter set)
+        geo, el = amorphous_network(int(name[len("amorphous"):] or 2000), 4)
         params, _ = make_parameters(el, 104)
         return geo, params, Options(kT=1.0e-3, scc_tol=1.0e-8)
     raise KeyError(name)

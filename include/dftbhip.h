@@ -178,6 +178,11 @@ int dftbhip_ctx_scc(dftbhip_ctx* ctx, double* energies, double* charges, int* ni
  * (one full iteration incl. mixing; converged flag and max |q_out - q_in|), end (energies and
  * charges of the last iteration, as dftbhip_ctx_scc) */
 int dftbhip_ctx_scc_begin(dftbhip_ctx* ctx);
+/* warm = 1: every SCC cycle after the first starts from the input charges the previous cycle
+ * ended with (on the same atoms), as the reference's geometry loop keeps qInput from one
+ * geometry step to the next (src/dftbp/dftbplus/main.F90:219-357, getNextInputCharges
+ * :4772-4963); warm = 0 (the default): every cycle starts from the neutral reference charges. */
+int dftbhip_ctx_set_charge_restart(dftbhip_ctx* ctx, int warm);
 int dftbhip_ctx_scc_step(dftbhip_ctx* ctx, int* converged, double* qerr);
 int dftbhip_ctx_scc_end(dftbhip_ctx* ctx, double* energies, double* charges, int* niter,
                         int* converged);
@@ -197,8 +202,9 @@ int dftbhip_ctx_nr_orbitals(dftbhip_ctx* ctx);
 int dftbhip_ctx_timings(dftbhip_ctx* ctx, double* t, int n);
 
 /* ---- molecule batch: many independent non-periodic molecules in one set of launches -------
- * Replaces a loop over single-geometry calculations (one dftbp_* instance per molecule,
- * src/dftbp/api/mm/mmapi.F90 / api/dftbplus.h) with one block-diagonal system: per molecule its
+ * Replaces a loop over single-geometry calculations (one dftbp_* instance per molecule:
+ * dftbp_init / dftbp_process_input / dftbp_get_energy, src/dftbp/api/dftbplus.h:215-550,
+ * src/dftbp/api/capi.F90:90-135) with one block-diagonal system: per molecule its
  * own electron count, Fermi level, Broyden mixer and convergence. FP64 only; at most 512
  * orbitals per molecule. coords(3, sum nAtomMol) bohr, species 0-based, molecules one after
  * the other. energies(8, nMol) as dftbhip_ctx_scc; charges(sum nAtomMol) = q0 - q. */
@@ -217,6 +223,28 @@ int dftbhip_batch_scc_step(dftbhip_batch* batch, int* nactive);
 int dftbhip_batch_scc_end(dftbhip_batch* batch, double* energies, double* charges, int* niter,
                           int* converged);
 int dftbhip_batch_gradients(dftbhip_batch* batch, double* grad);
+
+/* ---- host-side pieces of the SCC cycle (no device work, callable without a GPU) ---------
+ * Fermi filling of one spin channel over nKPoints k-points, as Efilling with the Fermi
+ * distribution (src/dftbp/dftb/etemp.F90:59-194: zero-electron and full-band shortcuts, middle
+ * gap for integer counts, bisection, final rescale to the exact count). eigenvals(nLevels,
+ * nKPoints) ascending per k-point, kWeight(nKPoints); filling(nLevels, nKPoints) in [0, 1];
+ * Ebs, TS, E0 k-weighted. The SCC cycle calls it with half the electron count and doubles the
+ * results (getFillingsAndBandEnergies, src/dftbp/dftbplus/main.F90:4544-4651). */
+int dftbhip_efilling(int nLevels, int nKPoints, const double* eigenvals, const double* kWeight,
+                     double nElectrons, double kT, double* filling, double* Ef, double* Ebs,
+                     double* TS, double* E0);
+/* Modified Broyden mixer, TBroydenMixerReal (src/dftbp/mixer/broydenmixer.F90:112-342):
+ * create = TBroydenMixerReal_init (maxSccIter, mixParam, omega0, minWeight, maxWeight,
+ * weightFac), reset = reset(nElem), mix = mix(qInpResult, qDiff): qInpResult holds the input
+ * charges on entry and the mixed charges on exit. More than maxSccIter mix calls after a reset
+ * is an error ("Broyden mixer: Maximal nr. of steps exceeded"), as in the reference. */
+typedef struct dftbhip_mixer dftbhip_mixer;
+int dftbhip_mixer_create(dftbhip_mixer** mixer, int maxSccIter, double mixParam, double omega0,
+                         double minWeight, double maxWeight, double weightFac);
+int dftbhip_mixer_reset(dftbhip_mixer* mixer, int nElem);
+int dftbhip_mixer_mix(dftbhip_mixer* mixer, double* qInpResult, const double* qDiff);
+int dftbhip_mixer_destroy(dftbhip_mixer* mixer);
 
 /* ---- measurement: per-launch HIP-event timing of the tridiagonalisation's symmetric
  *      product (the HBM-bound kernel of the eigensolver). stats: out[0] launches, out[1] summed

@@ -788,6 +788,10 @@ def fermi_filling_k(eig: np.ndarray, nel: float, kT: float, kw: np.ndarray):
         f, eb, ts = fill(ef)
         return 2 * f * scale, ef, 2 * eb, 2 * ts, 2 * (eb - 0.5 * ts)
 
+    if ne < EPS:
+        # no electrons (etemp.F90:113-122): empty levels, E_F well below the lowest level
+        return (np.zeros_like(eig), eig.min() - 1000.0 * (kT + float(np.finfo(np.float32).eps)),
+                0.0, 0.0, 0.0)
     if nlev <= ne:
         return done(eig.max() + 1000.0 * (kT + float(np.finfo(np.float32).eps)))
     if abs(ne - round(ne)) <= ELEC_TOL:
@@ -840,6 +844,10 @@ def fermi_filling(eig: np.ndarray, nel: float, kT: float):
     ne = 0.5 * nel
     eig = np.asarray(eig)
     n = len(eig)
+    if ne < EPS:
+        # no electrons (etemp.F90:113-122): empty levels, E_F well below the lowest level
+        return (np.zeros_like(eig), eig.min() - 1000.0 * (kT + float(np.finfo(np.float32).eps)),
+                0.0, 0.0, 0.0)
     if n <= ne:
         ef = eig.max() + 1000.0 * (kT + float(np.finfo(np.float32).eps))
         f, eb, ts = _electron_fill(ef, eig, kT)

@@ -78,3 +78,67 @@ def test_kpoints_together_equal_one_by_one():
     assert abs(e1 - e0) / geo.natom < 1e-13
     assert np.max(np.abs(q1 - q0)) < 1e-12
     assert np.max(np.abs(g1 - g0)) < 1e-11
+
+
+@pytest.fixture(scope="module")
+def amorphous64():
+    """A 64-atom cell of the amorphous C/Si/H/O network recipe (seed 4, 2.3 g/cm^3, 247
+    orbitals): the config-5 workload at a size the oracle solves in seconds per mesh."""
+    from dftbplus_amd import synth
+    return synth.workload("amorphous64")
+
+
+@pytest.mark.parametrize("mesh", ["gc222", "mp222"])
+def test_amorphous_network_kpoints_match_oracle(amorphous64, mesh):
+    """The config-5 recipe (C/Si/H/O, the only workload with Si) with the Gamma-centred 2x2x2
+    mesh (eight real problems, tridiagonalised together) and the Monkhorst-Pack 2x2x2 mesh
+    (k = +-1/4, genuinely complex) against the oracle's complex path at the FP64 bounds."""
+    from dftbplus_amd import api
+    from oracle import dftb_ref as R
+    geo, params, opts = amorphous64
+    o = dataclasses.replace(opts, kpoints=MESHES[mesh])
+    calc = R.Calculator(params, geo, o)
+    res = calc.scc()
+    og = calc.gradients(res)
+    dp = api.DftbPlus(params, geo, o)
+    e, q, g = dp.get_energy(), dp.get_gross_charges(), dp.get_gradients()
+    nat = geo.natom
+    print(f"amorphous64 {mesh}: it {dp.n_iter}/{res.n_iter} dE/atom "
+          f"{abs(e - res.energy_mermin) / nat:.2e} dq {np.max(np.abs(q - res.charges)):.2e} "
+          f"dF {np.max(np.abs(g - og)):.2e}")
+    assert dp.converged and res.converged
+    assert abs(e - res.energy_mermin) / nat < 1e-8
+    assert np.max(np.abs(q - res.charges)) < 1e-8
+    assert np.max(np.abs(g - og)) < 1e-7
+    dp.close()
+
+
+def test_md_steps_warm_start_equal_cold(amorphous64):
+    """MD-like steps (every coordinate moved by up to +-0.05 bohr per step) with the SCC
+    warm-started from the previous step's input charges reach the fixed points of cold starts
+    (to the SCC tolerance) in fewer iterations."""
+    from dftbplus_amd import api
+    geo, params, opts = amorphous64
+    o = dataclasses.replace(opts, kpoints=MESHES["gc222"])
+    rng = np.random.default_rng(4)
+    warm = api.DftbPlus(params, geo, o)
+    warm.set_charge_restart(True)
+    warm.get_energy()
+    coords = geo.coords.copy()
+    it_w, it_c = [], []
+    for step in range(3):
+        coords = coords + rng.uniform(-0.05, 0.05, coords.shape)
+        warm.set_geometry(coords, geo.latvecs)
+        ew, qw, gw = warm.get_energy(), warm.get_gross_charges(), warm.get_gradients()
+        cold = api.DftbPlus(params, type(geo)(coords, geo.species, geo.latvecs), o)
+        ec, qc, gc = cold.get_energy(), cold.get_gross_charges(), cold.get_gradients()
+        it_w.append(warm.n_iter)
+        it_c.append(cold.n_iter)
+        print(f"step {step}: warm {warm.n_iter} it, cold {cold.n_iter} it, dE {abs(ew - ec):.2e}")
+        assert warm.converged and cold.converged
+        assert abs(ew - ec) / geo.natom < 1e-9
+        assert np.max(np.abs(qw - qc)) < 5e-8
+        assert np.max(np.abs(gw - gc)) < 1e-7
+        cold.close()
+    assert sum(it_w) < sum(it_c)
+    warm.close()

@@ -77,6 +77,33 @@ def test_fractional_gen_and_supercell_folding():
         hsd.load_input(os.path.join(GOLD, "si2_richardson.hsd"))
 
 
+def test_supercell_folding_with_shift_outside_unit_interval():
+    """getSuperSampling (periodic.F90:1160-1280) folds the points into [0, 1) before the
+    inversion reduction: 2 x 2 x 2 with shifts 1.5 gives the points (i + 1.5) / 2 mod 1 in
+    (i1, i2, i3) order with i3 fastest, and each later point at modulo(-k, 1) merges into the
+    first. The expected list below is worked out by hand from those steps."""
+    text = """
+    KPointsAndWeights = SupercellFolding {
+      2 0 0
+      0 2 0
+      0 0 2
+      1.5 1.5 1.5
+    }
+    """
+    ham = hsd.parse_hsd(text)
+    k, w = hsd._kpoints(ham, True)
+    expected = np.array([[0.75, 0.75, 0.75], [0.75, 0.75, 0.25], [0.75, 0.25, 0.75],
+                         [0.75, 0.25, 0.25]])
+    np.testing.assert_array_equal(k, expected)
+    np.testing.assert_array_equal(w, np.full(4, 0.25))
+    # an odd folding with a shift below zero keeps the Gamma point and pairs the rest
+    text3 = text.replace("2 0 0", "3 0 0").replace("0 2 0", "0 1 0").replace("0 0 2", "0 0 1")
+    text3 = text3.replace("1.5 1.5 1.5", "-1.0 0.0 0.0")
+    k3, w3 = hsd._kpoints(hsd.parse_hsd(text3), True)
+    np.testing.assert_allclose(k3, [[2.0 / 3.0, 0, 0], [0.0, 0, 0]], atol=1e-15)
+    np.testing.assert_allclose(w3, [2.0 / 3.0, 1.0 / 3.0], rtol=1e-15)
+
+
 def test_hsd_syntax_forms(tmp_path):
     (tmp_path / "inc.hsd").write_text("Inner = 3\n")
     (tmp_path / "vals.txt").write_text("1 2\n3 4\n")

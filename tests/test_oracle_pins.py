@@ -157,6 +157,62 @@ def test_expgamma_limits_and_symmetry():
         assert R.expgamma_prime(2.5, ua, ub) == pytest.approx(fd, rel=1e-7)
 
 
+def _slater_gamma(r, ua, ub, deriv=False):
+    """Coulomb interaction of two normalised exponential charge densities tau^3/(8 pi) e^(-tau r),
+    tau = 3.2 U, from their Fourier transforms F(k) = tau^4 / (tau^2 + k^2)^2:
+    gamma(R) = (2/pi) int_0^inf F_a F_b sin(kR)/(kR) dk (its R-derivative with deriv=True),
+    and at R = 0 gamma(0) = (2/pi) int_0^inf F_a F_b dk."""
+    from scipy.integrate import quad
+    ta, tb = 3.2 * ua, 3.2 * ub
+
+    def ff(k):
+        return (ta ** 4 / (ta ** 2 + k ** 2) ** 2) * (tb ** 4 / (tb ** 2 + k ** 2) ** 2)
+
+    def kern(k):
+        x = k * r
+        if r == 0.0:
+            return 1.0
+        if not deriv:
+            return math.sin(x) / x if x > 1e-8 else 1.0 - x * x / 6
+        return k * (math.cos(x) / x - math.sin(x) / x ** 2) if x > 1e-3 else k * (-x / 3 + x ** 3 / 30)
+
+    val, _ = quad(lambda k: ff(k) * kern(k), 0, np.inf, limit=500, epsabs=1e-14, epsrel=1e-13)
+    return 2.0 / math.pi * val
+
+
+@pytest.mark.parametrize("ua,ub", [(0.3, 0.47), (0.47, 0.3), (0.35, 0.5), (0.4, 0.4)])
+def test_expgamma_against_slater_density_coulomb_integral(ua, ub):
+    """expGamma (shortgammafuncs.F90:97-148, gammaSubExprn_ :267-299) is 1/R minus the Coulomb
+    integral of two Slater-type (exponential) charge densities of exponents 3.2 U_a, 3.2 U_b,
+    and its on-site value is minus that integral at R = 0 (= -U for equal U). Both gsub terms
+    of the unequal-U branch are checked, so altering either one fails."""
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for r in (0.3, 0.5, 1.0, 1.7, 2.5, 4.0, 6.0, 9.0):
+            ref = 1.0 / r - _slater_gamma(r, ua, ub)
+            assert abs(R.expgamma(r, ua, ub) - ref) < 1e-12 * max(1.0, 1.0 / r), (r, ua, ub)
+            dref = -1.0 / r ** 2 - _slater_gamma(r, ua, ub, deriv=True)
+            assert abs(R.expgamma_prime(r, ua, ub) - dref) < 1e-12 * max(1.0, 1.0 / r ** 2), (r,)
+        g0 = _slater_gamma(0.0, ua, ub)
+    assert R.expgamma(0.0, ua, ub) == pytest.approx(-g0, rel=1e-13, abs=1e-14)
+    if ua == ub:
+        assert R.expgamma(0.0, ua, ub) == -ua
+
+
+def test_expgamma_altered_term_fails_the_pin():
+    """The integral pin separates the reference's formula from a plausible transcription error
+    (the second gammaSubExprn_ term with its exponents swapped)."""
+    import warnings
+    ua, ub, r = 0.3, 0.47, 1.7
+    ta, tb = 3.2 * ua, 3.2 * ub
+    wrong = R._gsub(r, ta, tb) + R._gsub(r, ta, tb)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ref = 1.0 / r - _slater_gamma(r, ua, ub)
+    assert abs(wrong - ref) > 1e-3 * abs(ref)
+
+
 def test_coulomb_derivative_fixture():
     """Reference unit test coulombderivcluster: dV/dR of a +1/-1 pair at 1 bohr."""
     fx = json.load(open(os.path.join(GOLDEN, "coulomb_cluster.json")))
@@ -209,9 +265,99 @@ def test_fermi_filling_counts_and_two_level_closed_form():
     assert ts == pytest.approx(4 * kT * -(x * math.log(x) + (1 - x) * math.log(1 - x)), rel=1e-10)
 
 
+def test_fermi_rescale_branch_closed_form():
+    """Efilling's final rescale (etemp.F90:186-189): a three-fold degenerate level holding a
+    fractional share of the electrons at kT = 1e-8 Ha, where the electron count changes by
+    7.5e7 per Ha of E_F, so the bisection stops at the floating-point resolution of E_F with a
+    count error far above 2 eps. The rescaled fillings hold the exact count, are the Fermi
+    function at the returned E_F times nEl/count, and split the degenerate level evenly."""
+    eig = np.array([-0.8, -0.5, 0.3, 0.3, 0.3, 0.9, 1.4])
+    kT = 1e-8
+    nel = 2 * (2 + 1.37)                 # 2 levels full, 1.37 electrons per spin in the triple
+    f, ef, eb, ts, e0 = R.fermi_filling(eig, nel, kT)
+    with np.errstate(over="ignore"):
+        raw = 1.0 / (1.0 + np.exp((eig - ef) / kT))
+    count = math.fsum(raw)
+    assert abs(count - nel / 2) > 1e-12          # the branch is exercised
+    assert abs(f.sum() - nel) < 1e-13
+    np.testing.assert_allclose(f, 2 * raw * (nel / 2) / count, rtol=1e-14, atol=1e-300)
+    # the triple is split evenly; the rescale also scales the full levels, so the triple holds
+    # 1.37 electrons per spin only to the bisection's count error (< elecTolMax = 1e-7)
+    assert f[2] == f[3] == f[4]
+    np.testing.assert_allclose(f[2:5], 2 * 1.37 / 3, rtol=1e-7)
+    # E_F from the Fermi function of the triple: x = 1.37 / 3 = 1 / (1 + exp((0.3 - E_F) / kT))
+    x = 1.37 / 3
+    assert abs(ef - (0.3 - kT * math.log(1.0 / x - 1.0))) < 1e-14
+
+
+def test_fermi_zero_electrons():
+    """No electrons (etemp.F90:113-122): zero fillings and energies, E_F 1000 (kT + eps_single)
+    below the lowest level."""
+    eig = np.array([-0.3, 0.1, 0.5])
+    kT = 0.01
+    f, ef, eb, ts, e0 = R.fermi_filling(eig, 0.0, kT)
+    assert np.all(f == 0.0) and eb == 0.0 and ts == 0.0 and e0 == 0.0
+    assert ef == -0.3 - 1000.0 * (kT + float(np.finfo(np.float32).eps))
+    fk, efk, ebk, tsk, e0k = R.fermi_filling_k(np.array([eig, eig + 0.05]), 0.0, kT,
+                                               np.array([0.5, 0.5]))
+    assert np.all(fk == 0.0) and efk == ef and ebk == 0.0
+
+
+# ---------------------------------------------------------------- spline repulsive
+def test_spline_repulsive_matches_its_source_function():
+    """The spline repulsive (splinerep.F90:136-190, evaluated by skf.SplineRep.value) reproduces
+    the function synth._spline_rep interpolates, f(r) = A exp(-B r) (1 - r/rc)^3: value and
+    slope at every knot (cubic Hermite segments), the Hermite error bound max|f^(4)| h^4 / 384
+    at mid-segments, the exponential head continuous in value and slope at the first knot, and
+    the quintic last segment vanishing at rc."""
+    seed = 7
+    rep = synth._spline_rep(np.random.default_rng(seed))
+    rng = np.random.default_rng(seed)
+    A, B, rc = rng.uniform(2.0, 4.0), rng.uniform(1.2, 1.8), synth.REP_CUTOFF
+
+    def f(r):
+        return A * math.exp(-B * r) * (1 - r / rc) ** 3
+
+    def fp(r, h=1e-5):
+        return (f(r + h) - f(r - h)) / (2 * h)
+
+    xs = rep.xstart
+    assert xs[0] == 1.0 and rep.cutoff == rc
+    for x in xs:
+        e, d = rep.value(x)
+        assert abs(e - f(x)) < 1e-13 * abs(f(x)) + 1e-15
+        assert abs(d - fp(x)) < 1e-8 * abs(fp(x)) + 1e-10
+    for x0, x1 in zip(xs[:-1], xs[1:]):
+        h = x1 - x0
+        xm = 0.5 * (x0 + x1)
+        # fourth derivative on the segment by a five-point difference, with a safety factor 4
+        hh = 0.05
+        d4 = max(abs((f(t - 2 * hh) - 4 * f(t - hh) + 6 * f(t) - 4 * f(t + hh) + f(t + 2 * hh))
+                     / hh ** 4) for t in np.linspace(x0, x1, 5))
+        bound = 4 * d4 * h ** 4 / 384 + 1e-14
+        assert abs(rep.value(xm)[0] - f(xm)) < bound, (xm, rep.value(xm)[0], f(xm), bound)
+        # a segment that drops its cubic term misses the function by far more than the bound
+        c = rep.spcoeffs[int(np.searchsorted(xs, xm, side="right")) - 1]
+        dr = xm - x0
+        assert abs(c[0] + c[1] * dr + c[2] * dr ** 2 - f(xm)) > bound
+    # exponential head below the first knot: continuous value and slope at r = 1
+    e_lo, d_lo = rep.value(1.0 - 1e-12)
+    assert abs(e_lo - f(1.0)) < 1e-10 and abs(d_lo - fp(1.0)) < 1e-7
+    # last segment: zero at the cutoff, and nothing beyond
+    e_end, _ = rep.value(rc - 1e-9)
+    assert abs(e_end) < 1e-12
+    assert rep.value(rc) == (0.0, 0.0) and rep.value(rc + 1.0) == (0.0, 0.0)
+    # the evaluated slope is the derivative of the evaluated energy everywhere
+    for r in np.linspace(0.6, rc - 0.01, 41):
+        e1, d1 = rep.value(r)
+        fd = (rep.value(r + 1e-6)[0] - rep.value(r - 1e-6)[0]) / 2e-6
+        assert abs(d1 - fd) < 1e-6 * max(1.0, abs(d1))
+
+
 # ---------------------------------------------------------------- Broyden mixer
 def test_broyden_golden_sequence():
-    """Reference regression sequence test/src/dftbp/unit/mixer/broyden.F90:24-58."""
+    """Reference regression sequence test/src/dftbp/unit/mixer/broyden.F90:24-58, to the
+    reference's own tolerance of 100 eps."""
     fx = json.load(open(os.path.join(GOLDEN, "broyden_h2o.json")))
     ch = np.array(fx["charges"])
     df = np.array(fx["differences"])
@@ -220,7 +366,7 @@ def test_broyden_golden_sequence():
     q = ch[0].copy()
     for i in range(4):
         q = mix.mix(q, df[i])
-        assert np.max(np.abs(q - ch[i + 1])) < 100 * np.finfo(float).eps * 10
+        assert np.max(np.abs(q - ch[i + 1])) < 100 * np.finfo(float).eps
 
 
 # ---------------------------------------------------------------- SCC energy and forces
