@@ -102,6 +102,9 @@ EXPORTS = {
     "dftbhip_kernel_stats": [_dp],
     "dftbhip_dm_real": [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int],
     "dftbhip_edm_real": [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int],
+    "dftbhip_dm_cmplx": [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int],
+    "dftbhip_hegvd": [C.c_int, C.c_char, C.c_char, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp],
+    "dftbhip_edm_cmplx": [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int],
     "dftbhip_sk_create": [C.POINTER(C.c_void_p), C.POINTER(Params)],
     "dftbhip_sk_destroy": [C.c_void_p],
     "dftbhip_build_h0_s": [C.c_void_p, C.c_int, C.c_int, _dp, _ip, _ip, C.c_int, _ip, _ip, _ip,
@@ -257,6 +260,21 @@ def sygvd(h: np.ndarray, s: np.ndarray, jobz: str = "V"):
         raise DftbHipError(f"sygvd info={rc}")
     _check(rc, "dftbhip_sygvd")
     return w, a, b
+
+
+def hegvd(h: np.ndarray, s: np.ndarray, jobz: str = "V"):
+    """Complex Hermitian generalized eigenproblem (lower triangles read): eigenvalues, the
+    S-orthonormal eigenvectors (jobz 'V') and the Cholesky factor L (lower triangle)."""
+    n = h.shape[0]
+    a = np.asfortranarray(h, dtype=np.complex128).copy(order="F")
+    b = np.asfortranarray(s, dtype=np.complex128).copy(order="F")
+    w = np.zeros(n)
+    rc = lib().dftbhip_hegvd(1, jobz.encode(), b"L", n, a.ctypes.data_as(_dp), n,
+                             b.ctypes.data_as(_dp), n, _d(w))
+    if rc > 0:
+        raise DftbHipError(f"hegvd: info={rc}")
+    _check(rc, "dftbhip_hegvd")
+    return w, (a if jobz.upper() == "V" else None), np.tril(b)
 
 
 def dsytrd(a: np.ndarray):
@@ -416,6 +434,24 @@ def edm_real(evec, filling, eigen, nlevels=None) -> np.ndarray:
     e = _f64(eigen)
     dm = np.zeros((n, n), order="F")
     _check(lib().dftbhip_edm_real(n, nl, _d(ev), n, _d(f), _d(e), _d(dm), n), "dftbhip_edm_real")
+    return dm
+
+
+def dm_cmplx(evec: np.ndarray, filling: np.ndarray, nlevels: Optional[int] = None,
+             eigen: Optional[np.ndarray] = None) -> np.ndarray:
+    """Lower triangle of C diag(f) C^H (with eigen: C diag(f eigen) C^H) for complex C."""
+    n = evec.shape[0]
+    nl = evec.shape[1] if nlevels is None else nlevels
+    ev = np.asfortranarray(evec, dtype=np.complex128)
+    f = _f64(filling)
+    dm = np.zeros((n, n), dtype=np.complex128, order="F")
+    evp = ev.ctypes.data_as(_dp)
+    dmp = dm.ctypes.data_as(_dp)
+    if eigen is None:
+        _check(lib().dftbhip_dm_cmplx(n, nl, evp, n, _d(f), dmp, n), "dftbhip_dm_cmplx")
+    else:
+        e = _f64(eigen)
+        _check(lib().dftbhip_edm_cmplx(n, nl, evp, n, _d(f), _d(e), dmp, n), "dftbhip_edm_cmplx")
     return dm
 
 

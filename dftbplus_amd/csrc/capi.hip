@@ -25,6 +25,7 @@
 #include "neigh.h"
 #include "eigen.h"
 #include "sparse.h"
+#include "zhegv.h"
 
 namespace dftbhip {
 
@@ -40,11 +41,14 @@ struct Runtime {
   int device = -1;
   hipStream_t stream = nullptr;
   EigenSolver eig;
-  DevBuf<double> stageA, stageB, stageW;
+  ZhegvSolver zeig;
+  DevBuf<double> stageA, stageB, stageC, stageW;
   void release() {
     eig.release();
+    zeig.release();
     stageA.release();
     stageB.release();
+    stageC.release();
     stageW.release();
   }
 };
@@ -340,6 +344,109 @@ static int dm_common(int n, int nlev, const double* evec, int lde, const double*
                           n, hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
   return 0;
+}
+
+extern "C" int dftbhip_hegvd(int itype, char jobz, char uplo, int n, double* a, int lda,
+                             double* b, int ldb, double* w) {
+  DH_CHECK(itype == 1, "hegvd: only itype = 1");
+  DH_CHECK(uplo == 'L' || uplo == 'l', "hegvd: only uplo = 'L'");
+  DH_CHECK(jobz == 'V' || jobz == 'v' || jobz == 'N' || jobz == 'n', "hegvd: bad jobz");
+  DH_CHECK(n >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), "hegvd: bad dimensions");
+  DH_CHECK(a && b && w, "hegvd: null pointer");
+  if (n == 0) return 0;
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  ZhegvSolver& Z = g_rt.zeig;
+  DH_TRY(Z.ensure(n));
+  const size_t nn = (size_t)n * n;
+  DevBuf<double>& A = g_rt.stageA;
+  DevBuf<double>& B = g_rt.stageB;
+  DH_TRY(A.alloc(2 * nn));
+  DH_TRY(B.alloc(2 * nn));
+  DH_HIP(hipMemcpy2DAsync(A.p, 2 * n * sizeof(double), a, 2 * (size_t)lda * sizeof(double),
+                          2 * n * sizeof(double), n, hipMemcpyHostToDevice, s));
+  DH_HIP(hipMemcpy2DAsync(B.p, 2 * n * sizeof(double), b, 2 * (size_t)ldb * sizeof(double),
+                          2 * n * sizeof(double), n, hipMemcpyHostToDevice, s));
+  const bool vec = jobz == 'V' || jobz == 'v';
+  std::vector<double> wh;
+  int info = 0;
+  DH_TRY(Z.solve(s, A.p, n, B.p, n, vec, wh, &info));
+  if (info > n) {
+    set_error("hegvd: overlap matrix not positive definite");
+    return info;
+  }
+  if (info != 0) {
+    set_error("hegvd: divide and conquer failed");
+    return info;
+  }
+  std::copy(wh.begin(), wh.end(), w);
+  if (vec) {
+    DH_TRY(merge_complex(s, Z.Hr.p, Z.Hi.p, n, A.p, n, n, n, 0));
+    DH_HIP(hipMemcpy2DAsync(a, 2 * (size_t)lda * sizeof(double), A.p, 2 * n * sizeof(double),
+                            2 * n * sizeof(double), n, hipMemcpyDeviceToHost, s));
+  }
+  // the Cholesky factor into the lower triangle of b, its strict upper part as given
+  DH_TRY(merge_complex(s, Z.Lr.p, Z.Li.p, n, B.p, n, n, n, 2));
+  DH_HIP(hipMemcpy2DAsync(b, 2 * (size_t)ldb * sizeof(double), B.p, 2 * n * sizeof(double),
+                          2 * n * sizeof(double), n, hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// P = C diag(w) C^H of complex C (interleaved, n x nlev, lde complex elements) through the
+// planes: Re P = Cr W Cr^T + Ci W Ci^T, Im P = Ci W Cr^T - Cr W Ci^T (lower-output GEMMs with
+// the weights applied in the loads); the lower triangle of P goes to dm (interleaved, lddm),
+// zeros above it.
+static int dm_cmplx_common(int n, int nlev, const double* evec, int lde, const double* wts,
+                           double* dm, int lddm) {
+  DH_CHECK(n >= 0 && nlev >= 0 && nlev <= n && lde >= std::max(1, n) && lddm >= std::max(1, n),
+           "complex density matrix: bad dimensions");
+  if (n == 0) return 0;
+  DH_TRY(ensure_runtime());
+  hipStream_t s = g_rt.stream;
+  const size_t nn = (size_t)n * n, nl = (size_t)n * std::max(nlev, 1);
+  DevBuf<double>& Z = g_rt.stageA;    // interleaved C in, interleaved P out
+  DevBuf<double>& C = g_rt.stageB;    // Cr | Ci
+  DevBuf<double>& P = g_rt.stageC;    // Pr | Pi
+  DevBuf<double>& f = g_rt.stageW;
+  DH_TRY(Z.alloc(2 * nn));
+  DH_TRY(C.alloc(2 * nl));
+  DH_TRY(P.alloc(2 * nn));
+  DH_TRY(f.upload(wts, std::max(nlev, 1), s));
+  double *Cr = C.p, *Ci = C.p + nl, *Pr = P.p, *Pi = P.p + nn;
+  if (nlev > 0) {
+    DH_HIP(hipMemcpy2DAsync(Z.p, 2 * n * sizeof(double), evec, 2 * (size_t)lde * sizeof(double),
+                            2 * n * sizeof(double), nlev, hipMemcpyHostToDevice, s));
+    DH_TRY(split_complex(s, Z.p, n, Cr, Ci, n, n, nlev));
+  }
+  DH_TRY(gemm<double>(s, false, true, n, n, nlev, 1.0, Cr, n, Cr, n, 0.0, Pr, n, true, kKFull,
+                      f.p));
+  DH_TRY(gemm<double>(s, false, true, n, n, nlev, 1.0, Ci, n, Ci, n, 1.0, Pr, n, true, kKFull,
+                      f.p));
+  DH_TRY(gemm<double>(s, false, true, n, n, nlev, 1.0, Ci, n, Cr, n, 0.0, Pi, n, true, kKFull,
+                      f.p));
+  DH_TRY(gemm<double>(s, false, true, n, n, nlev, -1.0, Cr, n, Ci, n, 1.0, Pi, n, true, kKFull,
+                      f.p));
+  DH_TRY(merge_complex(s, Pr, Pi, n, Z.p, n, n, n, 1));
+  DH_HIP(hipMemcpy2DAsync(dm, 2 * (size_t)lddm * sizeof(double), Z.p, 2 * n * sizeof(double),
+                          2 * n * sizeof(double), n, hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int dftbhip_dm_cmplx(int n, int nlevels, const double* evec, int lde,
+                                const double* filling, double* dm, int lddm) {
+  DH_CHECK(evec && filling && dm, "dm_cmplx: null pointer");
+  return dm_cmplx_common(n, nlevels, evec, lde, filling, dm, lddm);
+}
+
+extern "C" int dftbhip_edm_cmplx(int n, int nlevels, const double* evec, int lde,
+                                 const double* filling, const double* eigen, double* edm,
+                                 int lddm) {
+  DH_CHECK(evec && filling && eigen && edm, "edm_cmplx: null pointer");
+  std::vector<double> w(std::max(nlevels, 1));
+  for (int i = 0; i < nlevels; ++i) w[i] = filling[i] * eigen[i];
+  return dm_cmplx_common(n, nlevels, evec, lde, w.data(), edm, lddm);
 }
 
 extern "C" int dftbhip_dm_real(int n, int nlevels, const double* evec, int lde,

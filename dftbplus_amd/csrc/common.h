@@ -177,6 +177,13 @@ template <typename T>
 int scale_cols(hipStream_t s, const T* src, long lds, const T* w, T* dst, long ldd, int m, int n);
 template <typename Tin, typename Tout>
 int convert2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n);
+// interleaved complex (re, im) matrix z (leading dimension ldz in complex elements) to the planes
+// re, im (ld) and back; merge mode 0: every element, 1: the lower triangle and zeros above it,
+// 2: the lower triangle only (the strict upper part of z untouched)
+int split_complex(hipStream_t s, const double* z, long ldz, double* re, double* im, long ld,
+                  int m, int n);
+int merge_complex(hipStream_t s, const double* re, const double* im, long ld, double* z, long ldz,
+                  int m, int n, int mode);
 int symmetrize_lower(hipStream_t s, double* a, long lda, int n);
 int symmetrize_lower(hipStream_t s, float* a, long lda, int n);
 int zero_strict_upper(hipStream_t s, double* a, long lda, int n);

@@ -273,6 +273,50 @@ int trtri_from_diag(hipStream_t s, const double* l, long ldl, double* linv, long
   return 0;
 }
 
+// ------------------------------------------------------------------------------------------
+// Interleaved complex (re, im pairs, Fortran complex(dp)) <-> split planes
+// ------------------------------------------------------------------------------------------
+__global__ void split_complex_kernel(const double* z, long ldz, double* re, double* im, long ld,
+                                     int m, int n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  const int i = (int)(e % m), j = (int)(e / m);
+  const double* src = z + 2 * (i + (long)j * ldz);
+  re[i + (long)j * ld] = src[0];
+  im[i + (long)j * ld] = src[1];
+}
+
+// mode 0: every element; 1: the lower triangle, zeros above it; 2: the lower triangle only
+__global__ void merge_complex_kernel(const double* re, const double* im, long ld, double* z,
+                                     long ldz, int m, int n, int mode) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  const int i = (int)(e % m), j = (int)(e / m);
+  double* dst = z + 2 * (i + (long)j * ldz);
+  const bool keep = mode == 0 || i >= j;
+  if (!keep && mode == 2) return;
+  dst[0] = keep ? re[i + (long)j * ld] : 0.0;
+  dst[1] = keep ? im[i + (long)j * ld] : 0.0;
+}
+
+int split_complex(hipStream_t s, const double* z, long ldz, double* re, double* im, long ld,
+                  int m, int n) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(split_complex_kernel, dim3(cdiv((long)m * n, 256)), dim3(256), 0, s, z, ldz,
+                     re, im, ld, m, n);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int merge_complex(hipStream_t s, const double* re, const double* im, long ld, double* z, long ldz,
+                  int m, int n, int mode) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(merge_complex_kernel, dim3(cdiv((long)m * n, 256)), dim3(256), 0, s, re, im,
+                     ld, z, ldz, m, n, mode);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
 template int fill<double>(hipStream_t, double*, size_t, double);
 template int fill<float>(hipStream_t, float*, size_t, float);
 template int fill<int>(hipStream_t, int*, size_t, int);

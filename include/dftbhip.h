@@ -28,6 +28,15 @@ int dftbhip_last_error(char* buf, int len);
 int dftbhip_sygvd(int itype, char jobz, char uplo, int n, double* a, int lda, double* b, int ldb,
                   double* w);
 
+/* ---- complex Hermitian twin: replaces magmaHegvd (dcomplex) / zhegvd
+ *      (src/dftbp/math/eigensolver.F90:113-117, 1143-1245). a and b are complex(dp) arrays
+ *      (interleaved re, im; lda, ldb in complex elements), lower triangles read. On exit a holds
+ *      the S-orthonormal eigenvectors (C^H S C = I, jobz = 'V'), the lower triangle of b the
+ *      Cholesky factor L (S = L L^H; its strict upper part is left as given), w the ascending
+ *      eigenvalues. info > n: leading minor info-n of b is not positive definite. */
+int dftbhip_hegvd(int itype, char jobz, char uplo, int n, double* a, int lda, double* b, int ldb,
+                  double* w);
+
 /* ---- BLAS / LAPACK style drop-ins (the dgemm and dstedc DFTB+ reaches through
  *      src/dftbp/math/blasroutines.F90 and the divide-and-conquer solvers of
  *      src/dftbp/math/eigensolver.F90:561-660). dstedc: on exit d holds the ascending
@@ -82,6 +91,16 @@ int dftbhip_dm_real(int n, int nlevels, const double* evec, int lde, const doubl
                     double* dm, int lddm);
 int dftbhip_edm_real(int n, int nlevels, const double* evec, int lde, const double* filling,
                      const double* eigen, double* edm, int lddm);
+/* ---- complex twins: replace makeDensityMtxCmplxGPU / makeEnergyDensityMtxCmplxGPU
+ *      (src/dftbp/dftb/densitymatrix.F90:1401-1539, 1545-1691). evec and dm/edm are complex(dp)
+ *      arrays, i.e. interleaved (re, im) doubles with leading dimensions lde, lddm counted in
+ *      complex elements. The lower triangle of dm = C diag(f) C^H (edm: weights f*eig) over
+ *      the first nlevels columns is set, the strict upper triangle is zeroed; evec is not
+ *      modified. */
+int dftbhip_dm_cmplx(int n, int nlevels, const double* evec, int lde, const double* filling,
+                     double* dm, int lddm);
+int dftbhip_edm_cmplx(int n, int nlevels, const double* evec, int lde, const double* filling,
+                      const double* eigen, double* edm, int lddm);
 
 /* ---- Slater-Koster parameters and repulsives (the content of the .skf files after
  *      getFullTable, src/dftbp/dftbplus/parser.F90:3668-4130). Species have at most two

@@ -99,6 +99,63 @@ def test_sygvd_not_positive_definite():
         api.sygvd(np.eye(4), s)
 
 
+def _random_hpd(rng, n):
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return x @ x.conj().T / n + np.eye(n)
+
+
+def _check_hegvd(api, h, s, tol=1e-11):
+    """Eigenvalues against scipy (LAPACK zhegvd), residual, S-orthonormality, the factor."""
+    n = h.shape[0]
+    w, c, l = api.hegvd(np.tril(h), np.tril(s))
+    ref = scipy.linalg.eigh(h, s, eigvals_only=True)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.max(np.abs(w - ref)) < tol * scale
+    assert np.max(np.abs(h @ c - s @ c * w)) < 10 * tol * scale
+    assert np.max(np.abs(c.conj().T @ s @ c - np.eye(n))) < tol
+    assert np.max(np.abs(l @ l.conj().T - s)) < 1e-12 * np.abs(s).max()
+    assert np.all(np.diag(l).real > 0) and np.max(np.abs(np.diag(l).imag)) == 0.0
+    return w, c
+
+
+@pytest.mark.parametrize("n", [1, 2, 65, 200, 257])
+def test_hegvd_random(n):
+    """magmaHegvd (dcomplex) twin: random Hermitian H, Hermitian positive definite S."""
+    api = _api()
+    rng = np.random.default_rng(11 * n)
+    s = _random_hpd(rng, n)
+    h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    h = h + h.conj().T
+    _check_hegvd(api, h, s)
+
+
+def test_hegvd_degenerate_levels():
+    """Levels of multiplicity 1, 2 and 3 (H = X^-H D X^-1, S = X^-H X^-1 for a random complex X,
+    so H X = S X D): every degenerate cluster still gives S-orthonormal eigenvectors that span
+    it."""
+    api = _api()
+    rng = np.random.default_rng(5)
+    n = 96
+    d = np.sort(np.concatenate([rng.uniform(-1, 1, n - 12), np.repeat([0.25, -0.4], 3),
+                                np.repeat([0.6, 0.7, -0.1], 2)]))
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 3 * np.eye(n)
+    xi = np.linalg.inv(x)
+    h = xi.conj().T @ np.diag(d) @ xi
+    h = 0.5 * (h + h.conj().T)
+    s = xi.conj().T @ xi
+    s = 0.5 * (s + s.conj().T)
+    w, c = _check_hegvd(api, h, s, tol=1e-10)
+    np.testing.assert_allclose(w, d, atol=1e-10)
+
+
+def test_hegvd_not_positive_definite():
+    api = _api()
+    s = np.eye(4, dtype=complex)
+    s[2, 2] = -1.0
+    with pytest.raises(api.DftbHipError, match="info=7"):
+        api.hegvd(np.eye(4, dtype=complex), s)
+
+
 def test_density_matrices():
     api = _api()
     rng = np.random.default_rng(3)
@@ -113,6 +170,30 @@ def test_density_matrices():
     lo = np.tril_indices(n)
     assert np.allclose(dm[lo], ref[lo], rtol=1e-12, atol=1e-11)
     assert np.allclose(edm[lo], refe[lo], rtol=1e-12, atol=1e-11)
+
+
+@pytest.mark.parametrize("n,nl", [(200, 120), (257, 257), (65, 1)])
+def test_complex_density_matrices(n, nl):
+    """makeDensityMtxCmplxGPU / makeEnergyDensityMtxCmplxGPU twins: lower triangle of
+    C diag(f) C^H and C diag(f eps) C^H (negative weights included) for complex C, zeros above
+    the diagonal; against numpy in complex128."""
+    api = _api()
+    rng = np.random.default_rng(7)
+    c = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    f = np.sort(rng.uniform(0, 2, n))[::-1]
+    eig = rng.uniform(-1, 1, n)
+    dm = api.dm_cmplx(c, f, nl)
+    edm = api.dm_cmplx(c, f, nl, eigen=eig)
+    ref = (c[:, :nl] * f[:nl]) @ c[:, :nl].conj().T
+    refe = (c[:, :nl] * (f[:nl] * eig[:nl])) @ c[:, :nl].conj().T
+    lo = np.tril_indices(n)
+    up = np.triu_indices(n, 1)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.max(np.abs(dm[lo] - ref[lo])) < 1e-12 * scale
+    assert np.max(np.abs(edm[lo] - refe[lo])) < 1e-12 * scale
+    assert np.all(dm[up] == 0) and np.all(edm[up] == 0)
+    # Hermitian: the diagonal is real to rounding
+    assert np.max(np.abs(np.diag(dm).imag)) < 1e-12 * scale
 
 
 @pytest.mark.parametrize("system", ["cluster216", "diamond2"])
