@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libdftbhip.so")
 SOURCES = ["gemm.hip", "dense.hip", "sytrd.hip", "band.hip", "stedc.hip", "eigen.hip", "sparse.hip",
-           "batch.hip", "neigh.hip", "zhegv.hip", "capi.hip"]
+           "batch.hip", "batch_scc.hip", "neigh.hip", "zhegv.hip", "capi.hip"]
 ARCH = os.environ.get("DFTBHIP_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -52,4 +52,30 @@ int launch_batch_gemv(hipStream_t s, const BatchLayout& L, int nat, const double
 int launch_batch_invr_grad(hipStream_t s, const BatchLayout& L, int nat, const double* coords,
                            const double* dq, double* grad);
 
+// Device state of a molecule batch's SCC cycle (batch_scc.hip): per atom (nat) and per molecule
+// (nmol) arrays, the Broyden histories (maxHist steps: ww, cc per molecule; aa, B maxHist^2 per
+// molecule; dF, uu one vector of all atoms per step) and four counters: iteration, molecules
+// still iterating, failure code (1 eigensolver, 2 Fermi level, 3 mixer history) and the lowest
+// failing molecule.
+struct BatchSccDev {
+  int nmol, nat, maxHist;
+  const int* atom0;
+  const long* orb0;
+  const double* q0;
+  const double* nel;
+  double *qin, *qout, *dq, *tmp;
+  int *conv, *active, *iters, *nocc, *mixIt, *counters;
+  double *err, *ef, *eband, *ts, *e0, *eNonScc, *eScc;
+  double *ww, *aa, *B, *cc, *dF, *uu, *qlast, *dlast;
+  double kT, sccTol, alpha, omega0, minw, maxw, wfac;
+  int minSccIter, maxSccIter;
+};
+int launch_bscc_reset(hipStream_t s, const BatchSccDev& d);
+int launch_bscc_begin(hipStream_t s, const BatchSccDev& d);
+int launch_bscc_info(hipStream_t s, const BatchSccDev& d, const int* info);
+int launch_bscc_fermi(hipStream_t s, const BatchSccDev& d, const double* w, double* fill);
+int launch_bscc_outputs(hipStream_t s, const BatchSccDev& d, const long* atomStart,
+                        const int* atomMol, const double* qorb, const double* eorb);
+int launch_bscc_mix(hipStream_t s, const BatchSccDev& d, const double* vout);
+
 }  // namespace dftbhip

@@ -2097,7 +2097,25 @@ struct BatchContext {
   static constexpr int kSx = kBatchClasses - 1;
   hipStream_t sx[kSx] = {};
   hipEvent_t fork = nullptr, join[kSx] = {};
+  // SCC iteration with the Fermi levels, energies, convergence and Broyden mixing on the device
+  // (sccMode 1: launched one by one; 2: the warm iterations replay one captured HIP graph) or
+  // on host threads (sccMode 0)
+  int sccMode = 2;
+  DevBuf<double> q0D, qinD, qoutD, tmpD, nelD, errD, efD, ebD, tsD, e0D, enD, esD;
+  DevBuf<double> wwD, ccD, aaD, BD, dFD, uuD, qlastD, dlastD;
+  DevBuf<int> convD, itersD, mixItD, cntD;
+  BatchSccDev dev{};
+  PinnedHost pCnt;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
   ~BatchContext() {
+    drop_graph();
     for (hipStream_t x : sx)
       if (x) (void)hipStreamDestroy(x);
     for (hipEvent_t e : join)
@@ -2109,6 +2127,7 @@ struct BatchContext {
 static int batch_geometry(BatchContext& b, int nmol, const int* natMol, const double* coords,
                           const int* species) {
   hipStream_t s = b.s;
+  b.drop_graph();
   b.nmol = nmol;
   b.atom0.assign(nmol + 1, 0);
   for (int m = 0; m < nmol; ++m) {
@@ -2253,9 +2272,20 @@ static int batch_potential(BatchContext& b, const std::vector<double>& dqa,
 // converged keep their input charges, so their state stays that of their last iteration.
 // The SCC cycle of every molecule (main.F90:1429-1563 per molecule); molecules that have
 // converged keep their input charges, so their state stays that of their last iteration.
+static int batch_dev_begin(BatchContext& b);
+static int batch_dev_step(BatchContext& b);
+static int g_batchSccMode();
+
 static int batch_scc_begin(BatchContext& b) {
   const dftbhip_options& o = b.opt;
   const int nmol = b.nmol, nat = b.nat, n = b.norbTot;
+  b.sccMode = g_batchSccMode();
+  if (b.sccMode > 0) {
+    b.it = 0;
+    b.nactive = nmol;
+    b.sccDone = false;
+    return batch_dev_begin(b);
+  }
   b.qin = b.q0atom;
   b.mixers.assign(nmol, Broyden());
   for (int m = 0; m < nmol; ++m)
@@ -2309,6 +2339,7 @@ static int parallel_for(int n, int minPar, const std::function<int(int)>& f) {
 }
 
 static int batch_scc_step(BatchContext& b) {
+  if (b.sccMode > 0) return batch_dev_step(b);
   hipStream_t s = b.s;
   const dftbhip_options& o = b.opt;
   const int nmol = b.nmol, nat = b.nat, n = b.norbTot;
@@ -2418,9 +2449,209 @@ static int batch_scc_step(BatchContext& b) {
   return 0;
 }
 
+// ---- the SCC cycle on the device ---------------------------------------------------------
+// molecules selected with set_tuning parameter 4: 0 host threads, 1 device kernels, 2 device
+// kernels with the warm iterations replayed from one captured graph (the default)
+static int g_batchScc = 2;
+static int g_batchSccMode() { return g_batchScc; }
+
+static int batch_dev_begin(BatchContext& b) {
+  hipStream_t s = b.s;
+  const dftbhip_options& o = b.opt;
+  const int nmol = b.nmol, nat = b.nat;
+  const int H = std::max(o.max_scc_iter, 1);
+  b.drop_graph();
+  DH_TRY(b.q0D.upload(b.q0atom.data(), nat, s));
+  DH_TRY(b.nelD.upload(b.nel.data(), nmol, s));
+  for (DevBuf<double>* x : {&b.qinD, &b.qoutD, &b.tmpD, &b.qlastD, &b.dlastD}) DH_TRY(x->alloc(nat));
+  for (DevBuf<double>* x : {&b.errD, &b.efD, &b.ebD, &b.tsD, &b.e0D, &b.enD, &b.esD})
+    DH_TRY(x->alloc(nmol));
+  DH_TRY(b.wwD.alloc((size_t)nmol * H));
+  DH_TRY(b.ccD.alloc((size_t)nmol * H));
+  DH_TRY(b.aaD.alloc((size_t)nmol * H * H));
+  DH_TRY(b.BD.alloc((size_t)nmol * H * H));
+  DH_TRY(b.dFD.alloc((size_t)H * nat));
+  DH_TRY(b.uuD.alloc((size_t)H * nat));
+  for (DevBuf<int>* x : {&b.convD, &b.itersD, &b.mixItD}) DH_TRY(x->alloc(nmol));
+  DH_TRY(b.cntD.alloc(4));
+  DH_TRY(b.pCnt.ensure(2));
+  BatchSccDev& d = b.dev;
+  d.nmol = nmol;
+  d.nat = nat;
+  d.maxHist = H;
+  d.atom0 = b.atom0D.p;
+  d.orb0 = b.orb0D.p;
+  d.q0 = b.q0D.p;
+  d.nel = b.nelD.p;
+  d.qin = b.qinD.p;
+  d.qout = b.qoutD.p;
+  d.dq = b.dq.p;
+  d.tmp = b.tmpD.p;
+  d.conv = b.convD.p;
+  d.active = b.activeD.p;
+  d.iters = b.itersD.p;
+  d.nocc = b.noccD.p;
+  d.mixIt = b.mixItD.p;
+  d.counters = b.cntD.p;
+  d.err = b.errD.p;
+  d.ef = b.efD.p;
+  d.eband = b.ebD.p;
+  d.ts = b.tsD.p;
+  d.e0 = b.e0D.p;
+  d.eNonScc = b.enD.p;
+  d.eScc = b.esD.p;
+  d.ww = b.wwD.p;
+  d.aa = b.aaD.p;
+  d.B = b.BD.p;
+  d.cc = b.ccD.p;
+  d.dF = b.dFD.p;
+  d.uu = b.uuD.p;
+  d.qlast = b.qlastD.p;
+  d.dlast = b.dlastD.p;
+  d.kT = o.kT;
+  d.sccTol = o.scc_tol;
+  d.alpha = o.mix_param;
+  d.omega0 = o.omega0;
+  d.minw = o.min_weight;
+  d.maxw = o.max_weight;
+  d.wfac = o.weight_fac;
+  d.minSccIter = o.min_scc_iter;
+  d.maxSccIter = o.max_scc_iter;
+  DH_TRY(launch_bscc_reset(s, d));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// one SCC iteration of every active molecule as launches on b.s (and the size classes' streams,
+// forked from and joined to it), ending with the copy of the counters to pinned host memory;
+// nothing here waits on the host, so the sequence can be captured
+static int batch_dev_enqueue(BatchContext& b, bool warm) {
+  hipStream_t s = b.s;
+  const BatchSccDev& d = b.dev;
+  const int n = b.norbTot;
+  DH_TRY(launch_bscc_begin(s, d));
+  DH_TRY(launch_batch_gemv(s, b.L, b.nat, b.gamma.p, b.dq.p, b.vatom.p));
+  hipLaunchKernelGGL(expand_kernel, dim3(cdiv(b.nat, 256)), dim3(256), 0, s, b.nat,
+                     b.dblk.atomStart.p, b.vatom.p, b.vorb.p);
+  DH_LAUNCH_CHECK();
+  DH_TRY(launch_shift(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.h0.p, b.ov.p,
+                      b.vorb.p, b.hs.p));
+  DH_TRY(launch_batch_unpack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.hs.p,
+                             b.Hd.p, b.denseSize));
+  DH_HIP(hipEventRecord(b.fork, s));
+  for (int q = 0; q < BatchContext::kSx; ++q) DH_HIP(hipStreamWaitEvent(b.sx[q], b.fork, 0));
+  hipStream_t st[kBatchClasses] = {s};
+  for (int q = 0; q < BatchContext::kSx; ++q) st[q + 1] = b.sx[q];
+  DH_TRY(launch_batch_sygv(st, b.L, b.Sd.p, b.Hd.p, b.work.p, b.w.p, b.Cv.p, b.infoD.p,
+                           b.activeD.p, warm));
+  for (int q = 0; q < BatchContext::kSx; ++q) {
+    DH_HIP(hipEventRecord(b.join[q], b.sx[q]));
+    DH_HIP(hipStreamWaitEvent(s, b.join[q], 0));
+  }
+  DH_TRY(launch_bscc_info(s, d, b.infoD.p));
+  DH_TRY(launch_bscc_fermi(s, d, b.w.p, b.f.p));
+  DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p, b.activeD.p));
+  DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p,
+                           b.rho.p));
+  DH_TRY(launch_mulliken(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.ov.p, b.h0.p,
+                         b.rho.p, b.qorb.p, b.eorb.p, n));
+  DH_TRY(launch_bscc_outputs(s, d, b.dblk.atomStart.p, b.atomMolD.p, b.qorb.p, b.eorb.p));
+  DH_TRY(launch_batch_gemv(s, b.L, b.nat, b.gamma.p, b.dq.p, b.vatom.p));
+  DH_TRY(launch_bscc_mix(s, d, b.vatom.p));
+  DH_HIP(hipMemcpyAsync(b.pCnt.p, b.cntD.p + 1, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+static int batch_dev_step(BatchContext& b) {
+  hipStream_t s = b.s;
+  const dftbhip_options& o = b.opt;
+  DH_CHECK(b.nactive > 0 && b.it < o.max_scc_iter, "batch scc_step: nothing left to iterate");
+  const int it = ++b.it;
+  const bool warm = it > 1;
+  if (b.sccMode == 2 && warm) {
+    if (!b.gexec) {
+      DH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int rc = batch_dev_enqueue(b, true);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(s, &g);
+      if (rc != 0) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      DH_HIP(ec);
+      b.graph = g;
+      DH_HIP(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
+    }
+    DH_HIP(hipGraphLaunch(b.gexec, s));
+  } else {
+    DH_TRY(batch_dev_enqueue(b, warm));
+  }
+  DH_HIP(hipStreamSynchronize(s));
+  const int* c = reinterpret_cast<const int*>(b.pCnt.p);
+  if (c[1] != 0) {
+    const int m = c[2];
+    if (c[1] == 1) {
+      std::vector<int> info(b.nmol);
+      DH_HIP(hipMemcpy(info.data(), b.infoD.p, b.nmol * sizeof(int), hipMemcpyDeviceToHost));
+      const int im = (m >= 0 && m < b.nmol) ? info[m] : 0;
+      set_error(im == kBatchJacobiNoConv
+                    ? "batch: Jacobi eigensolver of molecule " + std::to_string(m) +
+                          " did not converge within its sweep limit"
+                : im < 0 ? "batch: molecule " + std::to_string(m) + " of " +
+                               std::to_string(-im) + " orbitals in a smaller size class"
+                         : "batch: overlap of molecule " + std::to_string(m) +
+                               " not positive definite (minor " + std::to_string(im) + ")");
+    } else if (c[1] == 2) {
+      set_error("batch: Fermi level search did not converge for molecule " + std::to_string(m));
+    } else {
+      set_error("batch: Broyden mixer of molecule " + std::to_string(m) +
+                ": Maximal nr. of steps exceeded");
+    }
+    return kErrState;
+  }
+  b.nactive = c[0];
+  return 0;
+}
+
+// the device state the host-side end of the cycle and the gradients read
+static int batch_dev_fetch(BatchContext& b) {
+  hipStream_t s = b.s;
+  const int nmol = b.nmol, nat = b.nat, n = b.norbTot;
+  std::vector<double> ef(nmol), eb(nmol), ts(nmol), e0(nmol);
+  b.qout.resize(nat);
+  b.eNonScc.resize(nmol);
+  b.eScc.resize(nmol);
+  b.err.resize(nmol);
+  b.iters.resize(nmol);
+  b.conv.resize(nmol);
+  b.nocc.resize(nmol);
+  b.fill.resize(n);
+  b.eigv.resize(n);
+  b.fr.resize(nmol);
+  DH_HIP(hipMemcpyAsync(b.qout.data(), b.qoutD.p, nat * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.eNonScc.data(), b.enD.p, nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.eScc.data(), b.esD.p, nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.err.data(), b.errD.p, nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.iters.data(), b.itersD.p, nmol * sizeof(int), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.conv.data(), b.convD.p, nmol * sizeof(int), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.nocc.data(), b.noccD.p, nmol * sizeof(int), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.fill.data(), b.f.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(b.eigv.data(), b.w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(ef.data(), b.efD.p, nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(eb.data(), b.ebD.p, nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(ts.data(), b.tsD.p, nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(e0.data(), b.e0D.p, nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  for (int m = 0; m < nmol; ++m) b.fr[m] = FillResult{ef[m], eb[m], ts[m], e0[m]};
+  b.dqout.resize(nat);
+  for (int a = 0; a < nat; ++a) b.dqout[a] = b.qout[a] - b.q0atom[a];
+  return 0;
+}
+
 static int batch_scc_end(BatchContext& b, double* energies, double* charges, int* niter,
                          int* converged) {
   DH_CHECK(b.it > 0, "batch scc_end: no SCC iteration was run");
+  if (b.sccMode > 0) DH_TRY(batch_dev_fetch(b));
   hipStream_t s = b.s;
   const int nmol = b.nmol, nat = b.nat;
   // repulsive energies and the final energies per molecule
@@ -2893,6 +3124,10 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
     case 3:
       DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 3 takes 0 or 1");
       dftbhip::g_devNeighbours = value;
+      return 0;
+    case 4:
+      DH_CHECK(value >= 0 && value <= 2, "set_tuning: parameter 4 takes 0, 1 or 2");
+      dftbhip::g_batchScc = value;
       return 0;
     default:
       dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));

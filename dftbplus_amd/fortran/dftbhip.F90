@@ -5,10 +5,12 @@
 !> replace (magmaHegvd in src/dftbp/math/eigensolver.F90:1143-1245, makeDensityMtxRealGPU /
 !> makeEnergyDensityMtxRealGPU in src/dftbp/dftb/densitymatrix.F90:1105-1395).
 module dftbhip
-  use iso_c_binding, only : c_int, c_double, c_char, c_ptr, c_null_char, c_null_ptr, c_loc
+  use iso_c_binding, only : c_int, c_double, c_double_complex, c_char, c_ptr, c_null_char,&
+      & c_null_ptr, c_loc
   implicit none
   private
   public :: dftbhip_gpus_available, dftbhip_hegvd_dreal, dftbhip_dm_dreal, dftbhip_edm_dreal
+  public :: dftbhip_hegvd_dcomplex, dftbhip_dm_dcomplex, dftbhip_edm_dcomplex
   public :: dftbhip_error_message, dftbhip_update_neighbour_list
 
   interface
@@ -45,6 +47,36 @@ module dftbhip
       real(c_double), intent(inout) :: edm(lddm, *)
       integer(c_int) :: rc
     end function c_edm
+
+    function c_hegvd(itype, jobz, uplo, n, a, lda, b, ldb, w) result(rc)&
+        & bind(C, name="dftbhip_hegvd")
+      import :: c_int, c_double, c_double_complex, c_char
+      integer(c_int), value :: itype, n, lda, ldb
+      character(kind=c_char), value :: jobz, uplo
+      complex(c_double_complex), intent(inout) :: a(lda, *), b(ldb, *)
+      real(c_double), intent(out) :: w(*)
+      integer(c_int) :: rc
+    end function c_hegvd
+
+    function c_dm_cmplx(n, nlevels, evec, lde, filling, dm, lddm) result(rc)&
+        & bind(C, name="dftbhip_dm_cmplx")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: n, nlevels, lde, lddm
+      complex(c_double_complex), intent(in) :: evec(lde, *)
+      real(c_double), intent(in) :: filling(*)
+      complex(c_double_complex), intent(inout) :: dm(lddm, *)
+      integer(c_int) :: rc
+    end function c_dm_cmplx
+
+    function c_edm_cmplx(n, nlevels, evec, lde, filling, eigen, edm, lddm) result(rc)&
+        & bind(C, name="dftbhip_edm_cmplx")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: n, nlevels, lde, lddm
+      complex(c_double_complex), intent(in) :: evec(lde, *)
+      real(c_double), intent(in) :: filling(*), eigen(*)
+      complex(c_double_complex), intent(inout) :: edm(lddm, *)
+      integer(c_int) :: rc
+    end function c_edm_cmplx
 
     function c_neigh(nAtom, coords, latVecs, cutoff, nAll, mNeighbour, coordsAll, img2CentCell,&
         & iNeighbour, nNeighbour, neighDist2) result(rc) bind(C, name="dftbhip_neighbour_list")
@@ -88,6 +120,52 @@ contains
     n = size(a, dim=1)
     info = c_sygvd(1_c_int, jobz, "L", n, a, n, b, size(b, dim=1), w)
   end subroutine dftbhip_hegvd_dreal
+
+  !> Generalized Hermitian eigenproblem, itype 1, lower storage (replaces magmaHegvd for
+  !> complex matrices); info as dftbhip_hegvd_dreal.
+  subroutine dftbhip_hegvd_dcomplex(a, b, w, jobz, info)
+    complex(c_double_complex), intent(inout) :: a(:,:), b(:,:)
+    real(c_double), intent(out) :: w(:)
+    character, intent(in) :: jobz
+    integer, intent(out) :: info
+    integer(c_int) :: n
+    n = size(a, dim=1)
+    info = c_hegvd(1_c_int, jobz, "L", n, a, n, b, size(b, dim=1), w)
+  end subroutine dftbhip_hegvd_dcomplex
+
+  !> Complex density matrix from eigenvectors and fillings (lower triangle; replaces
+  !> makeDensityMtxCmplxGPU).
+  subroutine dftbhip_dm_dcomplex(dm, eigenvecs, filling, info)
+    complex(c_double_complex), intent(inout) :: dm(:,:)
+    complex(c_double_complex), intent(in) :: eigenvecs(:,:)
+    real(c_double), intent(in) :: filling(:)
+    integer, intent(out) :: info
+    integer :: nLevels
+    nLevels = size(filling)
+    do while (nLevels > 0)
+      if (filling(nLevels) > epsilon(1.0_c_double)) exit
+      nLevels = nLevels - 1
+    end do
+    info = c_dm_cmplx(int(size(eigenvecs, dim=1), c_int), int(nLevels, c_int), eigenvecs,&
+        & int(size(eigenvecs, dim=1), c_int), filling, dm, int(size(dm, dim=1), c_int))
+  end subroutine dftbhip_dm_dcomplex
+
+  !> Complex energy-weighted density matrix (lower triangle; replaces
+  !> makeEnergyDensityMtxCmplxGPU).
+  subroutine dftbhip_edm_dcomplex(edm, eigenvecs, filling, eigen, info)
+    complex(c_double_complex), intent(inout) :: edm(:,:)
+    complex(c_double_complex), intent(in) :: eigenvecs(:,:)
+    real(c_double), intent(in) :: filling(:), eigen(:)
+    integer, intent(out) :: info
+    integer :: nLevels
+    nLevels = size(filling)
+    do while (nLevels > 0)
+      if (filling(nLevels) > epsilon(1.0_c_double)) exit
+      nLevels = nLevels - 1
+    end do
+    info = c_edm_cmplx(int(size(eigenvecs, dim=1), c_int), int(nLevels, c_int), eigenvecs,&
+        & int(size(eigenvecs, dim=1), c_int), filling, eigen, edm, int(size(edm, dim=1), c_int))
+  end subroutine dftbhip_edm_dcomplex
 
   !> Density matrix from eigenvectors and fillings (lower triangle).
   subroutine dftbhip_dm_dreal(dm, eigenvecs, filling, info)

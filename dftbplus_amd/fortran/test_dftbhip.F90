@@ -1,11 +1,13 @@
-!> Fortran caller of the bind(C) module: generalized eigenproblem of a small symmetric pair and
-!> the density matrix, checked by residuals, and the neighbour list of a periodic cell checked
-!> by its invariants (prints PASS/FAIL; exit status 1 on failure).
+!> Fortran caller of the bind(C) module: generalized eigenproblems of a small symmetric and a
+!> small Hermitian pair checked by residuals, their density matrices checked against the explicit
+!> sums over the filled levels, and the neighbour list of a periodic cell checked by its
+!> invariants (prints PASS/FAIL; exit status 1 on failure).
 program test_dftbhip
   use dftbhip
   implicit none
   integer, parameter :: n = 40
-  real(8) :: h(n,n), s(n,n), h0(n,n), s0(n,n), w(n), f(n), dm(n,n), r
+  real(8) :: h(n,n), s(n,n), h0(n,n), s0(n,n), w(n), f(n), dm(n,n), r, ref
+  complex(8) :: hc(n,n), sc(n,n), hc0(n,n), sc0(n,n), dmc(n,n), refc
   integer :: info, i, j, k, ngpu, nAll
   real(8) :: x0(3,2), lat(3,3), cut
   real(8), allocatable :: xall(:,:), d2(:,:)
@@ -36,6 +38,54 @@ program test_dftbhip
   call dftbhip_dm_dreal(dm, h, f, info)
   if (r > 1.0d-10 .or. info /= 0) then
     print '(a)', "FAIL"
+    error stop 1
+  end if
+  ! lower triangle of sum_k f_k c_k c_k^T
+  r = 0.0d0
+  do j = 1, n
+    do i = j, n
+      ref = sum(f * h(i, :) * h(j, :))
+      r = max(r, abs(dm(i, j) - ref))
+    end do
+  end do
+  print '(a,es10.3)', "density matrix ", r
+  if (r > 1.0d-12) then
+    print '(a)', "FAIL: density matrix"
+    error stop 1
+  end if
+  ! complex Hermitian pair: H with an imaginary part antisymmetric in (i, j), S Hermitian
+  ! positive definite
+  do j = 1, n
+    do i = 1, n
+      hc0(i,j) = cmplx(h0(i,j), 0.05d0 * real(i - j, 8) / real(i + j, 8), 8)
+      sc0(i,j) = cmplx(s0(i,j), merge(0.0d0, 0.02d0 * sign(1.0d0, real(i - j, 8)) /&
+          & real(abs(i - j) + 1, 8)**2, i == j), 8)
+    end do
+  end do
+  hc = hc0
+  sc = sc0
+  call dftbhip_hegvd_dcomplex(hc, sc, w, "V", info)
+  if (info /= 0) then
+    print '(a,i0,1x,a)', "FAIL: complex info = ", info, trim(dftbhip_error_message())
+    error stop 1
+  end if
+  r = maxval(abs(matmul(hc0, hc) - matmul(sc0, hc) * spread(w, 1, n)))
+  print '(a,es10.3)', "complex residual ", r
+  call dftbhip_dm_dcomplex(dmc, hc, f, info)
+  if (r > 1.0d-10 .or. info /= 0) then
+    print '(a)', "FAIL: complex eigenproblem"
+    error stop 1
+  end if
+  r = 0.0d0
+  do j = 1, n
+    do i = j, n
+      refc = sum(f * hc(i, :) * conjg(hc(j, :)))
+      r = max(r, abs(dmc(i, j) - refc))
+    end do
+  end do
+  print '(a,es10.3)', "complex density matrix ", r
+  if (r > 1.0d-12) then
+    print '(a)', "FAIL: complex density matrix"
     error stop 1
   end if
   ! two atoms in a cubic cell of 3 bohr, cutoff 4.5: every atom lists itself first, then

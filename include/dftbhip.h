@@ -55,7 +55,10 @@ int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e, d
  * autotuned tables; param 2: k-points that are all real and of one size are tridiagonalised
  * together, one launch per column kernel for all points (1, the default) or one by one (0);
  * param 3: the neighbour list, pairs and SK blocks of a geometry are built on the device (1,
- * the default) or on the host (0). No counterpart in the reference. */
+ * the default) or on the host (0); param 4: the molecule batch's per-molecule Fermi levels,
+ * energies, convergence tests and Broyden mixing run on host threads (0), as device kernels
+ * (1), or as device kernels with every warm SCC iteration replayed from one captured HIP
+ * graph (2, the default). No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)
  * builds it: atom a lists itself, then the images of atoms b >= a within cutoff by increasing

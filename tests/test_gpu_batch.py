@@ -138,3 +138,72 @@ def test_batch_level_at_zero_converges():
         assert np.max(np.abs(res.charges - b.charges[m])) < 1e-8
         assert np.max(np.abs(og - g[m])) < 1e-7
     b.close()
+
+
+def _batch_scc(mode, params, mols, opts):
+    from dftbplus_amd import api
+    api.set_tuning(4, mode)
+    try:
+        b = api.MoleculeBatch(params, mols, opts)
+        e = b.scc()
+        g = b.gradients()
+        out = (e.copy(), [q.copy() for q in b.charges], g, b.n_iter.copy(), b.converged.copy())
+        b.close()
+    finally:
+        api.set_tuning(4, 2)
+    return out
+
+
+def test_batch_device_scc_equals_host_scc():
+    """Fermi levels, energies, convergence tests and Broyden mixing as device kernels (modes 1
+    and 2) give the results of the host-thread implementation (mode 0) to rounding: the Fermi
+    sums are combined in another order."""
+    from dftbplus_amd import synth
+    mols, params, opts = synth.workload("molecules24")
+    e0, q0, g0, n0, c0 = _batch_scc(0, params, mols, opts)
+    for mode in (1, 2):
+        e1, q1, g1, n1, c1 = _batch_scc(mode, params, mols, opts)
+        assert np.array_equal(c0, c1)
+        assert np.max(np.abs(n1 - n0)) <= 1
+        for m in range(len(mols)):
+            if not c0[m]:
+                continue
+            assert abs(e1[m] - e0[m]) / mols[m].natom < 1e-10, (mode, m)
+            assert np.max(np.abs(q1[m] - q0[m])) < 1e-8, (mode, m)
+            assert np.max(np.abs(g1[m] - g0[m])) < 1e-8, (mode, m)
+
+
+def test_batch_graph_replays_equal_eager_launches():
+    """Every replay of the captured SCC iteration (mode 2) equals the same iteration launched
+    kernel by kernel (mode 1), bit for bit, with the host reading the state between replays."""
+    from dftbplus_amd import api, synth
+    mols, params, opts = synth.workload("molecules24")
+    try:
+        api.set_tuning(4, 1)
+        eager = api.MoleculeBatch(params, mols, opts)
+        eager.scc_begin()
+        api.set_tuning(4, 2)
+        graph = api.MoleculeBatch(params, mols, opts)
+        graph.scc_begin()
+    finally:
+        api.set_tuning(4, 2)
+    steps = 0
+    while True:
+        na_e = eager.scc_step()
+        na_g = graph.scc_step()
+        steps += 1
+        assert na_e == na_g
+        ee = eager.scc_end()
+        eg = graph.scc_end()
+        assert np.array_equal(ee, eg), steps
+        for qa, qb in zip(eager.charges, graph.charges):
+            assert np.array_equal(qa, qb), steps
+        assert np.array_equal(eager.n_iter, graph.n_iter)
+        if na_e == 0 or steps >= opts.max_scc_iter:
+            break
+    assert steps > 5
+    ge, gg = eager.gradients(), graph.gradients()
+    for a, b in zip(ge, gg):
+        assert np.array_equal(a, b)
+    eager.close()
+    graph.close()
