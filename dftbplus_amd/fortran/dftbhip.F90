@@ -11,6 +11,8 @@ module dftbhip
   private
   public :: dftbhip_gpus_available, dftbhip_hegvd_dreal, dftbhip_dm_dreal, dftbhip_edm_dreal
   public :: dftbhip_hegvd_dcomplex, dftbhip_dm_dcomplex, dftbhip_edm_dcomplex
+  public :: dftbhip_efilling, dftbhip_mixer_create, dftbhip_mixer_reset, dftbhip_mixer_mix
+  public :: dftbhip_mixer_destroy
   public :: dftbhip_error_message, dftbhip_update_neighbour_list
 
   interface
@@ -89,6 +91,49 @@ module dftbhip
       type(c_ptr), value :: coordsAll, img2CentCell, iNeighbour, nNeighbour, neighDist2
       integer(c_int) :: rc
     end function c_neigh
+
+    !> Efilling (src/dftbp/dftb/etemp.F90:59-194), Fermi distribution, one spin channel
+    function dftbhip_efilling(nLevels, nKPoints, eigenvals, kWeight, nElectrons, kT, filling,&
+        & Ef, Ebs, TS, E0) result(rc) bind(C, name="dftbhip_efilling")
+      import :: c_int, c_double
+      integer(c_int), value :: nLevels, nKPoints
+      real(c_double), intent(in) :: eigenvals(nLevels, *), kWeight(*)
+      real(c_double), value :: nElectrons, kT
+      real(c_double), intent(out) :: filling(nLevels, *), Ef, Ebs, TS, E0
+      integer(c_int) :: rc
+    end function dftbhip_efilling
+
+    !> TBroydenMixerReal (src/dftbp/mixer/broydenmixer.F90:112-342): init, reset, mix
+    function dftbhip_mixer_create(mixer, maxSccIter, mixParam, omega0, minWeight, maxWeight,&
+        & weightFac) result(rc) bind(C, name="dftbhip_mixer_create")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), intent(out) :: mixer
+      integer(c_int), value :: maxSccIter
+      real(c_double), value :: mixParam, omega0, minWeight, maxWeight, weightFac
+      integer(c_int) :: rc
+    end function dftbhip_mixer_create
+
+    function dftbhip_mixer_reset(mixer, nElem) result(rc) bind(C, name="dftbhip_mixer_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mixer
+      integer(c_int), value :: nElem
+      integer(c_int) :: rc
+    end function dftbhip_mixer_reset
+
+    function dftbhip_mixer_mix(mixer, qInpResult, qDiff) result(rc)&
+        & bind(C, name="dftbhip_mixer_mix")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: mixer
+      real(c_double), intent(inout) :: qInpResult(*)
+      real(c_double), intent(in) :: qDiff(*)
+      integer(c_int) :: rc
+    end function dftbhip_mixer_mix
+
+    function dftbhip_mixer_destroy(mixer) result(rc) bind(C, name="dftbhip_mixer_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mixer
+      integer(c_int) :: rc
+    end function dftbhip_mixer_destroy
 
     function c_last_error(buf, blen) result(rc) bind(C, name="dftbhip_last_error")
       import :: c_int, c_char

@@ -12,6 +12,31 @@ program test_dftbhip
   real(8) :: x0(3,2), lat(3,3), cut
   real(8), allocatable :: xall(:,:), d2(:,:)
   integer, allocatable :: i2c(:), inb(:,:), nnb(:)
+  ! host pieces (no device work): Fermi filling of four levels with two electrons per spin
+  ! channel at kT = 0.01 Ha, and two Broyden steps on a three-element vector
+  block
+    use iso_c_binding, only : c_ptr
+    real(8) :: e4(4), f4(4), kw(1), ef, ebs, ts, e0, q3(3), d3(3)
+    type(c_ptr) :: mixer
+    e4 = [-0.5d0, -0.3d0, 0.2d0, 0.4d0]
+    kw = 1.0d0
+    info = dftbhip_efilling(4, 1, e4, kw, 2.0d0, 0.01d0, f4, ef, ebs, ts, e0)
+    if (info /= 0 .or. abs(sum(f4) - 2.0d0) > 1.0d-12 .or. abs(ef + 0.05d0) > 1.0d-12) then
+      print '(a)', "FAIL: efilling"
+      error stop 1
+    end if
+    info = dftbhip_mixer_create(mixer, 10, 0.2d0, 0.01d0, 1.0d0, 1.0d5, 1.0d-2)
+    info = info + dftbhip_mixer_reset(mixer, 3)
+    q3 = [6.0d0, 1.0d0, 1.0d0]
+    d3 = [0.8d0, -0.4d0, -0.4d0]
+    info = info + dftbhip_mixer_mix(mixer, q3, d3)
+    if (info /= 0 .or. abs(q3(1) - 6.16d0) > 1.0d-14) then
+      print '(a)', "FAIL: mixer"
+      error stop 1
+    end if
+    info = dftbhip_mixer_destroy(mixer)
+    print '(a)', "host pieces ok"
+  end block
   call dftbhip_gpus_available(ngpu)
   print '(a,i0)', "gpus: ", ngpu
   if (ngpu < 1) then
