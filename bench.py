@@ -16,6 +16,9 @@ gradients is the leg's force evaluation. Legs:
   mixed_then_fp64  FP32 MFMA reduction, tridiagonalisation, back-transformation and density
                    matrices while the previous iteration's charge error is above 1e-3, plain
                    FP64 iterations after (the converged point is the FP64 one);
+  mixed_tail       FP32 iterations above 1e-3, FP32 eigenpairs refined by two Ogita-Aishima
+                   steps down to 1e-5, below that no eigensolve: one step refines the
+                   previous iteration's FP64 eigenpairs against the new H;
   mixed_fp32       FP32 iterations throughout (only with --legs; does not reach 1e-8 e).
 `value` is the SCC iterations/s of the fp64 leg's timed steps; every leg reports its timed
 rate, its converged cycle (iterations, seconds, force evaluations/s) and its energy, charge and
@@ -51,7 +54,8 @@ SYMV_TRAFFIC_N2048 = {"n": 2048, "alg_bytes_per_launch": 5611520, "hbm_read_byte
 LEGS = {"fp64": dict(precision=0),
         "mixed_then_fp64": dict(precision=1, refine_below=1e-3, refine_steps=0),
         "mixed_fp32": dict(precision=1, refine_below=0.0),
-        "mixed_then_refined": dict(precision=1, refine_below=1e-3, refine_steps=1)}
+        "mixed_then_refined": dict(precision=1, refine_below=1e-3, refine_steps=1),
+        "mixed_tail": dict(precision=1, refine_below=1e-3, refine_steps=2, tail_below=1e-5)}
 E_BOUND, F_BOUND = 1e-6, 1e-5
 
 
@@ -499,7 +503,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="diamond8")
-    ap.add_argument("--legs", default="fp64,mixed_then_fp64",
+    ap.add_argument("--legs", default="fp64,mixed_then_fp64,mixed_tail",
                     help="comma-separated subset of " + ",".join(LEGS))
     ap.add_argument("--mix", type=float, default=None,
                     help="Broyden mixing parameter (default: the workload's)")

@@ -85,6 +85,7 @@ class COptions(C.Structure):
         ("refine_below", C.c_double),
         ("refine_steps", C.c_int),
         ("tridiag", C.c_int),
+        ("tail_below", C.c_double),
     ]
 
 
@@ -224,7 +225,8 @@ def make_params(params: Parameters):
 
 
 def make_options(opts: Options, force_mode: int = 0, precision: int = 0,
-                 refine_below: float = 1e-4, refine_steps: int = 2, tridiag: int = 1) -> COptions:
+                 refine_below: float = 1e-4, refine_steps: int = 2, tridiag: int = 1,
+                 tail_below: float = 0.0) -> COptions:
     o = COptions()
     o.kT = opts.kT
     o.scc_tol = opts.scc_tol
@@ -242,6 +244,7 @@ def make_options(opts: Options, force_mode: int = 0, precision: int = 0,
     o.refine_below = refine_below
     o.refine_steps = refine_steps
     o.tridiag = tridiag
+    o.tail_below = tail_below
     return o
 
 
@@ -544,11 +547,11 @@ class DftbPlus:
 
     def __init__(self, params: Parameters, geometry: Geometry, options: Options = Options(),
                  force_mode: int = 0, precision: int = 0, refine_below: float = 1e-4,
-                 refine_steps: int = 2, tridiag: int = 1):
+                 refine_steps: int = 2, tridiag: int = 1, tail_below: float = 0.0):
         self.params = params
         self._p, self._keep = make_params(params)
         self._o = make_options(options, force_mode, precision, refine_below, refine_steps,
-                               tridiag)
+                               tridiag, tail_below)
         self.h = C.c_void_p()
         _check(lib().dftbhip_ctx_create(C.byref(self.h), C.byref(self._p), C.byref(self._o)),
                "dftbhip_ctx_create")

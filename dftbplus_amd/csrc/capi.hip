@@ -1360,6 +1360,7 @@ struct Context {
   double energies[8]{};
   int nocc = 0;
   bool xf64 = true;     // the eigenvectors in X are FP64 (else FP32, mixed path)
+  bool xFull = false;   // X holds all n refined FP64 eigenvectors of the last iteration
   bool sccDone = false;
   // SCC cycles start from the input charges the previous cycle ended with (the reference's
   // geometry loop keeps qInput between geometry steps, main.F90:219-357) instead of the
@@ -1613,6 +1614,7 @@ static int ctx_scc_begin(Context& c) {
   // warm start: the input charges of the previous cycle, when it was on the same atoms
   if (!(c.warmStart && st.it > 0 && st.qin.size() == c.q0atom.size())) st.qin = c.q0atom;
   st.mixer.reset(c.nat, c.opt, c.opt.max_scc_iter + 1);
+  c.xFull = false;
   st.it = 0;
   st.conv = false;
   st.err = 0.0;
@@ -1647,6 +1649,10 @@ static int ctx_scc_step(Context& c) {
   const bool below = o.precision == 1 && st.prevErr <= o.refine_below;
   const bool mixed = o.precision == 1 && !(below && o.refine_steps == 0);
   const bool refined = mixed && below;
+  // tail refinement: the previous iteration's n refined eigenpairs are refined against the new
+  // H by one Ogita-Aishima step, without an eigensolve
+  const bool keepAll = o.tail_below > 0.0;
+  const bool tail = refined && keepAll && c.xFull && c.nk == 0 && st.prevErr <= o.tail_below;
   if (c.nk > 0) {
     // k-points: per k H(k) (real n x n at time-reversal-invariant points, else the 2n
     // embedding), eigenpairs and all eigenvectors; then the k-weighted Fermi level and the
@@ -1806,7 +1812,39 @@ static int ctx_scc_step(Context& c) {
     float* X32 = reinterpret_cast<float*>(c.X.p);
     float* P32 = reinterpret_cast<float*>(c.P.p);
     float* H32 = refined ? c.eig.h32() : reinterpret_cast<float*>(c.Hd.p);
-    if (mixed && !refined) {
+    if (tail) {
+      DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p,
+                                   n, n));
+      DH_HIP(hipEventRecord(ev[1], s));
+      DH_HIP(hipEventRecord(ev[2], s));
+      DH_HIP(hipEventRecord(ev[6], s));
+      std::vector<double> lam;
+      DH_TRY(c.eig.refine(s, c.Hd.p, c.Sd.p, c.X.p, 1, n, c.X.p, n, lam));
+      DH_HIP(hipEventRecord(ev[7], s));
+      const double th3 = now();
+      std::vector<int> perm(n);
+      std::iota(perm.begin(), perm.end(), 0);
+      std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return lam[x] < lam[y]; });
+      std::vector<double> ls(n), fs;
+      for (int k = 0; k < n; ++k) ls[k] = lam[perm[k]];
+      DH_TRY(fermi_filling(ls, st.nel, o.kT, fs, st.fr));
+      c.fill.assign(n, 0.0);
+      for (int k = 0; k < n; ++k) c.fill[perm[k]] = fs[k];
+      c.eigv = lam;
+      // the density takes the columns up to the last one with a filling of at least eps
+      const int nocc = occupied_levels(c.fill);
+      DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+      c.times[kTHost] += now() - th3;
+      c.nocc = nocc;
+      c.xFull = true;
+      DH_HIP(hipEventRecord(ev[3], s));
+      double* Y = c.eig.T.p;
+      DH_TRY(scale_cols_sqrt<double>(s, c.X.p, n, c.f.p, Y, n, n, nocc));
+      DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, Y, n, Y, n, 0.0, c.P.p, n, true));
+      DH_HIP(hipEventRecord(ev[4], s));
+      DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
+                                 c.rho.p));
+    } else if (mixed && !refined) {
       DH_TRY(launch_unpack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, H32, n,
                                   n));
     } else {
@@ -1814,6 +1852,7 @@ static int ctx_scc_step(Context& c) {
                                    n, n));
       if (refined) DH_TRY((convert2d<double, float>(s, c.Hd.p, n, H32, n, n, n)));
     }
+    if (!tail) {
     DH_HIP(hipEventRecord(ev[1], s));
     int info = 0;
     if (mixed)
@@ -1838,7 +1877,8 @@ static int ctx_scc_step(Context& c) {
       // the density matrix with fillings recomputed from the refined eigenvalues
       DH_TRY(c.eig.vectors_mixed(s, n, P32, n));
       DH_HIP(hipEventRecord(ev[6], s));
-      const int nvec = std::min(n, nocc + kRefineMargin);
+      // with tail refinement every refined vector is kept: the next iteration refines them
+      const int nvec = keepAll ? n : std::min(n, nocc + kRefineMargin);
       std::vector<double> lam;
       DH_TRY(c.eig.refine(s, c.Hd.p, c.Sd.p, P32, o.refine_steps, nvec, c.X.p, n, lam));
       DH_HIP(hipEventRecord(ev[7], s));
@@ -1851,7 +1891,7 @@ static int ctx_scc_step(Context& c) {
       DH_TRY(fermi_filling(ls, st.nel, o.kT, fs, st.fr));
       for (int k = 0; k < n; ++k) c.fill[perm[k]] = fs[k];
       c.eigv = lam;
-      nocc = nvec;
+      nocc = keepAll ? occupied_levels(c.fill) : nvec;
       DH_HIP(hipMemcpyAsync(c.f.p, c.fill.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
       c.times[kTHost] += now() - th3;
     } else if (mixed) {
@@ -1876,6 +1916,8 @@ static int ctx_scc_step(Context& c) {
                                  c.rho.p));
     else
       DH_TRY(launch_pack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, P32, n, c.rho.p));
+    c.xFull = refined && keepAll;
+    }
   }
   DH_TRY(launch_mulliken(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.h0.p, c.rho.p,
                          c.qorb.p, c.eorb.p, n));

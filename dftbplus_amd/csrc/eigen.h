@@ -69,6 +69,11 @@ struct EigenSolver {
   // eigenvalues (Rayleigh quotients of the last input basis, unsorted) to lamHost.
   int refine(hipStream_t s, const double* H, const double* S, const float* X0, int steps,
              int nvec, double* X, long ldx, std::vector<double>& lamHost);
+  // the same from an FP64 basis X0 (n x n, leading dimension n); X0 may be X itself
+  int refine(hipStream_t s, const double* H, const double* S, const double* X0, int steps,
+             int nvec, double* X, long ldx, std::vector<double>& lamHost);
+  int refine_from_basis(hipStream_t s, const double* H, const double* S, int steps, int nvec,
+                        double* X, long ldx, std::vector<double>& lamHost);
 };
 
 }  // namespace dftbhip

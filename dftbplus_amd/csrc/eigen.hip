@@ -160,13 +160,27 @@ __global__ void oa_correction_kernel(const double* M, const double* G, const dou
 int EigenSolver::refine(hipStream_t s, const double* H, const double* S, const float* X0,
                         int steps, int nvec, double* X, long ldx, std::vector<double>& lamHost) {
   DH_CHECK(workSize >= 2 * (size_t)n * n, "refine: workspace too small");
+  DH_TRY((convert2d<float, double>(s, X0, n, Zd.p, n, n, n)));
+  return refine_from_basis(s, H, S, steps, nvec, X, ldx, lamHost);
+}
+
+int EigenSolver::refine(hipStream_t s, const double* H, const double* S, const double* X0,
+                        int steps, int nvec, double* X, long ldx, std::vector<double>& lamHost) {
+  DH_CHECK(workSize >= 2 * (size_t)n * n, "refine: workspace too small");
+  DH_TRY(copy2d<double>(s, X0, n, Zd.p, n, n, n));
+  return refine_from_basis(s, H, S, steps, nvec, X, ldx, lamHost);
+}
+
+// the steps on the basis already in Zd
+int EigenSolver::refine_from_basis(hipStream_t s, const double* H, const double* S, int steps,
+                                   int nvec, double* X, long ldx,
+                                   std::vector<double>& lamHost) {
   steps = std::max(steps, 1);
   double* Y = C.p;                  // H X or S X
   double* M = work.p;               // X^T H X
   double* G = T.p;                  // X^T S X
   double* E = work.p + (size_t)n * n;
   double* Xa = Zd.p;                // basis of the current step
-  DH_TRY((convert2d<float, double>(s, X0, n, Xa, n, n, n)));
   for (int st = 0; st < steps; ++st) {
     const bool last = st == steps - 1;
     const int nv = last ? nvec : n;

@@ -178,6 +178,12 @@ typedef struct {
   double refine_below;
   int refine_steps;
   int tridiag;          /* 0: two-stage (dense -> band -> tridiagonal), 1: one-stage Householder */
+  /* mixed precision only (0: off): with tail_below > 0 the refined iterations keep all n
+   * refined eigenvectors, and a refined iteration whose previous charge error is at most
+   * tail_below skips the eigensolve: one Ogita-Aishima step refines the previous iteration's
+   * FP64 eigenpairs against the new H (late in the SCC cycle H changes by much less than the
+   * level spacing from one iteration to the next) */
+  double tail_below;
 } dftbhip_options;
 
 typedef struct dftbhip_ctx dftbhip_ctx;

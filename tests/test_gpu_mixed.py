@@ -80,3 +80,29 @@ def test_mixed_then_fp64_converges_to_fp64(cluster216):
     FP64 iterations; converges to the FP64 fixed point within the FP64 tolerances."""
     t = _fp64_and_switch(cluster216, refine_below=1e-3, refine_steps=0)
     assert t["refine"] == 0.0
+
+
+def test_mixed_tail_refinement_converges_to_fp64(cluster216):
+    """Three phases: FP32 iterations above a charge error of 1e-3, FP32 eigenpairs refined by
+    two Ogita-Aishima steps down to 1e-5, and below that no eigensolve at all: one step refines
+    the previous iteration's FP64 eigenpairs against the new H. Converges to the FP64 fixed
+    point within the FP64 tolerances."""
+    t = _fp64_and_switch(cluster216, refine_below=1e-3, refine_steps=2, tail_below=1e-5)
+    assert t["refine"] > 0.0
+
+
+def test_mixed_tail_refinement_periodic(diamond2):
+    """The same three-phase cycle on the 64-atom periodic cell at kT = 0.01 Ha (many
+    partially filled levels near the Fermi level) against the FP64 oracle."""
+    from dftbplus_amd import api
+    geo, params, opts, calc, res = diamond2
+    dp = api.DftbPlus(params, geo, opts, precision=1, refine_below=1e-3, refine_steps=2,
+                      tail_below=1e-5)
+    e, q, g = dp.get_energy(), dp.get_gross_charges(), dp.get_gradients()
+    nat = geo.natom
+    de, dq, dg = abs(e - res.energy_mermin) / nat, np.max(np.abs(q - res.charges)), \
+        np.max(np.abs(g - res.gradients))
+    print(f"diamond2 tail: it {dp.n_iter}/{res.n_iter} dE/atom {de:.2e} dq {dq:.2e} dF {dg:.2e}")
+    assert dp.converged
+    assert de < 1e-8 and dq < 1e-8 and dg < 1e-7
+    dp.close()
