@@ -536,6 +536,7 @@ struct DevBlocks {
     pr.at2 = at2.p;
     pr.start = start.p;
     pr.species = this->species.p;
+    rowsValid = false;
     return 0;
   }
   // the same from lists already on the device (start null: zeros)
@@ -565,6 +566,43 @@ struct DevBlocks {
     pr.at2 = at2.p;
     pr.start = start.p;
     pr.species = this->species.p;
+    rowsValid = false;
+    return 0;
+  }
+  // row view of the blocks for the block-sparse products (built on first use per geometry):
+  // a counting sort of the block list by row atom, on the host
+  DevBuf<int> rowPtr;
+  DevBuf<int2> rowEnt;
+  RowBlocksDev rows{};
+  bool rowsValid = false;
+  int build_rows(int nat, hipStream_t s) {
+    if (rowsValid) return 0;
+    const int nb = pr.nblk;
+    std::vector<int> h1(std::max(nb, 1)), h2(std::max(nb, 1));
+    if (nb > 0) {
+      DH_HIP(hipMemcpyAsync(h1.data(), pr.at1, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+      DH_HIP(hipMemcpyAsync(h2.data(), pr.at2, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+      DH_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<int> ptr(nat + 1, 0);
+    for (int k = 0; k < nb; ++k) {
+      DH_CHECK(h1[k] >= 0 && h1[k] < nat && h2[k] >= 0 && h2[k] < nat,
+               "row view: block atom out of range");
+      ptr[h2[k] + 1] += 1;
+      if (h1[k] != h2[k]) ptr[h1[k] + 1] += 1;
+    }
+    for (int a = 0; a < nat; ++a) ptr[a + 1] += ptr[a];
+    std::vector<int2> ent(std::max(ptr[nat], 1));
+    std::vector<int> fillPos(ptr.begin(), ptr.end() - 1);
+    for (int k = 0; k < nb; ++k) {
+      ent[fillPos[h2[k]]++] = make_int2(2 * k, h1[k]);
+      if (h1[k] != h2[k]) ent[fillPos[h1[k]]++] = make_int2(2 * k + 1, h2[k]);
+    }
+    DH_TRY(rowPtr.upload(ptr.data(), ptr.size(), s));
+    DH_TRY(rowEnt.upload(ent.data(), ent.size(), s));
+    DH_HIP(hipStreamSynchronize(s));
+    rows = RowBlocksDev{nat, rowPtr.p, rowEnt.p};
+    rowsValid = true;
     return 0;
   }
 };
@@ -1361,6 +1399,11 @@ struct Context {
   int nocc = 0;
   bool xf64 = true;     // the eigenvectors in X are FP64 (else FP32, mixed path)
   bool xFull = false;   // X holds all n refined FP64 eigenvectors of the last iteration
+  // Ogita-Aishima refinement on the sparse H and S: the n eigenvectors transposed (two buffers,
+  // tCur the current one), the products (H X)^T, (S X)^T, M = X^T H X, G = X^T S X, the
+  // Rayleigh quotients
+  DevBuf<double> tXt[2], tYH, tYS, tM, tG, tLam;
+  int tCur = 0;
   bool sccDone = false;
   // SCC cycles start from the input charges the previous cycle ended with (the reference's
   // geometry loop keeps qInput between geometry steps, main.F90:219-357) instead of the
@@ -1602,6 +1645,46 @@ static int occupied_levels(const std::vector<double>& f) {
   return 0;
 }
 
+// Ogita-Aishima refinement (Ogita & Aishima 2018) of all n generalized eigenpairs held
+// transposed in c.tXt[c.tCur] (FP64, n x n), against the sparse H (c.hs) and S (c.ov) of the
+// current iteration. Per step: YH^T = (H X)^T and YS^T = (S X)^T by the block-sparse product
+// (H and S stay sparse: 2 nnz n flops instead of 2 n^3), M = X^T H X and G = X^T S X as
+// lower-output GEMMs (n^3 each), the correction E written transposed, and X <- X (I + E), i.e.
+// Xt <- Xt + E^T Xt (2 n^3). The refined vectors end in c.X (normal layout), the Rayleigh
+// quotients of the last input basis (unsorted) in lamHost.
+static int oa_refine_sparse(Context& c, int steps, std::vector<double>& lamHost) {
+  hipStream_t s = c.s;
+  const int n = c.norbTot;
+  const size_t nn = (size_t)n * n;
+  DH_TRY(c.dblk.build_rows(c.nat, s));
+  DH_TRY(c.tXt[1 - c.tCur].alloc(nn));
+  DH_TRY(c.tYH.alloc(nn));
+  DH_TRY(c.tYS.alloc(nn));
+  DH_TRY(c.tM.alloc(nn));
+  DH_TRY(c.tG.alloc(nn));
+  DH_TRY(c.tLam.alloc(n));
+  for (int st = 0; st < std::max(steps, 1); ++st) {
+    const double* Xt = c.tXt[c.tCur].p;
+    double* Xn = c.tXt[1 - c.tCur].p;
+    DH_TRY(launch_spmm_hs_t(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, c.dblk.start.p,
+                            c.hs.p, c.ov.p, Xt, n, n, c.tYH.p, c.tYS.p, n));
+    DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYH.p, n, 0.0, c.tM.p, n, true));
+    DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYS.p, n, 0.0, c.tG.p, n, true));
+    DH_TRY(symmetrize_lower(s, c.tM.p, n, n));
+    DH_TRY(symmetrize_lower(s, c.tG.p, n, n));
+    double* Et = c.tYH.p;   // free once M is formed
+    DH_TRY(oa_correction_t(s, c.tM.p, c.tG.p, n, c.tLam.p, Et));
+    DH_TRY(copy2d<double>(s, Xt, n, Xn, n, n, n));
+    DH_TRY(gemm<double>(s, false, false, n, n, n, 1.0, Et, n, Xt, n, 1.0, Xn, n));
+    c.tCur = 1 - c.tCur;
+  }
+  DH_TRY((transpose2d<double, double>(s, c.tXt[c.tCur].p, n, c.X.p, n, n, n)));
+  lamHost.resize(n);
+  DH_HIP(hipMemcpyAsync(lamHost.data(), c.tLam.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 // Levels beyond the occupied ones kept in the refined density matrix: refined eigenvalues
 // move by the FP32 error of the tridiagonalisation, which can reorder levels at the cut.
 constexpr int kRefineMargin = 64;
@@ -1813,13 +1896,11 @@ static int ctx_scc_step(Context& c) {
     float* P32 = reinterpret_cast<float*>(c.P.p);
     float* H32 = refined ? c.eig.h32() : reinterpret_cast<float*>(c.Hd.p);
     if (tail) {
-      DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p,
-                                   n, n));
       DH_HIP(hipEventRecord(ev[1], s));
       DH_HIP(hipEventRecord(ev[2], s));
       DH_HIP(hipEventRecord(ev[6], s));
       std::vector<double> lam;
-      DH_TRY(c.eig.refine(s, c.Hd.p, c.Sd.p, c.X.p, 1, n, c.X.p, n, lam));
+      DH_TRY(oa_refine_sparse(c, 1, lam));
       DH_HIP(hipEventRecord(ev[7], s));
       const double th3 = now();
       std::vector<int> perm(n);
@@ -1877,10 +1958,17 @@ static int ctx_scc_step(Context& c) {
       // the density matrix with fillings recomputed from the refined eigenvalues
       DH_TRY(c.eig.vectors_mixed(s, n, P32, n));
       DH_HIP(hipEventRecord(ev[6], s));
-      // with tail refinement every refined vector is kept: the next iteration refines them
+      // with tail refinement every refined vector is kept (the next iteration refines them) and
+      // the refinement runs on the sparse H and S with the vectors held transposed
       const int nvec = keepAll ? n : std::min(n, nocc + kRefineMargin);
       std::vector<double> lam;
-      DH_TRY(c.eig.refine(s, c.Hd.p, c.Sd.p, P32, o.refine_steps, nvec, c.X.p, n, lam));
+      if (keepAll) {
+        DH_TRY(c.tXt[c.tCur].alloc((size_t)n * n));
+        DH_TRY((transpose2d<float, double>(s, P32, n, c.tXt[c.tCur].p, n, n, n)));
+        DH_TRY(oa_refine_sparse(c, o.refine_steps, lam));
+      } else {
+        DH_TRY(c.eig.refine(s, c.Hd.p, c.Sd.p, P32, o.refine_steps, nvec, c.X.p, n, lam));
+      }
       DH_HIP(hipEventRecord(ev[7], s));
       const double th3 = now();
       std::vector<int> perm(n);

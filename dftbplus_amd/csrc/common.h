@@ -184,6 +184,9 @@ int split_complex(hipStream_t s, const double* z, long ldz, double* re, double* 
                   int m, int n);
 int merge_complex(hipStream_t s, const double* re, const double* im, long ld, double* z, long ldz,
                   int m, int n, int mode);
+// dst(j, i) = src(i, j) for the m x n matrix src (dst is n x m), converted to Tout
+template <typename Tin, typename Tout>
+int transpose2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n);
 int symmetrize_lower(hipStream_t s, double* a, long lda, int n);
 int symmetrize_lower(hipStream_t s, float* a, long lda, int n);
 int zero_strict_upper(hipStream_t s, double* a, long lda, int n);

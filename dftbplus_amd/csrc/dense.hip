@@ -142,6 +142,39 @@ int symmetrize_lower(hipStream_t s, float* a, long lda, int n) {
   return symmetrize_lower_t<float>(s, a, lda, n);
 }
 
+// dst(j, i) = src(i, j) for the m x n matrix src, 32x32 tiles through LDS (grid-stride over
+// tile columns so that any n fits the grid)
+template <typename Tin, typename Tout>
+__global__ void transpose2d_kernel(const Tin* src, long lds, Tout* dst, long ldd, int m, int n) {
+  __shared__ Tout t[32][33];
+  const int tx = threadIdx.x, ty = threadIdx.y;   // 32 x 8
+  const int bi = blockIdx.x;
+  for (int bj = blockIdx.y; bj * 32 < n; bj += gridDim.y) {
+    for (int k = ty; k < 32; k += 8) {
+      const int i = bi * 32 + tx, j = bj * 32 + k;
+      t[k][tx] = (i < m && j < n) ? (Tout)src[i + (long)j * lds] : Tout(0);
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+      const int j = bj * 32 + tx, i = bi * 32 + k;   // dst(j, i) = src(i, j)
+      if (i < m && j < n) dst[j + (long)i * ldd] = t[tx][k];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename Tin, typename Tout>
+int transpose2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n) {
+  if (m <= 0 || n <= 0) return 0;
+  dim3 grid(cdiv(m, 32), std::min(cdiv(n, 32), 65535u));
+  hipLaunchKernelGGL((transpose2d_kernel<Tin, Tout>), grid, dim3(32, 8), 0, s, src, lds, dst, ldd,
+                     m, n);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+template int transpose2d<double, double>(hipStream_t, const double*, long, double*, long, int, int);
+template int transpose2d<float, double>(hipStream_t, const float*, long, double*, long, int, int);
+
 int zero_strict_upper(hipStream_t s, double* a, long lda, int n) {
   if (n <= 1) return 0;
   hipLaunchKernelGGL(zero_upper_kernel, dim3(cdiv(n, 256), n), dim3(256), 0, s, a, lda, n);

@@ -8,6 +8,12 @@
 
 namespace dftbhip {
 
+// Ogita-Aishima correction of n eigenpairs, transposed: lam_i = M_ii / G_ii and
+// Et(j, i) = E(i, j) (E as in EigenSolver::refine) from the symmetric M = X^T H X and
+// G = X^T S X (both triangles stored, leading dimension n)
+int oa_correction_t(hipStream_t s, const double* M, const double* G, int n, double* lam,
+                    double* Et);
+
 struct EigenSolver {
   int n = 0;
   bool factored = false;

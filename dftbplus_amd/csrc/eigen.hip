@@ -157,6 +157,37 @@ __global__ void oa_correction_kernel(const double* M, const double* G, const dou
   E[ij] = e;
 }
 
+// The same correction written transposed, Et(j, i) = E(i, j), from the symmetric M and G (both
+// triangles stored): thread (row j, column i) reads M and G at (j, i) = (i, j), so reads and
+// writes run along columns. lam is formed first by oa_lambda_kernel.
+__global__ void oa_correction_t_kernel(const double* M, const double* G, const double* lam, int n,
+                                       double* Et) {
+  const int jr = blockIdx.x * blockDim.x + threadIdx.x;   // row of Et: the vector index j of E
+  const int ic = blockIdx.y;                               // column of Et: the index i of E
+  if (jr >= n || ic >= n) return;
+  const long q = jr + (long)ic * n;
+  double e;
+  if (jr == ic) {
+    e = 0.5 * (1.0 - G[q]);
+  } else {
+    const double r = -G[q];
+    const double num = M[q] + lam[jr] * r, den = lam[jr] - lam[ic];
+    e = (std::fabs(num) < kOaMax * std::fabs(den)) ? num / den : 0.5 * r;
+  }
+  Et[q] = e;
+}
+
+int oa_correction_t(hipStream_t s, const double* M, const double* G, int n, double* lam,
+                    double* Et) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(oa_lambda_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, M, G, n, lam);
+  DH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(oa_correction_t_kernel, dim3(cdiv(n, 256), n), dim3(256), 0, s, M, G, lam, n,
+                     Et);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
 int EigenSolver::refine(hipStream_t s, const double* H, const double* S, const float* X0,
                         int steps, int nvec, double* X, long ldx, std::vector<double>& lamHost) {
   DH_CHECK(workSize >= 2 * (size_t)n * n, "refine: workspace too small");

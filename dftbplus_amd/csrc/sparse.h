@@ -98,6 +98,21 @@ int launch_gamma_grad(hipStream_t s, const NeighDev& nb, const double* coords,
                       int ntr, const double* tr, double alpha, double maxr, int ng,
                       const double* G, const double* w, const double* CSq, const double* dq,
                       double* grad);
+// Row view of the dense matrix that unpack builds from the sparse blocks: row atom A holds the
+// entries (2 k + t, B) of rowPtr[A] .. rowPtr[A + 1] - 1: block k of the pair (at1 = B,
+// at2 = A) with t = 0 and, for at1 != at2, block k of the pair (at1 = A, at2 = B) transposed
+// (t = 1). Image blocks of one atom pair are separate entries (they are summed by the product).
+struct RowBlocksDev {
+  int nat;
+  const int* rowPtr;
+  const int2* ent;
+};
+// YH^T = (H X)^T and YS^T = (S X)^T with Xt = X^T (nvec x n, leading dimension ldx) and the
+// outputs nvec x n (leading dimension ldy); H, S: sparse h and ov in the block layout of rb
+int launch_spmm_hs_t(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+                     const long* iAtomStart, const long* blkStart, const double* h,
+                     const double* ov, const double* Xt, long ldx, int nvec, double* yh,
+                     double* ys, long ldy);
 int launch_repulsive(hipStream_t s, const NeighDev& nb, const double* coords, const RepDev* reps,
                      int nsp, double* eatom, double* grad);
 

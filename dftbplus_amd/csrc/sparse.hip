@@ -857,6 +857,85 @@ int launch_repulsive(hipStream_t s, const NeighDev& nb, const double* coords, co
   return 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Block-sparse products with a transposed dense operand: YH^T = (H X)^T and YS^T = (S X)^T for
+// X given as Xt = X^T (nvec x n, column-major, so the nvec coefficients of one orbital are
+// contiguous). H and S are the matrices unpack builds from the sparse blocks (image blocks of one
+// atom pair summed), taken row atom by row atom from the row view rb. A workgroup owns one row
+// atom A and 256 consecutive vectors j (one per thread): per entry it reads the block (the same
+// addresses in every lane) and the <= 4 contiguous coefficient runs of the column atom, and keeps
+// the <= 4 outputs of A's orbitals for both matrices in registers. Workgroups are numbered so
+// that consecutive ones of an XCD (IDs congruent mod 8) walk the atoms of one vector tile: the
+// column-atom runs of neighbouring row atoms are then read from that XCD's L2.
+// ---------------------------------------------------------------------------------------------
+constexpr int kSpmmCols = 256;
+constexpr int kXcds = 8;
+
+__global__ __launch_bounds__(kSpmmCols) void spmm_hs_t_kernel(
+    int nat, int ntile, const int* __restrict__ rowPtr, const int2* __restrict__ ent,
+    const int* __restrict__ norbAtom, const long* __restrict__ iAtomStart,
+    const long* __restrict__ blkStart, const double* __restrict__ h,
+    const double* __restrict__ ov, const double* __restrict__ Xt, long ldx, int nvec,
+    double* __restrict__ yh, double* __restrict__ ys, long ldy) {
+  const long w = blockIdx.x;
+  const int xcd = (int)(w % kXcds);
+  const long l = w / kXcds;
+  const int tile = xcd + kXcds * (int)(l / nat);
+  const int A = (int)(l % nat);
+  if (tile >= ntile) return;
+  const int j = tile * kSpmmCols + threadIdx.x;
+  const bool live = j < nvec;
+  const int jj = live ? j : 0;
+  const int nA = norbAtom[A];
+  const long oA = iAtomStart[A];
+  double ah[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
+  const int p1 = rowPtr[A + 1];
+  for (int p = rowPtr[A]; p < p1; ++p) {
+    const int2 e = ent[p];
+    const int k = e.x >> 1, tr = e.x & 1, B = e.y;
+    const int nB = norbAtom[B];
+    const long oB = iAtomStart[B], st = blkStart[k];
+    double xv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xv[c] = c < nB ? Xt[jj + (oB + c) * ldx] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r >= nA) break;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= nB) break;
+        // block k is nOrb(at2) x nOrb(at1), column-major: as is (rows of A) or transposed
+        const long q = st + (tr ? (c + (long)r * nB) : (r + (long)c * nA));
+        ah[r] = fma(h[q], xv[c], ah[r]);
+        as[r] = fma(ov[q], xv[c], as[r]);
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (r >= nA) break;
+    yh[j + (oA + r) * ldy] = ah[r];
+    ys[j + (oA + r) * ldy] = as[r];
+  }
+}
+
+int launch_spmm_hs_t(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+                     const long* iAtomStart, const long* blkStart, const double* h,
+                     const double* ov, const double* Xt, long ldx, int nvec, double* yh,
+                     double* ys, long ldy) {
+  if (rb.nat == 0 || nvec <= 0) return 0;
+  const int ntile = (int)cdiv(nvec, kSpmmCols);
+  const long tilesPad = (long)cdiv(ntile, kXcds) * kXcds;
+  const long nwg = tilesPad * rb.nat;
+  DH_CHECK(nwg < (1L << 31), "spmm: grid too large");
+  hipLaunchKernelGGL(spmm_hs_t_kernel, dim3((unsigned)nwg), dim3(kSpmmCols), 0, s, rb.nat, ntile,
+                     rb.rowPtr, rb.ent, norb, iAtomStart, blkStart, h, ov, Xt, ldx, nvec, yh, ys,
+                     ldy);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
 template int launch_unpack<double>(hipStream_t, const PairsDev&, const int*, const long*,
                                    const double*, double*, long, int);
 template int launch_unpack<float>(hipStream_t, const PairsDev&, const int*, const long*,

@@ -175,7 +175,9 @@ def test_batch_device_scc_equals_host_scc():
 
 def test_batch_graph_replays_equal_eager_launches():
     """Every replay of the captured SCC iteration (mode 2) equals the same iteration launched
-    kernel by kernel (mode 1), bit for bit, with the host reading the state between replays."""
+    kernel by kernel (mode 1), with the host reading the state between replays. The Mulliken
+    and force sums add blocks with floating-point atomics, whose order varies from run to run,
+    so two runs agree to rounding (here 1e-11), not bit for bit."""
     from dftbplus_amd import api, synth
     mols, params, opts = synth.workload("molecules24")
     try:
@@ -195,15 +197,15 @@ def test_batch_graph_replays_equal_eager_launches():
         assert na_e == na_g
         ee = eager.scc_end()
         eg = graph.scc_end()
-        assert np.array_equal(ee, eg), steps
+        assert np.max(np.abs(ee - eg) / np.maximum(1.0, np.abs(ee))) < 1e-11, steps
         for qa, qb in zip(eager.charges, graph.charges):
-            assert np.array_equal(qa, qb), steps
+            assert np.max(np.abs(qa - qb)) < 1e-11, steps
         assert np.array_equal(eager.n_iter, graph.n_iter)
         if na_e == 0 or steps >= opts.max_scc_iter:
             break
     assert steps > 5
     ge, gg = eager.gradients(), graph.gradients()
     for a, b in zip(ge, gg):
-        assert np.array_equal(a, b)
+        assert np.max(np.abs(a - b)) < 1e-11
     eager.close()
     graph.close()

@@ -106,3 +106,29 @@ def test_mixed_tail_refinement_periodic(diamond2):
     assert dp.converged
     assert de < 1e-8 and dq < 1e-8 and dg < 1e-7
     dp.close()
+
+
+@pytest.mark.parametrize("system", ["cluster216", "diamond2"])
+def test_mixed_early_tail_matches_oracle(system, request):
+    """FP32 iterations above a charge error of 1e-3, one iteration whose FP32 eigenpairs are
+    refined by two Ogita-Aishima steps, and every later iteration a single step on the
+    previous eigenpairs against the new sparse H and S (no eigensolve): the cycle reaches the
+    FP64 oracle's fixed point within the FP64 tolerances. The periodic cell has atoms that
+    neighbour their own images, so the block-sparse products must sum image blocks as the
+    dense unpack does."""
+    from dftbplus_amd import api
+    geo, params, opts, calc, res = request.getfixturevalue(system)
+    o = dataclasses.replace(opts, max_scc_iter=200)
+    dp = api.DftbPlus(params, geo, o, precision=1, refine_below=1e-3, refine_steps=2,
+                      tail_below=1e-3)
+    e, q, g = dp.get_energy(), dp.get_gross_charges(), dp.get_gradients()
+    t = dp.timings()
+    nat = geo.natom
+    de, dq, dg = abs(e - res.energy_mermin) / nat, np.max(np.abs(q - res.charges)), \
+        np.max(np.abs(g - res.gradients))
+    print(f"{system} early tail: it {dp.n_iter}/{res.n_iter} dE/atom {de:.2e} dq {dq:.2e} "
+          f"dF {dg:.2e} refine {t['refine']:.3f}s")
+    assert dp.converged
+    assert t["refine"] > 0.0
+    assert de < 1e-8 and dq < 1e-8 and dg < 1e-7
+    dp.close()
