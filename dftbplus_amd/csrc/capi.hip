@@ -13,6 +13,7 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <memory>
 #include <string>
@@ -53,6 +54,32 @@ struct Runtime {
   }
 };
 static Runtime& g_rt = *new Runtime();
+
+// Workspace of the allocation-free entry point dftbhip_dsytrd_noalloc: a fixed device array
+// that every GPU the library runs on holds for the life of the process. Host code never hands
+// the symbol itself to a kernel (in host code it is only a stand-in): hipGetSymbolAddress gives
+// each device's own address of it once, with that device current, and that address is used.
+constexpr size_t kScratchBytes = 96ull << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratchPtr[kMaxDevices] = {};
+static std::mutex& g_scratchMutex = *new std::mutex();
+
+// the current device's address of the workspace array (looked up on the first call per device)
+static int scratch_ptr(void** out) {
+  int dev = 0;
+  DH_HIP(hipGetDevice(&dev));
+  DH_CHECK(dev >= 0 && dev < kMaxDevices, "scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    DH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DH_CHECK(p != nullptr, "scratch: no device address");
+    g_scratchPtr[dev] = p;
+  }
+  *out = g_scratchPtr[dev];
+  return 0;
+}
 
 static int ensure_runtime() {
   if (g_rt.stream) return 0;
@@ -259,7 +286,9 @@ extern "C" int dftbhip_get_gpus_available(int* ngpus) {
 extern "C" int dftbhip_init(int device) {
   if (g_rt.stream) return 0;
   g_rt.device = device;
-  return ensure_runtime();
+  DH_TRY(ensure_runtime());
+  void* p = nullptr;
+  return scratch_ptr(&p);   // the workspace address of this device, looked up at set-up
 }
 
 extern "C" int dftbhip_finalize(void) {
@@ -3216,6 +3245,33 @@ extern "C" int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, d
     DH_HIP(hipMemcpyAsync(e, E.p, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
     DH_HIP(hipMemcpyAsync(tau, TAU.p, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, s));
   }
+  DH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Tridiagonalisation of device-resident arrays with the workspace in the fixed device array
+// (no allocation): the same kernels as dftbhip_dsytrd, on the library stream, synchronised
+// before returning. Calls that overlap on one GPU share the array and race.
+extern "C" int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e,
+                                      double* tau) {
+  DH_CHECK(n >= 0 && a && d && (n < 2 || (e && tau)) && lda >= std::max(n, 1),
+           "dsytrd_noalloc: bad arguments");
+  if (n == 0) return 0;
+  if (!dftbhip::g_rt.stream) {
+    dftbhip::set_error("dsytrd_noalloc: call dftbhip_init first");
+    return dftbhip::kErrState;
+  }
+  const size_t need = dftbhip::sytrd_work_bytes(n, 8);
+  if (need > dftbhip::kScratchBytes) {
+    dftbhip::set_error("dsytrd_noalloc: n = " + std::to_string(n) + " needs " +
+                       std::to_string(need) + " bytes of workspace, more than the fixed " +
+                       std::to_string(dftbhip::kScratchBytes));
+    return dftbhip::kErrArg;
+  }
+  void* ws = nullptr;
+  DH_TRY(dftbhip::scratch_ptr(&ws));
+  hipStream_t s = dftbhip::g_rt.stream;
+  DH_TRY(dftbhip::sytrd_lower(s, a, lda, n, d, e, tau, static_cast<double*>(ws), need / 8));
   DH_HIP(hipStreamSynchronize(s));
   return 0;
 }

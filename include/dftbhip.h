@@ -49,6 +49,12 @@ int dftbhip_dstedc(int n, double* d, double* e, double* z, int ldz);
  * (the reduction DFTB+ reaches through the dsygvd / dsyevd routes of
  * src/dftbp/math/eigensolver.F90:561-660). */
 int dftbhip_dsytrd(char uplo, int n, double* a, int lda, double* d, double* e, double* tau);
+/* dftbhip_dsytrd on device arrays (a, d, e, tau device pointers) without any allocation: no
+ * counterpart in the reference; for callers that cannot allocate device memory (code under
+ * graph capture, serving loops). The workspace is a fixed 96 MiB device array per GPU
+ * (n up to about 17500); larger n is rejected with kErrArg. dftbhip_init must have been
+ * called. Calls that overlap on one GPU share the workspace and race. */
+int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e, double* tau);
 /* Kernel tuning. param 0: row tiles per workgroup of the tridiagonalisation's SYMV (2, 4, 8,
  * 16, or 0 = chosen from the trailing size per column, the default: the autotuned table of the
  * arithmetic type when there is one, else fixed thresholds); param 1, value 0: drop the
