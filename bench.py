@@ -16,9 +16,12 @@ gradients is the leg's force evaluation. Legs:
   mixed_then_fp64  FP32 MFMA reduction, tridiagonalisation, back-transformation and density
                    matrices while the previous iteration's charge error is above 1e-3, plain
                    FP64 iterations after (the converged point is the FP64 one);
-  mixed_tail       FP32 iterations above 1e-3, FP32 eigenpairs refined by two Ogita-Aishima
-                   steps down to 1e-5, below that no eigensolve: one step refines the
-                   previous iteration's FP64 eigenpairs against the new H;
+  mixed_tail       FP32 iterations above 1e-3, then one iteration whose FP32 eigenpairs are
+                   refined by two Ogita-Aishima steps, and after it no eigensolve: one step
+                   refines the previous iteration's FP64 eigenpairs against the new sparse H
+                   and S (block-sparse products, three n^3 GEMMs);
+  mixed_tail_late  as mixed_tail, with the refined-seed iterations down to 1e-5 (only with
+                   --legs);
   mixed_fp32       FP32 iterations throughout (only with --legs; does not reach 1e-8 e).
 `value` is the SCC iterations/s of the fp64 leg's timed steps; every leg reports its timed
 rate, its converged cycle (iterations, seconds, force evaluations/s) and its energy, charge and
@@ -55,7 +58,8 @@ LEGS = {"fp64": dict(precision=0),
         "mixed_then_fp64": dict(precision=1, refine_below=1e-3, refine_steps=0),
         "mixed_fp32": dict(precision=1, refine_below=0.0),
         "mixed_then_refined": dict(precision=1, refine_below=1e-3, refine_steps=1),
-        "mixed_tail": dict(precision=1, refine_below=1e-3, refine_steps=2, tail_below=1e-5)}
+        "mixed_tail": dict(precision=1, refine_below=1e-3, refine_steps=2, tail_below=1e-3),
+        "mixed_tail_late": dict(precision=1, refine_below=1e-3, refine_steps=2, tail_below=1e-5)}
 E_BOUND, F_BOUND = 1e-6, 1e-5
 
 
@@ -90,18 +94,24 @@ def blas_threads():
                               os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
 
 
-def cpu_iteration(workload):
+def cpu_iteration(workload, timing_gamma=False):
     """Two SCC iterations of the oracle on the host, each timed on its own (potential, H,
     dense generalized eigensolve, Fermi filling, density matrix, pack, Mulliken, energies and
     Broyden mixing, the steps of its SCC loop); returns the orbital count, the set-up time and
     the two iteration times (the second one is the baseline: the first one also pays one-off
-    costs of the first LAPACK calls)."""
+    costs of the first LAPACK calls). timing_gamma: the atomic gamma matrix is the cluster 1/R
+    instead of the Ewald sum, whose serial set-up takes minutes at 4096 atoms; an iteration
+    costs the same (one product of the matrix with the charges)."""
     import numpy as np
     from dftbplus_amd import synth
     from oracle import dftb_ref as R
     geo, params, opts = synth.workload(workload)
     t0 = time.perf_counter()
-    calc = R.Calculator(params, geo, opts)
+    gamma = None
+    if timing_gamma:
+        gamma = R.invr_cluster(geo.coords)
+        gamma[np.diag_indices(geo.natom)] = 0.4
+    calc = R.Calculator(params, geo, opts, gamma=gamma)
     S = calc.dense(calc.s)
     setup = time.perf_counter() - t0
     n = int(calc.off[-1])
@@ -189,6 +199,24 @@ def run_cycle_leg(api, geo, params, opts, leg, steps, warmup):
         "symv": {"launches": int(launches), "seconds": ksec, "bytes": kbytes,
                  "achieved": achieved},
     }
+
+
+def mode_cpu(args):
+    """The CPU baseline alone: SCC iterations of the oracle on the host for --workload (at the
+    full size of the 4096-atom cell, with the cluster 1/R gamma to skip the serial Ewald set-up)
+    and, for comparison, the bounded sample --cpu-sample scaled by N^3."""
+    model, ncpu = host_cpu()
+    out = {"mode": "cpu", "cpu_model": model, "host_cpus": ncpu, "blas_threads": blas_threads()}
+    for wl, tg in ((args.cpu_sample, False), (args.workload, True)):
+        print(f"[cpu] {wl}: set-up and two SCC iterations", file=sys.stderr, flush=True)
+        n, setup, first, second = cpu_iteration(wl, timing_gamma=tg)
+        out[wl] = {"orbitals": n, "setup_s": setup, "first_s": first, "second_s": second,
+                   "cluster_gamma": tg}
+        print(f"[cpu] {wl}: {n} orbitals, set-up {setup:.1f}s, iterations {first:.2f}s "
+              f"{second:.2f}s", file=sys.stderr, flush=True)
+    ns, nf = out[args.cpu_sample]["orbitals"], out[args.workload]["orbitals"]
+    out["sample_scaled_s"] = out[args.cpu_sample]["second_s"] * (nf / ns) ** 3
+    print(json.dumps(out), flush=True)
 
 
 def mode_sytrd(args):
@@ -330,8 +358,16 @@ def mode_kernels(args):
     rng = np.random.default_rng(0)
     a = rng.standard_normal((args.n, args.n))
     b = rng.standard_normal((args.n, args.n))
-    for tb in ("N", "T"):
-        api.dgemm(a, b, "N", tb)
+    # the GEMMs in both tile orders (launch grid, XCD-grouped): in the kernel trace the launches
+    # follow this sequence
+    for order in (0, 1):
+        api.set_tuning(5, order)
+        for ta, tb in (("N", "N"), ("N", "T"), ("T", "N")):
+            t0 = time.perf_counter()
+            api.dgemm(a, b, ta, tb)
+            print(f"[kernels] gemm order {order} {ta}{tb} n={args.n}: "
+                  f"{time.perf_counter() - t0:.3f}s with transfers", file=sys.stderr, flush=True)
+    api.set_tuning(5, 1)
     print(json.dumps({"mode": "kernels", "workload": args.workload, "atoms": geo.natom,
                       "orbitals": n, "sk_blocks": nblk, "sparse_elements": nnz,
                       "filled_levels": nocc, "gemm_n": args.n}), flush=True)
@@ -516,7 +552,7 @@ def main():
                     help="keep the fixed SYMV tile thresholds instead of autotuning them")
     ap.add_argument("--mode", default="scc",
                     choices=["scc", "sytrd", "batch", "kpoints", "autograd", "neighbours",
-                             "kernels"],
+                             "kernels", "cpu"],
                     help="scc: the headline SCC benchmark (default); sytrd: the tridiagonalisation "
                          "alone on a random matrix (SYMV counter runs); batch: the molecule batch "
                          "against a sampled loop of single-molecule runs; kpoints: SCC iterations "
@@ -525,7 +561,8 @@ def main():
                          "neighbour list on the device and the geometry set-up with it and "
                          "with the host-built list; kernels: one SCC iteration, forces and a "
                          "GEMM of size --n with the sizes that fix their bytes and flops (for "
-                         "kernel traces and counter runs)")
+                         "kernel traces and counter runs); cpu: the CPU baseline alone, at the "
+                         "full size and on the bounded sample")
     ap.add_argument("--n", type=int, default=8192,
                     help="matrix size of --mode sytrd and of the GEMM of --mode kernels")
     ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
@@ -549,7 +586,7 @@ def main():
     if args.mode != "scc":
         {"sytrd": mode_sytrd, "batch": mode_batch, "kpoints": mode_kpoints,
          "neighbours": mode_neighbours, "kernels": mode_kernels,
-         "autograd": mode_autograd}[args.mode](args)
+         "autograd": mode_autograd, "cpu": mode_cpu}[args.mode](args)
         return
     import dataclasses
     geo, params, opts = synth.workload(args.workload)
@@ -625,7 +662,7 @@ def main():
     if not args.no_cpu_baseline:
         model, ncpu = host_cpu()
         wl = args.workload if args.cpu_full else args.cpu_sample
-        ns, setup, first, second = cpu_iteration(wl)
+        ns, setup, first, second = cpu_iteration(wl, timing_gamma=args.cpu_full)
         scale = (n / ns) ** 3
         rec["cpu_baseline"] = {
             "value": 1.0 / (second * scale), "unit": "SCC iterations/s", "cores": blas_threads(),

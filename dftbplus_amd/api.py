@@ -86,6 +86,7 @@ class COptions(C.Structure):
         ("refine_steps", C.c_int),
         ("tridiag", C.c_int),
         ("tail_below", C.c_double),
+        ("tail_steps", C.c_int),
     ]
 
 
@@ -124,6 +125,7 @@ EXPORTS = {
     "dftbhip_ctx_scc_step": [C.c_void_p, _ip, _dp],
     "dftbhip_ctx_scc_end": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_ctx_gradients": [C.c_void_p, _dp],
+    "dftbhip_ctx_scc_state": [C.c_void_p, _dp, C.c_int],
     "dftbhip_ctx_eigenvalues": [C.c_void_p, _dp, _dp, C.c_int],
     "dftbhip_ctx_eigenvalues_k": [C.c_void_p, C.c_int, _dp, _dp, C.c_int],
     "dftbhip_ctx_orbital_populations": [C.c_void_p, _dp, C.c_int],
@@ -226,7 +228,7 @@ def make_params(params: Parameters):
 
 def make_options(opts: Options, force_mode: int = 0, precision: int = 0,
                  refine_below: float = 1e-4, refine_steps: int = 2, tridiag: int = 1,
-                 tail_below: float = 0.0) -> COptions:
+                 tail_below: float = 0.0, tail_steps: int = 1) -> COptions:
     o = COptions()
     o.kT = opts.kT
     o.scc_tol = opts.scc_tol
@@ -245,6 +247,7 @@ def make_options(opts: Options, force_mode: int = 0, precision: int = 0,
     o.refine_steps = refine_steps
     o.tridiag = tridiag
     o.tail_below = tail_below
+    o.tail_steps = tail_steps
     return o
 
 
@@ -547,11 +550,12 @@ class DftbPlus:
 
     def __init__(self, params: Parameters, geometry: Geometry, options: Options = Options(),
                  force_mode: int = 0, precision: int = 0, refine_below: float = 1e-4,
-                 refine_steps: int = 2, tridiag: int = 1, tail_below: float = 0.0):
+                 refine_steps: int = 2, tridiag: int = 1, tail_below: float = 0.0,
+                 tail_steps: int = 1):
         self.params = params
         self._p, self._keep = make_params(params)
         self._o = make_options(options, force_mode, precision, refine_below, refine_steps,
-                               tridiag, tail_below)
+                               tridiag, tail_below, tail_steps)
         self.h = C.c_void_p()
         _check(lib().dftbhip_ctx_create(C.byref(self.h), C.byref(self._p), C.byref(self._o)),
                "dftbhip_ctx_create")
@@ -623,6 +627,16 @@ class DftbPlus:
         self.converged = bool(conv.value)
         self._grad = None
         self._dirty = False
+
+    def scc_state(self) -> dict:
+        """Iteration number, charge error, electronic energy of the last iteration and its
+        change from the one before, input electrons of the last iteration, pairs outside the
+        first-order range of the last refinement step."""
+        out = np.zeros(6)
+        _check(lib().dftbhip_ctx_scc_state(self.h, _d(out), 6), "dftbhip_ctx_scc_state")
+        return {"iteration": int(out[0]), "scc_error": float(out[1]), "e_elec": float(out[2]),
+                "diff_elec": float(out[2] - out[3]) if np.isfinite(out[3]) else float(out[2]),
+                "q_input_sum": float(out[4]), "oa_clipped_pairs": int(out[5])}
 
     def get_nr_atoms(self) -> int:
         return self.natom

@@ -1362,6 +1362,8 @@ struct SccState {
   bool conv = false, active = false;
   double err = 0.0, nel = 0.0, eNonScc = 0.0, eScc = 0.0;
   double prevErr = HUGE_VAL;   // charge error of the previous iteration (mixed-path switch)
+  // electronic energy of the last and of the previous iteration, input electrons of the last
+  double eElec = 0.0, eElecPrev = std::numeric_limits<double>::quiet_NaN(), qinSum = 0.0;
   FillResult fr{};
 };
 
@@ -1432,7 +1434,9 @@ struct Context {
   // tCur the current one), the products (H X)^T, (S X)^T, M = X^T H X, G = X^T S X, the
   // Rayleigh quotients
   DevBuf<double> tXt[2], tYH, tYS, tM, tG, tLam;
+  DevBuf<int> tClip;
   int tCur = 0;
+  int lastClipped = 0;   // pairs outside the first-order range in the last refinement step
   bool sccDone = false;
   // SCC cycles start from the input charges the previous cycle ended with (the reference's
   // geometry loop keeps qInput between geometry steps, main.F90:219-357) instead of the
@@ -1692,7 +1696,9 @@ static int oa_refine_sparse(Context& c, int steps, std::vector<double>& lamHost)
   DH_TRY(c.tM.alloc(nn));
   DH_TRY(c.tG.alloc(nn));
   DH_TRY(c.tLam.alloc(n));
+  DH_TRY(c.tClip.alloc(1));
   for (int st = 0; st < std::max(steps, 1); ++st) {
+    DH_TRY(fill<int>(s, c.tClip.p, 1, 0));
     const double* Xt = c.tXt[c.tCur].p;
     double* Xn = c.tXt[1 - c.tCur].p;
     DH_TRY(launch_spmm_hs_t(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, c.dblk.start.p,
@@ -1702,7 +1708,7 @@ static int oa_refine_sparse(Context& c, int steps, std::vector<double>& lamHost)
     DH_TRY(symmetrize_lower(s, c.tM.p, n, n));
     DH_TRY(symmetrize_lower(s, c.tG.p, n, n));
     double* Et = c.tYH.p;   // free once M is formed
-    DH_TRY(oa_correction_t(s, c.tM.p, c.tG.p, n, c.tLam.p, Et));
+    DH_TRY(oa_correction_t(s, c.tM.p, c.tG.p, n, c.tLam.p, Et, c.tClip.p));
     DH_TRY(copy2d<double>(s, Xt, n, Xn, n, n, n));
     DH_TRY(gemm<double>(s, false, false, n, n, n, 1.0, Et, n, Xt, n, 1.0, Xn, n));
     c.tCur = 1 - c.tCur;
@@ -1710,6 +1716,7 @@ static int oa_refine_sparse(Context& c, int steps, std::vector<double>& lamHost)
   DH_TRY((transpose2d<double, double>(s, c.tXt[c.tCur].p, n, c.X.p, n, n, n)));
   lamHost.resize(n);
   DH_HIP(hipMemcpyAsync(lamHost.data(), c.tLam.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DH_HIP(hipMemcpyAsync(&c.lastClipped, c.tClip.p, sizeof(int), hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
   return 0;
 }
@@ -1731,6 +1738,8 @@ static int ctx_scc_begin(Context& c) {
   st.conv = false;
   st.err = 0.0;
   st.prevErr = HUGE_VAL;
+  st.eElec = 0.0;
+  st.eElecPrev = std::numeric_limits<double>::quiet_NaN();
   st.active = true;
   c.sccDone = false;
   return 0;
@@ -1764,7 +1773,12 @@ static int ctx_scc_step(Context& c) {
   // tail refinement: the previous iteration's n refined eigenpairs are refined against the new
   // H by one Ogita-Aishima step, without an eigensolve
   const bool keepAll = o.tail_below > 0.0;
-  const bool tail = refined && keepAll && c.xFull && c.nk == 0 && st.prevErr <= o.tail_below;
+  // the iteration that starts the tail (the first one below refine_below) keeps all n FP64
+  // eigenvectors: refined FP32 ones (refine_steps > 0) or those of a plain FP64 iteration
+  // (refine_steps = 0, the more accurate seed: the FP32 tridiagonalisation mixes levels closer
+  // than its error, which the refinement cannot separate)
+  const bool seedAll = keepAll && below && c.nk == 0;
+  const bool tail = below && keepAll && c.xFull && c.nk == 0 && st.prevErr <= o.tail_below;
   if (c.nk > 0) {
     // k-points: per k H(k) (real n x n at time-reversal-invariant points, else the 2n
     // embedding), eigenpairs and all eigenvectors; then the k-weighted Fermi level and the
@@ -1929,7 +1943,7 @@ static int ctx_scc_step(Context& c) {
       DH_HIP(hipEventRecord(ev[2], s));
       DH_HIP(hipEventRecord(ev[6], s));
       std::vector<double> lam;
-      DH_TRY(oa_refine_sparse(c, 1, lam));
+      DH_TRY(oa_refine_sparse(c, std::max(o.tail_steps, 1), lam));
       DH_HIP(hipEventRecord(ev[7], s));
       const double th3 = now();
       std::vector<int> perm(n);
@@ -2014,7 +2028,7 @@ static int ctx_scc_step(Context& c) {
     } else if (mixed) {
       DH_TRY(c.eig.vectors_mixed(s, nocc, X32, n));
     } else {
-      DH_TRY(c.eig.vectors(s, nocc, c.X.p, n));
+      DH_TRY(c.eig.vectors(s, seedAll ? n : nocc, c.X.p, n));
     }
     c.nocc = nocc;
     DH_HIP(hipEventRecord(ev[3], s));
@@ -2033,7 +2047,12 @@ static int ctx_scc_step(Context& c) {
                                  c.rho.p));
     else
       DH_TRY(launch_pack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, P32, n, c.rho.p));
-    c.xFull = refined && keepAll;
+    c.xFull = seedAll && (refined || !mixed);
+    if (c.xFull && !mixed) {
+      // the plain FP64 seed: its vectors transposed for the tail's block-sparse products
+      DH_TRY(c.tXt[c.tCur].alloc((size_t)n * n));
+      DH_TRY((transpose2d<double, double>(s, c.X.p, n, c.tXt[c.tCur].p, n, n, n)));
+    }
     }
   }
   DH_TRY(launch_mulliken(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.h0.p, c.rho.p,
@@ -2081,6 +2100,10 @@ static int ctx_scc_step(Context& c) {
   c.dqout = dqo;
   st.err = err;
   st.prevErr = err;
+  st.eElecPrev = st.it > 1 ? st.eElec : std::numeric_limits<double>::quiet_NaN();
+  st.eElec = st.eNonScc + st.eScc;
+  st.qinSum = 0.0;
+  for (int a = 0; a < nat; ++a) st.qinSum += st.qin[a];
   st.conv = err < o.scc_tol && st.it >= o.min_scc_iter;
   const double th3 = now();
   if (!st.conv && st.it != o.max_scc_iter) st.mixer.mix(st.qin, diff);
@@ -2119,6 +2142,17 @@ static int ctx_scc_end(Context& c, double* energies, double* charges, int* niter
   if (converged) *converged = st.conv ? 1 : 0;
   st.active = false;
   ++c.ncalls;
+  return 0;
+}
+
+// out[0..5]: iteration number, charge error, electronic energy of the last iteration and of
+// the one before (NaN after the first), input electrons of the last iteration, pairs outside
+// the first-order range in the last Ogita-Aishima step of the sparse refinement
+static int ctx_scc_state(Context& c, double* out, int n) {
+  const SccState& st = c.st;
+  const double v[6] = {(double)st.it, st.err, st.eElec, st.eElecPrev, st.qinSum,
+                       (double)c.lastClipped};
+  for (int k = 0; k < std::min(n, 6); ++k) out[k] = v[k];
   return 0;
 }
 
@@ -3041,6 +3075,11 @@ extern "C" int dftbhip_ctx_scc_end(dftbhip_ctx* ctx, double* energies, double* c
   return ctx_scc_end(ctx->c, energies, charges, niter, converged);
 }
 
+extern "C" int dftbhip_ctx_scc_state(dftbhip_ctx* ctx, double* out, int n) {
+  DH_CHECK(ctx && out && n >= 0, "scc_state: bad arguments");
+  return ctx_scc_state(ctx->c, out, n);
+}
+
 extern "C" int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad) {
   DH_CHECK(ctx && grad, "gradients: null pointer");
   return ctx_gradients(ctx->c, grad);
@@ -3315,6 +3354,10 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
       DH_CHECK(value >= 0 && value <= 2, "set_tuning: parameter 4 takes 0, 1 or 2");
       dftbhip::g_batchScc = value;
       return 0;
+    case 5:
+      return dftbhip::set_gemm_order(value);
+    case 6:
+      return dftbhip::set_oa_mode(value);
     default:
       dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));
       return dftbhip::kErrArg;

@@ -154,6 +154,9 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
          const T* B, long ldb, T beta, T* C, long ldc, bool lowerOut = false,
          GemmKLimit klim = kKFull, const T* w = nullptr, int splitK = 1);
 
+// tile order of full-output GEMMs: 0 the launch grid's, 1 XCD-grouped (the default)
+int set_gemm_order(int order);
+
 // Batched GEMM over device-resident descriptors; maxM/maxN bound the tile grid.
 template <typename T>
 int gemm_batched(hipStream_t s, bool ta, bool tb, const GemmDesc<T>* dDesc, int nBatch, int maxM,

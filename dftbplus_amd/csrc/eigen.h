@@ -10,9 +10,13 @@ namespace dftbhip {
 
 // Ogita-Aishima correction of n eigenpairs, transposed: lam_i = M_ii / G_ii and
 // Et(j, i) = E(i, j) (E as in EigenSolver::refine) from the symmetric M = X^T H X and
-// G = X^T S X (both triangles stored, leading dimension n)
+// G = X^T S X (both triangles stored, leading dimension n); nClipped (device, may be null)
+// counts the pairs outside the first-order range
 int oa_correction_t(hipStream_t s, const double* M, const double* G, int n, double* lam,
-                    double* Et);
+                    double* Et, int* nClipped = nullptr);
+// pairs outside the first-order range: 0 re-orthonormalised only (the default), 1 rotated by
+// the angle of the exact 2x2 problem
+int set_oa_mode(int mode);
 
 struct EigenSolver {
   int n = 0;

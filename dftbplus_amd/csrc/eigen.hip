@@ -160,8 +160,12 @@ __global__ void oa_correction_kernel(const double* M, const double* G, const dou
 // The same correction written transposed, Et(j, i) = E(i, j), from the symmetric M and G (both
 // triangles stored): thread (row j, column i) reads M and G at (j, i) = (i, j), so reads and
 // writes run along columns. lam is formed first by oa_lambda_kernel.
+// jacobi = 1: a pair outside the first-order range (near-degenerate against its coupling) is
+// rotated by the angle of the exact 2x2 problem, theta = atan(2 M_ij / (lam_j - lam_i)) / 2
+// (|theta| <= pi/4, equal to the first-order E_ij for a weak coupling), instead of only being
+// re-orthonormalised
 __global__ void oa_correction_t_kernel(const double* M, const double* G, const double* lam, int n,
-                                       double* Et) {
+                                       double* Et, int jacobi, int* nClipped) {
   const int jr = blockIdx.x * blockDim.x + threadIdx.x;   // row of Et: the vector index j of E
   const int ic = blockIdx.y;                               // column of Et: the index i of E
   if (jr >= n || ic >= n) return;
@@ -172,18 +176,37 @@ __global__ void oa_correction_t_kernel(const double* M, const double* G, const d
   } else {
     const double r = -G[q];
     const double num = M[q] + lam[jr] * r, den = lam[jr] - lam[ic];
-    e = (std::fabs(num) < kOaMax * std::fabs(den)) ? num / den : 0.5 * r;
+    if (std::fabs(num) < kOaMax * std::fabs(den)) {
+      e = num / den;
+    } else if (nClipped && jr > ic) {
+      atomicAdd(nClipped, 1);   // each pair once
+      e = jacobi ? 0.5 * r + sin(den != 0.0 ? 0.5 * atan(2.0 * M[q] / den)
+                                           : copysign(0.25 * M_PI, M[q]))
+                 : 0.5 * r;
+    } else if (jacobi) {
+      const double th = den != 0.0 ? 0.5 * atan(2.0 * M[q] / den) : copysign(0.25 * M_PI, M[q]);
+      e = 0.5 * r + sin(th);
+    } else {
+      e = 0.5 * r;
+    }
   }
   Et[q] = e;
 }
 
+static int g_oaJacobi = 0;
+int set_oa_mode(int mode) {
+  DH_CHECK(mode == 0 || mode == 1, "oa mode: 0 or 1");
+  g_oaJacobi = mode;
+  return 0;
+}
+
 int oa_correction_t(hipStream_t s, const double* M, const double* G, int n, double* lam,
-                    double* Et) {
+                    double* Et, int* nClipped) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(oa_lambda_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, M, G, n, lam);
   DH_LAUNCH_CHECK();
   hipLaunchKernelGGL(oa_correction_t_kernel, dim3(cdiv(n, 256), n), dim3(256), 0, s, M, G, lam, n,
-                     Et);
+                     Et, g_oaJacobi, nClipped);
   DH_LAUNCH_CHECK();
   return 0;
 }

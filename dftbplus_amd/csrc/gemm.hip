@@ -16,6 +16,14 @@ namespace dftbhip {
 
 constexpr int kBM = 128, kBN = 128, kBK = 16, kNT = 256, kPad = 1;
 
+// tile order of full-output GEMMs: 0 the launch grid's (x = tile row), 1 XCD-grouped
+static int g_gemmOrder = 1;
+int set_gemm_order(int order) {
+  DH_CHECK(order == 0 || order == 1, "gemm order: 0 or 1");
+  g_gemmOrder = order;
+  return 0;
+}
+
 template <typename T, bool TA, bool TB>
 __device__ __forceinline__ void gemm_tile(int tm, int tn, int M, int N, int K, T alpha,
                                           const T* __restrict__ A, long lda,
@@ -178,13 +186,37 @@ __device__ __forceinline__ void lower_tile_index(int x, int& tm, int& tn) {
   tn = x - t * (t + 1) / 2;
 }
 
+// XCD-grouped tile order of a full output: workgroup IDs are dealt to the 8 XCDs round-robin,
+// so the IDs of one XCD (congruent mod 8) are given a contiguous range of a tile sequence that
+// walks groups of kGroupRows tile rows column by column. The tiles an XCD runs at one time then
+// share a few A row panels and B column panels in its L2, instead of every tile of the chip
+// reading its own pair of panels. nPad = tiles rounded up to a multiple of 8; IDs past the last
+// tile return.
+constexpr int kXcds = 8, kGroupRows = 16;
+__device__ __forceinline__ bool grouped_tile_index(int w, int gm, int gn, int nPad, int& tm,
+                                                   int& tn) {
+  const int per = nPad / kXcds;
+  const int o = (w % kXcds) * per + w / kXcds;
+  if (o >= gm * gn) return false;
+  const int band = gn * kGroupRows;
+  const int g = o / band, rem = o % band;
+  const int rows = min(kGroupRows, gm - g * kGroupRows);
+  tn = rem / rows;
+  tm = g * kGroupRows + rem % rows;
+  return true;
+}
+
 template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(int M, int N, int K, T alpha, const T* A,
                                                    long lda, const T* B, long ldb, T beta, T* C,
-                                                   long ldc, int lowerOut, int klim, const T* w) {
+                                                   long ldc, int lowerOut, int klim, const T* w,
+                                                   int grouped) {
   int tm, tn;
   if (lowerOut) {
     lower_tile_index(blockIdx.x, tm, tn);
+  } else if (grouped) {
+    const int gm = (M + kBM - 1) / kBM, gn = (N + kBN - 1) / kBN;
+    if (!grouped_tile_index((int)blockIdx.x, gm, gn, (int)gridDim.x, tm, tn)) return;
   } else {
     tm = blockIdx.x;
     tn = blockIdx.y;
@@ -216,15 +248,19 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
   }
   splitK = std::max(splitK, 1);
   dim3 grid;
+  const bool grouped = !lowerOut && g_gemmOrder == 1;
   if (lowerOut) {
     const long t = cdiv(M, kBM);
     grid = dim3((unsigned)(t * (t + 1) / 2), 1, 1);
+  } else if (grouped) {
+    const long tiles = (long)cdiv(M, kBM) * cdiv(N, kBN);
+    grid = dim3((unsigned)((tiles + kXcds - 1) / kXcds * kXcds), 1, splitK);
   } else {
     grid = dim3(cdiv(M, kBM), cdiv(N, kBN), splitK);
   }
 #define DH_GEMM_CASE(TA_, TB_)                                                           \
   hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(kNT), 0, s, M, N, K, alpha, A, \
-                     lda, B, ldb, beta, C, ldc, (int)lowerOut, (int)klim, w)
+                     lda, B, ldb, beta, C, ldc, (int)lowerOut, (int)klim, w, (int)grouped)
   if (!ta && !tb) DH_GEMM_CASE(false, false);
   else if (!ta && tb) DH_GEMM_CASE(false, true);
   else if (ta && !tb) DH_GEMM_CASE(true, false);

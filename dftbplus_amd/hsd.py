@@ -27,7 +27,7 @@ from typing import Dict, List, Optional, Tuple
 import numpy as np
 
 from . import skf as skfmod
-from . import tagged
+from . import outfiles, tagged
 from .model import Geometry, Options, Parameters, Species, reference_charges
 
 # src/dftbp/common/constants.F90:28-40
@@ -354,6 +354,8 @@ class Input:
     forces: bool = False
     autotest_tag: bool = False
     results_tag: bool = False
+    detailed_out: bool = True      # Options { WriteDetailedOut }, default Yes
+    band_out: bool = True          # Options { WriteBandOut }, default Yes
 
 
 _UNSUPPORTED = ["SpinPolarisation", "SpinOrbit", "Dispersion", "ThirdOrder", "ThirdOrderFull",
@@ -557,7 +559,9 @@ def load_input(path: str) -> Input:
     return Input(geo, params, opts,
                  forces=_yes(analysis.child("CalculateForces") if analysis else None, False),
                  autotest_tag=_yes(options.child("WriteAutotestTag") if options else None, False),
-                 results_tag=_yes(options.child("WriteResultsTag") if options else None, False))
+                 results_tag=_yes(options.child("WriteResultsTag") if options else None, False),
+                 detailed_out=_yes(options.child("WriteDetailedOut") if options else None, True),
+                 band_out=_yes(options.child("WriteBandOut") if options else None, True))
 
 
 # ------------------------------------------------------------------------------------------
@@ -582,6 +586,16 @@ def run(path: str, out_dir: Optional[str] = None, **backend) -> dict:
         tagged.write_tags(os.path.join(out_dir, "autotest.tag"), autotest_items(res, geo))
     if inp.results_tag:
         tagged.write_tags(os.path.join(out_dir, "results.tag"), results_items(res, geo))
+    if inp.band_out:
+        nk = res["eigenvalues"].shape[1]
+        kw = np.ones(nk) if inp.options.kweights is None else np.asarray(inp.options.kweights)
+        outfiles.write_band_out(os.path.join(out_dir, "band.out"), res["eigenvalues"],
+                                res["filling"], kw / kw.sum())
+    if inp.detailed_out:
+        st = res["scc_state"]
+        with open(os.path.join(out_dir, "detailed.out"), "w") as fh:
+            fh.write(outfiles.detailed_out(res, geo, inp.params, st["iteration"],
+                                           st["diff_elec"], st["scc_error"], st["q_input_sum"]))
     return res
 
 
@@ -590,6 +604,7 @@ def _results(dp, inp: Input, geo: Geometry) -> dict:
     en = dict(dp.energies)
     res = {"mermin_energy": e, "energies": en, "converged": dp.converged, "n_iter": dp.n_iter,
            "n_electrons": float(reference_charges(inp.params, geo.species).sum()),
+           "scc_state": dp.scc_state(),
            "gross_charges": dp.get_gross_charges(), "orbital_populations":
            dp.get_orbital_populations(), "norb_atom":
            np.array([inp.params.species[s].norb for s in geo.species])}

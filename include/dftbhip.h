@@ -64,7 +64,11 @@ int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e, doub
  * the default) or on the host (0); param 4: the molecule batch's per-molecule Fermi levels,
  * energies, convergence tests and Broyden mixing run on host threads (0), as device kernels
  * (1), or as device kernels with every warm SCC iteration replayed from one captured HIP
- * graph (2, the default). No counterpart in the reference. */
+ * graph (2, the default); param 5: tile order of full-output GEMMs, 0 the launch grid's or 1
+ * XCD-grouped (the default: the tiles an XCD runs together share their panels in its L2);
+ * param 6: Ogita-Aishima refinement of the sparse tail, pairs outside the first-order range
+ * re-orthonormalised only (0, the default) or rotated by the exact 2x2 angle (1). No
+ * counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)
  * builds it: atom a lists itself, then the images of atoms b >= a within cutoff by increasing
@@ -190,6 +194,9 @@ typedef struct {
    * FP64 eigenpairs against the new H (late in the SCC cycle H changes by much less than the
    * level spacing from one iteration to the next) */
   double tail_below;
+  /* Ogita-Aishima steps per tail iteration (0 is taken as 1); more steps keep the eigenpairs
+   * closer to the new H, which the SCC convergence of large systems with dense spectra needs */
+  int tail_steps;
 } dftbhip_options;
 
 typedef struct dftbhip_ctx dftbhip_ctx;
@@ -220,6 +227,13 @@ int dftbhip_ctx_set_charge_restart(dftbhip_ctx* ctx, int warm);
 int dftbhip_ctx_scc_step(dftbhip_ctx* ctx, int* converged, double* qerr);
 int dftbhip_ctx_scc_end(dftbhip_ctx* ctx, double* energies, double* charges, int* niter,
                         int* converged);
+/* State of the last SCC iteration for the SCC block of detailed.out (writeDetailedOut1,
+ * src/dftbp/dftbplus/mainio.F90:2794-2926): out[0..n-1] of iteration number, charge error
+ * (sccErrorQ), electronic energy, electronic energy of the iteration before (NaN after the
+ * first; diffElec is their difference), input electrons of the last iteration and the number
+ * of eigenvector pairs outside the first-order range of the last Ogita-Aishima refinement
+ * step of the mixed-precision tail (no counterpart in the reference). */
+int dftbhip_ctx_scc_state(dftbhip_ctx* ctx, double* out, int n);
 /* gradient dE/dR (3, nAtom); needs a converged dftbhip_ctx_scc */
 int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad);
 int dftbhip_ctx_eigenvalues(dftbhip_ctx* ctx, double* eig, double* fill, int n);

@@ -1004,7 +1004,9 @@ class Result:
 class Calculator:
     """Serial CPU restatement of one SCC-DFTB energy + gradient evaluation."""
 
-    def __init__(self, params, geo, opts):
+    def __init__(self, params, geo, opts, gamma: Optional[np.ndarray] = None):
+        """gamma: a precomputed atomic gamma matrix to use instead of building it (the CPU timing
+        of an SCC iteration, whose cost does not depend on its values)."""
         self.params, self.geo, self.opts = params, geo, opts
         sp = geo.species
         self.norb_atom = np.array([params.species[s].norb for s in sp])
@@ -1018,7 +1020,8 @@ class Calculator:
         self.nl = neighbour_list(geo.coords, geo.latvecs, self.cutoff)
         self.sp = sparse_descriptor(self.nl, self.norb_atom, params.sk_cutoff()).index_maps(self.off)
         self.h0, self.s = build_h0_s(params, sp, self.nl, self.sp)
-        self.gamma = gamma_matrix(params, sp, self.nl, geo.coords, geo.latvecs, opts.ewald_tol)
+        self.gamma = (gamma_matrix(params, sp, self.nl, geo.coords, geo.latvecs, opts.ewald_tol)
+                      if gamma is None else gamma)
         self.q0 = np.array([sum(params.species[s].occ) for s in sp])
         self.nel = float(self.q0.sum())
         self.erep, self.grad_rep = repulsive(params, sp, self.nl)
