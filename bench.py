@@ -16,12 +16,13 @@ gradients is the leg's force evaluation. Legs:
   mixed_then_fp64  FP32 MFMA reduction, tridiagonalisation, back-transformation and density
                    matrices while the previous iteration's charge error is above 1e-3, plain
                    FP64 iterations after (the converged point is the FP64 one);
-  mixed_tail       FP32 iterations above 1e-3, then one iteration whose FP32 eigenpairs are
-                   refined by two Ogita-Aishima steps, and after it no eigensolve: one step
-                   refines the previous iteration's FP64 eigenpairs against the new sparse H
-                   and S (block-sparse products, three n^3 GEMMs);
-  mixed_tail_late  as mixed_tail, with the refined-seed iterations down to 1e-5 (only with
-                   --legs);
+  mixed_tail       FP32 iterations above 1e-3, then one plain FP64 iteration that keeps all n
+                   eigenvectors, and after it no eigensolve: two Ogita-Aishima steps refine the
+                   previous iteration's FP64 eigenpairs against the new sparse H and S
+                   (block-sparse products, two n^3 FP64 GEMMs for X^T H X and X^T S X, the
+                   correction product X E in FP32);
+  mixed_tail_fp32seed  as mixed_tail, seeded by FP32 eigenpairs refined by two steps (only
+                   with --legs);
   mixed_fp32       FP32 iterations throughout (only with --legs; does not reach 1e-8 e).
 `value` is the SCC iterations/s of the fp64 leg's timed steps; every leg reports its timed
 rate, its converged cycle (iterations, seconds, force evaluations/s) and its energy, charge and
@@ -58,8 +59,10 @@ LEGS = {"fp64": dict(precision=0),
         "mixed_then_fp64": dict(precision=1, refine_below=1e-3, refine_steps=0),
         "mixed_fp32": dict(precision=1, refine_below=0.0),
         "mixed_then_refined": dict(precision=1, refine_below=1e-3, refine_steps=1),
-        "mixed_tail": dict(precision=1, refine_below=1e-3, refine_steps=2, tail_below=1e-3),
-        "mixed_tail_late": dict(precision=1, refine_below=1e-3, refine_steps=2, tail_below=1e-5)}
+        "mixed_tail": dict(precision=1, refine_below=1e-3, refine_steps=0, tail_below=1e-3,
+                           tail_steps=2),
+        "mixed_tail_fp32seed": dict(precision=1, refine_below=1e-3, refine_steps=2,
+                                    tail_below=1e-3, tail_steps=2)}
 E_BOUND, F_BOUND = 1e-6, 1e-5
 
 
@@ -199,6 +202,39 @@ def run_cycle_leg(api, geo, params, opts, leg, steps, warmup):
         "symv": {"launches": int(launches), "seconds": ksec, "bytes": kbytes,
                  "achieved": achieved},
     }
+
+
+def mode_tail(args):
+    """The mixed-precision tail on its own, for kernel traces: one plain FP64 iteration seeds it
+    and every later iteration is one Ogita-Aishima step on the sparse H and S (the cycle is not
+    meant to converge from neutral charges this way); prints the iteration times and the sizes
+    that fix the flops of the step."""
+    import math
+    from dftbplus_amd import api, synth
+    geo, params, opts = synth.workload(args.workload)
+    dp = api.DftbPlus(params, geo, opts, precision=1, refine_below=math.inf, refine_steps=0,
+                      tail_below=math.inf, tail_steps=2)
+    n = dp.get_nr_orbitals()
+    dp.scc_begin()
+    times = []
+    # the seed, then --steps tail iterations for each vector-tile width of the block-sparse
+    # products (in the kernel trace the spmm_hs_t_kernel<w> launches follow this order)
+    plan = [None] + [w for w in (64, 128, 256) for _ in range(args.steps)]
+    for it, cols in enumerate(plan):
+        if cols:
+            api.set_tuning(9, cols)
+        t0 = time.perf_counter()
+        dp.scc_step()
+        times.append((cols, time.perf_counter() - t0))
+        print(f"[tail] iteration {it + 1} ({'FP64 seed' if cols is None else f'tail, {cols}-vector tiles'}): "
+              f"{times[-1][1]:.3f}s", file=sys.stderr, flush=True)
+    api.set_tuning(9, 256)
+    dp.scc_end()
+    dp.close()
+    print(json.dumps({"mode": "tail", "workload": args.workload, "orbitals": n,
+                      "seed_s": times[0][1], "tail_s": {str(c): [t for cc, t in times if cc == c]
+                                                         for c in (64, 128, 256)}}),
+          flush=True)
 
 
 def mode_cpu(args):
@@ -539,10 +575,12 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="diamond8")
-    ap.add_argument("--legs", default="fp64,mixed_then_fp64,mixed_tail",
+    ap.add_argument("--legs", default="fp64,mixed_tail",
                     help="comma-separated subset of " + ",".join(LEGS))
-    ap.add_argument("--mix", type=float, default=None,
-                    help="Broyden mixing parameter (default: the workload's)")
+    ap.add_argument("--mix", type=float, default=0.05,
+                    help="Broyden mixing parameter (default 0.05: at 0.2 the charge of the "
+                         "4096-atom cell sloshes and the cycle does not converge in 100 "
+                         "iterations; 0 takes the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", default="diamond5",
                     help="workload of the bounded CPU-baseline sample")
@@ -552,7 +590,7 @@ def main():
                     help="keep the fixed SYMV tile thresholds instead of autotuning them")
     ap.add_argument("--mode", default="scc",
                     choices=["scc", "sytrd", "batch", "kpoints", "autograd", "neighbours",
-                             "kernels", "cpu"],
+                             "kernels", "cpu", "tail"],
                     help="scc: the headline SCC benchmark (default); sytrd: the tridiagonalisation "
                          "alone on a random matrix (SYMV counter runs); batch: the molecule batch "
                          "against a sampled loop of single-molecule runs; kpoints: SCC iterations "
@@ -562,7 +600,8 @@ def main():
                          "with the host-built list; kernels: one SCC iteration, forces and a "
                          "GEMM of size --n with the sizes that fix their bytes and flops (for "
                          "kernel traces and counter runs); cpu: the CPU baseline alone, at the "
-                         "full size and on the bounded sample")
+                         "full size and on the bounded sample; tail: one FP64 iteration and "
+                         "--steps iterations of the mixed-precision tail (kernel traces)")
     ap.add_argument("--n", type=int, default=8192,
                     help="matrix size of --mode sytrd and of the GEMM of --mode kernels")
     ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
@@ -586,12 +625,12 @@ def main():
     if args.mode != "scc":
         {"sytrd": mode_sytrd, "batch": mode_batch, "kpoints": mode_kpoints,
          "neighbours": mode_neighbours, "kernels": mode_kernels,
-         "autograd": mode_autograd, "cpu": mode_cpu}[args.mode](args)
+         "autograd": mode_autograd, "cpu": mode_cpu, "tail": mode_tail}[args.mode](args)
         return
     import dataclasses
     geo, params, opts = synth.workload(args.workload)
-    if args.mix is not None:
-        opts = dataclasses.replace(opts, mix_param=args.mix)
+    if args.mix:
+        opts = dataclasses.replace(opts, mix_param=args.mix, max_scc_iter=300)
     legs = [x for x in args.legs.split(",") if x]
     if "fp64" not in legs:
         legs.insert(0, "fp64")   # the reference leg of the error comparison

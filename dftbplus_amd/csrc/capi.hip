@@ -600,8 +600,8 @@ struct DevBlocks {
   }
   // row view of the blocks for the block-sparse products (built on first use per geometry):
   // a counting sort of the block list by row atom, on the host
-  DevBuf<int> rowPtr;
-  DevBuf<int2> rowEnt;
+  DevBuf<int> rowPtr, gPtr;
+  DevBuf<int2> rowEnt, rowEntS, grp;
   RowBlocksDev rows{};
   bool rowsValid = false;
   int build_rows(int nat, hipStream_t s) {
@@ -627,10 +627,29 @@ struct DevBlocks {
       ent[fillPos[h2[k]]++] = make_int2(2 * k, h1[k]);
       if (h1[k] != h2[k]) ent[fillPos[h1[k]]++] = make_int2(2 * k + 1, h2[k]);
     }
+    // a copy with each row's entries by column atom (block order kept within one), and the
+    // groups of entries with one column atom in it; the products keep the block order, in which
+    // the measured block-sparse product runs more than twice as fast
+    std::vector<int2> ents(ent);
+    std::vector<int> gp(nat + 1, 0);
+    std::vector<int2> gr;
+    gr.reserve(ptr[nat] + 1);
+    for (int a = 0; a < nat; ++a) {
+      std::stable_sort(ents.begin() + ptr[a], ents.begin() + ptr[a + 1],
+                       [](const int2& x, const int2& y) { return x.y < y.y; });
+      for (int p = ptr[a]; p < ptr[a + 1]; ++p)
+        if (p == ptr[a] || ents[p].y != ents[p - 1].y) gr.push_back(make_int2(p, p + 1));
+        else gr.back().y = p + 1;
+      gp[a + 1] = (int)gr.size();
+    }
+    if (gr.empty()) gr.push_back(make_int2(0, 0));
     DH_TRY(rowPtr.upload(ptr.data(), ptr.size(), s));
     DH_TRY(rowEnt.upload(ent.data(), ent.size(), s));
+    DH_TRY(rowEntS.upload(ents.data(), ents.size(), s));
+    DH_TRY(gPtr.upload(gp.data(), gp.size(), s));
+    DH_TRY(grp.upload(gr.data(), gr.size(), s));
     DH_HIP(hipStreamSynchronize(s));
-    rows = RowBlocksDev{nat, rowPtr.p, rowEnt.p};
+    rows = RowBlocksDev{nat, rowPtr.p, rowEnt.p, rowEntS.p, gPtr.p, grp.p};
     rowsValid = true;
     return 0;
   }
@@ -710,7 +729,9 @@ extern "C" int dftbhip_unpack_hs_real(int nAtom, const int* iAtomStart, const in
   DevBuf<double> x, D;
   DH_TRY(x.upload(orig, sparseSize, s));
   DH_TRY(D.alloc((size_t)n * n));
-  DH_TRY(launch_unpack<double>(s, db.pr, db.norb.p, db.atomStart.p, x.p, D.p, n, n));
+  DH_TRY(db.build_rows(nAtom, s));
+  DH_TRY(launch_unpack_rows<double>(s, db.rows, db.norb.p, db.atomStart.p, db.start.p, x.p, D.p,
+                                    n, n));
   DH_HIP(hipMemcpy2DAsync(square, ldsq * sizeof(double), D.p, n * sizeof(double),
                           n * sizeof(double), n, hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
@@ -778,8 +799,9 @@ extern "C" int dftbhip_mulliken(int nAtom, const int* iAtomStart, const int* img
   DH_TRY(ds.upload(over, sparseSize, s));
   DH_TRY(dr.upload(rho, sparseSize, s));
   DH_TRY(q.alloc(n));
-  DH_TRY(launch_mulliken(s, db.pr, db.norb.p, db.atomStart.p, ds.p, nullptr, dr.p, q.p, nullptr,
-                         n));
+  DH_TRY(db.build_rows(nAtom, s));
+  DH_TRY(launch_mulliken_rows(s, db.rows, db.norb.p, db.atomStart.p, db.start.p, ds.p, nullptr,
+                              dr.p, q.p, nullptr));
   DH_HIP(hipMemcpyAsync(qorb, q.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
   return 0;
@@ -1373,6 +1395,12 @@ static int g_kBatch = 1;
 // neighbour list, pairs and SK blocks of a geometry built on the device (1, the default) or on
 // the host (0); set through dftbhip_set_tuning parameter 3
 static int g_devNeighbours = 1;
+// the Broyden history restarts at the iteration that seeds the mixed-precision tail (1) or is
+// kept (0, the default); set through dftbhip_set_tuning parameter 7
+static int g_tailMixReset = 0;
+// the correction product X E of the sparse refinement in FP32 (1, the default) or FP64 (0); set
+// through dftbhip_set_tuning parameter 8
+static int g_tailUpdate32 = 1;
 
 struct Context {
   dftbhip_options opt{};
@@ -1436,6 +1464,11 @@ struct Context {
   DevBuf<double> tXt[2], tYH, tYS, tM, tG, tLam;
   DevBuf<int> tClip;
   int tCur = 0;
+  // the same per k-point (all time-reversal invariant): the transposed eigenvectors of point k
+  // in tXtK[2 k + tCurK[k]]
+  std::vector<DevBuf<double>> tXtK;
+  std::vector<int> tCurK;
+  bool xFullK = false;   // XK holds all n FP64 eigenvectors of every point (the tail's seed)
   int lastClipped = 0;   // pairs outside the first-order range in the last refinement step
   bool sccDone = false;
   // SCC cycles start from the input charges the previous cycle ended with (the reference's
@@ -1516,6 +1549,7 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
     DH_TRY(c.dnb.upload(c.nbp, c.species, c.norbAtom, c.atomStart, s));
     DH_TRY(c.coordsAll.upload(c.nl.coordsAll.data(), c.nl.coordsAll.size(), s));
   }
+  DH_TRY(c.dblk.build_rows(nat, s));   // row view of the blocks (Mulliken, tail products)
   c.nbdev.n = c.dnb.pr.nblk;
   c.nbdev.at1 = c.dnb.pr.at1;
   c.nbdev.img2 = c.dnb.pr.img2;
@@ -1615,7 +1649,8 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
   c.eig.twoStage = c.opt.tridiag == 0;
   int info = 0;
   if (c.nk == 0) {
-    DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.Sd.p,
+    DH_TRY(launch_unpack_rows<double>(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p,
+                                      c.dblk.start.p, c.ov.p, c.Sd.p,
                                  n, n));
     DH_TRY(c.eig.factor_overlap(s, c.Sd.p, n, &info));
   } else {
@@ -1642,9 +1677,14 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
     return nd + info;
   }
   if (c.opt.precision == 1) {
-    DH_CHECK(c.nk == 0, "set_geometry: k-points run in FP64 (precision 0)");
-    DH_TRY(c.eig.prepare_mixed(s));
-    DH_TRY(c.f32.alloc(n));
+    // k-points: the eigensolves stay FP64 (refine_steps = 0), the mixed form is the tail
+    DH_CHECK(c.nk == 0 || (c.opt.refine_steps == 0 && c.opt.tail_below > 0.0),
+             "set_geometry: with k-points the mixed precision runs FP64 eigensolves and the "
+             "tail only (refine_steps 0, tail_below > 0)");
+    if (c.nk == 0) {
+      DH_TRY(c.eig.prepare_mixed(s));
+      DH_TRY(c.f32.alloc(n));
+    }
   }
   if (c.nk > 0) DH_TRY(c.f.alloc(nd));
   DH_HIP(hipStreamSynchronize(s));
@@ -1679,18 +1719,20 @@ static int occupied_levels(const std::vector<double>& f) {
 }
 
 // Ogita-Aishima refinement (Ogita & Aishima 2018) of all n generalized eigenpairs held
-// transposed in c.tXt[c.tCur] (FP64, n x n), against the sparse H (c.hs) and S (c.ov) of the
-// current iteration. Per step: YH^T = (H X)^T and YS^T = (S X)^T by the block-sparse product
+// transposed in xt[cur] (FP64, n x n; c.tXt at the Gamma point, c.tXtK[k] at a
+// time-reversal-invariant k-point, whose matrices are the real sums of the blocks with the
+// phases +-1 of kf), against the sparse H (c.hs) and S (c.ov) of the current iteration. Per step: YH^T = (H X)^T and YS^T = (S X)^T by the block-sparse product
 // (H and S stay sparse: 2 nnz n flops instead of 2 n^3), M = X^T H X and G = X^T S X as
 // lower-output GEMMs (n^3 each), the correction E written transposed, and X <- X (I + E), i.e.
-// Xt <- Xt + E^T Xt (2 n^3). The refined vectors end in c.X (normal layout), the Rayleigh
+// Xt <- Xt + E^T Xt (2 n^3). The refined vectors end in Xout (normal layout), the Rayleigh
 // quotients of the last input basis (unsorted) in lamHost.
-static int oa_refine_sparse(Context& c, int steps, std::vector<double>& lamHost) {
+static int oa_refine_sparse_on(Context& c, int steps, DevBuf<double>* xt, int& cur, double* Xout,
+                               const double* kf, std::vector<double>& lamHost) {
   hipStream_t s = c.s;
   const int n = c.norbTot;
   const size_t nn = (size_t)n * n;
   DH_TRY(c.dblk.build_rows(c.nat, s));
-  DH_TRY(c.tXt[1 - c.tCur].alloc(nn));
+  DH_TRY(xt[1 - cur].alloc(nn));
   DH_TRY(c.tYH.alloc(nn));
   DH_TRY(c.tYS.alloc(nn));
   DH_TRY(c.tM.alloc(nn));
@@ -1699,26 +1741,44 @@ static int oa_refine_sparse(Context& c, int steps, std::vector<double>& lamHost)
   DH_TRY(c.tClip.alloc(1));
   for (int st = 0; st < std::max(steps, 1); ++st) {
     DH_TRY(fill<int>(s, c.tClip.p, 1, 0));
-    const double* Xt = c.tXt[c.tCur].p;
-    double* Xn = c.tXt[1 - c.tCur].p;
+    const double* Xt = xt[cur].p;
+    double* Xn = xt[1 - cur].p;
     DH_TRY(launch_spmm_hs_t(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, c.dblk.start.p,
-                            c.hs.p, c.ov.p, Xt, n, n, c.tYH.p, c.tYS.p, n));
+                            c.hs.p, c.ov.p, Xt, n, n, c.tYH.p, c.tYS.p, n,
+                            kf ? c.blkT.p : nullptr, kf));
     DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYH.p, n, 0.0, c.tM.p, n, true));
     DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYS.p, n, 0.0, c.tG.p, n, true));
     DH_TRY(symmetrize_lower(s, c.tM.p, n, n));
     DH_TRY(symmetrize_lower(s, c.tG.p, n, n));
     double* Et = c.tYH.p;   // free once M is formed
     DH_TRY(oa_correction_t(s, c.tM.p, c.tG.p, n, c.tLam.p, Et, c.tClip.p));
-    DH_TRY(copy2d<double>(s, Xt, n, Xn, n, n, n));
-    DH_TRY(gemm<double>(s, false, false, n, n, n, 1.0, Et, n, Xt, n, 1.0, Xn, n));
-    c.tCur = 1 - c.tCur;
+    if (g_tailUpdate32) {
+      // the correction product in FP32: its rounding is a relative 1e-7-level error of the
+      // correction, which the next step corrects like any other basis error (the fixed point
+      // stays that of the FP64 residuals M and G); FP32 operands in the free YS and M buffers
+      float* E32 = reinterpret_cast<float*>(c.tYS.p);
+      float* X32 = E32 + (size_t)n * n;
+      float* D32 = reinterpret_cast<float*>(c.tM.p);
+      DH_TRY((convert2d<double, float>(s, Et, n, E32, n, n, n)));
+      DH_TRY((convert2d<double, float>(s, Xt, n, X32, n, n, n)));
+      DH_TRY(gemm<float>(s, false, false, n, n, n, 1.0f, E32, n, X32, n, 0.0f, D32, n));
+      DH_TRY(add_f32(s, Xt, n, D32, n, Xn, n, n, n));
+    } else {
+      DH_TRY(copy2d<double>(s, Xt, n, Xn, n, n, n));
+      DH_TRY(gemm<double>(s, false, false, n, n, n, 1.0, Et, n, Xt, n, 1.0, Xn, n));
+    }
+    cur = 1 - cur;
   }
-  DH_TRY((transpose2d<double, double>(s, c.tXt[c.tCur].p, n, c.X.p, n, n, n)));
+  DH_TRY((transpose2d<double, double>(s, xt[cur].p, n, Xout, n, n, n)));
   lamHost.resize(n);
   DH_HIP(hipMemcpyAsync(lamHost.data(), c.tLam.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipMemcpyAsync(&c.lastClipped, c.tClip.p, sizeof(int), hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
   return 0;
+}
+
+static int oa_refine_sparse(Context& c, int steps, std::vector<double>& lamHost) {
+  return oa_refine_sparse_on(c, steps, c.tXt, c.tCur, c.X.p, nullptr, lamHost);
 }
 
 // Levels beyond the occupied ones kept in the refined density matrix: refined eigenvalues
@@ -1734,6 +1794,7 @@ static int ctx_scc_begin(Context& c) {
   if (!(c.warmStart && st.it > 0 && st.qin.size() == c.q0atom.size())) st.qin = c.q0atom;
   st.mixer.reset(c.nat, c.opt, c.opt.max_scc_iter + 1);
   c.xFull = false;
+  c.xFullK = false;
   st.it = 0;
   st.conv = false;
   st.err = 0.0;
@@ -1779,6 +1840,7 @@ static int ctx_scc_step(Context& c) {
   // than its error, which the refinement cannot separate)
   const bool seedAll = keepAll && below && c.nk == 0;
   const bool tail = below && keepAll && c.xFull && c.nk == 0 && st.prevErr <= o.tail_below;
+  bool refineTimed = refined || tail;   // events 6 and 7 bracket a refinement
   if (c.nk > 0) {
     // k-points: per k H(k) (real n x n at time-reversal-invariant points, else the 2n
     // embedding), eigenpairs and all eigenvectors; then the k-weighted Fermi level and the
@@ -1786,8 +1848,32 @@ static int ctx_scc_step(Context& c) {
     c.xf64 = true;
     DH_HIP(hipEventRecord(ev[1], s));
     std::vector<std::vector<double>> ec(c.nk);
-    bool together = g_kBatch && c.nk > 1 && !c.eig.twoStage;
-    for (int k = 0; k < c.nk && together; ++k) together = kpoint_dim(&c.kpts[3 * k], n) == n;
+    bool allReal = true;
+    for (int k = 0; k < c.nk; ++k) allReal = allReal && kpoint_dim(&c.kpts[3 * k], n) == n;
+    // the tail at k-points (every point time-reversal invariant, mixed precision): no
+    // eigensolve, Ogita-Aishima steps on each point's previous FP64 eigenpairs against its new
+    // sparse H(k) and S(k); the eigenvalues of a point in the order of its vectors
+    const bool ktail = keepAll && below && allReal && c.xFullK && st.prevErr <= o.tail_below;
+    std::vector<std::vector<int>> kperm;
+    if (ktail) {
+      kperm.resize(c.nk);
+      DH_HIP(hipEventRecord(ev[6], s));
+      for (int k = 0; k < c.nk; ++k) {
+        std::vector<double> lam;
+        DH_TRY(oa_refine_sparse_on(c, std::max(o.tail_steps, 1), &c.tXtK[2 * (size_t)k],
+                                   c.tCurK[k], c.XK[k].p, &c.kpts[3 * k], lam));
+        std::vector<int>& perm = kperm[k];
+        perm.resize(n);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return lam[x] < lam[y]; });
+        ec[k].resize(n);
+        for (int j = 0; j < n; ++j) ec[k][j] = lam[perm[j]];
+        c.eigK[k] = lam;
+      }
+      DH_HIP(hipEventRecord(ev[7], s));
+      refineTimed = true;
+    }
+    bool together = !ktail && g_kBatch && c.nk > 1 && !c.eig.twoStage && allReal;
     if (together) {
       // every point real and of size n: the reductions per point, one tridiagonalisation pass
       // over all points (one launch per column kernel for all of them), then per point the
@@ -1894,7 +1980,7 @@ static int ctx_scc_step(Context& c) {
       }
       for (int k = 0; k < K; ++k) ec[k] = c.eigK[k];
     }
-    for (int k = 0; k < c.nk && !together; ++k) {
+    for (int k = 0; k < c.nk && !together && !ktail; ++k) {
       const int ndk = kpoint_dim(&c.kpts[3 * k], n);
       DH_TRY(launch_unpack_kpt(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.blkT.p,
                                &c.kpts[3 * k], c.hs.p, c.Hd.p, ndk, n));
@@ -1912,6 +1998,14 @@ static int ctx_scc_step(Context& c) {
     DH_HIP(hipEventRecord(ev[2], s));
     const double th = now();
     DH_TRY(fermi_filling_k(ec, c.kw, st.nel, o.kT, c.fillK, st.fr));
+    if (ktail) {
+      // fillings back into the order of the vectors
+      for (int k = 0; k < c.nk; ++k) {
+        std::vector<double> fk(n);
+        for (int j = 0; j < n; ++j) fk[kperm[k][j]] = c.fillK[k][j];
+        c.fillK[k].swap(fk);
+      }
+    }
     c.times[kTHost] += now() - th;
     DH_HIP(hipEventRecord(ev[3], s));
     DH_TRY(fill<double>(s, c.rho.p, c.nnz, 0.0));
@@ -1930,6 +2024,17 @@ static int ctx_scc_step(Context& c) {
     }
     c.eigv.clear();
     DH_HIP(hipEventRecord(ev[4], s));
+    // an FP64 iteration below the switch point seeds the k-point tail: each point's n vectors
+    // transposed for the block-sparse products
+    if (keepAll && below && allReal && !ktail) {
+      if (c.tXtK.size() < 2 * (size_t)c.nk) c.tXtK.resize(2 * (size_t)c.nk);
+      c.tCurK.assign(c.nk, 0);
+      for (int k = 0; k < c.nk; ++k) {
+        DH_TRY(c.tXtK[2 * (size_t)k].alloc((size_t)n * n));
+        DH_TRY((transpose2d<double, double>(s, c.XK[k].p, n, c.tXtK[2 * (size_t)k].p, n, n, n)));
+      }
+      c.xFullK = true;
+    }
   } else {
     // FP64: the plain path. Mixed: FP32 reduction, tridiagonalisation, back-transformation and
     // density matrix. Mixed + refined: the FP32 eigenvectors of all levels are refined in FP64
@@ -1969,10 +2074,12 @@ static int ctx_scc_step(Context& c) {
       DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
                                  c.rho.p));
     } else if (mixed && !refined) {
-      DH_TRY(launch_unpack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, H32, n,
+      DH_TRY(launch_unpack_rows<float>(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p,
+                                       c.dblk.start.p, c.hs.p, H32, n,
                                   n));
     } else {
-      DH_TRY(launch_unpack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.hs.p, c.Hd.p,
+      DH_TRY(launch_unpack_rows<double>(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p,
+                                        c.dblk.start.p, c.hs.p, c.Hd.p,
                                    n, n));
       if (refined) DH_TRY((convert2d<double, float>(s, c.Hd.p, n, H32, n, n, n)));
     }
@@ -2055,8 +2162,8 @@ static int ctx_scc_step(Context& c) {
     }
     }
   }
-  DH_TRY(launch_mulliken(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.ov.p, c.h0.p, c.rho.p,
-                         c.qorb.p, c.eorb.p, n));
+  DH_TRY(launch_mulliken_rows(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, c.dblk.start.p,
+                              c.ov.p, c.h0.p, c.rho.p, c.qorb.p, c.eorb.p));
   DH_HIP(hipMemcpyAsync(qorbH.data(), c.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipMemcpyAsync(eorbH.data(), c.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipEventRecord(ev[5], s));
@@ -2066,10 +2173,10 @@ static int ctx_scc_step(Context& c) {
   c.times[kTHam] += ms * 1e-3;
   DH_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
   c.times[kTBack] += ms * 1e-3;
-  if (refined) {
+  if (refineTimed) {
     DH_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
     c.times[kTRefine] += ms * 1e-3;
-    c.times[kTBack] -= ms * 1e-3;
+    if (refined || tail) c.times[kTBack] -= ms * 1e-3;   // inside the back-transform window
   }
   DH_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
   c.times[kTDM] += ms * 1e-3;
@@ -2106,6 +2213,9 @@ static int ctx_scc_step(Context& c) {
   for (int a = 0; a < nat; ++a) st.qinSum += st.qin[a];
   st.conv = err < o.scc_tol && st.it >= o.min_scc_iter;
   const double th3 = now();
+  // the iteration that seeds the tail may restart the Broyden history (the earlier FP32
+  // iterations' output charges carry the FP32 noise that the tail's converged charges do not)
+  if (g_tailMixReset && seedAll && !tail && c.xFull) st.mixer.reset(c.nat, c.opt, o.max_scc_iter + 1);
   if (!st.conv && st.it != o.max_scc_iter) st.mixer.mix(st.qin, diff);
   c.times[kTHost] += now() - th3;
   c.times[kTIter] = now() - tIt;
@@ -2392,6 +2502,7 @@ static int batch_geometry(BatchContext& b, int nmol, const int* natMol, const do
   }
   b.nnz = st;
   DH_TRY(b.dblk.upload(b.blk, b.species, b.norbAtom, b.atomStart, s));
+  DH_TRY(b.dblk.build_rows(b.nat, s));   // row view of the blocks (Mulliken)
   DH_TRY(b.dnb.upload(b.nbp, b.species, b.norbAtom, b.atomStart, s));
   b.nbdev.n = b.dnb.pr.nblk;
   b.nbdev.at1 = b.dnb.pr.at1;
@@ -2600,8 +2711,8 @@ static int batch_scc_step(BatchContext& b) {
   DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p, b.activeD.p));
   DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p,
                            b.rho.p));
-  DH_TRY(launch_mulliken(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.ov.p, b.h0.p,
-                         b.rho.p, b.qorb.p, b.eorb.p, n));
+  DH_TRY(launch_mulliken_rows(s, b.dblk.rows, b.dblk.norb.p, b.dblk.atomStart.p, b.dblk.start.p,
+                              b.ov.p, b.h0.p, b.rho.p, b.qorb.p, b.eorb.p));
   DH_HIP(hipMemcpyAsync(qorbH, b.qorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipMemcpyAsync(eorbH, b.eorb.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   DH_HIP(hipStreamSynchronize(s));
@@ -2746,8 +2857,8 @@ static int batch_dev_enqueue(BatchContext& b, bool warm) {
   DH_TRY(launch_batch_density(s, b.L, b.Cv.p, b.f.p, b.noccD.p, b.P.p, b.activeD.p));
   DH_TRY(launch_batch_pack(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.L, b.P.p,
                            b.rho.p));
-  DH_TRY(launch_mulliken(s, b.dblk.pr, b.dblk.norb.p, b.dblk.atomStart.p, b.ov.p, b.h0.p,
-                         b.rho.p, b.qorb.p, b.eorb.p, n));
+  DH_TRY(launch_mulliken_rows(s, b.dblk.rows, b.dblk.norb.p, b.dblk.atomStart.p, b.dblk.start.p,
+                              b.ov.p, b.h0.p, b.rho.p, b.qorb.p, b.eorb.p));
   DH_TRY(launch_bscc_outputs(s, d, b.dblk.atomStart.p, b.atomMolD.p, b.qorb.p, b.eorb.p));
   DH_TRY(launch_batch_gemv(s, b.L, b.nat, b.gamma.p, b.dq.p, b.vatom.p));
   DH_TRY(launch_bscc_mix(s, d, b.vatom.p));
@@ -3358,6 +3469,16 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
       return dftbhip::set_gemm_order(value);
     case 6:
       return dftbhip::set_oa_mode(value);
+    case 7:
+      DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 7 takes 0 or 1");
+      dftbhip::g_tailMixReset = value;
+      return 0;
+    case 8:
+      DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 8 takes 0 or 1");
+      dftbhip::g_tailUpdate32 = value;
+      return 0;
+    case 9:
+      return dftbhip::set_spmm_cols(value);
     default:
       dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));
       return dftbhip::kErrArg;

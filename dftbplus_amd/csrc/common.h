@@ -190,6 +190,9 @@ int merge_complex(hipStream_t s, const double* re, const double* im, long ld, do
 // dst(j, i) = src(i, j) for the m x n matrix src (dst is n x m), converted to Tout
 template <typename Tin, typename Tout>
 int transpose2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n);
+// dst = a + b with b in FP32 (m x n)
+int add_f32(hipStream_t s, const double* a, long lda, const float* b, long ldb, double* dst,
+            long ldd, int m, int n);
 int symmetrize_lower(hipStream_t s, double* a, long lda, int n);
 int symmetrize_lower(hipStream_t s, float* a, long lda, int n);
 int zero_strict_upper(hipStream_t s, double* a, long lda, int n);

@@ -172,6 +172,23 @@ int transpose2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, in
   DH_LAUNCH_CHECK();
   return 0;
 }
+// dst(i, j) = a(i, j) + b(i, j), b in FP32 (an FP32 product added to an FP64 matrix)
+__global__ void add_f32_kernel(const double* a, long lda, const float* b, long ldb, double* dst,
+                               long ldd, int m, int n) {
+  for (int j = blockIdx.y; j < n; j += gridDim.y)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+      dst[i + (long)j * ldd] = a[i + (long)j * lda] + (double)b[i + (long)j * ldb];
+}
+
+int add_f32(hipStream_t s, const double* a, long lda, const float* b, long ldb, double* dst,
+            long ldd, int m, int n) {
+  if (m <= 0 || n <= 0) return 0;
+  dim3 grid(std::min(cdiv(m, 256), 64u), std::min(n, 65535));
+  hipLaunchKernelGGL(add_f32_kernel, grid, dim3(256), 0, s, a, lda, b, ldb, dst, ldd, m, n);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
 template int transpose2d<double, double>(hipStream_t, const double*, long, double*, long, int, int);
 template int transpose2d<float, double>(hipStream_t, const float*, long, double*, long, int, int);
 

@@ -168,29 +168,37 @@ __global__ void oa_correction_t_kernel(const double* M, const double* G, const d
                                        double* Et, int jacobi, int* nClipped) {
   const int jr = blockIdx.x * blockDim.x + threadIdx.x;   // row of Et: the vector index j of E
   const int ic = blockIdx.y;                               // column of Et: the index i of E
-  if (jr >= n || ic >= n) return;
+  const bool in = jr < n && ic < n;
   const long q = jr + (long)ic * n;
-  double e;
-  if (jr == ic) {
-    e = 0.5 * (1.0 - G[q]);
-  } else {
-    const double r = -G[q];
-    const double num = M[q] + lam[jr] * r, den = lam[jr] - lam[ic];
-    if (std::fabs(num) < kOaMax * std::fabs(den)) {
-      e = num / den;
-    } else if (nClipped && jr > ic) {
-      atomicAdd(nClipped, 1);   // each pair once
-      e = jacobi ? 0.5 * r + sin(den != 0.0 ? 0.5 * atan(2.0 * M[q] / den)
-                                           : copysign(0.25 * M_PI, M[q]))
-                 : 0.5 * r;
-    } else if (jacobi) {
-      const double th = den != 0.0 ? 0.5 * atan(2.0 * M[q] / den) : copysign(0.25 * M_PI, M[q]);
-      e = 0.5 * r + sin(th);
+  double e = 0.0;
+  bool clipped = false;
+  if (in) {
+    if (jr == ic) {
+      e = 0.5 * (1.0 - G[q]);
     } else {
-      e = 0.5 * r;
+      const double r = -G[q];
+      const double num = M[q] + lam[jr] * r, den = lam[jr] - lam[ic];
+      if (std::fabs(num) < kOaMax * std::fabs(den)) {
+        e = num / den;
+      } else {
+        clipped = jr > ic;   // each pair counted once
+        if (jacobi) {
+          const double th =
+              den != 0.0 ? 0.5 * atan(2.0 * M[q] / den) : copysign(0.25 * M_PI, M[q]);
+          e = 0.5 * r + sin(th);
+        } else {
+          e = 0.5 * r;
+        }
+      }
     }
+    Et[q] = e;
   }
-  Et[q] = e;
+  // one atomic per wave for the count of pairs outside the first-order range
+  if (nClipped) {
+    const unsigned long long m = __ballot(clipped);
+    if (m != 0ull && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1))
+      atomicAdd(nClipped, (int)__popcll(m));
+  }
 }
 
 static int g_oaJacobi = 0;

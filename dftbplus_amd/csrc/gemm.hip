@@ -248,7 +248,9 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
   }
   splitK = std::max(splitK, 1);
   dim3 grid;
-  const bool grouped = !lowerOut && g_gemmOrder == 1;
+  // grouped only for full-K products: with a triangular k-limit the work of a tile row grows
+  // or shrinks along the rows, and XCDs given contiguous bands of rows would be unbalanced
+  const bool grouped = !lowerOut && klim == kKFull && g_gemmOrder == 1;
   if (lowerOut) {
     const long t = cdiv(M, kBM);
     grid = dim3((unsigned)(t * (t + 1) / 2), 1, 1);

@@ -105,14 +105,36 @@ int launch_gamma_grad(hipStream_t s, const NeighDev& nb, const double* coords,
 struct RowBlocksDev {
   int nat;
   const int* rowPtr;
-  const int2* ent;
+  const int2* ent;    // in block order within a row
+  // the same entries with each row sorted by column atom; the entries of one column atom (its
+  // image blocks) form a group, grp[g] = (first, one past the last entry of entS), the groups
+  // of row A at gPtr[A] .. gPtr[A + 1] - 1
+  const int2* entS;
+  const int* gPtr;
+  const int2* grp;
 };
+// dense = the matrix of the sparse x (both triangles, image blocks summed; zero elsewhere) from
+// the row view: every dense element written once, without atomics
+template <typename T>
+int launch_unpack_rows(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+                       const long* iAtomStart, const long* blkStart, const double* x, T* dense,
+                       long ld, int n);
 // YH^T = (H X)^T and YS^T = (S X)^T with Xt = X^T (nvec x n, leading dimension ldx) and the
-// outputs nvec x n (leading dimension ldy); H, S: sparse h and ov in the block layout of rb
+// outputs nvec x n (leading dimension ldy); H, S: sparse h and ov in the block layout of rb,
+// at the Gamma point (blkT null) or at the time-reversal-invariant k-point kf (fractional; the
+// blocks times their phases +-1 from the fractional image translations blkT)
 int launch_spmm_hs_t(hipStream_t s, const RowBlocksDev& rb, const int* norb,
                      const long* iAtomStart, const long* blkStart, const double* h,
                      const double* ov, const double* Xt, long ldx, int nvec, double* yh,
-                     double* ys, long ldy);
+                     double* ys, long ldy, const double* blkT = nullptr,
+                     const double* kf = nullptr);
+// vectors per tile of the block-sparse products (64, 128 or 256, the default)
+int set_spmm_cols(int cols);
+// Mulliken populations qorb and band-energy contributions eorb (may be null) per orbital from
+// the row view (no atomics; every orbital of the rb.nat atoms written)
+int launch_mulliken_rows(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+                         const long* iAtomStart, const long* blkStart, const double* ov,
+                         const double* h0, const double* rho, double* qorb, double* eorb);
 int launch_repulsive(hipStream_t s, const NeighDev& nb, const double* coords, const RepDev* reps,
                      int nsp, double* eatom, double* grad);
 

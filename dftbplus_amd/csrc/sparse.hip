@@ -861,31 +861,79 @@ int launch_repulsive(hipStream_t s, const NeighDev& nb, const double* coords, co
 // Block-sparse products with a transposed dense operand: YH^T = (H X)^T and YS^T = (S X)^T for
 // X given as Xt = X^T (nvec x n, column-major, so the nvec coefficients of one orbital are
 // contiguous). H and S are the matrices unpack builds from the sparse blocks (image blocks of one
-// atom pair summed), taken row atom by row atom from the row view rb. A workgroup owns one row
-// atom A and 256 consecutive vectors j (one per thread): per entry it reads the block (the same
-// addresses in every lane) and the <= 4 contiguous coefficient runs of the column atom, and keeps
-// the <= 4 outputs of A's orbitals for both matrices in registers. Workgroups are numbered so
-// that consecutive ones of an XCD (IDs congruent mod 8) walk the atoms of one vector tile: the
-// column-atom runs of neighbouring row atoms are then read from that XCD's L2.
+// atom pair summed), taken row atom by row atom from the row view rb. A workgroup of 256 threads
+// owns 256 / kCols consecutive row atoms and a tile of kCols consecutive vectors j (one thread per
+// atom and vector): per entry a thread reads the block (shared by the threads of its atom) and
+// the <= 4 contiguous coefficient runs of the column atom, and keeps the <= 4 outputs of its
+// atom's orbitals for both matrices in registers. Workgroups are numbered so that consecutive
+// ones of an XCD (IDs congruent mod 8) walk the atoms of one vector tile: every coefficient run
+// of that tile is read by all the row atoms neighbouring its atom, and narrow tiles (kCols = 32:
+// 32 vectors x n orbitals x 8 B = 4 MB at n = 15769) keep the whole tile in the XCD's L2 for
+// those reads.
 // ---------------------------------------------------------------------------------------------
-constexpr int kSpmmCols = 256;
 constexpr int kXcds = 8;
+static int g_spmmCols = 256;
+int set_spmm_cols(int cols) {
+  DH_CHECK(cols == 64 || cols == 128 || cols == 256, "spmm tile: 64, 128 or 256 vectors");
+  g_spmmCols = cols;
+  return 0;
+}
 
-__global__ __launch_bounds__(kSpmmCols) void spmm_hs_t_kernel(
+// one entry: y(r) += B(r, c) x(c) for H and S, B the block as stored (TR false: element (r, c) at
+// r + c NA) or transposed (TR true: at c + r NB); the block's NA NB values of each matrix are
+// loaded first, together (wave-uniform scalar loads), then used from registers
+template <int NA, int NB, bool TR>
+__device__ __forceinline__ void spmm_entry(const double* __restrict__ h,
+                                           const double* __restrict__ ov, long st,
+                                           const double (&xv)[4], double (&ah)[4],
+                                           double (&as)[4]) {
+  double bh[NA * NB], bs[NA * NB];
+#pragma unroll
+  for (int q = 0; q < NA * NB; ++q) {
+    bh[q] = h[st + q];
+    bs[q] = ov[st + q];
+  }
+#pragma unroll
+  for (int r = 0; r < NA; ++r)
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int q = TR ? (c + r * NB) : (r + c * NA);
+      ah[r] = fma(bh[q], xv[c], ah[r]);
+      as[r] = fma(bs[q], xv[c], as[r]);
+    }
+}
+
+template <int NA, int NB>
+__device__ __forceinline__ void spmm_entry(const double* __restrict__ h,
+                                           const double* __restrict__ ov, long st, int tr,
+                                           const double (&xv)[4], double (&ah)[4],
+                                           double (&as)[4]) {
+  if (tr) spmm_entry<NA, NB, true>(h, ov, st, xv, ah, as);
+  else spmm_entry<NA, NB, false>(h, ov, st, xv, ah, as);
+}
+
+template <int kCols>
+__global__ __launch_bounds__(256) void spmm_hs_t_kernel(
     int nat, int ntile, const int* __restrict__ rowPtr, const int2* __restrict__ ent,
     const int* __restrict__ norbAtom, const long* __restrict__ iAtomStart,
     const long* __restrict__ blkStart, const double* __restrict__ h,
     const double* __restrict__ ov, const double* __restrict__ Xt, long ldx, int nvec,
-    double* __restrict__ yh, double* __restrict__ ys, long ldy) {
+    double* __restrict__ yh, double* __restrict__ ys, long ldy, const double* __restrict__ blkT,
+    int m0, int m1, int m2) {
+  static_assert(kCols % 64 == 0, "one row atom per wave at most");
+  constexpr int kApw = 256 / kCols;   // row atoms per workgroup
+  const int nag = (nat + kApw - 1) / kApw;
   const long w = blockIdx.x;
   const int xcd = (int)(w % kXcds);
   const long l = w / kXcds;
-  const int tile = xcd + kXcds * (int)(l / nat);
-  const int A = (int)(l % nat);
-  if (tile >= ntile) return;
-  const int j = tile * kSpmmCols + threadIdx.x;
+  const int tile = xcd + kXcds * (int)(l / nag);
+  // the row atom is the same in the whole wave: its entries and blocks are scalar loads
+  const int A = __builtin_amdgcn_readfirstlane((int)(l % nag) * kApw +
+                                               (int)threadIdx.x / kCols);
+  if (tile >= ntile || A >= nat) return;
+  const int j = tile * kCols + (int)threadIdx.x % kCols;
   const bool live = j < nvec;
-  const int jj = live ? j : 0;
+  const double* __restrict__ xj = Xt + (live ? j : 0);
   const int nA = norbAtom[A];
   const long oA = iAtomStart[A];
   double ah[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
@@ -895,20 +943,34 @@ __global__ __launch_bounds__(kSpmmCols) void spmm_hs_t_kernel(
     const int k = e.x >> 1, tr = e.x & 1, B = e.y;
     const int nB = norbAtom[B];
     const long oB = iAtomStart[B], st = blkStart[k];
+    // the column atom's coefficient runs, loaded unconditionally (runs past nB repeat the last
+    // one and are not used)
     double xv[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) xv[c] = c < nB ? Xt[jj + (oB + c) * ldx] : 0.0;
+    for (int c = 0; c < 4; ++c) xv[c] = xj[(oB + min(c, nB - 1)) * ldx];
+    // at a time-reversal-invariant k-point (m = 2k integral) the block's phase is (-1)^(m.T)
+    if (blkT) {
+      const long par = llround(m0 * blkT[3 * k] + m1 * blkT[3 * k + 1] + m2 * blkT[3 * k + 2]);
+      if (par & 1)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (r >= nA) break;
+        for (int c = 0; c < 4; ++c) xv[c] = -xv[c];
+    }
+    if (nA == 4 && nB == 4) spmm_entry<4, 4>(h, ov, st, tr, xv, ah, as);
+    else if (nA == 4 && nB == 1) spmm_entry<4, 1>(h, ov, st, tr, xv, ah, as);
+    else if (nA == 1 && nB == 4) spmm_entry<1, 4>(h, ov, st, tr, xv, ah, as);
+    else if (nA == 1 && nB == 1) spmm_entry<1, 1>(h, ov, st, tr, xv, ah, as);
+    else {
+      // other shell sets (at most 4 orbitals per atom): unrolled with guards, so that the
+      // accumulators stay in registers
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if (c >= nB) break;
-        // block k is nOrb(at2) x nOrb(at1), column-major: as is (rows of A) or transposed
-        const long q = st + (tr ? (c + (long)r * nB) : (r + (long)c * nA));
-        ah[r] = fma(h[q], xv[c], ah[r]);
-        as[r] = fma(ov[q], xv[c], as[r]);
-      }
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (r < nA && c < nB) {
+            const long q = st + (tr ? (c + (long)r * nB) : (r + (long)c * nA));
+            ah[r] = fma(h[q], xv[c], ah[r]);
+            as[r] = fma(ov[q], xv[c], as[r]);
+          }
     }
   }
   if (!live) return;
@@ -923,15 +985,160 @@ __global__ __launch_bounds__(kSpmmCols) void spmm_hs_t_kernel(
 int launch_spmm_hs_t(hipStream_t s, const RowBlocksDev& rb, const int* norb,
                      const long* iAtomStart, const long* blkStart, const double* h,
                      const double* ov, const double* Xt, long ldx, int nvec, double* yh,
-                     double* ys, long ldy) {
+                     double* ys, long ldy, const double* blkT, const double* kf) {
   if (rb.nat == 0 || nvec <= 0) return 0;
-  const int ntile = (int)cdiv(nvec, kSpmmCols);
+  int m[3] = {0, 0, 0};
+  if (blkT) {
+    DH_CHECK(kf && kpoint_is_real(kf), "spmm: the k-point is not time-reversal invariant");
+    for (int q = 0; q < 3; ++q) m[q] = (int)std::nearbyint(2.0 * kf[q]);
+  }
+  const int cols = g_spmmCols, apw = 256 / cols;
+  const int ntile = (int)cdiv(nvec, cols);
   const long tilesPad = (long)cdiv(ntile, kXcds) * kXcds;
-  const long nwg = tilesPad * rb.nat;
+  const long nwg = tilesPad * cdiv(rb.nat, apw);
   DH_CHECK(nwg < (1L << 31), "spmm: grid too large");
-  hipLaunchKernelGGL(spmm_hs_t_kernel, dim3((unsigned)nwg), dim3(kSpmmCols), 0, s, rb.nat, ntile,
-                     rb.rowPtr, rb.ent, norb, iAtomStart, blkStart, h, ov, Xt, ldx, nvec, yh, ys,
-                     ldy);
+#define DH_SPMM(C_)                                                                          \
+  hipLaunchKernelGGL(spmm_hs_t_kernel<C_>, dim3((unsigned)nwg), dim3(256), 0, s, rb.nat, ntile, \
+                     rb.rowPtr, rb.ent, norb, iAtomStart, blkStart, h, ov, Xt, ldx, nvec, yh,  \
+                     ys, ldy, blkT, m[0], m[1], m[2])
+  if (cols == 64) DH_SPMM(64);
+  else if (cols == 128) DH_SPMM(128);
+  else DH_SPMM(256);
+#undef DH_SPMM
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+// Sparse -> dense from the row view (unpackHS_real, sparse2dense.F90:266-321, both triangles):
+// one wave per row atom, a lane per group of entries with one column atom (the image blocks of
+// the pair, summed in registers), every element of the atom pair's block written once.
+template <typename T>
+__global__ __launch_bounds__(256) void unpack_rows_kernel(
+    RowBlocksDev rb, const int* __restrict__ norbAtom, const long* __restrict__ iAtomStart,
+    const long* __restrict__ blkStart, const double* __restrict__ x, T* __restrict__ dense,
+    long ld) {
+  const int A = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (A >= rb.nat) return;
+  const int nA = norbAtom[A];
+  const long oA = iAtomStart[A];
+  const int g1 = rb.gPtr[A + 1];
+  for (int g = rb.gPtr[A] + lane; g < g1; g += 64) {
+    const int2 gr = rb.grp[g];
+    const int B = rb.entS[gr.x].y;
+    const int nB = norbAtom[B];
+    const long oB = iAtomStart[B];
+    double v[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[r][c] = 0.0;
+    for (int p = gr.x; p < gr.y; ++p) {
+      const int2 en = rb.entS[p];
+      const int k = en.x >> 1, tr = en.x & 1;
+      const long st = blkStart[k];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r >= nA) break;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (c >= nB) break;
+          v[r][c] += x[st + (tr ? (c + (long)r * nB) : (r + (long)c * nA))];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (c >= nB) break;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r >= nA) break;
+        dense[(oA + r) + (oB + c) * ld] = (T)v[r][c];
+      }
+    }
+  }
+}
+
+template <typename T>
+int launch_unpack_rows(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+                       const long* iAtomStart, const long* blkStart, const double* x, T* dense,
+                       long ld, int n) {
+  DH_TRY(fill2d<T>(s, dense, ld, n, n, T(0)));
+  if (rb.nat == 0) return 0;
+  hipLaunchKernelGGL((unpack_rows_kernel<T>), dim3(cdiv(rb.nat, 4)), dim3(256), 0, s, rb, norb,
+                     iAtomStart, blkStart, x, dense, ld);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+template int launch_unpack_rows<double>(hipStream_t, const RowBlocksDev&, const int*,
+                                        const long*, const long*, const double*, double*, long,
+                                        int);
+template int launch_unpack_rows<float>(hipStream_t, const RowBlocksDev&, const int*, const long*,
+                                       const long*, const double*, float*, long, int);
+
+// Mulliken populations and band-structure energy per orbital (mullikenPerOrbital,
+// populations.F90:120-193: the block of a pair (atom1, atom2) adds its column sums to atom1's
+// orbitals and, when atom1 != atom2 as central atoms, its row sums to atom2's) from the row view:
+// one wave per row atom, a lane per entry, the per-orbital partial sums combined by an xor
+// butterfly, so every orbital is written once, without atomics, and the result does not depend
+// on the order in which the hardware runs the waves. In the row view an entry (k, t, B) of row
+// atom A holds element (r of A, c of B) at r + c nA (t = 0, pair (B, A)) or c + r nB (t = 1,
+// pair (A, B)); a pair of an atom with itself or its image (t = 0, B = A) adds its column sums,
+// i.e. element (c, r) of the stored block.
+__global__ __launch_bounds__(256) void mulliken_rows_kernel(
+    RowBlocksDev rb, const int* __restrict__ norbAtom, const long* __restrict__ iAtomStart,
+    const long* __restrict__ blkStart, const double* __restrict__ ov,
+    const double* __restrict__ h0, const double* __restrict__ rho, double* __restrict__ qorb,
+    double* __restrict__ eorb) {
+  const int A = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (A >= rb.nat) return;
+  const int nA = norbAtom[A];
+  double q[4] = {0.0, 0.0, 0.0, 0.0}, e[4] = {0.0, 0.0, 0.0, 0.0};
+  const int p1 = rb.rowPtr[A + 1];
+  for (int p = rb.rowPtr[A] + lane; p < p1; p += 64) {
+    const int2 en = rb.ent[p];
+    const int k = en.x >> 1, tr = en.x & 1, B = en.y;
+    const int nB = norbAtom[B];
+    const long st = blkStart[k];
+    const bool self = !tr && B == A;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r >= nA) break;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= nB) break;
+        const long idx = st + (tr ? (c + (long)r * nB) : (self ? (c + (long)r * nA)
+                                                              : (r + (long)c * nA)));
+        const double d = rho[idx];
+        q[r] = fma(ov[idx], d, q[r]);
+        if (h0) e[r] = fma(h0[idx], d, e[r]);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    for (int o = 32; o > 0; o >>= 1) {
+      q[r] += __shfl_xor(q[r], o, 64);
+      e[r] += __shfl_xor(e[r], o, 64);
+    }
+  if (lane == 0) {
+    const long oA = iAtomStart[A];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r >= nA) break;
+      qorb[oA + r] = q[r];
+      if (eorb) eorb[oA + r] = e[r];
+    }
+  }
+}
+
+int launch_mulliken_rows(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+                         const long* iAtomStart, const long* blkStart, const double* ov,
+                         const double* h0, const double* rho, double* qorb, double* eorb) {
+  if (rb.nat == 0) return 0;
+  hipLaunchKernelGGL(mulliken_rows_kernel, dim3(cdiv(rb.nat, 4)), dim3(256), 0, s, rb, norb,
+                     iAtomStart, blkStart, ov, h0, rho, qorb, eorb);
   DH_LAUNCH_CHECK();
   return 0;
 }

@@ -67,8 +67,11 @@ int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e, doub
  * graph (2, the default); param 5: tile order of full-output GEMMs, 0 the launch grid's or 1
  * XCD-grouped (the default: the tiles an XCD runs together share their panels in its L2);
  * param 6: Ogita-Aishima refinement of the sparse tail, pairs outside the first-order range
- * re-orthonormalised only (0, the default) or rotated by the exact 2x2 angle (1). No
- * counterpart in the reference. */
+ * re-orthonormalised only (0, the default) or rotated by the exact 2x2 angle (1); param 7:
+ * the Broyden history restarts at the iteration that seeds the mixed-precision tail (1) or is
+ * kept (0, the default); param 8: the correction product X E of the tail's refinement in FP32
+ * (1, the default) or FP64 (0); param 9: vectors per tile of the tail's block-sparse products
+ * (64, 128 or 256, the default). No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)
  * builds it: atom a lists itself, then the images of atoms b >= a within cutoff by increasing

@@ -142,3 +142,31 @@ def test_md_steps_warm_start_equal_cold(amorphous64):
         cold.close()
     assert sum(it_w) < sum(it_c)
     warm.close()
+
+
+def test_amorphous_kpoint_tail_matches_oracle(amorphous64):
+    """Mixed precision at k-points (Gamma-centred 2x2x2, every point time-reversal invariant):
+    plain FP64 iterations down to a charge error of 1e-3, then per point two Ogita-Aishima steps
+    on its previous eigenpairs against its sparse H(k) and S(k) (block phases +-1), no
+    eigensolve: the oracle's complex-path fixed point at the FP64 bounds."""
+    from dftbplus_amd import api
+    from oracle import dftb_ref as R
+    geo, params, opts = amorphous64
+    o = dataclasses.replace(opts, kpoints=MESHES["gc222"], max_scc_iter=200)
+    calc = R.Calculator(params, geo, o)
+    res = calc.scc()
+    og = calc.gradients(res)
+    dp = api.DftbPlus(params, geo, o, precision=1, refine_below=1e-3, refine_steps=0,
+                      tail_below=1e-3, tail_steps=2)
+    e, q, g = dp.get_energy(), dp.get_gross_charges(), dp.get_gradients()
+    t = dp.timings()
+    nat = geo.natom
+    print(f"amorphous64 k-point tail: it {dp.n_iter}/{res.n_iter} dE/atom "
+          f"{abs(e - res.energy_mermin) / nat:.2e} dq {np.max(np.abs(q - res.charges)):.2e} "
+          f"dF {np.max(np.abs(g - og)):.2e} refine {t['refine']:.3f}s")
+    assert dp.converged and res.converged
+    assert t["refine"] > 0.0
+    assert abs(e - res.energy_mermin) / nat < 1e-8
+    assert np.max(np.abs(q - res.charges)) < 1e-8
+    assert np.max(np.abs(g - og)) < 1e-7
+    dp.close()

@@ -45,12 +45,12 @@ def test_mixed_fp32_error_bounds(system, request):
     dp.close()
 
 
-def _fp64_and_switch(cluster216, **switch):
+def _fp64_and_switch(system, **switch):
     """The plain FP64 path and a switching mixed path, both converged to 1e-10 (a cycle that
     takes a different path, here through FP32 iterations, agrees with another only to about
     its SCC tolerance); the FP64 path itself is pinned to the oracle by test_gpu_scc."""
     from dftbplus_amd import api
-    geo, params, opts, calc, res = cluster216
+    geo, params, opts, calc, res = system
     o = dataclasses.replace(opts, scc_tol=1e-10, max_scc_iter=200)
     ref = api.DftbPlus(params, geo, o)
     e0, q0, g0 = ref.get_energy(), ref.get_gross_charges(), ref.get_gradients()
@@ -132,3 +132,15 @@ def test_mixed_early_tail_matches_oracle(system, request):
     assert t["refine"] > 0.0
     assert de < 1e-8 and dq < 1e-8 and dg < 1e-7
     dp.close()
+
+
+@pytest.mark.parametrize("system", ["cluster216", "diamond2"])
+def test_tail_fp64_seed_two_steps_matches_fp64(system, request):
+    """The tail as the benchmark runs it: FP32 iterations above 1e-3, one plain FP64 iteration
+    that keeps all eigenvectors, then two Ogita-Aishima steps per iteration on the sparse H and
+    S with the correction product in FP32: the FP64 path's fixed point within the FP64
+    tolerances (both converged to 1e-10, since cycles along different paths agree only to
+    about their SCC tolerance; the FP64 path is pinned to the oracle by test_gpu_scc)."""
+    t = _fp64_and_switch(request.getfixturevalue(system), refine_below=1e-3, refine_steps=0,
+                         tail_below=1e-3, tail_steps=2)
+    assert t["refine"] > 0.0
