@@ -18,11 +18,14 @@ gradients is the leg's force evaluation. Legs:
                    FP64 iterations after (the converged point is the FP64 one);
   mixed_tail       FP32 iterations above 1e-3, then one plain FP64 iteration that keeps all n
                    eigenvectors, and after it no eigensolve: two Ogita-Aishima steps refine the
-                   previous iteration's FP64 eigenpairs against the new sparse H and S
-                   (block-sparse products, two n^3 FP64 GEMMs for X^T H X and X^T S X, the
-                   correction product X E in FP32);
+                   previous iteration's FP64 eigenpairs against the new sparse H and S in
+                   residual form (block-sparse products; X^T S X on its diagonal tile band, the
+                   Rayleigh quotients and the residuals H X - S X diag(lam) in FP64; the
+                   products with the residuals and with the correction in FP16, FP32 sums);
   mixed_tail_fp32seed  as mixed_tail, seeded by FP32 eigenpairs refined by two steps (only
                    with --legs);
+  mixed_tail32     as mixed_tail with X^T H X and X^T S X as FP64 GEMMs and the correction
+                   product in FP32 (only with --legs);
   mixed_fp32       FP32 iterations throughout (only with --legs; does not reach 1e-8 e).
 `value` is the SCC iterations/s of the fp64 leg's timed steps; every leg reports its timed
 rate, its converged cycle (iterations, seconds, force evaluations/s) and its energy, charge and
@@ -62,7 +65,11 @@ LEGS = {"fp64": dict(precision=0),
         "mixed_tail": dict(precision=1, refine_below=1e-3, refine_steps=0, tail_below=1e-3,
                            tail_steps=2),
         "mixed_tail_fp32seed": dict(precision=1, refine_below=1e-3, refine_steps=2,
-                                    tail_below=1e-3, tail_steps=2)}
+                                    tail_below=1e-3, tail_steps=2),
+        "mixed_tail32": dict(precision=1, refine_below=1e-3, refine_steps=0, tail_below=1e-3,
+                             tail_steps=2, tuning={8: 1})}
+# library defaults of the tuning parameters a leg or mode may change (restored after it)
+TUNING_DEFAULTS = {8: 2, 9: 1, 10: 1}
 E_BOUND, F_BOUND = 1e-6, 1e-5
 
 
@@ -146,8 +153,12 @@ def run_cycle_leg(api, geo, params, opts, leg, steps, warmup):
     import numpy as np
     L = api.lib()
     tag = f"[bench:{leg}]"
+    kw = dict(LEGS[leg])
+    tuning = kw.pop("tuning", {})
+    for p, v in tuning.items():
+        api.set_tuning(p, v)
     t0 = time.perf_counter()
-    dp = api.DftbPlus(params, geo, opts, **LEGS[leg])
+    dp = api.DftbPlus(params, geo, opts, **kw)
     setup_s = time.perf_counter() - t0
     n = dp.get_nr_orbitals()
     print(f"{tag} {geo.natom} atoms, {n} orbitals, setup {setup_s:.2f}s", file=sys.stderr,
@@ -166,8 +177,15 @@ def run_cycle_leg(api, geo, params, opts, leg, steps, warmup):
         conv, err = dp.scc_step()
         it += 1
         now = time.perf_counter()
+        extra = ""
+        if kw.get("tail_below", 0.0) > 0.0:
+            st = dp.scc_state()
+            if st["oa_clipped_pairs"] or st["oa_far_pairs"]:
+                extra = f" (pairs outside the first-order range {st['oa_clipped_pairs']}, " \
+                        f"beyond the band {st['oa_far_pairs']})"
         print(f"{tag} iteration {it}: {now - t_it:.3f}s charge error {err:.3e}"
-              + (" (timed)" if warmup < it <= warmup + steps else ""), file=sys.stderr, flush=True)
+              + (" (timed)" if warmup < it <= warmup + steps else "") + extra, file=sys.stderr,
+              flush=True)
         if it == warmup + steps:
             timed_s = now - t_win
             L.dftbhip_kernel_stats(api._d(st))
@@ -190,6 +208,8 @@ def run_cycle_leg(api, geo, params, opts, leg, steps, warmup):
           f"{'converged' if conv_it else 'NOT converged'} in {conv_it or it} iterations, "
           f"{conv_s or 0:.1f}s; gradients {1e3 * grad_s:.1f} ms", file=sys.stderr, flush=True)
     dp.close()
+    for p in tuning:
+        api.set_tuning(p, TUNING_DEFAULTS[p])
     launches, ksec, kbytes = st
     achieved = kbytes / ksec / 1e9 if ksec > 0 else None
     return {
@@ -206,9 +226,11 @@ def run_cycle_leg(api, geo, params, opts, leg, steps, warmup):
 
 def mode_tail(args):
     """The mixed-precision tail on its own, for kernel traces: one plain FP64 iteration seeds it
-    and every later iteration is one Ogita-Aishima step on the sparse H and S (the cycle is not
-    meant to converge from neutral charges this way); prints the iteration times and the sizes
-    that fix the flops of the step."""
+    and every later iteration is two Ogita-Aishima steps on the sparse H and S (the cycle is not
+    meant to converge from neutral charges this way); prints the iteration times for each
+    arithmetic of the steps (dftbhip_set_tuning parameter 8: 1 FP64 X^T H X and X^T S X with
+    the correction product in FP32, 2 the residual form with FP16 products) and, for the
+    residual form, each number of vectors per lane of the block-sparse products (parameter 9)."""
     import math
     from dftbplus_amd import api, synth
     geo, params, opts = synth.workload(args.workload)
@@ -217,24 +239,30 @@ def mode_tail(args):
     n = dp.get_nr_orbitals()
     dp.scc_begin()
     times = []
-    # the seed, then --steps tail iterations for each vector-tile width of the block-sparse
-    # products (in the kernel trace the spmm_hs_t_kernel<w> launches follow this order)
-    plan = [None] + [w for w in (64, 128, 256) for _ in range(args.steps)]
-    for it, cols in enumerate(plan):
-        if cols:
-            api.set_tuning(9, cols)
+    # the seed, then --steps tail iterations for each variant (in the kernel trace the
+    # iterations follow this order)
+    variants = [(1, 2), (2, 1), (2, 2), (2, 4)]
+    plan = [None] + [v for v in variants for _ in range(args.steps)]
+    for it, var in enumerate(plan):
+        if var:
+            api.set_tuning(8, var[0])
+            api.set_tuning(9, var[1])
         t0 = time.perf_counter()
         dp.scc_step()
-        times.append((cols, time.perf_counter() - t0))
-        print(f"[tail] iteration {it + 1} ({'FP64 seed' if cols is None else f'tail, {cols}-vector tiles'}): "
-              f"{times[-1][1]:.3f}s", file=sys.stderr, flush=True)
-    api.set_tuning(9, 256)
+        times.append((var, time.perf_counter() - t0))
+        st = dp.scc_state()
+        what = "FP64 seed" if var is None else f"tail, arithmetic {var[0]}, {var[1]} vectors/lane"
+        print(f"[tail] iteration {it + 1} ({what}): {times[-1][1]:.3f}s, pairs outside the "
+              f"first-order range {st['oa_clipped_pairs']} (beyond the band of X^T S X: "
+              f"{st['oa_far_pairs']})", file=sys.stderr, flush=True)
+    api.set_tuning(8, TUNING_DEFAULTS[8])
+    api.set_tuning(9, TUNING_DEFAULTS[9])
     dp.scc_end()
     dp.close()
     print(json.dumps({"mode": "tail", "workload": args.workload, "orbitals": n,
-                      "seed_s": times[0][1], "tail_s": {str(c): [t for cc, t in times if cc == c]
-                                                         for c in (64, 128, 256)}}),
-          flush=True)
+                      "seed_s": times[0][1],
+                      "tail_s": {f"{v[0]}/{v[1]}": [t for vv, t in times if vv == v]
+                                 for v in variants}}), flush=True)
 
 
 def mode_cpu(args):

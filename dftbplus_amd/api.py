@@ -100,6 +100,9 @@ EXPORTS = {
                       _dp, C.c_int, C.c_double, _dp, C.c_int],
     "dftbhip_dstedc": [C.c_int, _dp, _dp, _dp, C.c_int],
     "dftbhip_probe_mfma_f64": [_dp],
+    "dftbhip_probe_mfma_f16": [_dp],
+    "dftbhip_gemm_f16_tn": [C.c_int, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_int,
+                            C.c_void_p, C.c_int, C.c_void_p, C.c_int],
     "dftbhip_kernel_profile": [C.c_int],
     "dftbhip_kernel_stats": [_dp],
     "dftbhip_dm_real": [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int],
@@ -631,12 +634,14 @@ class DftbPlus:
     def scc_state(self) -> dict:
         """Iteration number, charge error, electronic energy of the last iteration and its
         change from the one before, input electrons of the last iteration, pairs outside the
-        first-order range of the last refinement step."""
-        out = np.zeros(6)
-        _check(lib().dftbhip_ctx_scc_state(self.h, _d(out), 6), "dftbhip_ctx_scc_state")
+        first-order range of the last refinement step and, for the residual form, of those
+        beyond the diagonal band of X^T S X in the last refinement."""
+        out = np.zeros(7)
+        _check(lib().dftbhip_ctx_scc_state(self.h, _d(out), 7), "dftbhip_ctx_scc_state")
         return {"iteration": int(out[0]), "scc_error": float(out[1]), "e_elec": float(out[2]),
                 "diff_elec": float(out[2] - out[3]) if np.isfinite(out[3]) else float(out[2]),
-                "q_input_sum": float(out[4]), "oa_clipped_pairs": int(out[5])}
+                "q_input_sum": float(out[4]), "oa_clipped_pairs": int(out[5]),
+                "oa_far_pairs": int(out[6])}
 
     def get_nr_atoms(self) -> int:
         return self.natom

@@ -604,6 +604,7 @@ struct DevBlocks {
   DevBuf<int2> rowEnt, rowEntS, grp;
   RowBlocksDev rows{};
   bool rowsValid = false;
+  int nGroups = 0;   // groups (row atom, column atom) of the row view
   int build_rows(int nat, hipStream_t s) {
     if (rowsValid) return 0;
     const int nb = pr.nblk;
@@ -650,6 +651,7 @@ struct DevBlocks {
     DH_TRY(grp.upload(gr.data(), gr.size(), s));
     DH_HIP(hipStreamSynchronize(s));
     rows = RowBlocksDev{nat, rowPtr.p, rowEnt.p, rowEntS.p, gPtr.p, grp.p};
+    nGroups = gp[nat];
     rowsValid = true;
     return 0;
   }
@@ -1398,9 +1400,16 @@ static int g_devNeighbours = 1;
 // the Broyden history restarts at the iteration that seeds the mixed-precision tail (1) or is
 // kept (0, the default); set through dftbhip_set_tuning parameter 7
 static int g_tailMixReset = 0;
-// the correction product X E of the sparse refinement in FP32 (1, the default) or FP64 (0); set
-// through dftbhip_set_tuning parameter 8
-static int g_tailUpdate32 = 1;
+// arithmetic of a step of the sparse refinement (dftbhip_set_tuning parameter 8):
+//  0: M = X^T H X and G = X^T S X in FP64, the correction product X E in FP64;
+//  1: the same with X E in FP32;
+//  2: residual form (the default): G in FP64, the Rayleigh quotients and the residuals
+//     R = H X - S X diag(lam) in FP64, X^T R and X E as FP16 products (FP32 accumulation) of
+//     row-scaled operands.
+static int g_tailUpdate32 = 2;
+// residual form: tile rows of X^T S X below its diagonal tiles that each step forms (128 x 128
+// tiles; -1: the whole lower triangle); dftbhip_set_tuning parameter 10
+static int g_tailBand = 1;
 
 struct Context {
   dftbhip_options opt{};
@@ -1463,6 +1472,15 @@ struct Context {
   // Rayleigh quotients
   DevBuf<double> tXt[2], tYH, tYS, tM, tG, tLam;
   DevBuf<int> tClip;
+  // residual form: FP16 operands (padded, f16_ld x f16_cols), row reductions and scales
+  DevBuf<_Float16> hX, hXt, hR, hE;
+  DevBuf<double> tPart, tRsc;
+  DevBuf<float> tRinv;
+  DevBuf<int> tFar;
+  int lastFar = 0;
+  // folded 4 x 4 blocks of H and S per row-view group for the block-sparse products
+  DevBuf<double> tFold;
+  DevBuf<int> tColOff;   // pairs beyond the band of G in the last refinement (whole G formed)
   int tCur = 0;
   // the same per k-point (all time-reversal invariant): the transposed eigenvectors of point k
   // in tXtK[2 k + tCurK[k]]
@@ -1739,13 +1757,81 @@ static int oa_refine_sparse_on(Context& c, int steps, DevBuf<double>* xt, int& c
   DH_TRY(c.tG.alloc(nn));
   DH_TRY(c.tLam.alloc(n));
   DH_TRY(c.tClip.alloc(1));
+  const int ng = c.dblk.nGroups;
+  DH_TRY(c.tFold.alloc((size_t)std::max(ng, 1) * 32));
+  DH_TRY(c.tColOff.alloc((size_t)std::max(ng, 1)));
+  DH_TRY(launch_spmm_fold(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, c.dblk.start.p,
+                          c.hs.p, c.ov.p, kf ? c.blkT.p : nullptr, kf, c.tFold.p, c.tColOff.p));
+  const int mode = g_tailUpdate32;
+  const long ldh = f16_ld(n);
+  if (mode == 2) {
+    // the padding of the FP16 operands must be zero: cleared once per call (the steps write the
+    // n x n parts only)
+    const size_t nh = (size_t)ldh * f16_cols(n);
+    for (DevBuf<_Float16>* b : {&c.hX, &c.hXt, &c.hR, &c.hE}) {
+      DH_TRY(b->alloc(nh));
+      DH_TRY(fill<_Float16>(s, b->p, nh, (_Float16)0.0f));
+    }
+    DH_TRY(c.tPart.alloc((size_t)64 * n));
+    DH_TRY(c.tFar.alloc(1));
+    c.lastFar = 0;
+    DH_TRY(c.tRsc.alloc(n));
+    DH_TRY(c.tRinv.alloc(n));
+  }
   for (int st = 0; st < std::max(steps, 1); ++st) {
     DH_TRY(fill<int>(s, c.tClip.p, 1, 0));
     const double* Xt = xt[cur].p;
     double* Xn = xt[1 - cur].p;
-    DH_TRY(launch_spmm_hs_t(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, c.dblk.start.p,
-                            c.hs.p, c.ov.p, Xt, n, n, c.tYH.p, c.tYS.p, n,
-                            kf ? c.blkT.p : nullptr, kf));
+    DH_TRY(launch_spmm_hs_t(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, n, c.tFold.p,
+                            c.tColOff.p, Xt, n, n, c.tYH.p, c.tYS.p, n));
+    if (mode == 2) {
+      // G = X^T S X in FP64; lam_j = x_j^T H x_j / G_jj; R = H X - S X diag(lam) in FP64, each
+      // vector's residual scaled by a power of two into the FP16 range; Ft = R^T X (FP16
+      // operands, FP32 sums, unscaled per row); E from Ft and G in FP16 (scaled by 2^15);
+      // Xt <- Xt + E^T Xt with the product in FP16. The rounding of the FP16 products is a
+      // relative 2^-11-level error of the correction, removed by the next step like any other
+      // error of the basis: the fixed point is that of the FP64 residuals.
+      // G on the diagonal tile band only: its diagonal, and its pairs outside the first-order
+      // range, which with eigenvectors in the order of their eigenvalues are close in index; a
+      // step that finds such a pair beyond the band forms the whole lower triangle and repeats
+      // the correction
+      const int band = g_tailBand;
+      if (band >= 0)
+        DH_TRY(gemm_band<double>(s, false, true, n, n, 1.0, Xt, n, c.tYS.p, n, c.tG.p, n, band));
+      else
+        DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYS.p, n, 0.0, c.tG.p, n,
+                            true));
+      DH_TRY(oa_rayleigh(s, Xt, c.tYH.p, n, n, n, c.tG.p, n, c.tPart.p, c.tLam.p));
+      DH_TRY(oa_residual_scale(s, c.tYH.p, c.tYS.p, n, n, n, c.tLam.p, c.tPart.p, c.tRsc.p,
+                               c.tRinv.p));
+      DH_TRY(f16_convert(s, c.tYH.p, c.tYS.p, c.tLam.p, c.tRsc.p, n, n, n, nullptr, 0, c.hR.p,
+                         ldh));
+      DH_TRY(f16_convert(s, Xt, nullptr, nullptr, nullptr, n, n, n, c.hXt.p, ldh, c.hX.p, ldh));
+      float* Ft = reinterpret_cast<float*>(c.tM.p);
+      DH_TRY(gemm_f16_tn(s, n, n, n, 1.0f, c.hR.p, ldh, c.hX.p, ldh, Ft, n, c.tRinv.p));
+      DH_TRY(fill<int>(s, c.tFar.p, 1, 0));
+      DH_TRY(oa_correction_f16(s, Ft, n, c.tG.p, n, c.tLam.p, n, c.hE.p, ldh, band, c.tClip.p,
+                               c.tFar.p));
+      int far = 0;
+      if (band >= 0) {
+        DH_HIP(hipMemcpyAsync(&far, c.tFar.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        DH_HIP(hipStreamSynchronize(s));
+      }
+      if (far > 0) {
+        c.lastFar += far;
+        DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYS.p, n, 0.0, c.tG.p, n,
+                            true));
+        DH_TRY(fill<int>(s, c.tClip.p, 1, 0));
+        DH_TRY(oa_correction_f16(s, Ft, n, c.tG.p, n, c.tLam.p, n, c.hE.p, ldh, -1, c.tClip.p,
+                                 nullptr));
+      }
+      float* D32 = reinterpret_cast<float*>(c.tYH.p);
+      DH_TRY(gemm_f16_tn(s, n, n, n, 1.0f / kOaF16Scale, c.hE.p, ldh, c.hXt.p, ldh, D32, n,
+                         nullptr));
+      DH_TRY(add_f32(s, Xt, n, D32, n, Xn, n, n, n));
+      cur = 1 - cur;
+      continue;
+    }
     DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYH.p, n, 0.0, c.tM.p, n, true));
     DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYS.p, n, 0.0, c.tG.p, n, true));
     DH_TRY(symmetrize_lower(s, c.tM.p, n, n));
@@ -2260,9 +2346,9 @@ static int ctx_scc_end(Context& c, double* energies, double* charges, int* niter
 // the first-order range in the last Ogita-Aishima step of the sparse refinement
 static int ctx_scc_state(Context& c, double* out, int n) {
   const SccState& st = c.st;
-  const double v[6] = {(double)st.it, st.err, st.eElec, st.eElecPrev, st.qinSum,
-                       (double)c.lastClipped};
-  for (int k = 0; k < std::min(n, 6); ++k) out[k] = v[k];
+  const double v[7] = {(double)st.it, st.err, st.eElec, st.eElecPrev, st.qinSum,
+                       (double)c.lastClipped, (double)c.lastFar};
+  for (int k = 0; k < std::min(n, 7); ++k) out[k] = v[k];
   return 0;
 }
 
@@ -3474,11 +3560,15 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
       dftbhip::g_tailMixReset = value;
       return 0;
     case 8:
-      DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 8 takes 0 or 1");
+      DH_CHECK(value >= 0 && value <= 2, "set_tuning: parameter 8 takes 0, 1 or 2");
       dftbhip::g_tailUpdate32 = value;
       return 0;
     case 9:
       return dftbhip::set_spmm_cols(value);
+    case 10:
+      DH_CHECK(value >= -1 && value <= 8, "set_tuning: parameter 10 takes -1 .. 8");
+      dftbhip::g_tailBand = value;
+      return 0;
     default:
       dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));
       return dftbhip::kErrArg;

@@ -154,8 +154,26 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
          const T* B, long ldb, T beta, T* C, long ldc, bool lowerOut = false,
          GemmKLimit klim = kKFull, const T* w = nullptr, int splitK = 1);
 
+// C = alpha op(A) op(B) (n x n, K) on the 128 x 128 diagonal tiles and the `band` tile rows
+// below them only: element (i, j) with i / 128 - j / 128 in [0, band] is written, the rest of C
+// is untouched
+template <typename T>
+int gemm_band(hipStream_t s, bool ta, bool tb, int n, int K, T alpha, const T* A, long lda,
+              const T* B, long ldb, T* C, long ldc, int band);
+constexpr int kGemmTile = 128;
+
 // tile order of full-output GEMMs: 0 the launch grid's, 1 XCD-grouped (the default)
 int set_gemm_order(int order);
+
+// FP16 product with FP32 accumulation: C = alpha diag(rowScale) A^T B (rowScale may be null),
+// A (K x M) and B (K x N) column-major FP16 with K contiguous. The operands are padded: lda and
+// ldb multiples of kF16LdMult and >= f16_ld(K), at least f16_cols(M) (f16_cols(N)) columns, the
+// padding (rows K .. f16_ld(K) - 1 and the columns past M or N) zero.
+constexpr int kF16LdMult = 64, kF16ColMult = 128;
+inline long f16_ld(long k) { return (k + kF16LdMult - 1) / kF16LdMult * kF16LdMult; }
+inline long f16_cols(long m) { return (m + kF16ColMult - 1) / kF16ColMult * kF16ColMult; }
+int gemm_f16_tn(hipStream_t s, int M, int N, int K, float alpha, const _Float16* A, long lda,
+                const _Float16* B, long ldb, float* C, long ldc, const float* rowScale);
 
 // Batched GEMM over device-resident descriptors; maxM/maxN bound the tile grid.
 template <typename T>
@@ -190,6 +208,11 @@ int merge_complex(hipStream_t s, const double* re, const double* im, long ld, do
 // dst(j, i) = src(i, j) for the m x n matrix src (dst is n x m), converted to Tout
 template <typename Tin, typename Tout>
 int transpose2d(hipStream_t s, const Tin* src, long lds, Tout* dst, long ldd, int m, int n);
+// FP16 copies of the m x n matrix x, or of the residual x - diag(lam) y (y given), rows scaled
+// by rsc (given): same(i, j) (ld1) and/or the transpose tr(j, i) (ld2); either may be null
+int f16_convert(hipStream_t s, const double* x, const double* y, const double* lam,
+                const double* rsc, long ldx, int m, int n, _Float16* same, long ld1, _Float16* tr,
+                long ld2);
 // dst = a + b with b in FP32 (m x n)
 int add_f32(hipStream_t s, const double* a, long lda, const float* b, long ldb, double* dst,
             long ldd, int m, int n);

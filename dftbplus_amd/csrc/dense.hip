@@ -189,6 +189,49 @@ int add_f32(hipStream_t s, const double* a, long lda, const float* b, long ldb, 
   return 0;
 }
 
+// FP16 operands of the refinement's products, from the m x n FP64 matrix x (or the residual
+// x - diag(lam) y when y is given), each row i scaled by rsc[i] (when given): same(i, j) and
+// tr(j, i). 64 x 64 tiles through LDS, so that reads and both writes run along columns.
+__global__ void f16_convert_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                   const double* __restrict__ lam, const double* __restrict__ rsc,
+                                   long ldx, int m, int n, _Float16* __restrict__ same, long ld1,
+                                   _Float16* __restrict__ tr, long ld2) {
+  __shared__ float t[64][65];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int i = i0 + tx;
+  const double li = (y && i < m) ? lam[i] : 0.0;
+  const double si = (rsc && i < m) ? rsc[i] : 1.0;
+  for (int jj = ty; jj < 64; jj += 4) {
+    const int j = j0 + jj;
+    float v = 0.0f;
+    if (i < m && j < n) {
+      double d = x[i + (long)j * ldx];
+      if (y) d -= li * y[i + (long)j * ldx];
+      v = (float)(d * si);
+      if (same) same[i + (long)j * ld1] = (_Float16)v;
+    }
+    t[jj][tx] = v;
+  }
+  if (!tr) return;
+  __syncthreads();
+  for (int ii = ty; ii < 64; ii += 4) {
+    const int j = j0 + tx, r = i0 + ii;
+    if (r < m && j < n) tr[j + (long)r * ld2] = (_Float16)t[tx][ii];
+  }
+}
+
+int f16_convert(hipStream_t s, const double* x, const double* y, const double* lam,
+                const double* rsc, long ldx, int m, int n, _Float16* same, long ld1, _Float16* tr,
+                long ld2) {
+  if (m <= 0 || n <= 0) return 0;
+  DH_CHECK(!y || lam, "f16_convert: the residual needs lam");
+  hipLaunchKernelGGL(f16_convert_kernel, dim3(cdiv(m, 64), cdiv(n, 64)), dim3(256), 0, s, x, y,
+                     lam, rsc, ldx, m, n, same, ld1, tr, ld2);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
 template int transpose2d<double, double>(hipStream_t, const double*, long, double*, long, int, int);
 template int transpose2d<float, double>(hipStream_t, const float*, long, double*, long, int, int);
 
@@ -371,6 +414,7 @@ template int fill<double>(hipStream_t, double*, size_t, double);
 template int fill<float>(hipStream_t, float*, size_t, float);
 template int fill<int>(hipStream_t, int*, size_t, int);
 template int fill<long>(hipStream_t, long*, size_t, long);
+template int fill<_Float16>(hipStream_t, _Float16*, size_t, _Float16);
 template int fill2d<double>(hipStream_t, double*, long, int, int, double);
 template int fill2d<float>(hipStream_t, float*, long, int, int, float);
 template int copy2d<double>(hipStream_t, const double*, long, double*, long, int, int);

@@ -18,6 +18,27 @@ int oa_correction_t(hipStream_t s, const double* M, const double* G, int n, doub
 // the angle of the exact 2x2 problem
 int set_oa_mode(int mode);
 
+// Residual form of the same step for nv vectors held transposed (Xt, YH^T = (H X)^T and
+// YS^T = (S X)^T, nv x n, leading dimension ld), with G = X^T S X (FP64; its diagonal and, for
+// the correction, its lower triangle, or only the band of 128 x 128 tiles of gemm_band):
+//  * oa_rayleigh: lam_j = x_j^T H x_j / G_jj (FP64 row dot products; part: 64 nv doubles);
+//  * oa_residual_scale: per vector j a power of two rsc_j with max_k |r_j(k)| rsc_j in
+//    [2^14, 2^15), r_j = (H - lam_j S) x_j, and rinv_j = 1 / rsc_j (FP32);
+//  * oa_correction_f16: from Ft(j, i) = x_i^T r_j (FP32, leading dimension ldf), the correction
+//    E(i, j) * 2^15 in FP16 at E16[i + j lde] (the off-diagonal first-order terms
+//    x_i^T r_j / (lam_j - lam_i); pairs outside the first-order range and the diagonal from G,
+//    as in oa_correction_t, which gives the same E up to the rounding of Ft). band >= 0: G
+//    holds the tile band of gemm_band only; pairs outside the first-order range beyond it are
+//    counted in nFar and left uncorrected (repeat with the whole lower triangle, band -1).
+constexpr float kOaF16Scale = 32768.0f;
+int oa_rayleigh(hipStream_t s, const double* Xt, const double* YH, long ld, int nv, int n,
+                const double* G, long ldg, double* part, double* lam);
+int oa_residual_scale(hipStream_t s, const double* YH, const double* YS, long ld, int nv, int n,
+                      const double* lam, double* part, double* rsc, float* rinv);
+int oa_correction_f16(hipStream_t s, const float* Ft, long ldf, const double* G, long ldg,
+                      const double* lam, int n, _Float16* E16, long lde, int band, int* nClipped,
+                      int* nFar);
+
 struct EigenSolver {
   int n = 0;
   bool factored = false;

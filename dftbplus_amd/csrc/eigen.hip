@@ -219,6 +219,149 @@ int oa_correction_t(hipStream_t s, const double* M, const double* G, int n, doub
   return 0;
 }
 
+// ---- residual form (FP16 products) ----------------------------------------------------------
+constexpr int kOaChunks = 64;   // k-chunks of the row reductions (partial results per chunk)
+
+// part[c nv + j] = sum over the chunk c of k of a(j, k) b(j, k) (mode 0), or the maximum of
+// |a(j, k) - lam_j b(j, k)| (mode 1); threads run along j, so every load is coalesced
+__global__ void oa_rowreduce_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                    long ld, int nv, int n, const double* __restrict__ lam,
+                                    int mode, double* __restrict__ part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  const int chunk = (n + kOaChunks - 1) / kOaChunks;
+  const int k0 = blockIdx.y * chunk, k1 = min(n, k0 + chunk);
+  double acc = 0.0;
+  if (mode == 0) {
+    for (int k = k0; k < k1; ++k) acc = fma(a[j + (long)k * ld], b[j + (long)k * ld], acc);
+  } else {
+    const double l = lam[j];
+    for (int k = k0; k < k1; ++k)
+      acc = fmax(acc, fabs(a[j + (long)k * ld] - l * b[j + (long)k * ld]));
+  }
+  part[(long)blockIdx.y * nv + j] = acc;
+}
+
+__global__ void oa_rayleigh_kernel(const double* __restrict__ part, int nv,
+                                   const double* __restrict__ G, long ldg, double* __restrict__ lam) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  double m = 0.0;
+  for (int c = 0; c < kOaChunks; ++c) m += part[(long)c * nv + j];
+  lam[j] = m / G[j + (long)j * ldg];
+}
+
+__global__ void oa_scale_kernel(const double* __restrict__ part, int nv, double* __restrict__ rsc,
+                                float* __restrict__ rinv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  double mx = 0.0;
+  for (int c = 0; c < kOaChunks; ++c) mx = fmax(mx, part[(long)c * nv + j]);
+  // max |r_j| 2^p in [2^14, 2^15); p bounded so that 2^-p stays a normal FP32 number
+  const int p = mx > 0.0 ? min(100, max(-100, 14 - ilogb(mx))) : 0;
+  rsc[j] = ldexp(1.0, p);
+  rinv[j] = ldexpf(1.0f, -p);
+}
+
+int oa_rayleigh(hipStream_t s, const double* Xt, const double* YH, long ld, int nv, int n,
+                const double* G, long ldg, double* part, double* lam) {
+  if (nv <= 0) return 0;
+  hipLaunchKernelGGL(oa_rowreduce_kernel, dim3(cdiv(nv, 256), kOaChunks), dim3(256), 0, s, Xt, YH,
+                     ld, nv, n, (const double*)nullptr, 0, part);
+  DH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(oa_rayleigh_kernel, dim3(cdiv(nv, 256)), dim3(256), 0, s, part, nv, G, ldg,
+                     lam);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int oa_residual_scale(hipStream_t s, const double* YH, const double* YS, long ld, int nv, int n,
+                      const double* lam, double* part, double* rsc, float* rinv) {
+  if (nv <= 0) return 0;
+  hipLaunchKernelGGL(oa_rowreduce_kernel, dim3(cdiv(nv, 256), kOaChunks), dim3(256), 0, s, YH, YS,
+                     ld, nv, n, lam, 1, part);
+  DH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(oa_scale_kernel, dim3(cdiv(nv, 256)), dim3(256), 0, s, part, nv, rsc, rinv);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+// 64 x 64 tiles of (j, i): Ft and G read along j, E16 written along i through LDS
+// G is read from its lower triangle, (max(i, j), min(i, j)), and only where it is used: on the
+// diagonal and for the pairs outside the first-order range. band >= 0: G holds only the
+// 128 x 128 diagonal tiles and `band` tile rows below them; a pair outside the first-order
+// range whose element lies outside that band is counted in nFar and left uncorrected (the
+// caller then forms the whole lower triangle and repeats the correction).
+__global__ __launch_bounds__(256) void oa_correction_f16_kernel(
+    const float* __restrict__ Ft, long ldf, const double* __restrict__ G, long ldg,
+    const double* __restrict__ lam, int n, _Float16* __restrict__ E16, long lde, int jacobi,
+    int band, int* __restrict__ nClipped, int* __restrict__ nFar) {
+  __shared__ float t[64][65];
+  const int j0 = blockIdx.x * 64, i0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int j = j0 + tx;
+  const double lj = j < n ? lam[j] : 0.0;
+  int nclip = 0, nfar = 0;
+  for (int ii = ty; ii < 64; ii += 4) {
+    const int i = i0 + ii;
+    double e = 0.0;
+    if (j < n && i < n) {
+      const int hi = max(i, j), lo = min(i, j);
+      const long qg = hi + (long)lo * ldg;
+      if (i == j) {
+        e = 0.5 * (1.0 - G[qg]);
+      } else {
+        const double num = (double)Ft[j + (long)i * ldf], den = lj - lam[i];
+        if (fabs(num) < kOaMax * fabs(den)) {
+          e = num / den;
+        } else if (band >= 0 && hi / kGemmTile - lo / kGemmTile > band) {
+          nfar += j > i;
+        } else {
+          const double r = -G[qg];
+          nclip += j > i;   // each pair counted once
+          if (jacobi) {
+            const double mij = num - lj * r;   // x_i^T H x_j
+            const double th =
+                den != 0.0 ? 0.5 * atan(2.0 * mij / den) : copysign(0.25 * M_PI, mij);
+            e = 0.5 * r + sin(th);
+          } else {
+            e = 0.5 * r;
+          }
+        }
+      }
+    }
+    // |E| <= 1/2 for S-normalised vectors (|G_ij| <= 1) and the first-order bound kOaMax: the
+    // scaled value stays below the FP16 maximum
+    t[ii][tx] = (float)(e * kOaF16Scale);
+  }
+  // one atomic per wave and counter
+  int w = nclip, f = nfar;
+  for (int o = 32; o > 0; o >>= 1) {
+    w += __shfl_xor(w, o);
+    f += __shfl_xor(f, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (nClipped && w != 0) atomicAdd(nClipped, w);
+    if (nFar && f != 0) atomicAdd(nFar, f);
+  }
+  __syncthreads();
+  for (int jj = ty; jj < 64; jj += 4) {
+    const int i = i0 + tx, jo = j0 + jj;
+    if (i < n && jo < n) E16[i + (long)jo * lde] = (_Float16)t[tx][jj];
+  }
+}
+
+int oa_correction_f16(hipStream_t s, const float* Ft, long ldf, const double* G, long ldg,
+                      const double* lam, int n, _Float16* E16, long lde, int band, int* nClipped,
+                      int* nFar) {
+  if (n <= 0) return 0;
+  DH_CHECK(band < 0 || nFar, "oa_correction_f16: a banded G needs the far-pair counter");
+  hipLaunchKernelGGL(oa_correction_f16_kernel, dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, s,
+                     Ft, ldf, G, ldg, lam, n, E16, lde, g_oaJacobi, band, nClipped, nFar);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
 int EigenSolver::refine(hipStream_t s, const double* H, const double* S, const float* X0,
                         int steps, int nvec, double* X, long ldx, std::vector<double>& lamHost) {
   DH_CHECK(workSize >= 2 * (size_t)n * n, "refine: workspace too small");

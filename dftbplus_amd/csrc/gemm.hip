@@ -1,5 +1,6 @@
 // MFMA GEMM core for CDNA4 (gfx950), FP64 (v_mfma_f64_16x16x4f64) and FP32
-// (v_mfma_f32_16x16x4f32).
+// (v_mfma_f32_16x16x4f32), and an FP16 product with FP32 accumulation
+// (v_mfma_f32_16x16x32_f16) for the correction steps of the eigenpair refinement.
 //
 //   C = alpha * op(A) * diag(w) * op(B) + beta * C        (column-major, LAPACK convention)
 //
@@ -15,6 +16,7 @@
 namespace dftbhip {
 
 constexpr int kBM = 128, kBN = 128, kBK = 16, kNT = 256, kPad = 1;
+static_assert(kBM == kGemmTile && kBN == kGemmTile, "band tiles");
 
 // tile order of full-output GEMMs: 0 the launch grid's (x = tile row), 1 XCD-grouped
 static int g_gemmOrder = 1;
@@ -210,9 +212,14 @@ template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(int M, int N, int K, T alpha, const T* A,
                                                    long lda, const T* B, long ldb, T beta, T* C,
                                                    long ldc, int lowerOut, int klim, const T* w,
-                                                   int grouped) {
+                                                   int grouped, int band) {
   int tm, tn;
-  if (lowerOut) {
+  if (band >= 0) {
+    // the diagonal tiles and the band tile rows below them: tile (tn + b, tn), b = 0 .. band
+    tn = (int)blockIdx.x / (band + 1);
+    tm = tn + (int)blockIdx.x % (band + 1);
+    if (tm * kBM >= M) return;
+  } else if (lowerOut) {
     lower_tile_index(blockIdx.x, tm, tn);
   } else if (grouped) {
     const int gm = (M + kBM - 1) / kBM, gn = (N + kBN - 1) / kBN;
@@ -262,7 +269,7 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
   }
 #define DH_GEMM_CASE(TA_, TB_)                                                           \
   hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(kNT), 0, s, M, N, K, alpha, A, \
-                     lda, B, ldb, beta, C, ldc, (int)lowerOut, (int)klim, w, (int)grouped)
+                     lda, B, ldb, beta, C, ldc, (int)lowerOut, (int)klim, w, (int)grouped, -1)
   if (!ta && !tb) DH_GEMM_CASE(false, false);
   else if (!ta && tb) DH_GEMM_CASE(false, true);
   else if (ta && !tb) DH_GEMM_CASE(true, false);
@@ -271,6 +278,27 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
   DH_LAUNCH_CHECK();
   return 0;
 }
+
+template <typename T>
+int gemm_band(hipStream_t s, bool ta, bool tb, int n, int K, T alpha, const T* A, long lda,
+              const T* B, long ldb, T* C, long ldc, int band) {
+  if (n <= 0) return 0;
+  DH_CHECK(band >= 0, "gemm_band: negative band");
+  const long t = cdiv(n, kBM);
+  const dim3 grid((unsigned)(t * (band + 1)), 1, 1);
+#define DH_GEMM_BAND(TA_, TB_)                                                           \
+  hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(kNT), 0, s, n, n, K, alpha, A, \
+                     lda, B, ldb, T(0), C, ldc, 0, (int)kKFull, (const T*)nullptr, 0, band)
+  if (!ta && !tb) DH_GEMM_BAND(false, false);
+  else if (!ta && tb) DH_GEMM_BAND(false, true);
+  else if (ta && !tb) DH_GEMM_BAND(true, false);
+  else DH_GEMM_BAND(true, true);
+#undef DH_GEMM_BAND
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+template int gemm_band<double>(hipStream_t, bool, bool, int, int, double, const double*, long,
+                               const double*, long, double*, long, int);
 
 template <typename T>
 int gemm_batched(hipStream_t s, bool ta, bool tb, const GemmDesc<T>* dDesc, int nBatch, int maxM,
@@ -285,6 +313,125 @@ int gemm_batched(hipStream_t s, bool ta, bool tb, const GemmDesc<T>* dDesc, int 
   else if (ta && !tb) DH_GEMMB_CASE(true, false);
   else DH_GEMMB_CASE(true, true);
 #undef DH_GEMMB_CASE
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP16 product C = alpha diag(rs) A^T B (FP32 accumulation and output), A (K x M) and B (K x N)
+// column-major with K contiguous, so that every MFMA operand (8 consecutive k of one row or
+// column) is one 16-byte load. The operands are padded (f16_ld, f16_cols): no bounds checks in
+// the K loop. Workgroup tile 128 x 128, 4 waves of 64 x 64 (4 x 4 MFMA blocks of 16 x 16), K
+// staged through LDS in steps of 64 with a register prefetch of the next step; rows of the LDS
+// tiles padded by 16 bytes so that the 16 rows one operand load reads fall on distinct banks.
+// ------------------------------------------------------------------------------------------
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+constexpr int kHB = 128, kHK = 64, kHRow = kHK + 8;
+
+__device__ __forceinline__ void wave_lds_wait() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt untouched (gfx9 encoding)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__global__ __launch_bounds__(256) void gemm_f16_tn_kernel(int M, int N, int Kp, float alpha,
+                                                          const _Float16* __restrict__ A, long lda,
+                                                          const _Float16* __restrict__ B, long ldb,
+                                                          float* __restrict__ C, long ldc,
+                                                          const float* __restrict__ rs, int gm,
+                                                          int gn) {
+  int tm, tn;
+  if (!grouped_tile_index((int)blockIdx.x, gm, gn, (int)gridDim.x, tm, tn)) return;
+  __shared__ __attribute__((aligned(16))) _Float16 smem[2 * kHB * kHRow];
+  _Float16* As = smem;
+  _Float16* Bs = smem + kHB * kHRow;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long i0 = (long)tm * kHB, j0 = (long)tn * kHB;
+  v4f acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = v4f{0.f, 0.f, 0.f, 0.f};
+  // 16-byte chunk c = tid + 256 q of a tile: row c / 8, k offset 8 (c % 8)
+  v8h ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = tid + 256 * q, row = c >> 3, kc = (c & 7) * 8;
+      ra[q] = *reinterpret_cast<const v8h*>(A + (i0 + row) * lda + k0 + kc);
+      rb[q] = *reinterpret_cast<const v8h*>(B + (j0 + row) * ldb + k0 + kc);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = tid + 256 * q, row = c >> 3, kc = (c & 7) * 8;
+      *reinterpret_cast<v8h*>(As + row * kHRow + kc) = ra[q];
+      *reinterpret_cast<v8h*>(Bs + row * kHRow + kc) = rb[q];
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < Kp; k0 += kHK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + kHK < Kp) gload(k0 + kHK);
+#pragma unroll
+    for (int kk = 0; kk < kHK / 32; ++kk) {
+      const int ko = kk * 32 + (lane >> 4) * 8;
+      v8h fa[4], fb[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        fa[m] = *reinterpret_cast<const v8h*>(As + (wm * 64 + m * 16 + (lane & 15)) * kHRow + ko);
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+        fb[n] = *reinterpret_cast<const v8h*>(Bs + (wn * 64 + n * 16 + (lane & 15)) * kHRow + ko);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[m], fb[n], acc[m][n], 0, 0, 0);
+    }
+  }
+  // accumulator element r of lane l: D(4 (l / 16) + r, l % 16) of its 16 x 16 block (the FP32
+  // layout, pinned by the probe below); staged per wave as 16 columns x 64 rows and written one
+  // column per store (64 consecutive rows)
+  __syncthreads();
+  float* Cs = reinterpret_cast<float*>(smem) + wave * 16 * 65;
+  const long gi = i0 + wm * 64 + lane;
+  const float sc = alpha * ((rs && gi < M) ? rs[gi] : 1.0f);
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Cs[(lane & 15) * 65 + m * 16 + (lane >> 4) * 4 + r] = acc[m][n][r];
+    wave_lds_wait();
+    if (gi < M) {
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc) {
+        const long gj = j0 + wn * 64 + n * 16 + cc;
+        if (gj < N) C[gi + gj * ldc] = sc * Cs[cc * 65 + lane];
+      }
+    }
+    wave_lds_wait();
+  }
+}
+
+int gemm_f16_tn(hipStream_t s, int M, int N, int K, float alpha, const _Float16* A, long lda,
+                const _Float16* B, long ldb, float* C, long ldc, const float* rowScale) {
+  if (M <= 0 || N <= 0) return 0;
+  const long Kp = f16_ld(K);
+  DH_CHECK(lda % kF16LdMult == 0 && ldb % kF16LdMult == 0 && lda >= Kp && ldb >= Kp && ldc >= M,
+           "gemm_f16_tn: leading dimensions");
+  if (Kp == 0) return fill2d<float>(s, C, ldc, M, N, 0.0f);
+  const int gm = (int)cdiv(M, kHB), gn = (int)cdiv(N, kHB);
+  const long tiles = (long)gm * gn;
+  const long grid = (tiles + kXcds - 1) / kXcds * kXcds;
+  DH_CHECK(grid < (1L << 31), "gemm_f16_tn: grid too large");
+  hipLaunchKernelGGL(gemm_f16_tn_kernel, dim3((unsigned)grid), dim3(256), 0, s, M, N, (int)Kp,
+                     alpha, A, lda, B, ldb, C, ldc, rowScale, gm, gn);
   DH_LAUNCH_CHECK();
   return 0;
 }
@@ -327,7 +474,64 @@ __global__ void mfma_layout_probe_f32_kernel(double* out) {
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
   for (int r = 0; r < 4; ++r) out[256 + 4 * lane + r] = acc[r];
 }
+
+// The same for the FP16 16x16x32 MFMA (FP32 accumulator): element e of the A operand of lane l
+// is taken as A(l % 16, 8 (l / 16) + e) and of the B operand as B(8 (l / 16) + e, l % 16); a
+// product only needs A and B to agree on the k of each (lane, element), which they do here.
+__global__ void mfma_layout_probe_f16_kernel(double* out) {
+  const int lane = threadIdx.x;
+  const int i = lane & 15, j = lane & 15;
+  v8h a, b;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int k = 8 * (lane >> 4) + e;
+    a[e] = (_Float16)(k == 0 ? (float)i : (k == 1 ? 1.0f : 0.0f));
+    b[e] = (_Float16)(k == 0 ? 1.0f : (k == 1 ? 100.0f * j : 0.0f));
+  }
+  v4f acc = {0, 0, 0, 0};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[4 * lane + r] = acc[r];
+}
 }  // namespace dftbhip
+
+extern "C" int dftbhip_probe_mfma_f16(double* out) {
+  using namespace dftbhip;
+  DH_CHECK(out, "probe: null pointer");
+  DevBuf<double> d;
+  DH_TRY(d.alloc(256));
+  hipLaunchKernelGGL(mfma_layout_probe_f16_kernel, dim3(1), dim3(64), 0, 0, d.p);
+  DH_LAUNCH_CHECK();
+  DH_HIP(hipMemcpy(out, d.p, 256 * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// FP16 product on host arrays (FP16 bit patterns): c = alpha a^T b, a (k x m), b (k x n), c
+// (m x n), all column-major
+extern "C" int dftbhip_gemm_f16_tn(int m, int n, int k, float alpha, const uint16_t* a, int lda,
+                                   const uint16_t* b, int ldb, float* c, int ldc) {
+  using namespace dftbhip;
+  DH_CHECK(m >= 0 && n >= 0 && k >= 0, "gemm_f16_tn: negative size");
+  DH_CHECK(a && b && c, "gemm_f16_tn: null pointer");
+  DH_CHECK(lda >= std::max(1, k) && ldb >= std::max(1, k) && ldc >= std::max(1, m),
+           "gemm_f16_tn: bad leading dimension");
+  if (m == 0 || n == 0) return 0;
+  const long ldp = std::max<long>(f16_ld(k), kF16LdMult);
+  DevBuf<_Float16> A, B;
+  DevBuf<float> Cd;
+  DH_TRY(A.alloc((size_t)ldp * f16_cols(m)));
+  DH_TRY(B.alloc((size_t)ldp * f16_cols(n)));
+  DH_TRY(Cd.alloc((size_t)m * n));
+  DH_HIP(hipMemset(A.p, 0, A.n * sizeof(_Float16)));
+  DH_HIP(hipMemset(B.p, 0, B.n * sizeof(_Float16)));
+  if (k > 0) {
+    DH_HIP(hipMemcpy2D(A.p, ldp * 2, a, (size_t)lda * 2, (size_t)k * 2, m, hipMemcpyHostToDevice));
+    DH_HIP(hipMemcpy2D(B.p, ldp * 2, b, (size_t)ldb * 2, (size_t)k * 2, n, hipMemcpyHostToDevice));
+  }
+  DH_TRY(gemm_f16_tn(0, m, n, k, alpha, A.p, ldp, B.p, ldp, Cd.p, m, nullptr));
+  DH_HIP(hipMemcpy2D(c, (size_t)ldc * 4, Cd.p, (size_t)m * 4, (size_t)m * 4, n,
+                     hipMemcpyDeviceToHost));
+  return 0;
+}
 
 extern "C" int dftbhip_probe_mfma_f64(double* out) {
   using namespace dftbhip;

@@ -119,17 +119,22 @@ template <typename T>
 int launch_unpack_rows(hipStream_t s, const RowBlocksDev& rb, const int* norb,
                        const long* iAtomStart, const long* blkStart, const double* x, T* dense,
                        long ld, int n);
-// YH^T = (H X)^T and YS^T = (S X)^T with Xt = X^T (nvec x n, leading dimension ldx) and the
-// outputs nvec x n (leading dimension ldy); H, S: sparse h and ov in the block layout of rb,
-// at the Gamma point (blkT null) or at the time-reversal-invariant k-point kf (fractional; the
-// blocks times their phases +-1 from the fractional image translations blkT)
-int launch_spmm_hs_t(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+// Dense 4 x 4 blocks of H and S per group of the row view (row atom, column atom: image blocks
+// summed, at a time-reversal-invariant k-point kf (fractional; blkT the fractional image
+// translations) with their phases +-1), fold[32 g .. 32 g + 31] = H(r, c) at 4 r + c, then S,
+// zero past the atoms' orbitals; colOff[g] = first orbital of the column atom. fold holds 32
+// doubles and colOff one int per group of rb (gPtr[nat] groups).
+int launch_spmm_fold(hipStream_t s, const RowBlocksDev& rb, const int* norb,
                      const long* iAtomStart, const long* blkStart, const double* h,
-                     const double* ov, const double* Xt, long ldx, int nvec, double* yh,
-                     double* ys, long ldy, const double* blkT = nullptr,
-                     const double* kf = nullptr);
-// vectors per tile of the block-sparse products (64, 128 or 256, the default)
-int set_spmm_cols(int cols);
+                     const double* ov, const double* blkT, const double* kf, double* fold,
+                     int* colOff);
+// YH^T = (H X)^T and YS^T = (S X)^T with Xt = X^T (nvec x n, leading dimension ldx) and the
+// outputs nvec x n (leading dimension ldy), H and S from the folded blocks
+int launch_spmm_hs_t(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+                     const long* iAtomStart, int n, const double* fold, const int* colOff,
+                     const double* Xt, long ldx, int nvec, double* yh, double* ys, long ldy);
+// vectors per lane of the block-sparse products (1, the default, 2 or 4)
+int set_spmm_cols(int vpl);
 // Mulliken populations qorb and band-energy contributions eorb (may be null) per orbital from
 // the row view (no atomics; every orbital of the rb.nat atoms written)
 int launch_mulliken_rows(hipStream_t s, const RowBlocksDev& rb, const int* norb,

@@ -861,149 +861,157 @@ int launch_repulsive(hipStream_t s, const NeighDev& nb, const double* coords, co
 // Block-sparse products with a transposed dense operand: YH^T = (H X)^T and YS^T = (S X)^T for
 // X given as Xt = X^T (nvec x n, column-major, so the nvec coefficients of one orbital are
 // contiguous). H and S are the matrices unpack builds from the sparse blocks (image blocks of one
-// atom pair summed), taken row atom by row atom from the row view rb. A workgroup of 256 threads
-// owns 256 / kCols consecutive row atoms and a tile of kCols consecutive vectors j (one thread per
-// atom and vector): per entry a thread reads the block (shared by the threads of its atom) and
-// the <= 4 contiguous coefficient runs of the column atom, and keeps the <= 4 outputs of its
-// atom's orbitals for both matrices in registers. Workgroups are numbered so that consecutive
-// ones of an XCD (IDs congruent mod 8) walk the atoms of one vector tile: every coefficient run
-// of that tile is read by all the row atoms neighbouring its atom, and narrow tiles (kCols = 32:
-// 32 vectors x n orbitals x 8 B = 4 MB at n = 15769) keep the whole tile in the XCD's L2 for
-// those reads.
+// atom pair summed).
+//  * spmm_fold_kernel: once per H and S, the row view's groups (row atom A, column atom B) become
+//    dense 4 x 4 blocks of both matrices (image blocks summed with their k-point phases +-1,
+//    transposed where the stored block is (B, A), zero-padded past the atoms' orbitals), 32
+//    consecutive doubles per group in row order, with the column atom's first orbital beside;
+//  * spmm_fold_t_kernel: a workgroup of 4 waves owns one row atom and 256 kVpl consecutive
+//    vectors (kVpl per lane, 64 apart). Per group the wave reads the block's 32 values as
+//    wave-uniform scalar loads from contiguous memory, the 4 coefficient rows of the column atom
+//    (kVpl coalesced loads each) and does 32 kVpl FMAs; the outputs of the atom's 4 orbitals
+//    for both matrices stay in registers. Workgroups are numbered so that consecutive ones of an
+//    XCD (IDs congruent mod 8) walk the atoms of one vector tile, so the coefficient rows a tile
+//    reads stay in that XCD's L2 while its neighbouring row atoms read them.
 // ---------------------------------------------------------------------------------------------
 constexpr int kXcds = 8;
-static int g_spmmCols = 256;
-int set_spmm_cols(int cols) {
-  DH_CHECK(cols == 64 || cols == 128 || cols == 256, "spmm tile: 64, 128 or 256 vectors");
-  g_spmmCols = cols;
+static int g_spmmVpl = 1;
+int set_spmm_cols(int vpl) {
+  DH_CHECK(vpl == 1 || vpl == 2 || vpl == 4, "spmm: 1, 2 or 4 vectors per lane");
+  g_spmmVpl = vpl;
   return 0;
 }
 
-// one entry: y(r) += B(r, c) x(c) for H and S, B the block as stored (TR false: element (r, c) at
-// r + c NA) or transposed (TR true: at c + r NB); the block's NA NB values of each matrix are
-// loaded first, together (wave-uniform scalar loads), then used from registers
-template <int NA, int NB, bool TR>
-__device__ __forceinline__ void spmm_entry(const double* __restrict__ h,
-                                           const double* __restrict__ ov, long st,
-                                           const double (&xv)[4], double (&ah)[4],
-                                           double (&as)[4]) {
-  double bh[NA * NB], bs[NA * NB];
-#pragma unroll
-  for (int q = 0; q < NA * NB; ++q) {
-    bh[q] = h[st + q];
-    bs[q] = ov[st + q];
-  }
-#pragma unroll
-  for (int r = 0; r < NA; ++r)
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      const int q = TR ? (c + r * NB) : (r + c * NA);
-      ah[r] = fma(bh[q], xv[c], ah[r]);
-      as[r] = fma(bs[q], xv[c], as[r]);
-    }
-}
-
-template <int NA, int NB>
-__device__ __forceinline__ void spmm_entry(const double* __restrict__ h,
-                                           const double* __restrict__ ov, long st, int tr,
-                                           const double (&xv)[4], double (&ah)[4],
-                                           double (&as)[4]) {
-  if (tr) spmm_entry<NA, NB, true>(h, ov, st, xv, ah, as);
-  else spmm_entry<NA, NB, false>(h, ov, st, xv, ah, as);
-}
-
-template <int kCols>
-__global__ __launch_bounds__(256) void spmm_hs_t_kernel(
-    int nat, int ntile, const int* __restrict__ rowPtr, const int2* __restrict__ ent,
+__global__ __launch_bounds__(256) void spmm_fold_kernel(
+    const int* __restrict__ gPtr, const int2* __restrict__ grp, const int2* __restrict__ entS,
     const int* __restrict__ norbAtom, const long* __restrict__ iAtomStart,
     const long* __restrict__ blkStart, const double* __restrict__ h,
-    const double* __restrict__ ov, const double* __restrict__ Xt, long ldx, int nvec,
-    double* __restrict__ yh, double* __restrict__ ys, long ldy, const double* __restrict__ blkT,
-    int m0, int m1, int m2) {
-  static_assert(kCols % 64 == 0, "one row atom per wave at most");
-  constexpr int kApw = 256 / kCols;   // row atoms per workgroup
-  const int nag = (nat + kApw - 1) / kApw;
+    const double* __restrict__ ov, const double* __restrict__ blkT, int m0, int m1, int m2,
+    double* __restrict__ fold, int* __restrict__ colOff) {
+  const int A = blockIdx.x;
+  const int nA = norbAtom[A];
+  const int g0 = gPtr[A], g1 = gPtr[A + 1];
+  for (int t = threadIdx.x; t < (g1 - g0) * 32; t += blockDim.x) {
+    const int g = g0 + (t >> 5), el = t & 31;
+    const int mat = el >> 4, r = (el >> 2) & 3, c = el & 3;
+    const int2 gr = grp[g];
+    const int B = entS[gr.x].y, nB = norbAtom[B];
+    const double* __restrict__ src = mat ? ov : h;
+    double v = 0.0;
+    if (r < nA && c < nB) {
+      for (int e = gr.x; e < gr.y; ++e) {
+        const int k = entS[e].x >> 1, tr = entS[e].x & 1;
+        const long q = blkStart[k] + (tr ? (c + (long)r * nB) : (r + (long)c * nA));
+        double x = src[q];
+        if (blkT) {
+          const long par =
+              llround(m0 * blkT[3 * k] + m1 * blkT[3 * k + 1] + m2 * blkT[3 * k + 2]);
+          if (par & 1) x = -x;
+        }
+        v += x;
+      }
+    }
+    fold[(long)g * 32 + el] = v;
+    if (el == 0) colOff[g] = (int)iAtomStart[B];
+  }
+}
+
+template <int kVpl>
+__global__ __launch_bounds__(256) void spmm_fold_t_kernel(
+    int nat, int ntile, int n, const int* __restrict__ gPtr, const int* __restrict__ colOff,
+    const double* __restrict__ fold, const int* __restrict__ norbAtom,
+    const long* __restrict__ iAtomStart, const double* __restrict__ Xt, long ldx, int nvec,
+    double* __restrict__ yh, double* __restrict__ ys, long ldy) {
   const long w = blockIdx.x;
   const int xcd = (int)(w % kXcds);
   const long l = w / kXcds;
-  const int tile = xcd + kXcds * (int)(l / nag);
-  // the row atom is the same in the whole wave: its entries and blocks are scalar loads
-  const int A = __builtin_amdgcn_readfirstlane((int)(l % nag) * kApw +
-                                               (int)threadIdx.x / kCols);
-  if (tile >= ntile || A >= nat) return;
-  const int j = tile * kCols + (int)threadIdx.x % kCols;
-  const bool live = j < nvec;
-  const double* __restrict__ xj = Xt + (live ? j : 0);
+  const int tile = xcd + kXcds * (int)(l / nat);
+  const int A = (int)(l % nat);
+  if (tile >= ntile) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int jb = tile * 256 * kVpl + wave * 64 * kVpl + lane;
+  const double* __restrict__ xj[kVpl];
+#pragma unroll
+  for (int v = 0; v < kVpl; ++v) xj[v] = Xt + min(jb + 64 * v, nvec - 1);
+  double ah[kVpl][4], as[kVpl][4];
+#pragma unroll
+  for (int v = 0; v < kVpl; ++v)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ah[v][r] = as[v][r] = 0.0;
+  const int g1 = gPtr[A + 1];
+  for (int g = gPtr[A]; g < g1; ++g) {
+    const int oB = colOff[g];
+    double xv[kVpl][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      // rows past the column atom's orbitals meet zero block columns; clamped to stay inside Xt
+      const long row = min(oB + c, n - 1);
+#pragma unroll
+      for (int v = 0; v < kVpl; ++v) xv[v][c] = xj[v][row * ldx];
+    }
+    const double* __restrict__ b = fold + (long)g * 32;
+    double bh[16], bs[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      bh[q] = b[q];
+      bs[q] = b[16 + q];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < kVpl; ++v) {
+          ah[v][r] = fma(bh[r * 4 + c], xv[v][c], ah[v][r]);
+          as[v][r] = fma(bs[r * 4 + c], xv[v][c], as[v][r]);
+        }
+  }
   const int nA = norbAtom[A];
   const long oA = iAtomStart[A];
-  double ah[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
-  const int p1 = rowPtr[A + 1];
-  for (int p = rowPtr[A]; p < p1; ++p) {
-    const int2 e = ent[p];
-    const int k = e.x >> 1, tr = e.x & 1, B = e.y;
-    const int nB = norbAtom[B];
-    const long oB = iAtomStart[B], st = blkStart[k];
-    // the column atom's coefficient runs, loaded unconditionally (runs past nB repeat the last
-    // one and are not used)
-    double xv[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) xv[c] = xj[(oB + min(c, nB - 1)) * ldx];
-    // at a time-reversal-invariant k-point (m = 2k integral) the block's phase is (-1)^(m.T)
-    if (blkT) {
-      const long par = llround(m0 * blkT[3 * k] + m1 * blkT[3 * k + 1] + m2 * blkT[3 * k + 2]);
-      if (par & 1)
+  for (int v = 0; v < kVpl; ++v) {
+    const int j = jb + 64 * v;
+    if (j >= nvec) continue;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) xv[c] = -xv[c];
+    for (int r = 0; r < 4; ++r) {
+      if (r >= nA) break;
+      yh[j + (oA + r) * ldy] = ah[v][r];
+      ys[j + (oA + r) * ldy] = as[v][r];
     }
-    if (nA == 4 && nB == 4) spmm_entry<4, 4>(h, ov, st, tr, xv, ah, as);
-    else if (nA == 4 && nB == 1) spmm_entry<4, 1>(h, ov, st, tr, xv, ah, as);
-    else if (nA == 1 && nB == 4) spmm_entry<1, 4>(h, ov, st, tr, xv, ah, as);
-    else if (nA == 1 && nB == 1) spmm_entry<1, 1>(h, ov, st, tr, xv, ah, as);
-    else {
-      // other shell sets (at most 4 orbitals per atom): unrolled with guards, so that the
-      // accumulators stay in registers
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-          if (r < nA && c < nB) {
-            const long q = st + (tr ? (c + (long)r * nB) : (r + (long)c * nA));
-            ah[r] = fma(h[q], xv[c], ah[r]);
-            as[r] = fma(ov[q], xv[c], as[r]);
-          }
-    }
-  }
-  if (!live) return;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (r >= nA) break;
-    yh[j + (oA + r) * ldy] = ah[r];
-    ys[j + (oA + r) * ldy] = as[r];
   }
 }
 
-int launch_spmm_hs_t(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+int launch_spmm_fold(hipStream_t s, const RowBlocksDev& rb, const int* norb,
                      const long* iAtomStart, const long* blkStart, const double* h,
-                     const double* ov, const double* Xt, long ldx, int nvec, double* yh,
-                     double* ys, long ldy, const double* blkT, const double* kf) {
-  if (rb.nat == 0 || nvec <= 0) return 0;
+                     const double* ov, const double* blkT, const double* kf, double* fold,
+                     int* colOff) {
+  if (rb.nat == 0) return 0;
   int m[3] = {0, 0, 0};
   if (blkT) {
     DH_CHECK(kf && kpoint_is_real(kf), "spmm: the k-point is not time-reversal invariant");
     for (int q = 0; q < 3; ++q) m[q] = (int)std::nearbyint(2.0 * kf[q]);
   }
-  const int cols = g_spmmCols, apw = 256 / cols;
-  const int ntile = (int)cdiv(nvec, cols);
+  hipLaunchKernelGGL(spmm_fold_kernel, dim3(rb.nat), dim3(256), 0, s, rb.gPtr, rb.grp, rb.entS,
+                     norb, iAtomStart, blkStart, h, ov, blkT, m[0], m[1], m[2], fold, colOff);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_spmm_hs_t(hipStream_t s, const RowBlocksDev& rb, const int* norb,
+                     const long* iAtomStart, int n, const double* fold, const int* colOff,
+                     const double* Xt, long ldx, int nvec, double* yh, double* ys, long ldy) {
+  if (rb.nat == 0 || nvec <= 0) return 0;
+  const int vpl = g_spmmVpl;
+  const int ntile = (int)cdiv(nvec, 256 * vpl);
   const long tilesPad = (long)cdiv(ntile, kXcds) * kXcds;
-  const long nwg = tilesPad * cdiv(rb.nat, apw);
+  const long nwg = tilesPad * rb.nat;
   DH_CHECK(nwg < (1L << 31), "spmm: grid too large");
-#define DH_SPMM(C_)                                                                          \
-  hipLaunchKernelGGL(spmm_hs_t_kernel<C_>, dim3((unsigned)nwg), dim3(256), 0, s, rb.nat, ntile, \
-                     rb.rowPtr, rb.ent, norb, iAtomStart, blkStart, h, ov, Xt, ldx, nvec, yh,  \
-                     ys, ldy, blkT, m[0], m[1], m[2])
-  if (cols == 64) DH_SPMM(64);
-  else if (cols == 128) DH_SPMM(128);
-  else DH_SPMM(256);
+#define DH_SPMM(V_)                                                                           \
+  hipLaunchKernelGGL(spmm_fold_t_kernel<V_>, dim3((unsigned)nwg), dim3(256), 0, s, rb.nat, ntile, \
+                     n, rb.gPtr, colOff, fold, norb, iAtomStart, Xt, ldx, nvec, yh, ys, ldy)
+  if (vpl == 1) DH_SPMM(1);
+  else if (vpl == 2) DH_SPMM(2);
+  else DH_SPMM(4);
 #undef DH_SPMM
   DH_LAUNCH_CHECK();
   return 0;

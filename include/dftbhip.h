@@ -9,6 +9,8 @@
 #ifndef DFTBHIP_H
 #define DFTBHIP_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -69,9 +71,14 @@ int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e, doub
  * param 6: Ogita-Aishima refinement of the sparse tail, pairs outside the first-order range
  * re-orthonormalised only (0, the default) or rotated by the exact 2x2 angle (1); param 7:
  * the Broyden history restarts at the iteration that seeds the mixed-precision tail (1) or is
- * kept (0, the default); param 8: the correction product X E of the tail's refinement in FP32
- * (1, the default) or FP64 (0); param 9: vectors per tile of the tail's block-sparse products
- * (64, 128 or 256, the default). No counterpart in the reference. */
+ * kept (0, the default); param 8: arithmetic of a step of the tail's refinement, 0: X^T H X,
+ * X^T S X and the correction product X E in FP64, 1: the same with X E in FP32, 2 (the
+ * default): residual form, X^T S X, the Rayleigh quotients and the residuals
+ * H X - S X diag(lam) in FP64, the products with the residuals and with E in FP16 (FP32
+ * sums); param 10: residual
+ * form, tile rows of X^T S X below its 128 x 128 diagonal tiles formed per step (1, the
+ * default; -1 the whole lower triangle); param 9: vectors per lane of the tail's
+ * block-sparse products (1, the default, 2 or 4). No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)
  * builds it: atom a lists itself, then the images of atoms b >= a within cutoff by increasing
@@ -233,9 +240,11 @@ int dftbhip_ctx_scc_end(dftbhip_ctx* ctx, double* energies, double* charges, int
 /* State of the last SCC iteration for the SCC block of detailed.out (writeDetailedOut1,
  * src/dftbp/dftbplus/mainio.F90:2794-2926): out[0..n-1] of iteration number, charge error
  * (sccErrorQ), electronic energy, electronic energy of the iteration before (NaN after the
- * first; diffElec is their difference), input electrons of the last iteration and the number
+ * first; diffElec is their difference), input electrons of the last iteration, the number
  * of eigenvector pairs outside the first-order range of the last Ogita-Aishima refinement
- * step of the mixed-precision tail (no counterpart in the reference). */
+ * step of the mixed-precision tail and (residual form of the refinement) the number of such
+ * pairs beyond the diagonal band of X^T S X in the last refinement, for which the whole
+ * X^T S X was formed (the last two have no counterpart in the reference). */
 int dftbhip_ctx_scc_state(dftbhip_ctx* ctx, double* out, int n);
 /* gradient dE/dR (3, nAtom); needs a converged dftbhip_ctx_scc */
 int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad);
@@ -308,6 +317,13 @@ int dftbhip_kernel_stats(double* out);
  *      each), out[512]: FP64 in out[0:256], FP32 in out[256:512] ----------------------------
  * (no counterpart in the reference; used by the tests to pin the MFMA tiling) */
 int dftbhip_probe_mfma_f64(double* out);
+/* the same for the FP16 16x16x32 MFMA (FP32 accumulator), out[256] */
+int dftbhip_probe_mfma_f16(double* out);
+/* FP16 product on host arrays, c = alpha a^T b: a (k x m) and b (k x n) FP16 bit patterns, c
+ * (m x n) FP32, all column-major; the kernel of the refinement's correction products
+ * (no counterpart in the reference; used by the tests) */
+int dftbhip_gemm_f16_tn(int m, int n, int k, float alpha, const uint16_t* a, int lda,
+                        const uint16_t* b, int ldb, float* c, int ldc);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,35 @@ def test_mfma_layouts():
     assert np.array_equal(f32, e32)
 
 
+def test_mfma_f16_layout():
+    """Accumulator element r of lane l of the FP16 16x16x32 MFMA holds D(4(l//16) + r, l%16),
+    as for the FP32 16x16x4 one (D(i, j) = i + 100 j in the probe)."""
+    api = _api()
+    out = np.zeros(256)
+    api._check(api.lib().dftbhip_probe_mfma_f16(api._d(out)), "probe")
+    e = np.array([[4 * (l // 16) + r + 100 * (l % 16) for r in range(4)] for l in range(64)], float)
+    assert np.array_equal(out.reshape(64, 4), e)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 53, 29), (129, 128, 64), (300, 257, 999),
+                                   (128, 200, 0)])
+def test_gemm_f16_tn(m, n, k):
+    """c = alpha a^T b with FP16 a (k x m), b (k x n) and FP32 sums: against the float64 product
+    of the same FP16 values, to the FP32 rounding of the sums (relative 1e-5 of |a|^T |b|)."""
+    api = _api()
+    rng = np.random.default_rng(m * n + k)
+    a = np.asfortranarray(rng.standard_normal((k, m)).astype(np.float16))
+    b = np.asfortranarray(rng.standard_normal((k, n)).astype(np.float16))
+    c = np.zeros((m, n), dtype=np.float32, order="F")
+    api._check(api.lib().dftbhip_gemm_f16_tn(m, n, k, 0.75, a.ctypes.data, max(k, 1),
+                                              b.ctypes.data, max(k, 1), c.ctypes.data, m),
+               "gemm_f16_tn")
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    ref = 0.75 * a64.T @ b64
+    bound = 1e-5 * (np.abs(a64).T @ np.abs(b64)) + 1e-30
+    assert np.all(np.abs(c - ref) <= bound)
+
+
 @pytest.mark.parametrize("m,n,k,ta,tb", [(1, 1, 1, "N", "N"), (37, 53, 29, "N", "N"),
                                          (130, 257, 200, "T", "N"), (128, 128, 16, "N", "T"),
                                          (300, 129, 77, "T", "T")])

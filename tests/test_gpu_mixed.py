@@ -134,13 +134,37 @@ def test_mixed_early_tail_matches_oracle(system, request):
     dp.close()
 
 
+@pytest.mark.parametrize("band", [1, -1])
+@pytest.mark.parametrize("system", ["cluster216", "diamond2"])
+def test_tail_residual_f16_matches_fp64(system, band, request):
+    """The tail with the refinement in residual form (the default): X^T S X, the Rayleigh quotients and
+    the residuals H X - S X diag(lam) in FP64, the products with the residuals and with the
+    correction in FP16 (FP32 sums). The FP16 rounding only slows the contraction of a step, so
+    the cycle reaches the FP64 path's fixed point within the FP64 tolerances. X^T S X is formed
+    on its diagonal tile band (band 1, the default) or as the whole lower triangle (-1)."""
+    from dftbplus_amd import api
+    api.set_tuning(10, band)
+    try:
+        t = _fp64_and_switch(request.getfixturevalue(system), refine_below=1e-3, refine_steps=0,
+                             tail_below=1e-3, tail_steps=2)
+    finally:
+        api.set_tuning(10, 1)
+    assert t["refine"] > 0.0
+
+
 @pytest.mark.parametrize("system", ["cluster216", "diamond2"])
 def test_tail_fp64_seed_two_steps_matches_fp64(system, request):
-    """The tail as the benchmark runs it: FP32 iterations above 1e-3, one plain FP64 iteration
-    that keeps all eigenvectors, then two Ogita-Aishima steps per iteration on the sparse H and
-    S with the correction product in FP32: the FP64 path's fixed point within the FP64
-    tolerances (both converged to 1e-10, since cycles along different paths agree only to
-    about their SCC tolerance; the FP64 path is pinned to the oracle by test_gpu_scc)."""
-    t = _fp64_and_switch(request.getfixturevalue(system), refine_below=1e-3, refine_steps=0,
-                         tail_below=1e-3, tail_steps=2)
+    """The tail schedule the benchmark runs: FP32 iterations above 1e-3, one plain FP64
+    iteration that keeps all eigenvectors, then two Ogita-Aishima steps per iteration on the
+    sparse H and S, here with X^T H X and X^T S X as FP64 GEMMs and the correction product in
+    FP32 (tuning parameter 8 = 1): the FP64 path's fixed point within the FP64 tolerances (both
+    converged to 1e-10, since cycles along different paths agree only to about their SCC
+    tolerance; the FP64 path is pinned to the oracle by test_gpu_scc)."""
+    from dftbplus_amd import api
+    api.set_tuning(8, 1)
+    try:
+        t = _fp64_and_switch(request.getfixturevalue(system), refine_below=1e-3, refine_steps=0,
+                             tail_below=1e-3, tail_steps=2)
+    finally:
+        api.set_tuning(8, 2)
     assert t["refine"] > 0.0
