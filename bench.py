@@ -19,9 +19,10 @@ gradients is the leg's force evaluation. Legs:
   mixed_tail       FP32 iterations above 1e-3, then one plain FP64 iteration that keeps all n
                    eigenvectors, and after it no eigensolve: two Ogita-Aishima steps refine the
                    previous iteration's FP64 eigenpairs against the new sparse H and S in
-                   residual form (block-sparse products; X^T S X on its diagonal tile band, the
-                   Rayleigh quotients and the residuals H X - S X diag(lam) in FP64; the
-                   products with the residuals and with the correction in FP16, FP32 sums);
+                   residual form (block-sparse products; X^T S X and X^T H X on their diagonal
+                   tile band, the Rayleigh quotients, the in-band couplings and the residuals
+                   H X - S X diag(lam) with the in-band part removed, all in FP64; the products
+                   with the residuals and with the correction in FP16, FP32 sums);
   mixed_tail_fp32seed  as mixed_tail, seeded by FP32 eigenpairs refined by two steps (only
                    with --legs);
   mixed_tail32     as mixed_tail with X^T H X and X^T S X as FP64 GEMMs and the correction
@@ -69,7 +70,7 @@ LEGS = {"fp64": dict(precision=0),
         "mixed_tail32": dict(precision=1, refine_below=1e-3, refine_steps=0, tail_below=1e-3,
                              tail_steps=2, tuning={8: 1})}
 # library defaults of the tuning parameters a leg or mode may change (restored after it)
-TUNING_DEFAULTS = {8: 2, 9: 1, 10: 1}
+TUNING_DEFAULTS = {8: 2, 9: 1, 10: 1, 11: 0}
 E_BOUND, F_BOUND = 1e-6, 1e-5
 
 
@@ -179,10 +180,10 @@ def run_cycle_leg(api, geo, params, opts, leg, steps, warmup):
         now = time.perf_counter()
         extra = ""
         if kw.get("tail_below", 0.0) > 0.0:
-            st = dp.scc_state()
-            if st["oa_clipped_pairs"] or st["oa_far_pairs"]:
-                extra = f" (pairs outside the first-order range {st['oa_clipped_pairs']}, " \
-                        f"beyond the band {st['oa_far_pairs']})"
+            state = dp.scc_state()
+            if state["oa_clipped_pairs"] or state["oa_far_pairs"]:
+                extra = f" (pairs outside the first-order range {state['oa_clipped_pairs']}, " \
+                        f"beyond the band {state['oa_far_pairs']})"
         print(f"{tag} iteration {it}: {now - t_it:.3f}s charge error {err:.3e}"
               + (" (timed)" if warmup < it <= warmup + steps else "") + extra, file=sys.stderr,
               flush=True)

@@ -1410,6 +1410,9 @@ static int g_tailUpdate32 = 2;
 // residual form: tile rows of X^T S X below its diagonal tiles that each step forms (128 x 128
 // tiles; -1: the whole lower triangle); dftbhip_set_tuning parameter 10
 static int g_tailBand = 1;
+// residual form: iterations whose previous charge error is at most this use arithmetic 1
+// (0: never, the default); dftbhip_set_tuning parameter 11 (value e: 10^-e, 0: never)
+static double g_tailFineBelow = 0.0;
 
 struct Context {
   dftbhip_options opt{};
@@ -1477,7 +1480,14 @@ struct Context {
   DevBuf<double> tPart, tRsc;
   DevBuf<float> tRinv;
   DevBuf<int> tFar;
-  int lastFar = 0;
+  int lastFar = 0;   // pairs beyond the band of G in the last refinement (whole G formed)
+  DevBuf<double> tMb, tCt;   // band tiles of X^T H X; in-band couplings (transposed)
+  // descriptors of the in-band deflation YH <- YH - C^T YS, one batch per tile offset
+  struct DeflBatch {
+    DevBuf<GemmDesc<double>> desc;
+    int count = 0;
+  };
+  std::vector<DeflBatch> tDefl;
   // folded 4 x 4 blocks of H and S per row-view group for the block-sparse products
   DevBuf<double> tFold;
   DevBuf<int> tColOff;   // pairs beyond the band of G in the last refinement (whole G formed)
@@ -1762,7 +1772,8 @@ static int oa_refine_sparse_on(Context& c, int steps, DevBuf<double>* xt, int& c
   DH_TRY(c.tColOff.alloc((size_t)std::max(ng, 1)));
   DH_TRY(launch_spmm_fold(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, c.dblk.start.p,
                           c.hs.p, c.ov.p, kf ? c.blkT.p : nullptr, kf, c.tFold.p, c.tColOff.p));
-  const int mode = g_tailUpdate32;
+  const int mode =
+      (g_tailUpdate32 == 2 && c.st.prevErr <= g_tailFineBelow) ? 1 : g_tailUpdate32;
   const long ldh = f16_ld(n);
   if (mode == 2) {
     // the padding of the FP16 operands must be zero: cleared once per call (the steps write the
@@ -1775,6 +1786,31 @@ static int oa_refine_sparse_on(Context& c, int steps, DevBuf<double>* xt, int& c
     DH_TRY(c.tPart.alloc((size_t)64 * n));
     DH_TRY(c.tFar.alloc(1));
     c.lastFar = 0;
+    const int band = g_tailBand;
+    if (band >= 0) {
+      DH_TRY(c.tMb.alloc(nn));
+      DH_TRY(c.tCt.alloc(nn));
+      const int nt = (int)cdiv(n, kGemmTile);
+      c.tDefl.resize(2 * band + 1);
+      for (int o = -band; o <= band; ++o) {
+        std::vector<GemmDesc<double>> d;
+        for (int bj = 0; bj < nt; ++bj) {
+          const int bi = bj + o;
+          if (bi < 0 || bi >= nt) continue;
+          const int m = std::min(kGemmTile, n - bj * kGemmTile);
+          const int k = std::min(kGemmTile, n - bi * kGemmTile);
+          d.push_back(GemmDesc<double>{m, n, k,
+                                       c.tCt.p + (size_t)bj * kGemmTile +
+                                           (size_t)bi * kGemmTile * n,
+                                       (long)n, c.tYS.p + (size_t)bi * kGemmTile, (long)n,
+                                       c.tYH.p + (size_t)bj * kGemmTile, (long)n});
+        }
+        auto& b = c.tDefl[o + band];
+        b.count = (int)d.size();
+        if (!d.empty()) DH_TRY(b.desc.upload(d.data(), d.size(), s));
+      }
+    }
+
     DH_TRY(c.tRsc.alloc(n));
     DH_TRY(c.tRinv.alloc(n));
   }
@@ -1785,23 +1821,36 @@ static int oa_refine_sparse_on(Context& c, int steps, DevBuf<double>* xt, int& c
     DH_TRY(launch_spmm_hs_t(s, c.dblk.rows, c.dblk.norb.p, c.dblk.atomStart.p, n, c.tFold.p,
                             c.tColOff.p, Xt, n, n, c.tYH.p, c.tYS.p, n));
     if (mode == 2) {
-      // G = X^T S X in FP64; lam_j = x_j^T H x_j / G_jj; R = H X - S X diag(lam) in FP64, each
-      // vector's residual scaled by a power of two into the FP16 range; Ft = R^T X (FP16
-      // operands, FP32 sums, unscaled per row); E from Ft and G in FP16 (scaled by 2^15);
-      // Xt <- Xt + E^T Xt with the product in FP16. The rounding of the FP16 products is a
-      // relative 2^-11-level error of the correction, removed by the next step like any other
-      // error of the basis: the fixed point is that of the FP64 residuals.
-      // G on the diagonal tile band only: its diagonal, and its pairs outside the first-order
-      // range, which with eigenvectors in the order of their eigenvalues are close in index; a
-      // step that finds such a pair beyond the band forms the whole lower triangle and repeats
-      // the correction
+      // Residual form. G = X^T S X and M = X^T H X on their diagonal tile band (FP64), the
+      // Rayleigh quotients lam_j = M_jj / G_jj and the in-band couplings C_ij = x_i^T r_j =
+      // M_ij - lam_j G_ij (r_j = (H - lam_j S) x_j). The in-band part of each residual,
+      // sum_i S x_i C_ij, is removed from H X in FP64: what is left is small once the cycle
+      // converges (near-degenerate pairs outside the first-order range keep C_ij of the order of
+      // their level spacing, which would otherwise stay in every residual). The deflated
+      // residuals are scaled per vector by a power of two into the FP16 range; Ft = R^T X
+      // (FP16 operands, FP32 sums, unscaled per row) gives the couplings beyond the band; E
+      // from C (in the band), Ft (beyond) and G, per column scaled into FP16; Xt <- Xt + E^T Xt
+      // with the product in FP16. The rounding of the FP16 products is a relative 2^-11-level
+      // error of the correction, removed by the next step like any other basis error.
       const int band = g_tailBand;
-      if (band >= 0)
+      int gband = band;
+      if (band >= 0) {
         DH_TRY(gemm_band<double>(s, false, true, n, n, 1.0, Xt, n, c.tYS.p, n, c.tG.p, n, band));
-      else
+        DH_TRY(gemm_band<double>(s, false, true, n, n, 1.0, Xt, n, c.tYH.p, n, c.tMb.p, n, band));
+        DH_TRY(oa_band_coupling(s, c.tMb.p, c.tG.p, n, n, band, c.tLam.p, c.tCt.p, n));
+        // YH <- YH - C^T YS over the band, one batched launch per tile offset (disjoint outputs)
+        for (int o = -band; o <= band; ++o) {
+          auto& d = c.tDefl[o + band];
+          if (d.count > 0)
+            DH_TRY(gemm_batched<double>(s, false, false, d.desc.p, d.count, kGemmTile, n, -1.0,
+                                        1.0));
+        }
+      } else {
         DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYS.p, n, 0.0, c.tG.p, n,
                             true));
-      DH_TRY(oa_rayleigh(s, Xt, c.tYH.p, n, n, n, c.tG.p, n, c.tPart.p, c.tLam.p));
+        DH_TRY(oa_rayleigh(s, Xt, c.tYH.p, n, n, n, c.tG.p, n, c.tPart.p, c.tLam.p));
+      }
+      const double* Ct = band >= 0 ? c.tCt.p : nullptr;
       DH_TRY(oa_residual_scale(s, c.tYH.p, c.tYS.p, n, n, n, c.tLam.p, c.tPart.p, c.tRsc.p,
                                c.tRinv.p));
       DH_TRY(f16_convert(s, c.tYH.p, c.tYS.p, c.tLam.p, c.tRsc.p, n, n, n, nullptr, 0, c.hR.p,
@@ -1809,25 +1858,31 @@ static int oa_refine_sparse_on(Context& c, int steps, DevBuf<double>* xt, int& c
       DH_TRY(f16_convert(s, Xt, nullptr, nullptr, nullptr, n, n, n, c.hXt.p, ldh, c.hX.p, ldh));
       float* Ft = reinterpret_cast<float*>(c.tM.p);
       DH_TRY(gemm_f16_tn(s, n, n, n, 1.0f, c.hR.p, ldh, c.hX.p, ldh, Ft, n, c.tRinv.p));
+      unsigned* colMax = reinterpret_cast<unsigned*>(c.tRsc.p);   // free once R is converted
       DH_TRY(fill<int>(s, c.tFar.p, 1, 0));
-      DH_TRY(oa_correction_f16(s, Ft, n, c.tG.p, n, c.tLam.p, n, c.hE.p, ldh, band, c.tClip.p,
-                               c.tFar.p));
+      DH_TRY(oa_correction_f16(s, Ft, n, c.tG.p, n, Ct, n, c.tLam.p, n, band, gband, colMax,
+                               c.tClip.p, c.tFar.p));
       int far = 0;
-      if (band >= 0) {
+      if (gband >= 0) {
         DH_HIP(hipMemcpyAsync(&far, c.tFar.p, sizeof(int), hipMemcpyDeviceToHost, s));
         DH_HIP(hipStreamSynchronize(s));
       }
       if (far > 0) {
+        // a pair outside the first-order range beyond the band: the whole lower triangle of G
         c.lastFar += far;
+        gband = -1;
         DH_TRY(gemm<double>(s, false, true, n, n, n, 1.0, Xt, n, c.tYS.p, n, 0.0, c.tG.p, n,
                             true));
         DH_TRY(fill<int>(s, c.tClip.p, 1, 0));
-        DH_TRY(oa_correction_f16(s, Ft, n, c.tG.p, n, c.tLam.p, n, c.hE.p, ldh, -1, c.tClip.p,
-                                 nullptr));
+        DH_TRY(oa_correction_f16(s, Ft, n, c.tG.p, n, Ct, n, c.tLam.p, n, band, -1, colMax,
+                                 c.tClip.p, nullptr));
       }
+      // E per column scaled by a power of two into the FP16 range (the corrections shrink as
+      // the cycle converges; a fixed scale would leave the late ones subnormal)
+      DH_TRY(oa_correction_f16_write(s, Ft, n, c.tG.p, n, Ct, n, c.tLam.p, n, band, gband,
+                                     colMax, c.hE.p, ldh, c.tRinv.p));
       float* D32 = reinterpret_cast<float*>(c.tYH.p);
-      DH_TRY(gemm_f16_tn(s, n, n, n, 1.0f / kOaF16Scale, c.hE.p, ldh, c.hXt.p, ldh, D32, n,
-                         nullptr));
+      DH_TRY(gemm_f16_tn(s, n, n, n, 1.0f, c.hE.p, ldh, c.hXt.p, ldh, D32, n, c.tRinv.p));
       DH_TRY(add_f32(s, Xt, n, D32, n, Xn, n, n, n));
       cur = 1 - cur;
       continue;
@@ -3568,6 +3623,10 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
     case 10:
       DH_CHECK(value >= -1 && value <= 8, "set_tuning: parameter 10 takes -1 .. 8");
       dftbhip::g_tailBand = value;
+      return 0;
+    case 11:
+      DH_CHECK(value >= 0 && value <= 15, "set_tuning: parameter 11 takes 0 .. 15");
+      dftbhip::g_tailFineBelow = value == 0 ? 0.0 : std::pow(10.0, -value);
       return 0;
     default:
       dftbhip::set_error("set_tuning: unknown parameter " + std::to_string(param));

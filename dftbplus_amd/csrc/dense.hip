@@ -413,6 +413,7 @@ int merge_complex(hipStream_t s, const double* re, const double* im, long ld, do
 template int fill<double>(hipStream_t, double*, size_t, double);
 template int fill<float>(hipStream_t, float*, size_t, float);
 template int fill<int>(hipStream_t, int*, size_t, int);
+template int fill<unsigned>(hipStream_t, unsigned*, size_t, unsigned);
 template int fill<long>(hipStream_t, long*, size_t, long);
 template int fill<_Float16>(hipStream_t, _Float16*, size_t, _Float16);
 template int fill2d<double>(hipStream_t, double*, long, int, int, double);

@@ -24,20 +24,32 @@ int set_oa_mode(int mode);
 //  * oa_rayleigh: lam_j = x_j^T H x_j / G_jj (FP64 row dot products; part: 64 nv doubles);
 //  * oa_residual_scale: per vector j a power of two rsc_j with max_k |r_j(k)| rsc_j in
 //    [2^14, 2^15), r_j = (H - lam_j S) x_j, and rinv_j = 1 / rsc_j (FP32);
-//  * oa_correction_f16: from Ft(j, i) = x_i^T r_j (FP32, leading dimension ldf), the correction
-//    E(i, j) * 2^15 in FP16 at E16[i + j lde] (the off-diagonal first-order terms
-//    x_i^T r_j / (lam_j - lam_i); pairs outside the first-order range and the diagonal from G,
-//    as in oa_correction_t, which gives the same E up to the rounding of Ft). band >= 0: G
+//  * oa_correction_f16: from Ft(j, i) = x_i^T r_j (FP32, leading dimension ldf) the correction
+//    E(i, j) (the off-diagonal first-order terms x_i^T r_j / (lam_j - lam_i); pairs outside the
+//    first-order range and the diagonal from G, as in oa_correction_t, which gives the same E
+//    up to the rounding of Ft): colMax[j] = max_i |E(i, j)| (as FP32 bits) and the pair
+//    counters; then oa_correction_f16_write: E16[i + j lde] = E(i, j) 2^q_j in FP16, the
+//    largest scaled value of column j in [2^14, 2^15), and rinv[j] = 2^-q_j. gband >= 0: G
 //    holds the tile band of gemm_band only; pairs outside the first-order range beyond it are
-//    counted in nFar and left uncorrected (repeat with the whole lower triangle, band -1).
-constexpr float kOaF16Scale = 32768.0f;
+//    counted in nFar and left uncorrected (repeat with the whole lower triangle, gband -1).
 int oa_rayleigh(hipStream_t s, const double* Xt, const double* YH, long ld, int nv, int n,
                 const double* G, long ldg, double* part, double* lam);
 int oa_residual_scale(hipStream_t s, const double* YH, const double* YS, long ld, int nv, int n,
                       const double* lam, double* part, double* rsc, float* rinv);
+// Ct (may be null): the couplings of oa_band_coupling, used instead of Ft for the pairs whose
+// tiles are at most `band` apart; gband: the tile band G holds (-1: its whole lower triangle).
 int oa_correction_f16(hipStream_t s, const float* Ft, long ldf, const double* G, long ldg,
-                      const double* lam, int n, _Float16* E16, long lde, int band, int* nClipped,
-                      int* nFar);
+                      const double* Ct, long ldc, const double* lam, int n, int band, int gband,
+                      unsigned* colMax, int* nClipped, int* nFar);
+int oa_correction_f16_write(hipStream_t s, const float* Ft, long ldf, const double* G, long ldg,
+                            const double* Ct, long ldc, const double* lam, int n, int band,
+                            int gband, const unsigned* colMax, _Float16* E16, long lde,
+                            float* rinv);
+// lam_j = M_jj / G_jj and the in-band couplings Ct[j + i ldc] = M_ij - lam_j G_ij (FP64, 0 on
+// the diagonal) for the pairs whose 128 x 128 tiles are at most band apart, from the band tiles
+// of M = X^T H X and G = X^T S X (lower triangle, leading dimension ldb, both n x n)
+int oa_band_coupling(hipStream_t s, const double* Mb, const double* Gb, long ldb, int n, int band,
+                     double* lam, double* Ct, long ldc);
 
 struct EigenSolver {
   int n = 0;

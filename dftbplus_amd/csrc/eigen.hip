@@ -287,62 +287,115 @@ int oa_residual_scale(hipStream_t s, const double* YH, const double* YS, long ld
 }
 
 // 64 x 64 tiles of (j, i): Ft and G read along j, E16 written along i through LDS
+// In-band couplings Ct[j + i ldc] = x_i^T r_j = M_ij - lam_j G_ij (0 for i = j) for the pairs
+// whose 128 x 128 tiles are at most `band` apart, from the band tiles of M = X^T H X and
+// G = X^T S X (lower triangle, leading dimension ldb); threads run along j.
+__global__ void oa_band_coupling_kernel(const double* __restrict__ Mb, const double* __restrict__ Gb,
+                                        long ldb, const double* __restrict__ lam, int n, int band,
+                                        double* __restrict__ Ct, long ldc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int i = (j / kGemmTile - band) * kGemmTile + (int)blockIdx.y;
+  if (i < 0 || i >= n) return;
+  const long q = max(i, j) + (long)min(i, j) * ldb;
+  Ct[j + (long)i * ldc] = i == j ? 0.0 : Mb[q] - lam[j] * Gb[q];
+}
+
+int oa_band_coupling(hipStream_t s, const double* Mb, const double* Gb, long ldb, int n, int band,
+                     double* lam, double* Ct, long ldc) {
+  if (n <= 0) return 0;
+  DH_CHECK(band >= 0, "oa_band_coupling: negative band");
+  hipLaunchKernelGGL(oa_lambda_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, Mb, Gb, n, lam);
+  DH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(oa_band_coupling_kernel, dim3(cdiv(n, 256), kGemmTile * (2 * band + 1)),
+                     dim3(256), 0, s, Mb, Gb, ldb, lam, n, band, Ct, ldc);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
 // G is read from its lower triangle, (max(i, j), min(i, j)), and only where it is used: on the
-// diagonal and for the pairs outside the first-order range. band >= 0: G holds only the
-// 128 x 128 diagonal tiles and `band` tile rows below them; a pair outside the first-order
+// diagonal and for the pairs outside the first-order range. gband >= 0: G holds only the
+// 128 x 128 diagonal tiles and `gband` tile rows below them; a pair outside the first-order
 // range whose element lies outside that band is counted in nFar and left uncorrected (the
-// caller then forms the whole lower triangle and repeats the correction).
+// caller then forms the whole lower triangle and repeats the correction). Ct (may be null):
+// the FP64 couplings of the pairs whose tiles are at most `band` apart, used instead of Ft.
+__device__ __forceinline__ double oa_f16_element(const float* __restrict__ Ft, long ldf,
+                                                 const double* __restrict__ G, long ldg,
+                                                 const double* __restrict__ Ct, long ldc,
+                                                 const double* __restrict__ lam, double lj,
+                                                 int i, int j, int jacobi, int band, int gband,
+                                                 int& nclip, int& nfar) {
+  const int hi = max(i, j), lo = min(i, j);
+  const long qg = hi + (long)lo * ldg;
+  if (i == j) return 0.5 * (1.0 - G[qg]);
+  const int dt = hi / kGemmTile - lo / kGemmTile;
+  const double num = (Ct && dt <= band) ? Ct[j + (long)i * ldc] : (double)Ft[j + (long)i * ldf];
+  const double den = lj - lam[i];
+  if (fabs(num) < kOaMax * fabs(den)) return num / den;
+  if (gband >= 0 && dt > gband) {
+    nfar += j > i;
+    return 0.0;
+  }
+  const double r = -G[qg];
+  nclip += j > i;   // each pair counted once
+  if (!jacobi) return 0.5 * r;
+  const double mij = num - lj * r;   // x_i^T H x_j
+  const double th = den != 0.0 ? 0.5 * atan(2.0 * mij / den) : copysign(0.25 * M_PI, mij);
+  return 0.5 * r + sin(th);
+}
+
+// pass 0: the largest |E(i, j)| of each column j (non-negative floats ordered as their bit
+// patterns: one atomicMax per thread and column) and the pair counters; pass 1: E16(i, j) =
+// E(i, j) 2^q_j with q_j from that maximum (largest scaled value in [2^14, 2^15)), written
+// along i through LDS, and rinv[j] = 2^-q_j. 64 x 64 tiles of (j, i): Ft and G read along j.
 __global__ __launch_bounds__(256) void oa_correction_f16_kernel(
     const float* __restrict__ Ft, long ldf, const double* __restrict__ G, long ldg,
+    const double* __restrict__ Ct, long ldc,
     const double* __restrict__ lam, int n, _Float16* __restrict__ E16, long lde, int jacobi,
-    int band, int* __restrict__ nClipped, int* __restrict__ nFar) {
+    int band, int gband, int pass, unsigned* __restrict__ colMax, float* __restrict__ rinv,
+    int* __restrict__ nClipped, int* __restrict__ nFar) {
   __shared__ float t[64][65];
   const int j0 = blockIdx.x * 64, i0 = blockIdx.y * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int j = j0 + tx;
   const double lj = j < n ? lam[j] : 0.0;
   int nclip = 0, nfar = 0;
+  if (pass == 0) {
+    float mx = 0.0f;
+    if (j < n)
+      for (int ii = ty; ii < 64; ii += 4) {
+        const int i = i0 + ii;
+        if (i < n)
+          mx = fmaxf(mx, (float)fabs(oa_f16_element(Ft, ldf, G, ldg, Ct, ldc, lam, lj, i, j,
+                                                    jacobi, band, gband, nclip, nfar)));
+      }
+    if (j < n && mx > 0.0f) atomicMax(colMax + j, __float_as_uint(mx));
+    // one atomic per wave and counter
+    int w = nclip, f = nfar;
+    for (int o = 32; o > 0; o >>= 1) {
+      w += __shfl_xor(w, o);
+      f += __shfl_xor(f, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      if (nClipped && w != 0) atomicAdd(nClipped, w);
+      if (nFar && f != 0) atomicAdd(nFar, f);
+    }
+    return;
+  }
+  double sc = 1.0;
+  if (j < n) {
+    const float mx = __uint_as_float(colMax[j]);
+    const int q = mx > 0.0f ? min(100, max(-100, 14 - ilogbf(mx))) : 0;
+    sc = ldexp(1.0, q);
+    if (blockIdx.y == 0 && ty == 0) rinv[j] = ldexpf(1.0f, -q);
+  }
   for (int ii = ty; ii < 64; ii += 4) {
     const int i = i0 + ii;
     double e = 0.0;
-    if (j < n && i < n) {
-      const int hi = max(i, j), lo = min(i, j);
-      const long qg = hi + (long)lo * ldg;
-      if (i == j) {
-        e = 0.5 * (1.0 - G[qg]);
-      } else {
-        const double num = (double)Ft[j + (long)i * ldf], den = lj - lam[i];
-        if (fabs(num) < kOaMax * fabs(den)) {
-          e = num / den;
-        } else if (band >= 0 && hi / kGemmTile - lo / kGemmTile > band) {
-          nfar += j > i;
-        } else {
-          const double r = -G[qg];
-          nclip += j > i;   // each pair counted once
-          if (jacobi) {
-            const double mij = num - lj * r;   // x_i^T H x_j
-            const double th =
-                den != 0.0 ? 0.5 * atan(2.0 * mij / den) : copysign(0.25 * M_PI, mij);
-            e = 0.5 * r + sin(th);
-          } else {
-            e = 0.5 * r;
-          }
-        }
-      }
-    }
-    // |E| <= 1/2 for S-normalised vectors (|G_ij| <= 1) and the first-order bound kOaMax: the
-    // scaled value stays below the FP16 maximum
-    t[ii][tx] = (float)(e * kOaF16Scale);
-  }
-  // one atomic per wave and counter
-  int w = nclip, f = nfar;
-  for (int o = 32; o > 0; o >>= 1) {
-    w += __shfl_xor(w, o);
-    f += __shfl_xor(f, o);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (nClipped && w != 0) atomicAdd(nClipped, w);
-    if (nFar && f != 0) atomicAdd(nFar, f);
+    if (j < n && i < n)
+      e = oa_f16_element(Ft, ldf, G, ldg, Ct, ldc, lam, lj, i, j, jacobi, band, gband, nclip,
+                         nfar);
+    t[ii][tx] = (float)(e * sc);
   }
   __syncthreads();
   for (int jj = ty; jj < 64; jj += 4) {
@@ -352,12 +405,26 @@ __global__ __launch_bounds__(256) void oa_correction_f16_kernel(
 }
 
 int oa_correction_f16(hipStream_t s, const float* Ft, long ldf, const double* G, long ldg,
-                      const double* lam, int n, _Float16* E16, long lde, int band, int* nClipped,
-                      int* nFar) {
+                      const double* Ct, long ldc, const double* lam, int n, int band, int gband,
+                      unsigned* colMax, int* nClipped, int* nFar) {
   if (n <= 0) return 0;
-  DH_CHECK(band < 0 || nFar, "oa_correction_f16: a banded G needs the far-pair counter");
+  DH_CHECK(gband < 0 || nFar, "oa_correction_f16: a banded G needs the far-pair counter");
+  DH_TRY(fill<unsigned>(s, colMax, n, 0u));
   hipLaunchKernelGGL(oa_correction_f16_kernel, dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, s,
-                     Ft, ldf, G, ldg, lam, n, E16, lde, g_oaJacobi, band, nClipped, nFar);
+                     Ft, ldf, G, ldg, Ct, ldc, lam, n, (_Float16*)nullptr, 0L, g_oaJacobi, band,
+                     gband, 0, colMax, (float*)nullptr, nClipped, nFar);
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+
+int oa_correction_f16_write(hipStream_t s, const float* Ft, long ldf, const double* G, long ldg,
+                            const double* Ct, long ldc, const double* lam, int n, int band,
+                            int gband, const unsigned* colMax, _Float16* E16, long lde,
+                            float* rinv) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(oa_correction_f16_kernel, dim3(cdiv(n, 64), cdiv(n, 64)), dim3(256), 0, s,
+                     Ft, ldf, G, ldg, Ct, ldc, lam, n, E16, lde, g_oaJacobi, band, gband, 1,
+                     const_cast<unsigned*>(colMax), rinv, (int*)nullptr, (int*)nullptr);
   DH_LAUNCH_CHECK();
   return 0;
 }

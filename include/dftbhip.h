@@ -77,7 +77,8 @@ int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e, doub
  * H X - S X diag(lam) in FP64, the products with the residuals and with E in FP16 (FP32
  * sums); param 10: residual
  * form, tile rows of X^T S X below its 128 x 128 diagonal tiles formed per step (1, the
- * default; -1 the whole lower triangle); param 9: vectors per lane of the tail's
+ * default; -1 the whole lower triangle); param 11: residual form, iterations whose previous
+ * charge error is at most 10^-value use arithmetic 1 (0, the default: never); param 9: vectors per lane of the tail's
  * block-sparse products (1, the default, 2 or 4). No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)

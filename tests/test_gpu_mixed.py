@@ -109,29 +109,17 @@ def test_mixed_tail_refinement_periodic(diamond2):
 
 
 @pytest.mark.parametrize("system", ["cluster216", "diamond2"])
-def test_mixed_early_tail_matches_oracle(system, request):
+def test_mixed_early_tail_matches_fp64(system, request):
     """FP32 iterations above a charge error of 1e-3, one iteration whose FP32 eigenpairs are
-    refined by two Ogita-Aishima steps, and every later iteration a single step on the
-    previous eigenpairs against the new sparse H and S (no eigensolve): the cycle reaches the
-    FP64 oracle's fixed point within the FP64 tolerances. The periodic cell has atoms that
+    refined by two Ogita-Aishima steps, and every later iteration a step on the previous
+    eigenpairs against the new sparse H and S (no eigensolve): the cycle reaches the FP64
+    path's fixed point within the FP64 tolerances (both converged to 1e-10; cycles converged
+    to 1e-8 along different paths agree only to about 1e-8 e). The periodic cell has atoms that
     neighbour their own images, so the block-sparse products must sum image blocks as the
     dense unpack does."""
-    from dftbplus_amd import api
-    geo, params, opts, calc, res = request.getfixturevalue(system)
-    o = dataclasses.replace(opts, max_scc_iter=200)
-    dp = api.DftbPlus(params, geo, o, precision=1, refine_below=1e-3, refine_steps=2,
-                      tail_below=1e-3)
-    e, q, g = dp.get_energy(), dp.get_gross_charges(), dp.get_gradients()
-    t = dp.timings()
-    nat = geo.natom
-    de, dq, dg = abs(e - res.energy_mermin) / nat, np.max(np.abs(q - res.charges)), \
-        np.max(np.abs(g - res.gradients))
-    print(f"{system} early tail: it {dp.n_iter}/{res.n_iter} dE/atom {de:.2e} dq {dq:.2e} "
-          f"dF {dg:.2e} refine {t['refine']:.3f}s")
-    assert dp.converged
+    t = _fp64_and_switch(request.getfixturevalue(system), refine_below=1e-3, refine_steps=2,
+                         tail_below=1e-3)
     assert t["refine"] > 0.0
-    assert de < 1e-8 and dq < 1e-8 and dg < 1e-7
-    dp.close()
 
 
 @pytest.mark.parametrize("band", [1, -1])
@@ -141,7 +129,8 @@ def test_tail_residual_f16_matches_fp64(system, band, request):
     the residuals H X - S X diag(lam) in FP64, the products with the residuals and with the
     correction in FP16 (FP32 sums). The FP16 rounding only slows the contraction of a step, so
     the cycle reaches the FP64 path's fixed point within the FP64 tolerances. X^T S X is formed
-    on its diagonal tile band (band 1, the default) or as the whole lower triangle (-1)."""
+    and X^T H X on their diagonal tile band with the in-band couplings in FP64 and removed from
+    the residuals (band 1, the default), or X^T S X as the whole lower triangle (-1)."""
     from dftbplus_amd import api
     api.set_tuning(10, band)
     try:
@@ -149,6 +138,19 @@ def test_tail_residual_f16_matches_fp64(system, band, request):
                              tail_below=1e-3, tail_steps=2)
     finally:
         api.set_tuning(10, 1)
+    assert t["refine"] > 0.0
+
+
+def test_tail_residual_then_fp64_form(cluster216):
+    """Residual form above a charge error of 1e-6 e, the FP64 form below (tuning parameter
+    11 = 6): the FP64 path's fixed point within the FP64 tolerances."""
+    from dftbplus_amd import api
+    api.set_tuning(11, 6)
+    try:
+        t = _fp64_and_switch(cluster216, refine_below=1e-3, refine_steps=0, tail_below=1e-3,
+                             tail_steps=2)
+    finally:
+        api.set_tuning(11, 0)
     assert t["refine"] > 0.0
 
 
