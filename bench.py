@@ -226,44 +226,113 @@ def run_cycle_leg(api, geo, params, opts, leg, steps, warmup):
 
 
 def mode_tail(args):
-    """The mixed-precision tail on its own, for kernel traces: one plain FP64 iteration seeds it
-    and every later iteration is two Ogita-Aishima steps on the sparse H and S (the cycle is not
-    meant to converge from neutral charges this way); prints the iteration times for each
-    arithmetic of the steps (dftbhip_set_tuning parameter 8: 1 FP64 X^T H X and X^T S X with
-    the correction product in FP32, 2 the residual form with FP16 products) and, for the
-    residual form, each number of vectors per lane of the block-sparse products (parameter 9)."""
-    import math
+    """The mixed-precision tail of the cycle, for kernel traces of its iterations. Without
+    --charges-in: runs the mixed_tail leg's cycle until its charge error is at most the tail
+    threshold (1e-3) and saves the input charges of the next iteration to --charges-out (no
+    tail iteration runs). With --charges-in: starts the mixed_tail leg's cycle from those
+    charges (ReadInitialCharges): one FP32 iteration, the plain FP64 seed, then the tail
+    iterations to convergence, each printed with its time."""
+    import dataclasses
+    import numpy as np
     from dftbplus_amd import api, synth
     geo, params, opts = synth.workload(args.workload)
-    dp = api.DftbPlus(params, geo, opts, precision=1, refine_below=math.inf, refine_steps=0,
-                      tail_below=math.inf, tail_steps=2)
+    if args.mix:
+        opts = dataclasses.replace(opts, mix_param=args.mix, max_scc_iter=300)
+    kw = dict(LEGS["mixed_tail"])
+    dp = api.DftbPlus(params, geo, opts, **kw)
     n = dp.get_nr_orbitals()
+    if args.charges_in:
+        dp.set_input_charges(np.load(args.charges_in))
     dp.scc_begin()
-    times = []
-    # the seed, then --steps tail iterations for each variant (in the kernel trace the
-    # iterations follow this order)
-    variants = [(1, 2), (2, 1), (2, 2), (2, 4)]
-    plan = [None] + [v for v in variants for _ in range(args.steps)]
-    for it, var in enumerate(plan):
-        if var:
-            api.set_tuning(8, var[0])
-            api.set_tuning(9, var[1])
+    times, conv, it = [], False, 0
+    while not conv and it < opts.max_scc_iter:
         t0 = time.perf_counter()
-        dp.scc_step()
-        times.append((var, time.perf_counter() - t0))
+        conv, err = dp.scc_step()
+        it += 1
+        times.append(time.perf_counter() - t0)
         st = dp.scc_state()
-        what = "FP64 seed" if var is None else f"tail, arithmetic {var[0]}, {var[1]} vectors/lane"
-        print(f"[tail] iteration {it + 1} ({what}): {times[-1][1]:.3f}s, pairs outside the "
-              f"first-order range {st['oa_clipped_pairs']} (beyond the band of X^T S X: "
+        print(f"[tail] iteration {it}: {times[-1]:.3f}s charge error {err:.3e}, pairs outside "
+              f"the first-order range {st['oa_clipped_pairs']} (beyond the band: "
               f"{st['oa_far_pairs']})", file=sys.stderr, flush=True)
-    api.set_tuning(8, TUNING_DEFAULTS[8])
-    api.set_tuning(9, TUNING_DEFAULTS[9])
+        if not args.charges_in and err <= kw["tail_below"]:
+            # the FP32 iterations conserve the electron count only to their rounding (about
+            # 0.3 e of 16179 here): rescaled to the count, as a charge file must hold it
+            q = dp.get_input_charges()
+            nel = sum(sum(params.species[int(z)].occ) for z in geo.species)
+            np.save(args.charges_out, q * (nel / q.sum()))
+            print(f"[tail] saved the input charges of iteration {it + 1} to {args.charges_out}",
+                  file=sys.stderr, flush=True)
+            break
     dp.scc_end()
     dp.close()
     print(json.dumps({"mode": "tail", "workload": args.workload, "orbitals": n,
-                      "seed_s": times[0][1],
-                      "tail_s": {f"{v[0]}/{v[1]}": [t for vv, t in times if vv == v]
-                                 for v in variants}}), flush=True)
+                      "charges_in": args.charges_in, "iterations": it, "converged": conv,
+                      "iteration_s": times}), flush=True)
+
+
+def mode_cycle(args):
+    """One converged SCC cycle per leg of --legs, without the FP64 reference leg or the CPU
+    baseline (for kernel traces of a whole cycle, e.g. rocprofv3 -P over its tail); prints the
+    per-iteration lines and one JSON summary line per leg."""
+    import dataclasses
+    from dftbplus_amd import api, synth
+    geo, params, opts = synth.workload(args.workload)
+    if args.mix:
+        opts = dataclasses.replace(opts, mix_param=args.mix, max_scc_iter=300)
+    for leg in [x for x in args.legs.split(",") if x]:
+        r = run_cycle_leg(api, geo, params, opts, leg, args.steps, args.warmup)
+        print(json.dumps({"mode": "cycle", "workload": args.workload, "leg": leg,
+                          "orbitals": r["n"], "converged": r["converged"],
+                          "iterations": r["iterations"], "cycle_scc_s": r["scc_s"],
+                          "energy": r["energy"]}), flush=True)
+
+
+def mode_tracesum(args):
+    """Per-iteration kernel table of the mixed-precision tail from a rocprofv3 trace database
+    (--trace, the rocpd SQLite output of --kernel-trace): the kernels of the last --steps
+    tail iterations (from their first spmm_fold_t_kernel launch on, two launches per tail
+    iteration; the cycle must end in the tail), by total time."""
+    import collections
+    import re
+    import sqlite3
+    con = sqlite3.connect(args.trace)
+    rows = list(con.execute("select name, start, end, duration from kernels order by start"))
+    marks = [r for r in rows if "spmm_fold_t_kernel" in r[0]]
+    if not marks:
+        raise SystemExit("no spmm_fold_t_kernel launch in the trace")
+    nit = min(len(marks) // 2, max(args.steps, 1))
+    win = [r for r in rows if r[1] >= marks[-2 * nit][1]]
+    names = sorted({r[0] for r in win if r[0].startswith("_Z")})
+    plain = {}
+    if names:
+        import subprocess
+        try:   # the trace keeps some kernel names mangled
+            out = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True,
+                                 text=True, timeout=30).stdout.splitlines()
+            plain = dict(zip(names, out)) if len(out) == len(names) else {}
+        except (OSError, subprocess.SubprocessError):
+            plain = {}
+    agg = collections.OrderedDict()
+    for name, st, en, du in win:
+        name = plain.get(name, name)
+        m = re.match(r"_ZN7dftbhip(\d+)", name)   # a name the demangler left: its identifier
+        if m:
+            name = "dftbhip::" + name[m.end():m.end() + int(m.group(1))]
+        k = re.sub(r"^void ", "", re.sub(r"\(.*", "", name))[:64]
+        a = agg.setdefault(k, [0, 0])
+        a[0] += 1
+        a[1] += du
+    tot = sum(a[1] for a in agg.values())
+    span = (win[-1][2] - win[0][1]) / 1e6
+    print(f"tail iterations: {nit}; kernel time {tot / 1e6 / nit:.1f} ms and span "
+          f"{span / nit:.1f} ms per iteration")
+    print()
+    print("| kernel | launches / iteration | ms / iteration | average µs |")
+    print("|---|---:|---:|---:|")
+    for k, (c, d) in sorted(agg.items(), key=lambda x: -x[1][1]):
+        if d / nit < 1e4:   # below 0.01 ms per iteration
+            continue
+        print(f"| `{k}` | {c / nit:.1f} | {d / 1e6 / nit:.2f} | {d / c / 1e3:.1f} |")
 
 
 def mode_cpu(args):
@@ -619,7 +688,7 @@ def main():
                     help="keep the fixed SYMV tile thresholds instead of autotuning them")
     ap.add_argument("--mode", default="scc",
                     choices=["scc", "sytrd", "batch", "kpoints", "autograd", "neighbours",
-                             "kernels", "cpu", "tail"],
+                             "kernels", "cpu", "tail", "cycle", "tracesum"],
                     help="scc: the headline SCC benchmark (default); sytrd: the tridiagonalisation "
                          "alone on a random matrix (SYMV counter runs); batch: the molecule batch "
                          "against a sampled loop of single-molecule runs; kpoints: SCC iterations "
@@ -629,8 +698,17 @@ def main():
                          "with the host-built list; kernels: one SCC iteration, forces and a "
                          "GEMM of size --n with the sizes that fix their bytes and flops (for "
                          "kernel traces and counter runs); cpu: the CPU baseline alone, at the "
-                         "full size and on the bounded sample; tail: one FP64 iteration and "
-                         "--steps iterations of the mixed-precision tail (kernel traces)")
+                         "full size and on the bounded sample; tail: the mixed-precision "
+                         "tail from saved charges (--charges-in; without it: save them at the "
+                         "tail threshold to --charges-out), for kernel traces; "
+                         "cycle: one converged cycle per leg of --legs, no reference leg "
+                         "(kernel traces over a window of the cycle); tracesum: the tail's "
+                         "per-iteration kernel table from a trace database (--trace)")
+    ap.add_argument("--trace", default="", help="--mode tracesum: rocpd trace database")
+    ap.add_argument("--charges-in", default="",
+                    help="--mode tail: start the tail from these input charges (.npy)")
+    ap.add_argument("--charges-out", default="tail_charges.npy",
+                    help="--mode tail without --charges-in: where to save the charges")
     ap.add_argument("--n", type=int, default=8192,
                     help="matrix size of --mode sytrd and of the GEMM of --mode kernels")
     ap.add_argument("--nmol", type=int, default=4096, help="molecules of --mode batch")
@@ -654,7 +732,8 @@ def main():
     if args.mode != "scc":
         {"sytrd": mode_sytrd, "batch": mode_batch, "kpoints": mode_kpoints,
          "neighbours": mode_neighbours, "kernels": mode_kernels,
-         "autograd": mode_autograd, "cpu": mode_cpu, "tail": mode_tail}[args.mode](args)
+         "autograd": mode_autograd, "cpu": mode_cpu, "tail": mode_tail,
+         "cycle": mode_cycle, "tracesum": mode_tracesum}[args.mode](args)
         return
     import dataclasses
     geo, params, opts = synth.workload(args.workload)

@@ -125,6 +125,8 @@ EXPORTS = {
     "dftbhip_ctx_scc": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_ctx_scc_begin": [C.c_void_p],
     "dftbhip_ctx_set_charge_restart": [C.c_void_p, C.c_int],
+    "dftbhip_ctx_set_input_charges": [C.c_void_p, C.c_int, _dp],
+    "dftbhip_ctx_get_input_charges": [C.c_void_p, C.c_int, _dp],
     "dftbhip_ctx_scc_step": [C.c_void_p, _ip, _dp],
     "dftbhip_ctx_scc_end": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_ctx_gradients": [C.c_void_p, _dp],
@@ -608,6 +610,21 @@ class DftbPlus:
                "dftbhip_ctx_set_charge_restart")
 
     # one-iteration control of the SCC cycle (used to time single iterations)
+    def set_input_charges(self, populations: np.ndarray):
+        """Electron population per atom for the next SCC cycle (ReadInitialCharges): their
+        sum must equal the number of electrons to 1e-3."""
+        q = np.ascontiguousarray(populations, dtype=np.float64)
+        _check(lib().dftbhip_ctx_set_input_charges(self.h, len(q), _d(q)),
+               "dftbhip_ctx_set_input_charges")
+
+    def get_input_charges(self) -> np.ndarray:
+        """Electron population per atom that the next SCC iteration takes as input (the mixed
+        charges after the last step; what the reference writes to its charge file)."""
+        q = np.zeros(self.natom)
+        _check(lib().dftbhip_ctx_get_input_charges(self.h, len(q), _d(q)),
+               "dftbhip_ctx_get_input_charges")
+        return q
+
     def scc_begin(self):
         _check(lib().dftbhip_ctx_scc_begin(self.h), "dftbhip_ctx_scc_begin")
 

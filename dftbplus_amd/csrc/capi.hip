@@ -1481,6 +1481,7 @@ struct Context {
   DevBuf<float> tRinv;
   DevBuf<int> tFar;
   int lastFar = 0;   // pairs beyond the band of G in the last refinement (whole G formed)
+  std::vector<double> qinGiven;   // input charges for the next scc_begin (set_input_charges)
   DevBuf<double> tMb, tCt;   // band tiles of X^T H X; in-band couplings (transposed)
   // descriptors of the in-band deflation YH <- YH - C^T YS, one batch per tile offset
   struct DeflBatch {
@@ -1931,8 +1932,14 @@ static int ctx_scc_begin(Context& c) {
   SccState& st = c.st;
   st.nel = 0.0;
   for (double q : c.q0atom) st.nel += q;
-  // warm start: the input charges of the previous cycle, when it was on the same atoms
-  if (!(c.warmStart && st.it > 0 && st.qin.size() == c.q0atom.size())) st.qin = c.q0atom;
+  // given input charges (dftbhip_ctx_set_input_charges), else a warm start: the input charges
+  // of the previous cycle, when it was on the same atoms; else the neutral atoms
+  if (!c.qinGiven.empty() && c.qinGiven.size() == c.q0atom.size()) {
+    st.qin = c.qinGiven;
+    c.qinGiven.clear();
+  } else if (!(c.warmStart && st.it > 0 && st.qin.size() == c.q0atom.size())) {
+    st.qin = c.q0atom;
+  }
   st.mixer.reset(c.nat, c.opt, c.opt.max_scc_iter + 1);
   c.xFull = false;
   c.xFullK = false;
@@ -3300,6 +3307,37 @@ extern "C" int dftbhip_ctx_scc(dftbhip_ctx* ctx, double* energies, double* charg
                                int* converged) {
   DH_CHECK(ctx && ctx->c.nat > 0, "scc: no geometry");
   return ctx_scc(ctx->c, energies, charges, niter, converged);
+}
+
+extern "C" int dftbhip_ctx_set_input_charges(dftbhip_ctx* ctx, int nAtom, const double* qin) {
+  DH_CHECK(ctx && qin, "set_input_charges: null pointer");
+  dftbhip::Context& c = ctx->c;
+  DH_CHECK(c.nat > 0 && nAtom == c.nat, "set_input_charges: atom count differs from the geometry");
+  double nel = 0.0, sum = 0.0;
+  for (int a = 0; a < nAtom; ++a) {
+    nel += c.q0atom[a];
+    sum += qin[a];
+  }
+  // the electron-count check of initQFromFile (sccinit.F90:397-402)
+  if (std::fabs(nel - sum) >= 1e-3) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf,
+                  "External file of charges has a total charge:%18.6f, instead of %18.6f", sum,
+                  nel);
+    dftbhip::set_error(buf);
+    return dftbhip::kErrArg;
+  }
+  c.qinGiven.assign(qin, qin + nAtom);
+  return 0;
+}
+
+extern "C" int dftbhip_ctx_get_input_charges(dftbhip_ctx* ctx, int nAtom, double* qin) {
+  DH_CHECK(ctx && qin, "get_input_charges: null pointer");
+  const dftbhip::Context& c = ctx->c;
+  DH_CHECK(nAtom == c.nat && c.st.qin.size() == (size_t)c.nat,
+           "get_input_charges: no SCC cycle on this geometry, or the atom count differs");
+  std::copy(c.st.qin.begin(), c.st.qin.end(), qin);
+  return 0;
 }
 
 extern "C" int dftbhip_ctx_set_charge_restart(dftbhip_ctx* ctx, int warm) {

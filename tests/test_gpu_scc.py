@@ -28,3 +28,29 @@ def test_scc_energy_charges_forces(system, tridiag, request):
                  ("ts", res.ts)):
         assert abs(dp.energies[k] - v) / nat < E_TOL, k
     dp.close()
+
+
+def test_input_charges_restart(cluster216):
+    """ReadInitialCharges: a cycle started from the input charges another cycle ended with (what
+    the reference writes to its charge file) converges at once to the same point; charges that
+    do not sum to the electron count are refused with the reference's message."""
+    from dftbplus_amd import api
+    geo, params, opts, calc, res = cluster216
+    dp = api.DftbPlus(params, geo, opts)
+    e = dp.get_energy()
+    qin = dp.get_input_charges()
+    nat = geo.natom
+    assert dp.converged and dp.n_iter > 5
+    dp2 = api.DftbPlus(params, geo, opts)
+    dp2.set_input_charges(qin)
+    e2 = dp2.get_energy()
+    print(f"restart: {dp2.n_iter} iterations (cold {dp.n_iter}), dE/atom {abs(e2 - e) / nat:.2e}")
+    assert dp2.converged and dp2.n_iter <= 3
+    assert abs(e2 - e) / nat < E_TOL
+    assert np.max(np.abs(dp2.get_gross_charges() - dp.get_gross_charges())) < Q_TOL
+    bad = qin.copy()
+    bad[0] += 0.01
+    with pytest.raises(api.DftbHipError, match="External file of charges has a total charge"):
+        dp2.set_input_charges(bad)
+    dp.close()
+    dp2.close()
