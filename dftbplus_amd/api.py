@@ -652,13 +652,14 @@ class DftbPlus:
         """Iteration number, charge error, electronic energy of the last iteration and its
         change from the one before, input electrons of the last iteration, pairs outside the
         first-order range of the last refinement step and, for the residual form, of those
-        beyond the diagonal band of X^T S X in the last refinement."""
-        out = np.zeros(7)
-        _check(lib().dftbhip_ctx_scc_state(self.h, _d(out), 7), "dftbhip_ctx_scc_state")
+        beyond the diagonal band of X^T S X in the last refinement; whether the tail was
+        switched off for the rest of the cycle."""
+        out = np.zeros(8)
+        _check(lib().dftbhip_ctx_scc_state(self.h, _d(out), 8), "dftbhip_ctx_scc_state")
         return {"iteration": int(out[0]), "scc_error": float(out[1]), "e_elec": float(out[2]),
                 "diff_elec": float(out[2] - out[3]) if np.isfinite(out[3]) else float(out[2]),
                 "q_input_sum": float(out[4]), "oa_clipped_pairs": int(out[5]),
-                "oa_far_pairs": int(out[6])}
+                "oa_far_pairs": int(out[6]), "tail_off": bool(out[7])}
 
     def get_nr_atoms(self) -> int:
         return self.natom

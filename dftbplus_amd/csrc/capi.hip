@@ -292,6 +292,7 @@ extern "C" int dftbhip_init(int device) {
 }
 
 extern "C" int dftbhip_finalize(void) {
+  release_sytrd_graphs();
   g_rt.release();
   if (g_rt.stream) (void)hipStreamDestroy(g_rt.stream);
   g_rt.stream = nullptr;
@@ -549,6 +550,13 @@ struct DevBlocks {
   DevBuf<int> at1, img2, at2, species, norb;
   DevBuf<long> start, atomStart;
   PairsDev pr{};
+  // host copies of the orbital layout, for the tile mask of the packed density
+  std::vector<int> hNorb;
+  std::vector<long> hStart;
+  // 128 x 128 tiles (row tile >= column tile) of the dense matrices that pack reads: the
+  // density and energy-density products are formed on these tiles only
+  DevBuf<int2> packTiles;
+  int nPackTiles = 0;
   int upload(const HostBlocks& hb, const std::vector<int>& species, const std::vector<int>& norbAtom,
              const std::vector<long>& atomStart, hipStream_t s) {
     const size_t n = hb.at1.size();
@@ -559,6 +567,8 @@ struct DevBlocks {
     if (!species.empty()) DH_TRY(this->species.upload(species.data(), species.size(), s));
     DH_TRY(norb.upload(norbAtom.data(), norbAtom.size(), s));
     DH_TRY(this->atomStart.upload(atomStart.data(), atomStart.size(), s));
+    hNorb = norbAtom;
+    hStart = atomStart;
     pr.nblk = (int)n;
     pr.at1 = at1.p;
     pr.img2 = img2.p;
@@ -589,6 +599,8 @@ struct DevBlocks {
     if (!species.empty()) DH_TRY(this->species.upload(species.data(), species.size(), s));
     DH_TRY(norb.upload(norbAtom.data(), norbAtom.size(), s));
     DH_TRY(this->atomStart.upload(atomStart.data(), atomStart.size(), s));
+    hNorb = norbAtom;
+    hStart = atomStart;
     pr.nblk = n;
     pr.at1 = at1.p;
     pr.img2 = img2.p;
@@ -622,6 +634,28 @@ struct DevBlocks {
       if (h1[k] != h2[k]) ptr[h1[k] + 1] += 1;
     }
     for (int a = 0; a < nat; ++a) ptr[a + 1] += ptr[a];
+    // tiles of the dense matrices that pack reads: block k at rows of at2, columns of at1
+    // (at2 >= at1 in the half list, so on or below the diagonal)
+    if (hNorb.size() == (size_t)nat && hStart.size() >= (size_t)nat) {
+      const long nOrb = hStart[nat - 1] + hNorb[nat - 1];
+      const int nt = (int)cdiv(nOrb, kGemmTile);
+      std::vector<char> mark((size_t)nt * nt, 0);
+      for (int k = 0; k < nb; ++k) {
+        const long r0 = hStart[h2[k]], r1 = r0 + hNorb[h2[k]] - 1;
+        const long c0 = hStart[h1[k]], c1 = c0 + hNorb[h1[k]] - 1;
+        for (long tr = r0 / kGemmTile; tr <= r1 / kGemmTile; ++tr)
+          for (long tc = c0 / kGemmTile; tc <= c1 / kGemmTile; ++tc)
+            mark[(size_t)std::max(tr, tc) * nt + std::min(tr, tc)] = 1;
+      }
+      std::vector<int2> tl;
+      for (int tr = 0; tr < nt; ++tr)
+        for (int tc = 0; tc <= tr; ++tc)
+          if (mark[(size_t)tr * nt + tc]) tl.push_back(make_int2(tr, tc));
+      nPackTiles = (int)tl.size();
+      if (!tl.empty()) DH_TRY(packTiles.upload(tl.data(), tl.size(), s));
+    } else {
+      nPackTiles = 0;
+    }
     std::vector<int2> ent(std::max(ptr[nat], 1));
     std::vector<int> fillPos(ptr.begin(), ptr.end() - 1);
     for (int k = 0; k < nb; ++k) {
@@ -1482,6 +1516,7 @@ struct Context {
   DevBuf<int> tFar;
   int lastFar = 0;   // pairs beyond the band of G in the last refinement (whole G formed)
   std::vector<double> qinGiven;   // input charges for the next scc_begin (set_input_charges)
+  bool tailOff = false;   // the tail failed once in this cycle: no more tail iterations
   DevBuf<double> tMb, tCt;   // band tiles of X^T H X; in-band couplings (transposed)
   // descriptors of the in-band deflation YH <- YH - C^T YS, one batch per tile offset
   struct DeflBatch {
@@ -1722,6 +1757,18 @@ static int ctx_geometry(Context& c, int nat, const double* coords, const int* sp
   return 0;
 }
 
+// C = A B^T (n x n, K = k, w: per-k weights) for the density and energy-density matrices,
+// formed only on the tiles pack reads (the sparse blocks' tiles) when the row view holds them,
+// else on the whole lower triangle
+template <typename T>
+static int pack_product(Context& c, int n, int k, const T* A, const T* B, T* C,
+                        const T* w = nullptr) {
+  if (c.dblk.rowsValid && c.dblk.nPackTiles > 0)
+    return gemm_tiles<T>(c.s, false, true, n, n, k, T(1), A, n, B, n, T(0), C, n,
+                         c.dblk.packTiles.p, c.dblk.nPackTiles, w);
+  return gemm<T>(c.s, false, true, n, n, k, T(1), A, n, B, n, T(0), C, n, true, kKFull, w);
+}
+
 static int gamma_times(Context& c, const std::vector<double>& dqa, std::vector<double>* vout) {
   hipStream_t s = c.s;
   DH_HIP(hipMemcpyAsync(c.dq.p, dqa.data(), c.nat * sizeof(double), hipMemcpyHostToDevice, s));
@@ -1836,8 +1883,10 @@ static int oa_refine_sparse_on(Context& c, int steps, DevBuf<double>* xt, int& c
       const int band = g_tailBand;
       int gband = band;
       if (band >= 0) {
-        DH_TRY(gemm_band<double>(s, false, true, n, n, 1.0, Xt, n, c.tYS.p, n, c.tG.p, n, band));
-        DH_TRY(gemm_band<double>(s, false, true, n, n, 1.0, Xt, n, c.tYH.p, n, c.tMb.p, n, band));
+        DH_TRY(gemm_band<double>(s, false, true, n, n, 1.0, Xt, n, c.tYS.p, n, c.tG.p, n, band,
+                                 8));
+        DH_TRY(gemm_band<double>(s, false, true, n, n, 1.0, Xt, n, c.tYH.p, n, c.tMb.p, n, band,
+                                 8));
         DH_TRY(oa_band_coupling(s, c.tMb.p, c.tG.p, n, n, band, c.tLam.p, c.tCt.p, n));
         // YH <- YH - C^T YS over the band, one batched launch per tile offset (disjoint outputs)
         for (int o = -band; o <= band; ++o) {
@@ -1943,6 +1992,7 @@ static int ctx_scc_begin(Context& c) {
   st.mixer.reset(c.nat, c.opt, c.opt.max_scc_iter + 1);
   c.xFull = false;
   c.xFullK = false;
+  c.tailOff = false;
   st.it = 0;
   st.conv = false;
   st.err = 0.0;
@@ -1987,7 +2037,10 @@ static int ctx_scc_step(Context& c) {
   // (refine_steps = 0, the more accurate seed: the FP32 tridiagonalisation mixes levels closer
   // than its error, which the refinement cannot separate)
   const bool seedAll = keepAll && below && c.nk == 0;
-  const bool tail = below && keepAll && c.xFull && c.nk == 0 && st.prevErr <= o.tail_below;
+  const bool tail =
+      below && keepAll && c.xFull && c.nk == 0 && st.prevErr <= o.tail_below && !c.tailOff;
+  bool tailUsed = tail;
+  const double prevErrIn = st.prevErr;
   bool refineTimed = refined || tail;   // events 6 and 7 bracket a refinement
   if (c.nk > 0) {
     // k-points: per k H(k) (real n x n at time-reversal-invariant points, else the 2n
@@ -2001,7 +2054,9 @@ static int ctx_scc_step(Context& c) {
     // the tail at k-points (every point time-reversal invariant, mixed precision): no
     // eigensolve, Ogita-Aishima steps on each point's previous FP64 eigenpairs against its new
     // sparse H(k) and S(k); the eigenvalues of a point in the order of its vectors
-    const bool ktail = keepAll && below && allReal && c.xFullK && st.prevErr <= o.tail_below;
+    const bool ktail =
+        keepAll && below && allReal && c.xFullK && st.prevErr <= o.tail_below && !c.tailOff;
+    tailUsed = ktail;
     std::vector<std::vector<int>> kperm;
     if (ktail) {
       kperm.resize(c.nk);
@@ -2217,7 +2272,7 @@ static int ctx_scc_step(Context& c) {
       DH_HIP(hipEventRecord(ev[3], s));
       double* Y = c.eig.T.p;
       DH_TRY(scale_cols_sqrt<double>(s, c.X.p, n, c.f.p, Y, n, n, nocc));
-      DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, Y, n, Y, n, 0.0, c.P.p, n, true));
+      DH_TRY(pack_product<double>(c, n, nocc, Y, Y, c.P.p));
       DH_HIP(hipEventRecord(ev[4], s));
       DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
                                  c.rho.p));
@@ -2292,10 +2347,9 @@ static int ctx_scc_step(Context& c) {
       // scratch matrix, which is free once the eigenvectors are formed
       double* Y = c.eig.T.p;
       DH_TRY(scale_cols_sqrt<double>(s, c.X.p, n, c.f.p, Y, n, n, nocc));
-      DH_TRY(gemm<double>(s, false, true, n, n, nocc, 1.0, Y, n, Y, n, 0.0, c.P.p, n, true));
+      DH_TRY(pack_product<double>(c, n, nocc, Y, Y, c.P.p));
     } else
-      DH_TRY(gemm<float>(s, false, true, n, n, nocc, 1.0f, X32, n, X32, n, 0.0f, P32, n, true,
-                         kKFull, c.f32.p));
+      DH_TRY(pack_product<float>(c, n, nocc, X32, X32, P32, c.f32.p));
     DH_HIP(hipEventRecord(ev[4], s));
     if (c.xf64)
       DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
@@ -2348,6 +2402,14 @@ static int ctx_scc_step(Context& c) {
     err = std::max(err, std::fabs(diff[a]));
   }
   c.times[kTHost] += now() - th2;
+  if (tailUsed && err > 4.0 * prevErrIn && err > o.tail_below) {
+    // the tail's eigenpairs did not follow this change of H (many near-degenerate levels with
+    // different fillings, mixed beyond the first-order range): the charges are not used, the
+    // cycle goes on without the tail, and this iteration is repeated with an eigensolve
+    c.tailOff = true;
+    st.it -= 1;
+    return ctx_scc_step(c);
+  }
   DH_TRY(gamma_times(c, dqo, &vout));
   st.eScc = 0.0;
   for (int a = 0; a < nat; ++a) st.eScc += 0.5 * vout[a] * dqo[a];
@@ -2408,9 +2470,9 @@ static int ctx_scc_end(Context& c, double* energies, double* charges, int* niter
 // the first-order range in the last Ogita-Aishima step of the sparse refinement
 static int ctx_scc_state(Context& c, double* out, int n) {
   const SccState& st = c.st;
-  const double v[7] = {(double)st.it, st.err, st.eElec, st.eElecPrev, st.qinSum,
-                       (double)c.lastClipped, (double)c.lastFar};
-  for (int k = 0; k < std::min(n, 7); ++k) out[k] = v[k];
+  const double v[8] = {(double)st.it, st.err, st.eElec, st.eElecPrev, st.qinSum,
+                       (double)c.lastClipped, (double)c.lastFar, c.tailOff ? 1.0 : 0.0};
+  for (int k = 0; k < std::min(n, 8); ++k) out[k] = v[k];
   return 0;
 }
 
@@ -2451,8 +2513,7 @@ static int ctx_gradients(Context& c, double* grad) {
     float* X32 = reinterpret_cast<float*>(c.X.p);
     float* P32 = reinterpret_cast<float*>(c.P.p);
     DH_HIP(hipMemcpyAsync(c.f32.p, fe32.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
-    DH_TRY(gemm<float>(s, false, true, n, n, c.nocc, 1.0f, X32, n, X32, n, 0.0f, P32, n, true,
-                       kKFull, c.f32.p));
+    DH_TRY(pack_product<float>(c, n, c.nocc, X32, X32, P32, c.f32.p));
     DH_TRY(launch_pack<float>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, P32, n, c.erho.p));
     DH_HIP(hipStreamSynchronize(s));   // fe32 leaves scope
   } else {
@@ -2460,8 +2521,7 @@ static int ctx_gradients(Context& c, double* grad) {
     // W = (X diag(f e)) X^T: the scaled columns staged in the eigensolver's scratch matrix,
     // free once the eigenvectors are formed (the weights change sign, so no SYRK of square roots)
     DH_TRY(scale_cols<double>(s, c.X.p, n, c.feps.p, c.eig.T.p, n, n, c.nocc));
-    DH_TRY(gemm<double>(s, false, true, n, n, c.nocc, 1.0, c.eig.T.p, n, c.X.p, n, 0.0, c.P.p, n,
-                        true));
+    DH_TRY(pack_product<double>(c, n, c.nocc, c.eig.T.p, c.X.p, c.P.p));
     DH_TRY(launch_pack<double>(s, c.dblk.pr, c.dblk.norb.p, c.dblk.atomStart.p, c.P.p, n,
                                c.erho.p));
   }
@@ -3662,6 +3722,9 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
       DH_CHECK(value >= -1 && value <= 8, "set_tuning: parameter 10 takes -1 .. 8");
       dftbhip::g_tailBand = value;
       return 0;
+    case 12:
+      DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 12 takes 0 or 1");
+      return dftbhip::set_sytrd_graphs(value == 1);
     case 11:
       DH_CHECK(value >= 0 && value <= 15, "set_tuning: parameter 11 takes 0 .. 15");
       dftbhip::g_tailFineBelow = value == 0 ? 0.0 : std::pow(10.0, -value);

@@ -156,11 +156,17 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
 
 // C = alpha op(A) op(B) (n x n, K) on the 128 x 128 diagonal tiles and the `band` tile rows
 // below them only: element (i, j) with i / 128 - j / 128 in [0, band] is written, the rest of C
-// is untouched
+// is untouched (splitK > 1: K split over that many workgroups per tile, C first cleared whole)
 template <typename T>
 int gemm_band(hipStream_t s, bool ta, bool tb, int n, int K, T alpha, const T* A, long lda,
-              const T* B, long ldb, T* C, long ldc, int band);
+              const T* B, long ldb, T* C, long ldc, int band, int splitK = 1);
 constexpr int kGemmTile = 128;
+// the same product on the 128 x 128 tiles of C in tiles[0 .. ntiles-1] = (tile row, tile
+// column) only (w: per-k weights as in gemm); the rest of C is untouched
+template <typename T>
+int gemm_tiles(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T* A,
+               long lda, const T* B, long ldb, T beta, T* C, long ldc, const int2* tiles,
+               int ntiles, const T* w = nullptr);
 
 // tile order of full-output GEMMs: 0 the launch grid's, 1 XCD-grouped (the default)
 int set_gemm_order(int order);
@@ -244,6 +250,10 @@ int ormtr_lower_f32(hipStream_t s, const float* a, long lda, int n, const float*
                     long ldz, int ncols, void* work, size_t workBytes);
 size_t sytrd_work_bytes(int n, int elemSize);
 int set_symv_tiles(int tiles);   // row tiles per SYMV workgroup: 2, 4, 8, 16 or 0 (by size)
+// capture the tridiagonalisation's launch sequence in a HIP graph and replay it for repeated
+// calls on the same buffers (on by default); release_sytrd_graphs frees the kept graphs
+int set_sytrd_graphs(bool on);
+void release_sytrd_graphs();
 // autotuned tile tables of the SYMV (FP64: precision 0, FP32: 1), rows (first size, tiles);
 // detail: summed seconds per option (2, 4, 8, 16) and 256-row bucket of trailing sizes
 int symv_autotune(hipStream_t s, int n, int precision, std::vector<std::pair<int, int>>& rows,

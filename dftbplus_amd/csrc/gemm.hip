@@ -212,9 +212,14 @@ template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(int M, int N, int K, T alpha, const T* A,
                                                    long lda, const T* B, long ldb, T beta, T* C,
                                                    long ldc, int lowerOut, int klim, const T* w,
-                                                   int grouped, int band) {
+                                                   int grouped, int band,
+                                                   const int2* __restrict__ tileList) {
   int tm, tn;
-  if (band >= 0) {
+  if (tileList) {
+    const int2 t = tileList[blockIdx.x];
+    tm = t.x;
+    tn = t.y;
+  } else if (band >= 0) {
     // the diagonal tiles and the band tile rows below them: tile (tn + b, tn), b = 0 .. band
     tn = (int)blockIdx.x / (band + 1);
     tm = tn + (int)blockIdx.x % (band + 1);
@@ -269,7 +274,8 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
   }
 #define DH_GEMM_CASE(TA_, TB_)                                                           \
   hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(kNT), 0, s, M, N, K, alpha, A, \
-                     lda, B, ldb, beta, C, ldc, (int)lowerOut, (int)klim, w, (int)grouped, -1)
+                     lda, B, ldb, beta, C, ldc, (int)lowerOut, (int)klim, w, (int)grouped, -1, \
+                     (const int2*)nullptr)
   if (!ta && !tb) DH_GEMM_CASE(false, false);
   else if (!ta && tb) DH_GEMM_CASE(false, true);
   else if (ta && !tb) DH_GEMM_CASE(true, false);
@@ -281,14 +287,18 @@ int gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T*
 
 template <typename T>
 int gemm_band(hipStream_t s, bool ta, bool tb, int n, int K, T alpha, const T* A, long lda,
-              const T* B, long ldb, T* C, long ldc, int band) {
+              const T* B, long ldb, T* C, long ldc, int band, int splitK) {
   if (n <= 0) return 0;
   DH_CHECK(band >= 0, "gemm_band: negative band");
   const long t = cdiv(n, kBM);
-  const dim3 grid((unsigned)(t * (band + 1)), 1, 1);
+  // few tiles with a long K: split K so that the chip fills (partial sums added atomically)
+  splitK = std::max(1, std::min(splitK, K / (4 * kBK)));
+  if (splitK > 1) DH_TRY(fill2d<T>(s, C, ldc, n, n, T(0)));
+  const dim3 grid((unsigned)(t * (band + 1)), 1, (unsigned)splitK);
 #define DH_GEMM_BAND(TA_, TB_)                                                           \
   hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(kNT), 0, s, n, n, K, alpha, A, \
-                     lda, B, ldb, T(0), C, ldc, 0, (int)kKFull, (const T*)nullptr, 0, band)
+                     lda, B, ldb, T(0), C, ldc, 0, (int)kKFull, (const T*)nullptr, 0, band,   \
+                     (const int2*)nullptr)
   if (!ta && !tb) DH_GEMM_BAND(false, false);
   else if (!ta && tb) DH_GEMM_BAND(false, true);
   else if (ta && !tb) DH_GEMM_BAND(true, false);
@@ -298,7 +308,31 @@ int gemm_band(hipStream_t s, bool ta, bool tb, int n, int K, T alpha, const T* A
   return 0;
 }
 template int gemm_band<double>(hipStream_t, bool, bool, int, int, double, const double*, long,
-                               const double*, long, double*, long, int);
+                               const double*, long, double*, long, int, int);
+
+template <typename T>
+int gemm_tiles(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T* A,
+               long lda, const T* B, long ldb, T beta, T* C, long ldc, const int2* tiles,
+               int ntiles, const T* w) {
+  if (M <= 0 || N <= 0 || ntiles <= 0) return 0;
+  DH_CHECK(tiles, "gemm_tiles: null tile list");
+#define DH_GEMM_TILES(TA_, TB_)                                                              \
+  hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), dim3((unsigned)ntiles), dim3(kNT), 0, s, M, N, \
+                     K, alpha, A, lda, B, ldb, beta, C, ldc, 0, (int)kKFull, w, 0, -1, tiles)
+  if (!ta && !tb) DH_GEMM_TILES(false, false);
+  else if (!ta && tb) DH_GEMM_TILES(false, true);
+  else if (ta && !tb) DH_GEMM_TILES(true, false);
+  else DH_GEMM_TILES(true, true);
+#undef DH_GEMM_TILES
+  DH_LAUNCH_CHECK();
+  return 0;
+}
+template int gemm_tiles<double>(hipStream_t, bool, bool, int, int, int, double, const double*,
+                                long, const double*, long, double, double*, long, const int2*,
+                                int, const double*);
+template int gemm_tiles<float>(hipStream_t, bool, bool, int, int, int, float, const float*, long,
+                               const float*, long, float, float*, long, const int2*, int,
+                               const float*);
 
 template <typename T>
 int gemm_batched(hipStream_t s, bool ta, bool tb, const GemmDesc<T>* dDesc, int nBatch, int maxM,

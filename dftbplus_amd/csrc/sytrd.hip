@@ -11,6 +11,8 @@
 // and a chunk of <= 8 column blocks; row-block partial sums and transposed-tile partial sums go
 // to workspaces that the next kernel reduces (no atomics on the vector). This product is the
 // HBM-bound part of the eigensolver (it streams the trailing matrix once per column).
+#include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "common.h"
@@ -73,11 +75,15 @@ constexpr int kCHMin = 2;    // fewest row tiles per SYMV workgroup (sizes the s
 // row tiles per SYMV workgroup (2, 4, 8 or 16; 0: chosen from the trailing size), set through
 // set_symv_tiles
 static int g_symvTiles = 0;
+// bumped whenever the tile choice changes: captured launch sequences (below) are only replayed
+// while it is unchanged
+static unsigned g_tileGen = 0;
 
 int set_symv_tiles(int tiles) {
   DH_CHECK(tiles == 0 || tiles == 2 || tiles == 4 || tiles == 8 || tiles == 16,
            "symv tiles must be 0 (by size), 2, 4, 8 or 16");
   g_symvTiles = tiles;
+  ++g_tileGen;
   return 0;
 }
 
@@ -89,6 +95,7 @@ static std::vector<std::pair<int, int>> g_tileTable[2];
 int clear_symv_tables() {
   g_tileTable[0].clear();
   g_tileTable[1].clear();
+  ++g_tileGen;
   return 0;
 }
 
@@ -543,12 +550,8 @@ __global__ void panel_restore_kernel(T* a, long lda, int n, int j0, int pw, T* d
 // workspace block at work + k wsStride bytes (wsStride >= sytrd_layout(n)). The column kernels
 // run all problems in one launch each; the panel products and trailing updates run per problem.
 template <typename T>
-int sytrd_lower_batched_t(hipStream_t s, T* a, long lda, long sA, int n, T* d, T* e, T* tau,
+static int sytrd_launches(hipStream_t s, T* a, long lda, long sA, int n, T* d, T* e, T* tau,
                           long sV, void* work, size_t wsStride, int K) {
-  DH_CHECK(K >= 1, "sytrd: no problems");
-  DH_CHECK(wsStride >= sytrd_layout<T>(n, nullptr, nullptr), "sytrd: workspace too small");
-  DH_CHECK(K == 1 || (sA >= lda * (long)n && sV >= n && wsStride % 256 == 0),
-           "sytrd: problems overlap or are misaligned");
   std::vector<SytrdWs<T>> wk(K);
   for (int k = 0; k < K; ++k) {
     sytrd_layout<T>(n, &wk[k], static_cast<char*>(work) + (size_t)k * wsStride);
@@ -627,7 +630,99 @@ int sytrd_lower_batched_t(hipStream_t s, T* a, long lda, long sA, int n, T* d, T
       DH_LAUNCH_CHECK();
     }
   }
-  DH_TRY(symv_prof_collect(s));
+  return 0;
+}
+
+// The launch sequence of one tridiagonalisation (3 launches per column and the panel updates:
+// about 47 000 launches at n = 15769) depends only on the sizes, the buffers and the tile
+// choice, and holds no host synchronisation: it is captured once into a HIP graph and replayed
+// for every later call with the same arguments (an SCC cycle calls it once per iteration on
+// the same buffers), which removes the host launch cost between the short column kernels.
+// Cleared memory inside the sequence is written by fill kernels (no memset nodes). A few
+// sequences are kept, the oldest replaced; not while the per-launch SYMV timing is on.
+struct SytrdGraphKey {
+  const void* a;
+  long lda, sA, sV;
+  int n, K, prec;
+  const void *d, *e, *tau, *work;
+  size_t wsStride;
+  unsigned gen;
+  bool operator==(const SytrdGraphKey& o) const {
+    return a == o.a && lda == o.lda && sA == o.sA && sV == o.sV && n == o.n && K == o.K &&
+           prec == o.prec && d == o.d && e == o.e && tau == o.tau && work == o.work &&
+           wsStride == o.wsStride && gen == o.gen;
+  }
+};
+struct SytrdGraph {
+  SytrdGraphKey key;
+  hipGraphExec_t exec = nullptr;
+  unsigned long use = 0;
+};
+static std::vector<SytrdGraph> g_sytrdGraphs;
+static std::mutex g_sytrdGraphMutex;
+static unsigned long g_sytrdGraphUse = 0;
+static bool g_sytrdGraphOn = true;
+constexpr size_t kSytrdGraphsKept = 4;
+
+int set_sytrd_graphs(bool on) {
+  g_sytrdGraphOn = on;
+  return 0;
+}
+
+void release_sytrd_graphs() {
+  std::lock_guard<std::mutex> lock(g_sytrdGraphMutex);
+  for (auto& g : g_sytrdGraphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g_sytrdGraphs.clear();
+}
+
+template <typename T>
+int sytrd_lower_batched_t(hipStream_t s, T* a, long lda, long sA, int n, T* d, T* e, T* tau,
+                          long sV, void* work, size_t wsStride, int K) {
+  DH_CHECK(K >= 1, "sytrd: no problems");
+  DH_CHECK(wsStride >= sytrd_layout<T>(n, nullptr, nullptr), "sytrd: workspace too small");
+  DH_CHECK(K == 1 || (sA >= lda * (long)n && sV >= n && wsStride % 256 == 0),
+           "sytrd: problems overlap or are misaligned");
+  if (!g_sytrdGraphOn || g_symvProf.on || n < 2) {
+    DH_TRY(sytrd_launches<T>(s, a, lda, sA, n, d, e, tau, sV, work, wsStride, K));
+    DH_TRY(symv_prof_collect(s));
+    return 0;
+  }
+  std::lock_guard<std::mutex> lock(g_sytrdGraphMutex);
+  const SytrdGraphKey key{a, lda, sA, sV, n, K, (int)sizeof(T), d, e, tau, work, wsStride,
+                          g_tileGen};
+  for (auto& g : g_sytrdGraphs)
+    if (g.key == key) {
+      g.use = ++g_sytrdGraphUse;
+      DH_HIP(hipGraphLaunch(g.exec, s));
+      return 0;
+    }
+  DH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = sytrd_launches<T>(s, a, lda, sA, n, d, e, tau, sV, work, wsStride, K);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (rc != 0) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    DH_HIP(ec);
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  DH_HIP(ei);
+  if (g_sytrdGraphs.size() >= kSytrdGraphsKept) {
+    auto old = std::min_element(g_sytrdGraphs.begin(), g_sytrdGraphs.end(),
+                                [](const SytrdGraph& x, const SytrdGraph& y) {
+                                  return x.use < y.use;
+                                });
+    (void)hipGraphExecDestroy(old->exec);
+    g_sytrdGraphs.erase(old);
+  }
+  g_sytrdGraphs.push_back(SytrdGraph{key, exec, ++g_sytrdGraphUse});
+  DH_HIP(hipGraphLaunch(exec, s));
   return 0;
 }
 
@@ -846,6 +941,7 @@ static int symv_autotune_t(hipStream_t s, int n, std::vector<std::pair<int, int>
   int rc = 0;
   for (int o = 0; o < 4 && rc == 0; ++o) {
     g_symvTiles = kTuneOptions[o];
+    ++g_tileGen;
     // untimed pass on a small matrix first: the option's kernels are loaded before timing
     for (int nn : {kTuneBucket, n}) {
       hipLaunchKernelGGL(synth_sym_kernel<T>, dim3(cdiv(nn, 256), nn), dim3(256), 0, s, a.p,
@@ -865,6 +961,7 @@ static int symv_autotune_t(hipStream_t s, int n, std::vector<std::pair<int, int>
     for (const auto& r : rec) t[(size_t)o * nb + r.first / kTuneBucket] += 1e-3 * r.second;
   }
   g_symvTiles = tiles0;
+  ++g_tileGen;
   pf.on = on0;
   pf.launches = l0;
   pf.seconds = s0;
@@ -888,6 +985,7 @@ static int symv_autotune_t(hipStream_t s, int n, std::vector<std::pair<int, int>
   rows.front().first = 0;
   if (detail) *detail = t;
   g_tileTable[std::is_same<T, float>::value ? 1 : 0] = rows;
+  ++g_tileGen;
   return 0;
 }
 

@@ -78,7 +78,9 @@ int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e, doub
  * sums); param 10: residual
  * form, tile rows of X^T S X below its 128 x 128 diagonal tiles formed per step (1, the
  * default; -1 the whole lower triangle); param 11: residual form, iterations whose previous
- * charge error is at most 10^-value use arithmetic 1 (0, the default: never); param 9: vectors per lane of the tail's
+ * charge error is at most 10^-value use arithmetic 1 (0, the default: never); param 12: the
+ * tridiagonalisation's launch sequence is captured in a HIP graph and replayed for repeated
+ * calls on the same buffers (1, the default) or launched kernel by kernel (0); param 9: vectors per lane of the tail's
  * block-sparse products (1, the default, 2 or 4). No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)
@@ -258,7 +260,9 @@ int dftbhip_ctx_scc_end(dftbhip_ctx* ctx, double* energies, double* charges, int
  * of eigenvector pairs outside the first-order range of the last Ogita-Aishima refinement
  * step of the mixed-precision tail and (residual form of the refinement) the number of such
  * pairs beyond the diagonal band of X^T S X in the last refinement, for which the whole
- * X^T S X was formed (the last two have no counterpart in the reference). */
+ * X^T S X was formed, and 1 if the tail has been switched off for the rest of the cycle (a tail
+ * iteration whose charge error grew more than fourfold past tail_below is repeated with an
+ * eigensolve) (the last three have no counterpart in the reference). */
 int dftbhip_ctx_scc_state(dftbhip_ctx* ctx, double* out, int n);
 /* gradient dE/dR (3, nAtom); needs a converged dftbhip_ctx_scc */
 int dftbhip_ctx_gradients(dftbhip_ctx* ctx, double* grad);

@@ -54,3 +54,26 @@ def test_input_charges_restart(cluster216):
         dp2.set_input_charges(bad)
     dp.close()
     dp2.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sytrd_graph_replay_matches_launches(diamond2, precision):
+    """The tridiagonalisation replayed from its captured HIP graph (every SCC iteration after
+    the first on the same buffers) against the same cycle launched kernel by kernel: the same
+    iterations, energies, charges and gradients to rounding (FP64 and mixed paths)."""
+    from dftbplus_amd import api
+    geo, params, opts, calc, res = diamond2
+    out = []
+    for graphs in (1, 0):
+        api.set_tuning(12, graphs)
+        try:
+            dp = api.DftbPlus(params, geo, opts, precision=precision)
+            out.append((dp.get_energy(), dp.get_gross_charges(), dp.get_gradients(), dp.n_iter))
+            dp.close()
+        finally:
+            api.set_tuning(12, 1)
+    (e1, q1, g1, n1), (e0, q0, g0, n0) = out
+    print(f"graphs: {n1} iterations, launches: {n0}; dE {abs(e1 - e0):.2e}")
+    assert n1 == n0
+    assert abs(e1 - e0) < 1e-10 * geo.natom
+    assert np.max(np.abs(q1 - q0)) < 1e-10 and np.max(np.abs(g1 - g0)) < 1e-10
