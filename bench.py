@@ -818,6 +818,16 @@ def main():
                        f"iteration of {wl} ({ns} orbitals): {second:.2f} s"
                        + (f", scaled by (N/{ns})^3 to N={n}" if ns != n else "")
                        + f"; first iteration {first:.2f} s, set-up {setup:.1f} s")}
+        # the full-size measurement (bench.py --mode cpu, two oracle iterations of the
+        # 15769-orbital cell on the same host, about two minutes) where it is on record
+        full = os.path.join(ROOT, "profiles", f"cpu_baseline_{args.workload}.json")
+        if ns != n and os.path.exists(full):
+            with open(full) as fh:
+                fr = json.load(fh).get(args.workload)
+            if fr and fr.get("orbitals") == n:
+                rec["cpu_baseline"]["full_size_measured"] = {
+                    "value": 1.0 / fr["second_s"], "unit": "SCC iterations/s",
+                    "iteration_s": fr["second_s"], "source": os.path.relpath(full, ROOT)}
     print(json.dumps(rec), flush=True)
 
 
