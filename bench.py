@@ -513,7 +513,8 @@ def mode_kpoints(args):
     real) or shifted Monkhorst-Pack (k in {-1/4, 1/4}^3, complex). With --md-steps S: a cold
     converged SCC and forces, then S MD-like steps, each moving every coordinate by a uniform
     random amount in [-0.05, 0.05] bohr (seed 4) and running a converged SCC warm-started from
-    the previous step's input charges plus the forces."""
+    the previous step's input charges plus the forces; once per leg of --legs (the same
+    displacements), each leg's energies and forces compared with the first leg's."""
     import dataclasses
     import numpy as np
     from dftbplus_amd import api, synth
@@ -543,41 +544,89 @@ def mode_kpoints(args):
         out.update({"scc_iteration_s": times, "force_eval_s": time.perf_counter() - t2,
                     "phase_s": dp.timings()})
     else:
-        dp.set_charge_restart(True)
-        rng = np.random.default_rng(4)
-        coords = geo.coords.copy()
-        steps = []
-        for step in range(args.md_steps + 1):
-            if step > 0:
-                coords = coords + rng.uniform(-0.05, 0.05, coords.shape)
-                t_geo = time.perf_counter()
-                dp.set_geometry(coords, geo.latvecs)
-                geo_s = time.perf_counter() - t_geo
-            else:
-                geo_s = setup
-            t1 = time.perf_counter()
-            e = dp.get_energy()
-            scc_s = time.perf_counter() - t1
-            t2 = time.perf_counter()
-            g = dp.get_gradients()
-            grad_s = time.perf_counter() - t2
-            rec = {"step": step, "warm": step > 0, "iterations": dp.n_iter,
-                   "converged": dp.converged, "geometry_s": geo_s, "scc_s": scc_s,
-                   "gradients_s": grad_s, "energy": e, "max_force": float(np.abs(g).max())}
-            steps.append(rec)
-            print(f"[bench:kpoints] MD step {step}: {dp.n_iter} iterations, SCC {scc_s:.2f}s, "
-                  f"forces {grad_s:.2f}s, converged {dp.converged}", file=sys.stderr, flush=True)
-        warm = steps[1:]
-        per_step = [x["geometry_s"] + x["scc_s"] + x["gradients_s"] for x in warm]
-        out.update({"md_steps": args.md_steps, "displacement_bohr": 0.05, "steps": steps,
-                    "cold_force_eval_s": steps[0]["scc_s"] + steps[0]["gradients_s"],
-                    "warm_step_s_mean": float(np.mean(per_step)),
-                    "warm_force_evals_per_s": float(1.0 / np.mean(per_step)),
-                    "warm_iterations_mean": float(np.mean([x["iterations"] for x in warm])),
-                    "all_converged": all(x["converged"] for x in steps),
-                    "phase_s_last_step": dp.timings()})
+        dp.close()
+        legs = [x for x in args.legs.split(",") if x]
+        out["legs"] = {}
+        first = None
+        for leg in legs:
+            kw = dict(LEGS[leg])
+            tuning = kw.pop("tuning", {})
+            for p_, v_ in tuning.items():
+                api.set_tuning(p_, v_)
+            t0 = time.perf_counter()
+            dp = api.DftbPlus(params, geo, opts, **kw)
+            rec = md_steps(api, dp, geo, args, time.perf_counter() - t0, leg, opts.max_scc_iter)
+            dp.close()
+            for p_ in tuning:
+                api.set_tuning(p_, TUNING_DEFAULTS[p_])
+            if first is None:
+                first = rec
+            rec["max_dE_per_atom_vs_first_leg"] = float(max(
+                abs(a["energy"] - b["energy"]) for a, b in zip(rec["steps"], first["steps"])) /
+                geo.natom)
+            rec["max_dF_vs_first_leg"] = float(max(
+                np.abs(a["_grad"] - b["_grad"]).max()
+                for a, b in zip(rec["steps"], first["steps"])))
+            out["legs"][leg] = rec
+        for rec in out["legs"].values():
+            for x in rec["steps"]:
+                x.pop("_grad", None)
+        print(json.dumps(out), flush=True)
+        return
     print(json.dumps(out), flush=True)
     dp.close()
+
+
+def md_steps(api, dp, geo, args, setup, leg, max_it):
+    """--md-steps MD-like steps of one leg (see mode_kpoints), each SCC cycle stepped here with
+    one line per iteration (at most max_it); the gradients of each step are kept under "_grad"
+    for the comparison between legs."""
+    import numpy as np
+    dp.set_charge_restart(True)
+    rng = np.random.default_rng(4)
+    coords = geo.coords.copy()
+    steps = []
+    for step in range(args.md_steps + 1):
+        if step > 0:
+            coords = coords + rng.uniform(-0.05, 0.05, coords.shape)
+            t_geo = time.perf_counter()
+            dp.set_geometry(coords, geo.latvecs)
+            geo_s = time.perf_counter() - t_geo
+        else:
+            geo_s = setup
+        t1 = time.perf_counter()
+        dp.scc_begin()
+        it, conv = 0, False
+        while not conv and it < max_it:
+            t_it = time.perf_counter()
+            conv, err = dp.scc_step()
+            it += 1
+            print(f"[bench:kpoints:{leg}] MD step {step} iteration {it}: "
+                  f"{time.perf_counter() - t_it:.2f}s charge error {err:.3e}", file=sys.stderr,
+                  flush=True)
+        dp.scc_end()
+        e = dp.get_energy()
+        scc_s = time.perf_counter() - t1
+        t2 = time.perf_counter()
+        g = dp.get_gradients()
+        grad_s = time.perf_counter() - t2
+        rec = {"step": step, "warm": step > 0, "iterations": dp.n_iter,
+               "converged": dp.converged, "geometry_s": geo_s, "scc_s": scc_s,
+               "gradients_s": grad_s, "energy": e, "max_force": float(np.abs(g).max()),
+               "_grad": g}
+        steps.append(rec)
+        print(f"[bench:kpoints:{leg}] MD step {step}: {dp.n_iter} iterations, SCC "
+              f"{scc_s:.2f}s, forces {grad_s:.2f}s, converged {dp.converged}",
+              file=sys.stderr, flush=True)
+    warm = steps[1:]
+    per_step = [x["geometry_s"] + x["scc_s"] + x["gradients_s"] for x in warm]
+    return {"md_steps": args.md_steps, "displacement_bohr": 0.05, "steps": steps,
+            "cold_force_eval_s": steps[0]["scc_s"] + steps[0]["gradients_s"],
+            "warm_step_s_mean": float(np.mean(per_step)),
+            "warm_force_evals_per_s": float(1.0 / np.mean(per_step)),
+            "warm_iterations_mean": float(np.mean([x["iterations"] for x in warm])),
+            "all_converged": all(x["converged"] for x in steps),
+            "phase_s_last_step": dp.timings()}
 
 
 def mode_autograd(args):
