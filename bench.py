@@ -27,6 +27,7 @@ gradients is the leg's force evaluation. Legs:
                    with --legs);
   mixed_tail32     as mixed_tail with X^T H X and X^T S X as FP64 GEMMs and the correction
                    product in FP32 (only with --legs);
+  mixed_tail_early as mixed_tail with the switch at 3e-3 (only with --legs);
   mixed_fp32       FP32 iterations throughout (only with --legs; does not reach 1e-8 e).
 `value` is the SCC iterations/s of the fp64 leg's timed steps; every leg reports its timed
 rate, its converged cycle (iterations, seconds, force evaluations/s) and its energy, charge and
@@ -68,7 +69,9 @@ LEGS = {"fp64": dict(precision=0),
         "mixed_tail_fp32seed": dict(precision=1, refine_below=1e-3, refine_steps=2,
                                     tail_below=1e-3, tail_steps=2),
         "mixed_tail32": dict(precision=1, refine_below=1e-3, refine_steps=0, tail_below=1e-3,
-                             tail_steps=2, tuning={8: 1})}
+                             tail_steps=2, tuning={8: 1}),
+        "mixed_tail_early": dict(precision=1, refine_below=3e-3, refine_steps=0,
+                                 tail_below=3e-3, tail_steps=2)}
 # library defaults of the tuning parameters a leg or mode may change (restored after it)
 TUNING_DEFAULTS = {8: 2, 9: 1, 10: 1, 11: 0}
 E_BOUND, F_BOUND = 1e-6, 1e-5
@@ -279,12 +282,21 @@ def mode_cycle(args):
     geo, params, opts = synth.workload(args.workload)
     if args.mix:
         opts = dataclasses.replace(opts, mix_param=args.mix, max_scc_iter=300)
+    import numpy as np
+    res = {}
     for leg in [x for x in args.legs.split(",") if x]:
         r = run_cycle_leg(api, geo, params, opts, leg, args.steps, args.warmup)
+        res[leg] = r
+        first = next(iter(res.values()))
         print(json.dumps({"mode": "cycle", "workload": args.workload, "leg": leg,
                           "orbitals": r["n"], "converged": r["converged"],
                           "iterations": r["iterations"], "cycle_scc_s": r["scc_s"],
-                          "energy": r["energy"]}), flush=True)
+                          "energy": r["energy"],
+                          "vs_first_leg": {
+                              "dE_per_atom": abs(r["energy"] - first["energy"]) / geo.natom,
+                              "dq_max": float(np.max(np.abs(r["charges"] - first["charges"]))),
+                              "dF_max": float(np.max(np.abs(r["grad"] - first["grad"])))}}),
+              flush=True)
 
 
 def mode_tracesum(args):
