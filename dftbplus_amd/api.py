@@ -125,7 +125,7 @@ EXPORTS = {
     "dftbhip_ctx_scc": [C.c_void_p, _dp, _dp, _ip, _ip],
     "dftbhip_ctx_scc_begin": [C.c_void_p],
     "dftbhip_ctx_set_charge_restart": [C.c_void_p, C.c_int],
-    "dftbhip_ctx_set_input_charges": [C.c_void_p, C.c_int, _dp],
+    "dftbhip_ctx_set_input_charges": [C.c_void_p, C.c_int, _dp, C.c_int],
     "dftbhip_ctx_get_input_charges": [C.c_void_p, C.c_int, _dp],
     "dftbhip_ctx_scc_step": [C.c_void_p, _ip, _dp],
     "dftbhip_ctx_scc_end": [C.c_void_p, _dp, _dp, _ip, _ip],
@@ -610,11 +610,12 @@ class DftbPlus:
                "dftbhip_ctx_set_charge_restart")
 
     # one-iteration control of the SCC cycle (used to time single iterations)
-    def set_input_charges(self, populations: np.ndarray):
+    def set_input_charges(self, populations: np.ndarray, check_count: bool = True):
         """Electron population per atom for the next SCC cycle (ReadInitialCharges): their
-        sum must equal the number of electrons to 1e-3."""
+        sum must equal the number of electrons to 1e-3 unless check_count is False (the
+        reference's SkipChargeTest)."""
         q = np.ascontiguousarray(populations, dtype=np.float64)
-        _check(lib().dftbhip_ctx_set_input_charges(self.h, len(q), _d(q)),
+        _check(lib().dftbhip_ctx_set_input_charges(self.h, len(q), _d(q), int(check_count)),
                "dftbhip_ctx_set_input_charges")
 
     def get_input_charges(self) -> np.ndarray:

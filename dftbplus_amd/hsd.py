@@ -356,6 +356,13 @@ class Input:
     results_tag: bool = False
     detailed_out: bool = True      # Options { WriteDetailedOut }, default Yes
     band_out: bool = True          # Options { WriteBandOut }, default Yes
+    # charge files (parser.F90:3351, 4204-4216): Hamiltonian { ReadInitialCharges }, Options
+    # { ReadChargesAsText, WriteCharges (default Yes), WriteChargesAsText, SkipChargeTest }
+    read_charges: bool = False
+    read_charges_ascii: bool = False
+    write_charges: bool = True
+    write_charges_ascii: bool = False
+    skip_charge_test: bool = False
 
 
 _UNSUPPORTED = ["SpinPolarisation", "SpinOrbit", "Dispersion", "ThirdOrder", "ThirdOrderFull",
@@ -561,7 +568,15 @@ def load_input(path: str) -> Input:
                  autotest_tag=_yes(options.child("WriteAutotestTag") if options else None, False),
                  results_tag=_yes(options.child("WriteResultsTag") if options else None, False),
                  detailed_out=_yes(options.child("WriteDetailedOut") if options else None, True),
-                 band_out=_yes(options.child("WriteBandOut") if options else None, True))
+                 band_out=_yes(options.child("WriteBandOut") if options else None, True),
+                 read_charges=_yes(ham.child("ReadInitialCharges"), False),
+                 read_charges_ascii=_yes(options.child("ReadChargesAsText") if options else None,
+                                         False),
+                 write_charges=_yes(options.child("WriteCharges") if options else None, True),
+                 write_charges_ascii=_yes(options.child("WriteChargesAsText") if options
+                                          else None, False),
+                 skip_charge_test=_yes(options.child("SkipChargeTest") if options else None,
+                                       False))
 
 
 # ------------------------------------------------------------------------------------------
@@ -577,9 +592,29 @@ def run(path: str, out_dir: Optional[str] = None, **backend) -> dict:
         out_dir = path if os.path.isdir(path) else os.path.dirname(os.path.abspath(path))
     g = inp.geometry
     geo = Geometry(g.coords, g.species, g.latvecs)
+    in_dir = path if os.path.isdir(path) else os.path.dirname(os.path.abspath(path))
+    norb = [inp.params.species[s_].norb for s_ in geo.species]
     dp = api.DftbPlus(inp.params, geo, inp.options, **backend)
     try:
+        if inp.read_charges:
+            # initQFromFile: the orbital populations of the file, summed per atom (the
+            # context's charges are atom resolved)
+            q = outfiles.read_charges(os.path.join(in_dir, "charges"), norb,
+                                      inp.read_charges_ascii)
+            dp.set_input_charges(np.array([float(x.sum()) for x in q[0]]),
+                                 check_count=not inp.skip_charge_test)
         res = _results(dp, inp, geo)
+        if inp.write_charges:
+            # writeQToFile: the input charges of the next iteration, each atom's population in
+            # its first orbital (as the reference's files of atom-resolved charges hold them)
+            qin = dp.get_input_charges()
+            pops = []
+            for a, n_ in enumerate(norb):
+                v = np.zeros(n_)
+                v[0] = qin[a]
+                pops.append(v)
+            outfiles.write_charges(os.path.join(out_dir, "charges"), pops,
+                                   inp.write_charges_ascii)
     finally:
         dp.close()
     if inp.autotest_tag:

@@ -272,3 +272,159 @@ def detailed_out(res: dict, geo, params, scc_iter: int, diff_elec: float, scc_er
             L.append(format1ue("Volume", abs(float(np.linalg.det(geo.latvecs))), "au^3"))
             L.append("")
     return "\n".join(L) + "\n"
+
+
+# ------------------------------------------------------------------------------------------
+# charge files (charges.dat / charges.bin)
+# ------------------------------------------------------------------------------------------
+RESTART_FORMAT = 8          # restartFormat of sccinit.F90
+ELEC_TOL_MAX = 1.0e-7       # elecTolMax of accuracy.F90: the file's checksum tolerance
+
+
+class ChargeFileError(ValueError):
+    pass
+
+
+def _fortran_logical(tok: str) -> bool:
+    t = tok.strip().upper().lstrip(".")
+    if t.startswith("T"):
+        return True
+    if t.startswith("F"):
+        return False
+    raise ChargeFileError(f"not a logical: {tok!r}")
+
+
+def _fortran_real(tok: str) -> float:
+    return float(tok.replace("D", "E").replace("d", "e"))
+
+
+class _ListReader:
+    """List-directed READs: every READ starts a new record and takes values across records
+    when one holds too few; what a record holds beyond a READ's list is skipped."""
+
+    def __init__(self, text: str):
+        self.lines = text.splitlines()
+        self.pos = 0
+
+    def read(self, n: int):
+        vals = []
+        while len(vals) < n:
+            if self.pos >= len(self.lines):
+                raise ChargeFileError("Error during reading external file of charge data")
+            vals += self.lines[self.pos].replace(",", " ").split()
+            self.pos += 1
+        return vals[:n]
+
+
+class _RecordReader:
+    """gfortran unformatted sequential records: a 4-byte length before and after each."""
+
+    def __init__(self, data: bytes):
+        self.data = data
+        self.pos = 0
+
+    def record(self) -> bytes:
+        import struct
+        if self.pos + 4 > len(self.data):
+            raise ChargeFileError("Error during reading external file of charge data")
+        (n,) = struct.unpack_from("<i", self.data, self.pos)
+        rec = self.data[self.pos + 4:self.pos + 4 + n]
+        self.pos += 8 + n
+        return rec
+
+
+def read_charges(base: str, norb: Sequence[int], ascii: bool, n_spin: int = 1):
+    """The orbital populations of a charge file as initQFromFile (sccinit.F90:185-615) reads
+    them for a calculation without block charges, density matrices or multipoles: base + '.dat'
+    (list-directed text) or base + '.bin' (unformatted), formats 4 to 8. Returns a list of
+    n_spin lists with one array of populations per atom (atoms missing from the file: zeros).
+    Raises ChargeFileError with the reference's message on a format, atom-count, spin or
+    checksum mismatch."""
+    import struct
+    nat = len(norb)
+    if ascii:
+        with open(base + ".dat") as fh:
+            rd = _ListReader(fh.read())
+        fmt = int(rd.read(1)[0])
+    else:
+        with open(base + ".bin", "rb") as fh:
+            rr = _RecordReader(fh.read())
+        (fmt,) = struct.unpack_from("<i", rr.record(), 0)
+    nlog = {4: 3, 5: 3, 6: 4, 7: 5, 8: 5}.get(fmt)
+    if nlog is None:
+        raise ChargeFileError("Incompatible file type for external charge data")
+    has_natom = fmt >= 5
+    if ascii:
+        # one READ: the flags, (the atom count,) the spin count and a checksum per spin of the
+        # calculation
+        nhead = nlog + (2 if has_natom else 1)
+        head = rd.read(nhead + n_spin)
+        flags = [_fortran_logical(t) for t in head[:nlog]]
+        nat_file = int(head[nlog]) if has_natom else nat
+        spin = int(head[nhead - 1])
+        checksum = [_fortran_real(t) for t in head[nhead:]]
+    else:
+        rec = rr.record()
+        flags = [v != 0 for v in struct.unpack_from(f"<{nlog}i", rec, 0)]
+        o = 4 * nlog
+        if has_natom:
+            (nat_file,) = struct.unpack_from("<i", rec, o)
+            o += 4
+        else:
+            nat_file = nat
+        (spin,) = struct.unpack_from("<i", rec, o)
+        o += 4
+        checksum = list(struct.unpack_from(f"<{n_spin}d", rec, o))
+    if any(flags):
+        raise ChargeFileError("block charges, density matrices, k-point data or multipoles in "
+                              "the charge file are not supported here")
+    if nat_file > nat:
+        raise ChargeFileError("External charge file has more atoms than are present in the "
+                              "system")
+    if spin != n_spin:
+        raise ChargeFileError("Incorrect number of spins in restart file")
+    q = [[np.zeros(norb[a]) for a in range(nat)] for _ in range(spin)]
+    for s_ in range(spin):
+        for a in range(nat_file):
+            if ascii:
+                q[s_][a] = np.array([_fortran_real(t) for t in rd.read(norb[a])])
+            else:
+                q[s_][a] = np.array(struct.unpack_from(f"<{norb[a]}d", rr.record(), 0))
+    for s_ in range(spin):
+        tot = sum(float(q[s_][a].sum()) for a in range(nat_file))
+        if abs(checksum[s_] - tot) > ELEC_TOL_MAX:
+            raise ChargeFileError("Error during reading external file of charge data - internal "
+                                  "checksum failure, probably a damaged file")
+    return q
+
+
+def write_charges(base: str, populations: Sequence[np.ndarray], ascii: bool) -> str:
+    """A charge file in restart format 8 as writeQToFile (sccinit.F90:619-905) writes it for a
+    spin-unpolarised calculation without block charges, density matrices or multipoles:
+    populations holds one array of orbital populations per atom. Text: list-directed records
+    to base + '.dat'; else unformatted records to base + '.bin'. Returns the path."""
+    import struct
+    nat = len(populations)
+    tot = float(sum(np.sum(p) for p in populations))
+    if ascii:
+        path = base + ".dat"
+        lines = [list_directed(RESTART_FORMAT),
+                 " " + " ".join(["F"] * 5) + list_directed(nat, 1, tot)[1:]]
+        for p in populations:
+            lines.append(list_directed(*[float(x) for x in p]))
+        with open(path, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+    else:
+        path = base + ".bin"
+
+        def rec(payload: bytes) -> bytes:
+            n = struct.pack("<i", len(payload))
+            return n + payload + n
+        out = rec(struct.pack("<i", RESTART_FORMAT))
+        out += rec(struct.pack("<5i", 0, 0, 0, 0, 0) + struct.pack("<2i", nat, 1) +
+                   struct.pack("<d", tot))
+        for p in populations:
+            out += rec(struct.pack(f"<{len(p)}d", *[float(x) for x in p]))
+        with open(path, "wb") as fh:
+            fh.write(out)
+    return path

@@ -240,11 +240,13 @@ int dftbhip_ctx_set_charge_restart(dftbhip_ctx* ctx, int warm);
 /* Input charges of the next SCC cycle, as ReadInitialCharges = Yes reads them from a charge
  * file (initQFromFile, src/dftbp/dftb/sccinit.F90:185-615, called from
  * src/dftbp/dftbplus/initprogram.F90:4604-4630): qin[nAtom] are the atoms' electron
- * populations (the context's charges are atom resolved); their sum must equal the number of
- * electrons to 1e-3, as the reference checks ("External file of charges has a total charge:
- * ..., instead of ..."). Used by the next dftbhip_ctx_scc_begin / dftbhip_ctx_scc only; call
- * after dftbhip_ctx_set_geometry. */
-int dftbhip_ctx_set_input_charges(dftbhip_ctx* ctx, int nAtom, const double* qin);
+ * populations (the context's charges are atom resolved); with checkCount != 0 their sum must
+ * equal the number of electrons to 1e-3, as the reference checks ("External file of charges
+ * has a total charge: ..., instead of ..."); checkCount = 0 skips the check, as its
+ * SkipChargeTest = Yes does. Used by the next dftbhip_ctx_scc_begin / dftbhip_ctx_scc only;
+ * call after dftbhip_ctx_set_geometry. */
+int dftbhip_ctx_set_input_charges(dftbhip_ctx* ctx, int nAtom, const double* qin,
+                                  int checkCount);
 /* The input charges of the next SCC iteration (the mixed charges after the last step), as the
  * reference writes them to its charge file every iteration (writeCharges, mainio.F90:4319-4370
  * -> writeQToFile, src/dftbp/dftb/sccinit.F90:619-905, called from main.F90:1016): electron

@@ -58,3 +58,29 @@ def test_input_directory_single_point(kind, tmp_path):
 
 def dp_norb(params, geo):
     return int(sum(params.species[s].norb for s in geo.species))
+
+
+@pytest.mark.parametrize("ascii", [False, True])
+def test_read_initial_charges(ascii, tmp_path):
+    """WriteCharges / ReadInitialCharges through the input directory: the first run writes the
+    charge file (charges.bin by default, charges.dat with WriteChargesAsText), a second run with
+    ReadInitialCharges = Yes reads it and is converged within three iterations, at the same
+    energy and charges."""
+    from dftbplus_amd import hsd
+    geo, el, params, files, opts = _case("cluster27")
+    hsd.export_input(str(tmp_path), geo, el, files, opts)
+    inp = tmp_path / "dftb_in.hsd"
+    text = inp.read_text()
+    extra = "  WriteChargesAsText = Yes\n  ReadChargesAsText = Yes\n" if ascii else ""
+    assert "Options {" in text
+    text = text.replace("Options {", "Options {\n" + extra, 1)
+    inp.write_text(text)
+    res = hsd.run(str(tmp_path))
+    assert (tmp_path / ("charges.dat" if ascii else "charges.bin")).exists()
+    inp.write_text(text.replace("SCC = Yes", "SCC = Yes\n  ReadInitialCharges = Yes", 1))
+    res2 = hsd.run(str(tmp_path))
+    print(f"restart ({'text' if ascii else 'binary'}): {res2['n_iter']} iterations "
+          f"(cold {res['n_iter']})")
+    assert res2["converged"] and res2["n_iter"] <= 3 < res["n_iter"]
+    assert abs(res2["mermin_energy"] - res["mermin_energy"]) / geo.natom < 1e-10
+    assert np.max(np.abs(res2["gross_charges"] - res["gross_charges"])) < 1e-9

@@ -1,10 +1,13 @@
 """band.out and detailed.out writers (src/dftbp/dftbplus/mainio.F90:2459-2507, 2794-3734):
 Fortran edit descriptors against hand-derived records, band.out records in the layout the
 reference's own band.out reader accepts (tools/dptools bandout.py: optional index column,
-energy, filling), and the detailed.out blocks of an SCC single point."""
+energy, filling), and the detailed.out blocks of an SCC single point; charge files
+(src/dftbp/dftb/sccinit.F90:185-905) against the reference's own charges.dat files."""
+import os
 import re
 
 import numpy as np
+import pytest
 
 from dftbplus_amd import outfiles as O
 
@@ -89,3 +92,59 @@ def test_detailed_out_blocks():
     orb = lines.index("Orbital populations (up)")
     first = lines[orb + 2].split()
     assert first[0] == "1" and first[-1] in ("s", "p_y")
+
+
+def test_charge_files_of_the_reference(tmp_path):
+    """initQFromFile on the reference's own charge files (format 4, text): the populations per
+    atom and the checksum; a damaged value fails the checksum with the reference's message,
+    and a system with fewer atoms than the file is refused."""
+    import shutil
+    from dftbplus_amd import outfiles
+    gold = os.path.join(os.path.dirname(__file__), "golden", "dftbplus", "charges")
+    shutil.copy(os.path.join(gold, "2H2O-3rdorder_read.charges.dat"), tmp_path / "charges.dat")
+    q = outfiles.read_charges(str(tmp_path / "charges"), [4, 4, 1, 1, 1, 1], True)
+    sums = np.array([x.sum() for x in q[0]])
+    assert len(q) == 1 and np.allclose(sums[:2], [6.7277403366336541, 6.6870429048902755],
+                                       rtol=0, atol=1e-15)
+    assert abs(sums.sum() - 16.0) < 1e-7
+    assert np.all(q[0][0][1:] == 0.0)
+    # a change of 1e-6 e in one value: beyond the checksum tolerance (1e-7)
+    text = (tmp_path / "charges.dat").read_text().replace("0.63772931009254918",
+                                                           "0.63773031009254918")
+    (tmp_path / "charges.dat").write_text(text)
+    with pytest.raises(outfiles.ChargeFileError, match="internal checksum failure"):
+        outfiles.read_charges(str(tmp_path / "charges"), [4, 4, 1, 1, 1, 1], True)
+    shutil.copy(os.path.join(gold, "H3_skipcheck.charges.dat"), tmp_path / "charges.dat")
+    q = outfiles.read_charges(str(tmp_path / "charges"), [1, 1, 1], True)
+    assert abs(sum(x.sum() for x in q[0]) - 3.0000000000000009) < 1e-12
+    # format 4 holds no atom count: read for two atoms, the third population is left out and
+    # the checksum fails, as in the reference
+    with pytest.raises(outfiles.ChargeFileError, match="internal checksum failure"):
+        outfiles.read_charges(str(tmp_path / "charges"), [1, 1], True)
+    with pytest.raises(outfiles.ChargeFileError, match="number of spins"):
+        outfiles.read_charges(str(tmp_path / "charges"), [1, 1, 1], True, n_spin=2)
+    # format 8 holds it
+    outfiles.write_charges(str(tmp_path / "c8"), [np.array([1.0])] * 3, True)
+    with pytest.raises(outfiles.ChargeFileError, match="more atoms"):
+        outfiles.read_charges(str(tmp_path / "c8"), [1, 1], True)
+
+
+@pytest.mark.parametrize("ascii", [True, False])
+def test_charge_file_round_trip(tmp_path, ascii):
+    """writeQToFile (format 8, text or unformatted records) read back by initQFromFile: the
+    same populations, header flags all false, the checksum consistent."""
+    import struct
+    from dftbplus_amd import outfiles
+    pops = [np.array([6.25, 0.0, 0.0, 0.0]), np.array([0.875]), np.array([0.875])]
+    path = outfiles.write_charges(str(tmp_path / "charges"), pops, ascii)
+    q = outfiles.read_charges(str(tmp_path / "charges"), [4, 1, 1], ascii)
+    for a, b in zip(q[0], pops):
+        assert np.array_equal(a, b)
+    if ascii:
+        lines = open(path).read().splitlines()
+        assert lines[0].split() == ["8"] and lines[1].split()[:5] == ["F"] * 5
+        assert lines[1].split()[5:7] == ["3", "1"]
+    else:
+        data = open(path, "rb").read()
+        (n,) = struct.unpack_from("<i", data, 0)
+        assert n == 4 and struct.unpack_from("<i", data, 4)[0] == 8
