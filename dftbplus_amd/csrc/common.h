@@ -251,7 +251,7 @@ int ormtr_lower_f32(hipStream_t s, const float* a, long lda, int n, const float*
 size_t sytrd_work_bytes(int n, int elemSize);
 int set_symv_tiles(int tiles);   // row tiles per SYMV workgroup: 2, 4, 8, 16 or 0 (by size)
 // capture the tridiagonalisation's launch sequence in a HIP graph and replay it for repeated
-// calls on the same buffers (on by default); release_sytrd_graphs frees the kept graphs
+// calls on the same buffers (off by default); release_sytrd_graphs frees the kept graphs
 int set_sytrd_graphs(bool on);
 void release_sytrd_graphs();
 // autotuned tile tables of the SYMV (FP64: precision 0, FP32: 1), rows (first size, tiles);

@@ -938,17 +938,27 @@ __global__ __launch_bounds__(256) void spmm_fold_t_kernel(
   for (int v = 0; v < kVpl; ++v)
 #pragma unroll
     for (int r = 0; r < 4; ++r) ah[v][r] = as[v][r] = 0.0;
-  const int g1 = gPtr[A + 1];
-  for (int g = gPtr[A]; g < g1; ++g) {
+  const int g0 = gPtr[A], g1 = gPtr[A + 1];
+  // the coefficient rows of the next group are loaded while the current one is multiplied
+  // (rows past the column atom's orbitals meet zero block columns; clamped to stay inside Xt)
+  double xn[kVpl][4];
+  auto load_x = [&](int g) {
     const int oB = colOff[g];
-    double xv[kVpl][4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      // rows past the column atom's orbitals meet zero block columns; clamped to stay inside Xt
       const long row = min(oB + c, n - 1);
 #pragma unroll
-      for (int v = 0; v < kVpl; ++v) xv[v][c] = xj[v][row * ldx];
+      for (int v = 0; v < kVpl; ++v) xn[v][c] = xj[v][row * ldx];
     }
+  };
+  if (g0 < g1) load_x(g0);
+  for (int g = g0; g < g1; ++g) {
+    double xv[kVpl][4];
+#pragma unroll
+    for (int v = 0; v < kVpl; ++v)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xv[v][c] = xn[v][c];
+    if (g + 1 < g1) load_x(g + 1);
     const double* __restrict__ b = fold + (long)g * 32;
     double bh[16], bs[16];
 #pragma unroll

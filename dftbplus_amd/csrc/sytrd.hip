@@ -637,7 +637,8 @@ static int sytrd_launches(hipStream_t s, T* a, long lda, long sA, int n, T* d, T
 // about 47 000 launches at n = 15769) depends only on the sizes, the buffers and the tile
 // choice, and holds no host synchronisation: it is captured once into a HIP graph and replayed
 // for every later call with the same arguments (an SCC cycle calls it once per iteration on
-// the same buffers), which removes the host launch cost between the short column kernels.
+// the same buffers), which removes the host launch cost between the short column kernels
+// (when switched on: set_sytrd_graphs).
 // Cleared memory inside the sequence is written by fill kernels (no memset nodes). A few
 // sequences are kept, the oldest replaced; not while the per-launch SYMV timing is on.
 struct SytrdGraphKey {
@@ -661,7 +662,9 @@ struct SytrdGraph {
 static std::vector<SytrdGraph> g_sytrdGraphs;
 static std::mutex g_sytrdGraphMutex;
 static unsigned long g_sytrdGraphUse = 0;
-static bool g_sytrdGraphOn = true;
+// off by default: the replay saves about 2 % of an FP32 iteration, and rocprofv3's kernel
+// tracing takes the process down on the capture (a host fault inside the tool)
+static bool g_sytrdGraphOn = false;
 constexpr size_t kSytrdGraphsKept = 4;
 
 int set_sytrd_graphs(bool on) {

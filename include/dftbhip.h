@@ -80,7 +80,8 @@ int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e, doub
  * default; -1 the whole lower triangle); param 11: residual form, iterations whose previous
  * charge error is at most 10^-value use arithmetic 1 (0, the default: never); param 12: the
  * tridiagonalisation's launch sequence is captured in a HIP graph and replayed for repeated
- * calls on the same buffers (1, the default) or launched kernel by kernel (0); param 9: vectors per lane of the tail's
+ * calls on the same buffers (1) or launched kernel by kernel (0, the default: the replay saves
+ * about 2 % of an FP32 iteration, and rocprofv3 kernel tracing fails on the capture); param 9: vectors per lane of the tail's
  * block-sparse products (1, the default, 2 or 4). No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)

@@ -71,7 +71,7 @@ def test_sytrd_graph_replay_matches_launches(diamond2, precision):
             out.append((dp.get_energy(), dp.get_gross_charges(), dp.get_gradients(), dp.n_iter))
             dp.close()
         finally:
-            api.set_tuning(12, 1)
+            api.set_tuning(12, 0)
     (e1, q1, g1, n1), (e0, q0, g0, n0) = out
     print(f"graphs: {n1} iterations, launches: {n0}; dE {abs(e1 - e0):.2e}")
     assert n1 == n0
