@@ -549,6 +549,16 @@ struct BandWs {
   int* abortFlag; // 1
 };
 
+// Stage 2 on one workgroup of an ordinary launch instead of the cooperative grid: the sweeps
+// then run in order, so every progress wait is already met when it is polled. Taken when the
+// device refuses the cooperative launch (while another process holds the device's cooperative
+// launch resource, hipLaunchCooperativeKernel returns hipErrorIllegalState) or when forced.
+static bool g_sb2stSingle = false;
+int set_sb2st_single(bool on) {
+  g_sb2stSingle = on;
+  return 0;
+}
+
 template <typename T>
 static size_t band_layout(int n, BandWs<T>* ws, char* base) {
   size_t off = 0;
@@ -661,8 +671,19 @@ int sytrd_2stage_t(hipStream_t s, T* a, long lda, int n, T* d, T* e, void* work,
     int nn = n, nsw = nsweep, km = kmax;
     long spinLimit = 1L << 24;
     void* args[] = {&AB, &nn, &nsw, &km, &V2, &tau2, &prog, &abortFlag, &spinLimit};
-    DH_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(sb2st_kernel<T>), dim3(grid),
-                                      dim3(256), args, 0, s));
+    hipError_t err = hipErrorIllegalState;
+    if (!g_sb2stSingle)
+      err = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(sb2st_kernel<T>),
+                                       dim3(grid), dim3(256), args, 0, s);
+    if (err == hipErrorIllegalState || err == hipErrorCooperativeLaunchTooLarge ||
+        err == hipErrorNotSupported) {
+      (void)hipGetLastError();   // the refused launch ran nothing; prog is still zero
+      hipLaunchKernelGGL(sb2st_kernel<T>, dim3(1), dim3(256), 0, s, AB, nn, nsw, km, V2, tau2,
+                         prog, abortFlag, spinLimit);
+      DH_LAUNCH_CHECK();
+    } else {
+      DH_HIP(err);
+    }
     int hAbort = 0;
     DH_HIP(hipMemcpyAsync(&hAbort, ws.abortFlag, sizeof(int), hipMemcpyDeviceToHost, s));
     DH_HIP(hipStreamSynchronize(s));

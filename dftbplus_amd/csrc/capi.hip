@@ -3727,6 +3727,9 @@ extern "C" int dftbhip_set_tuning(int param, int value) {
     case 12:
       DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 12 takes 0 or 1");
       return dftbhip::set_sytrd_graphs(value == 1);
+    case 13:
+      DH_CHECK(value == 0 || value == 1, "set_tuning: parameter 13 takes 0 or 1");
+      return dftbhip::set_sb2st_single(value == 1);
     case 11:
       DH_CHECK(value >= 0 && value <= 15, "set_tuning: parameter 11 takes 0 .. 15");
       dftbhip::g_tailFineBelow = value == 0 ? 0.0 : std::pow(10.0, -value);

@@ -254,6 +254,9 @@ int set_symv_tiles(int tiles);   // row tiles per SYMV workgroup: 2, 4, 8, 16 or
 // calls on the same buffers (off by default); release_sytrd_graphs frees the kept graphs
 int set_sytrd_graphs(bool on);
 void release_sytrd_graphs();
+// two-stage tridiagonalisation: bulge chasing on one workgroup of an ordinary launch (the path
+// taken when the device refuses the cooperative launch) instead of the cooperative grid
+int set_sb2st_single(bool on);
 // autotuned tile tables of the SYMV (FP64: precision 0, FP32: 1), rows (first size, tiles);
 // detail: summed seconds per option (2, 4, 8, 16) and 256-row bucket of trailing sizes
 int symv_autotune(hipStream_t s, int n, int precision, std::vector<std::pair<int, int>>& rows,

@@ -82,7 +82,11 @@ int dftbhip_dsytrd_noalloc(int n, double* a, int lda, double* d, double* e, doub
  * tridiagonalisation's launch sequence is captured in a HIP graph and replayed for repeated
  * calls on the same buffers (1) or launched kernel by kernel (0, the default: the replay saves
  * about 2 % of an FP32 iteration, and rocprofv3 kernel tracing fails on the capture); param 9: vectors per lane of the tail's
- * block-sparse products (1, the default, 2 or 4). No counterpart in the reference. */
+ * block-sparse products (1, the default, 2 or 4); param 13: the two-stage tridiagonalisation's
+ * bulge chasing runs on one workgroup of an ordinary launch (1) or on the cooperative grid (0,
+ * the default; the one-workgroup launch is taken by itself when the device refuses the
+ * cooperative one, as it does while another process holds its cooperative-launch resource).
+ * No counterpart in the reference. */
 int dftbhip_set_tuning(int param, int value);
 /* Half neighbour list on the device, as updateNeighbourList (src/dftbp/dftb/periodic.F90:446-696)
  * builds it: atom a lists itself, then the images of atoms b >= a within cutoff by increasing

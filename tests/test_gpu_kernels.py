@@ -120,6 +120,25 @@ def test_sygvd_random(n):
     assert np.allclose(np.tril(l), np.linalg.cholesky(s), atol=1e-12)
 
 
+def test_sygvd_single_workgroup_bulge_chasing():
+    """The one-workgroup launch of the bulge chasing (the path taken when the device refuses
+    the cooperative launch) runs the same sweep steps on the same data: the same bits."""
+    api = _api()
+    n = 333
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((n, n))
+    s = x @ x.T / n + np.eye(n)
+    h = rng.standard_normal((n, n))
+    h = h + h.T
+    w0, c0, _ = api.sygvd(h, s)
+    api.set_tuning(13, 1)
+    try:
+        w1, c1, _ = api.sygvd(h, s)
+    finally:
+        api.set_tuning(13, 0)
+    assert np.array_equal(w0, w1) and np.array_equal(c0, c1)
+
+
 def test_sygvd_not_positive_definite():
     api = _api()
     s = np.eye(4)
